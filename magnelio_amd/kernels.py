@@ -1,0 +1,121 @@
+"""Python wrappers of the libmgh entry points over torch device tensors.
+
+Device buffers are torch tensors on the ROCm device (PyTorch is only the
+allocator and stream provider here); every call passes raw pointers, sizes
+and the current HIP stream through the C-ABI.  Shapes and dtypes are
+checked on the host before any launch.
+
+Kernel-tier counterparts in the reference:
+``update_E_fused_cuda`` / ``update_H_fused_cuda``
+(src/_operators/numba_kernels.py:518-552): in place, float32/float64 only,
+``TypeError`` for any other dtype (:483-488).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+DEFAULT_KCHUNK = 16
+
+
+def _dtype_code(t: torch.Tensor) -> int:
+    if t.dtype == torch.float32:
+        return 4
+    if t.dtype == torch.float64:
+        return 8
+    raise TypeError(f"HIP fused kernels support float32/float64, got {t.dtype}")
+
+
+def stream_handle(stream: torch.cuda.Stream | None = None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def _check_dev(*ts: torch.Tensor) -> None:
+    for t in ts:
+        if not t.is_cuda:
+            raise ValueError("HIP backend buffers must live on the GPU")
+        if not t.is_contiguous():
+            raise ValueError("HIP backend buffers must be contiguous")
+
+
+def n_e(nx: int, ny: int, nz: int) -> int:
+    return nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz
+
+
+def n_h(nx: int, ny: int, nz: int) -> int:
+    return (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
+
+
+def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: torch.Tensor,
+             dims: tuple[int, int, int], *, curl64: bool = False, kchunk: int = DEFAULT_KCHUNK,
+             stream: torch.cuda.Stream | None = None) -> None:
+    """E phase on K field sets: ``e`` is (K, n_E) or (n_E,), ``h`` likewise."""
+    nx, ny, nz = dims
+    ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
+    e2 = e.view(-1, ne)
+    h2 = h.view(-1, nh)
+    k = e2.shape[0]
+    if h2.shape[0] != k:
+        raise ValueError(f"E has {k} field sets, H has {h2.shape[0]}")
+    if alpha_e.numel() != ne or beta_e.numel() != ne:
+        raise ValueError("alpha_E/beta_E length does not match the grid")
+    code = _dtype_code(e)
+    for t in (h, alpha_e, beta_e):
+        if t.dtype != e.dtype:
+            raise TypeError("fields and coefficients must share one dtype")
+    _check_dev(e, h, alpha_e, beta_e)
+    _lib.call("mgh_update_e", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+              alpha_e.data_ptr(), beta_e.data_ptr(), int(bool(curl64)), int(kchunk),
+              stream_handle(stream))
+
+
+def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: torch.Tensor,
+             dims: tuple[int, int, int], *, kchunk: int = DEFAULT_KCHUNK,
+             stream: torch.cuda.Stream | None = None) -> None:
+    """H phase on K field sets (same conventions as :func:`update_e`)."""
+    nx, ny, nz = dims
+    ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
+    e2 = e.view(-1, ne)
+    h2 = h.view(-1, nh)
+    k = e2.shape[0]
+    if h2.shape[0] != k:
+        raise ValueError(f"E has {k} field sets, H has {h2.shape[0]}")
+    if alpha_h.numel() != nh or beta_h.numel() != nh:
+        raise ValueError("alpha_H/beta_H length does not match the grid")
+    code = _dtype_code(h)
+    for t in (e, alpha_h, beta_h):
+        if t.dtype != h.dtype:
+            raise TypeError("fields and coefficients must share one dtype")
+    _check_dev(e, h, alpha_h, beta_h)
+    _lib.call("mgh_update_h", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+              alpha_h.data_ptr(), beta_h.data_ptr(), int(kchunk), stream_handle(stream))
+
+
+class SqSum:
+    """Reusable float64 weighted-square-sum reduction (device result)."""
+
+    def __init__(self, nsets: int, device: torch.device):
+        lib = _lib.load()
+        self.nsets = nsets
+        self.work = torch.empty(int(lib.mgh_weighted_sq_sum_workspace(nsets)), dtype=torch.float64,
+                                device=device)
+
+    def __call__(self, x: torch.Tensor, w: torch.Tensor, out: torch.Tensor, *,
+                 accumulate: bool = False, scale: float = 1.0,
+                 stream: torch.cuda.Stream | None = None) -> None:
+        n = w.numel()
+        x2 = x.view(-1, n)
+        k = x2.shape[0]
+        if k > self.nsets:
+            raise ValueError(f"{k} sets exceed the reduction's capacity {self.nsets}")
+        if out.dtype != torch.float64 or out.numel() < k:
+            raise ValueError("out must be a float64 device tensor with one slot per set")
+        if x.dtype != w.dtype:
+            raise TypeError("values and weights must share one dtype")
+        _check_dev(x, w, out)
+        _lib.call("mgh_weighted_sq_sum", _dtype_code(x), n, k, x.data_ptr(), n, w.data_ptr(),
+                  out.data_ptr(), int(accumulate), float(scale), self.work.data_ptr(),
+                  self.work.numel(), stream_handle(stream))
