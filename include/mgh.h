@@ -62,6 +62,29 @@ int mgh_update_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
                  long long e_set_stride, void* h, long long h_set_stride, const void* alpha_h,
                  const void* beta_h, int kchunk, void* stream);
 
+/* ---- (a3)+(a7) / (a4)+(a7): curl phases with fused CPML -----------------
+ * Same as mgh_update_e / mgh_update_h, plus the CPML auxiliary recursion of
+ * every listed face fused into the boundary slabs.  Replaces
+ * CPMLBoundary.update_E / update_H (src/magnelio/boundaries/cpml.py:343-516),
+ * called right after the curl update in e_phase_device / h_phase_device
+ * (src/magnelio/solver/fit_td.py:870-876, :918-921).
+ * cpml_desc: host pointer to int64 words: [F, F records of 19 words]
+ *   axis, k0, n, b, c, ck (device ptrs, n values in the field dtype),
+ *   psi_e0, psi_e1, psi_h0, psi_h1 (device ptrs, field set 0),
+ *   stride_e0, stride_e1, stride_h0, stride_h1 (elements between sets),
+ *   air_e0, air_e1, air_h0, air_h1 (device uint8 0/1 masks or 0), spare.
+ * Term components per normal axis: x (Ey,Ez | Hy,Hz), y (Ex,Ez | Hx,Hz),
+ * z (Ex,Ey | Hx,Hy); psi slabs are x-fastest with the normal extent cut to
+ * n layers.  Faces apply in record order.  cpml_desc = NULL: no CPML. */
+int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                     long long e_set_stride, const void* h, long long h_set_stride,
+                     const void* alpha_e, const void* beta_e, int curl64, int kchunk,
+                     const void* cpml_desc, void* stream);
+int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                     long long e_set_stride, void* h, long long h_set_stride,
+                     const void* alpha_h, const void* beta_h, int kchunk, const void* cpml_desc,
+                     void* stream);
+
 /* ---- (a18) stored-energy sum ---------------------------------------------
  * Replaces the float64-accumulated reductions of the energy check
  * (src/magnelio/solver/fit_td.py:1014-1019).

@@ -41,6 +41,16 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
          _c_void_p, _c_int, _c_void_p],
     ),
+    "mgh_update_e_pml": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p],
+    ),
+    "mgh_update_h_pml": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_int, _c_void_p, _c_void_p],
+    ),
     "mgh_weighted_sq_sum_workspace": (_c_ll, [_c_int]),
     "mgh_weighted_sq_sum": (
         _c_int,

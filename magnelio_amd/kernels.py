@@ -49,10 +49,23 @@ def n_h(nx: int, ny: int, nz: int) -> int:
     return (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
 
 
+def _cpml_desc(cpml, nsets: int, dtype) -> int | None:
+    if cpml is None:
+        return None
+    if cpml.nsets != nsets:
+        raise ValueError(f"CPML state holds {cpml.nsets} field sets, the launch {nsets}")
+    if cpml.dtype != dtype:
+        raise TypeError("CPML state and fields must share one dtype")
+    return cpml.desc_ptr
+
+
 def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: torch.Tensor,
              dims: tuple[int, int, int], *, curl64: bool = False, kchunk: int = DEFAULT_KCHUNK,
-             stream: torch.cuda.Stream | None = None) -> None:
-    """E phase on K field sets: ``e`` is (K, n_E) or (n_E,), ``h`` likewise."""
+             cpml=None, stream: torch.cuda.Stream | None = None) -> None:
+    """E phase on K field sets: ``e`` is (K, n_E) or (n_E,), ``h`` likewise.
+
+    ``cpml`` is a :class:`magnelio_amd.boundaries.DeviceCPML` (psi slabs for
+    the same K sets) or None."""
     nx, ny, nz = dims
     ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
     e2 = e.view(-1, ne)
@@ -67,13 +80,14 @@ def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: to
         if t.dtype != e.dtype:
             raise TypeError("fields and coefficients must share one dtype")
     _check_dev(e, h, alpha_e, beta_e)
-    _lib.call("mgh_update_e", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
-              alpha_e.data_ptr(), beta_e.data_ptr(), int(bool(curl64)), int(kchunk),
+    desc = _cpml_desc(cpml, k, e.dtype)
+    _lib.call("mgh_update_e_pml", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+              alpha_e.data_ptr(), beta_e.data_ptr(), int(bool(curl64)), int(kchunk), desc,
               stream_handle(stream))
 
 
 def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: torch.Tensor,
-             dims: tuple[int, int, int], *, kchunk: int = DEFAULT_KCHUNK,
+             dims: tuple[int, int, int], *, kchunk: int = DEFAULT_KCHUNK, cpml=None,
              stream: torch.cuda.Stream | None = None) -> None:
     """H phase on K field sets (same conventions as :func:`update_e`)."""
     nx, ny, nz = dims
@@ -90,8 +104,9 @@ def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: to
         if t.dtype != h.dtype:
             raise TypeError("fields and coefficients must share one dtype")
     _check_dev(e, h, alpha_h, beta_h)
-    _lib.call("mgh_update_h", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
-              alpha_h.data_ptr(), beta_h.data_ptr(), int(kchunk), stream_handle(stream))
+    desc = _cpml_desc(cpml, k, h.dtype)
+    _lib.call("mgh_update_h_pml", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+              alpha_h.data_ptr(), beta_h.data_ptr(), int(kchunk), desc, stream_handle(stream))
 
 
 class SqSum:

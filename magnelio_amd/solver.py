@@ -1,0 +1,493 @@
+"""FIT-TD leapfrog driver of the HIP backend (``backend="hip"``).
+
+Mirrors ``FITTimeDomainSolver`` (src/magnelio/solver/fit_td.py:135-1386):
+same constructor fields (mesh, boundary_conditions, ports, total_time_steps,
+dt, energy_stop_db, port_signal_stop_db, precision) and the same
+``setup`` / ``run`` / ``state_dict`` / ``load_state_dict`` surface, plus
+``nsets`` / ``excitations``: K field sets of one mesh — one per excited
+(port, mode) channel of an S-parameter sweep — advance together, every launch
+reading the shared material coefficients once for all K.
+
+Step order (fit_td.py:950-1007), t = n dt:
+  E phase    curl update + fused CPML (+ ADE)          mgh_update_e_pml
+  sources    TF/SF E injection at t + dt                mgh_tfsf_apply
+  ports      modal DTBC/Mur step, write-back, V/I record mgh_port_step
+             lumped RLC elements                       mgh_lumped_step
+  H phase    curl update + fused CPML                  mgh_update_h_pml
+  sources    TF/SF H injection at t + 3dt/2
+  monitors   DFT accumulation                          mgh_dft_*
+  counter    n <- n + 1 on the device                  mgh_step_advance
+Nothing in the loop synchronises the host except the optional energy /
+port-signal stop checks every ``check_interval`` steps.  With
+``use_graphs=True`` one step is captured as a HIP graph (through torch's
+graph API) after two eager warm-up steps and replayed.
+
+All state lives on the device; ``state_dict`` exports it to host NumPy in
+the reference's layout and keys (fit_td.py:1264-1353).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, kernels, layout
+from . import mesh as _mesh
+from .boundaries import CPMLBoundary, DeviceCPML, PECBoundary, PMCBoundary
+from .ports import PortModel
+
+PRECISIONS = {"single": (np.float32, torch.float32), "double": (np.float64, torch.float64)}
+
+
+@dataclass
+class ChannelExcitation:
+    """Set s of the batch excites ``mode`` of port ``port`` with ``waveform``."""
+
+    port: str
+    mode: int
+    waveform: object  # callable t -> amplitude [sqrt(W)]
+
+
+class DevicePorts:
+    """Device arrays + descriptors of the modal ports for K field sets."""
+
+    WORDS = 29
+
+    def __init__(self, ports: list[PortModel], dims, nsets: int, n_steps: int, device,
+                 excitations: list[ChannelExcitation | None], dt: float, record: bool = True):
+        self.ports = ports
+        self.nsets = nsets
+        self.n_steps = n_steps
+        self.device = device
+        self._keep = []
+        self.desc = np.zeros(self.WORDS * len(ports), dtype=np.int64)
+        self.V_rec, self.I_rec, self.state, self.w_hist, self.s_hist, self.absmax = [], [], [], [], [], []
+        f64 = torch.float64
+        for pi, P in enumerate(ports):
+            pl = P.plane
+            n_u, n_v, M = pl.e_u_indices.size, pl.e_v_indices.size, P.n_modes
+
+            def dev(a, dtype=f64):
+                t = torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dtype)
+                self._keep.append(t)
+                return t
+
+            idx_port = layout.index_to_device(np.concatenate([pl.e_u_indices, pl.e_v_indices]), *dims, "E")
+            idx_int = layout.index_to_device(
+                np.concatenate([pl.e_u_indices_interior, pl.e_v_indices_interior]), *dims, "E")
+            idx_h = layout.index_to_device(np.concatenate([pl.h_v_indices, pl.h_u_indices]), *dims, "H")
+            params = np.zeros((M, 8))
+            for m in range(M):
+                r = P.dtbc_r[m]
+                if r is not None:
+                    q = P.dtbc_q[m]
+                    r2, q2 = r * r, q * q
+                    params[m, 0] = 1.0
+                    params[m, 1] = r2
+                    params[m, 2] = q2
+                    params[m, 5] = 2.0 - 2.0 * r2 - q2
+                params[m, 3] = P.mur_r[m]
+                params[m, 4] = P.record_scale[m]
+            kern = P.packed_dtbc_kernels(max(n_steps, 1))
+            state = dev(np.zeros((nsets, M, 4)))
+            cap = max(n_steps, 1)
+            any_dtbc = any(r is not None for r in P.dtbc_r)
+            w_hist = dev(np.zeros((nsets, M, cap if any_dtbc else 1)))
+            s_hist = dev(np.zeros((nsets, M, cap if any_dtbc else 1)))
+            exc = np.full(nsets, -1, dtype=np.int32)
+            src_tab = np.zeros((nsets, n_steps))
+            mur_tab = None
+            for s, ex in enumerate(excitations):
+                if ex is None or ex.port != P.name:
+                    continue
+                exc[s] = ex.mode
+                scale = float(P.source_scale[ex.mode])
+                if P.dtbc_r[ex.mode] is not None:
+                    # DTBC ghost source s^n = scale * waveform(t - dt), t = n dt + dt
+                    for n in range(n_steps):
+                        t = n * dt
+                        src_tab[s, n] = scale * ex.waveform((t + dt) - dt)
+                else:
+                    if mur_tab is None:
+                        mur_tab = np.zeros((nsets, n_steps, 4))
+                    mur_tab[s] = _mur_incident_table(ex.waveform, scale, float(P.tau[ex.mode]), dt, n_steps)
+            rec_cap = n_steps if record else 0
+            V = dev(np.zeros((nsets, M, max(rec_cap, 1))))
+            I = dev(np.zeros((nsets, M, max(rec_cap, 1))))
+            absmax = dev(np.zeros(nsets))
+            t_idx = [dev(idx_port, torch.int64), dev(idx_int, torch.int64), dev(idx_h, torch.int64)]
+            t_me = dev(np.concatenate([P.me_u_port, P.me_v_port]))
+            t_mi = dev(np.concatenate([P.me_u_int, P.me_v_int]))
+            t_mh = dev(np.concatenate([P.mh_v, P.mh_u]))
+            t_pe, t_ph = dev(P.prof_e()), dev(P.prof_h())
+            t_par = dev(params)
+            t_kern = dev(kern) if kern.size else None
+            t_exc = dev(exc, torch.int32)
+            t_src = dev(src_tab if n_steps > 0 else np.zeros((nsets, 1)))
+            t_mur = dev(mur_tab) if mur_tab is not None else None
+            w = self.desc[pi * self.WORDS:(pi + 1) * self.WORDS]
+            w[0], w[1], w[2] = n_u, n_v, M
+            w[3], w[4], w[5] = (t.data_ptr() for t in t_idx)
+            w[6], w[7], w[8] = t_me.data_ptr(), t_mi.data_ptr(), t_mh.data_ptr()
+            w[9], w[10], w[11] = t_pe.data_ptr(), t_ph.data_ptr(), t_par.data_ptr()
+            w[12] = t_kern.data_ptr() if t_kern is not None else 0
+            w[13] = kern.shape[1] if kern.size else 0
+            w[14] = 4096
+            w[15], w[16], w[17] = state.data_ptr(), w_hist.data_ptr(), s_hist.data_ptr()
+            w[18] = w_hist.shape[2] if any_dtbc else 0
+            w[19] = t_exc.data_ptr()
+            w[20], w[21] = t_src.data_ptr(), n_steps
+            w[22] = t_mur.data_ptr() if t_mur is not None else 0
+            w[23], w[24], w[25] = (V.data_ptr(), I.data_ptr(), rec_cap) if record else (0, 0, 0)
+            w[26] = absmax.data_ptr()
+            self.V_rec.append(V)
+            self.I_rec.append(I)
+            self.state.append(state)
+            self.w_hist.append(w_hist)
+            self.s_hist.append(s_hist)
+            self.absmax.append(absmax)
+
+    @property
+    def desc_ptr(self) -> int:
+        return int(self.desc.ctypes.data)
+
+
+def _mur_incident_table(waveform, scale: float, tau: float, dt: float, n_steps: int) -> np.ndarray:
+    """Per step: src(t^{n+1}), and the ring-buffer linear look-backs at tau,
+    tau + dt and dt (operator.py:1435-1463, _interp_delayed :1539-1562)."""
+    buf_len = max(int(math.ceil((tau + dt) / dt)) + 3, 5)
+    out = np.zeros((n_steps, 4))
+    buf: list[float] = []
+
+    def look(delay: float) -> float:
+        n = len(buf)
+        if n == 0 or dt <= 0.0:
+            return 0.0
+        frac = delay / dt
+        i0 = int(math.floor(frac))
+        a = frac - i0
+        if n - 1 - i0 < 0:
+            return 0.0
+        v0 = buf[n - 1 - i0]
+        v1 = buf[n - 2 - i0] if n - 2 - i0 >= 0 else 0.0
+        return (1.0 - a) * v0 + a * v1
+
+    for n in range(n_steps):
+        t = n * dt + dt
+        v = scale * waveform(t)
+        buf.append(v)
+        if len(buf) > buf_len:
+            buf.pop(0)
+        out[n] = (v, look(tau), look(tau + dt), look(dt))
+    return out
+
+
+@dataclass
+class FITTimeDomainSolver:
+    """HIP FIT-TD solver (see module docstring)."""
+
+    mesh: _mesh.Mesh
+    boundary_conditions: dict = field(default_factory=dict)
+    ports: list = field(default_factory=list)
+    sources: list = field(default_factory=list)
+    total_time_steps: int = 1000
+    dt: float = 1e-12
+    verbose: bool = False
+    energy_stop_db: float | None = None
+    port_signal_stop_db: float | None = None
+    energy_check_interval: int | None = None
+    monitors: list = field(default_factory=list)
+    lumped: list = field(default_factory=list)
+    precision: str = "single"
+    nsets: int = 1
+    excitations: list | None = None  # per set: ChannelExcitation | None
+    record: bool = True
+    curl64: bool = False
+    kchunk: int = kernels.DEFAULT_KCHUNK
+    use_graphs: bool = False
+    device: str = "cuda:0"
+
+    def __post_init__(self):
+        if self.precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {list(PRECISIONS)}, got {self.precision!r}")
+        if self.excitations is None:
+            self.excitations = [None] * self.nsets
+        if len(self.excitations) != self.nsets:
+            raise ValueError("one excitation entry (or None) per field set")
+        self._setup_done = False
+        # optional (start, end) event pairs recorded around the E-phase kernel
+        # of eager steps, on the launch stream (used by the benchmark)
+        self.timing_events: list = []
+        self.timed_events: list = []
+        self._resume_step = 0
+        self._stop_reason = None
+        self._graph = None
+
+    # ------------------------------------------------------------------
+    def setup(self) -> None:
+        _lib.load()
+        mesh = self.mesh
+        dims = mesh.dims
+        self.dims = dims
+        dt = self.dt
+        np_dt, t_dt = PRECISIONS[self.precision]
+        self._np_dtype, self._t_dtype = np_dt, t_dt
+        dev = torch.device(self.device)
+        self._dev = dev
+
+        m_eps = _mesh.build_m_eps(mesh)
+        m_mu = _mesh.build_m_mu(mesh)
+        m_sig = _mesh.build_m_sigma(mesh)
+        pec = mesh.pec_mask_edges
+        port_faces = []
+        for P in self.ports:
+            port_faces.append(P.face)
+            m_eps = _mesh.flatten_port_mass(m_eps, dims, P.face)
+            m_mu = _mesh.flatten_port_mu(m_mu, dims, P.face)
+        for f in port_faces:
+            pec = _mesh.flatten_port_pec(pec, dims, f)
+        self._port_faces = set(port_faces)
+        pec_e = _mesh.pec_flat(pec, dims)
+        self._pec_e = pec_e
+        w_e = w_h = None
+        self._dispersion = None
+        for src in self.sources:
+            if hasattr(src, "prepare"):
+                src.prepare(self)
+        disp_builder = getattr(self, "_dispersion_builder", None)
+        if disp_builder is not None:
+            self._dispersion = disp_builder(mesh, dt, m_eps)
+            w_e = self._dispersion.w_half
+        co = _mesh.update_coefficients(m_eps, m_mu, m_sig, _mesh.build_m_sigma_m(mesh), dt, pec_e, w_e, w_h)
+        self._coeffs = co
+        self._m_eps_host, self._m_mu_host = m_eps, m_mu
+
+        def up(a, side, dtype=t_dt):
+            return torch.from_numpy(np.ascontiguousarray(layout.to_device(np.asarray(a).astype(np_dt), *dims, side))).to(dev, dtype)
+
+        self.alpha_e, self.beta_e = up(co.alpha_e, "E"), up(co.beta_e, "E")
+        self.alpha_h, self.beta_h = up(co.alpha_h, "H"), up(co.beta_h, "H")
+        self.m_eps_dev, self.m_mu_dev = up(m_eps, "E"), up(m_mu, "H")
+        ne, nh = kernels.n_e(*dims), kernels.n_h(*dims)
+        K = self.nsets
+        self.e = torch.zeros((K, ne), dtype=t_dt, device=dev)
+        self.h = torch.zeros((K, nh), dtype=t_dt, device=dev)
+
+        cpml_faces = []
+        for face, bc in self.boundary_conditions.items():
+            if isinstance(bc, CPMLBoundary):
+                if face in self._port_faces:
+                    raise ValueError(f"face {face} carries both a port and a CPML")
+                bc.initialize(dt)
+                cpml_faces.append(bc)
+            elif not isinstance(bc, (PECBoundary, PMCBoundary)):
+                raise TypeError(f"unsupported boundary type on {face}: {type(bc).__name__}")
+        pec_cell = np.zeros(dims, dtype=bool)
+        pec_ids = [m for m, mat in mesh.material_library.items() if mat.is_pec]
+        if pec_ids:
+            pec_cell = np.isin(mesh.material_id, pec_ids)
+        self.cpml = (DeviceCPML(cpml_faces, dims, K, t_dt, dev, pec_e, pec_cell)
+                     if cpml_faces else None)
+        n_steps = int(self.total_time_steps)
+        self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.dports = (DevicePorts(self.ports, dims, K, n_steps, dev, self.excitations, dt, self.record)
+                       if self.ports else None)
+        for src in self.sources:
+            src.attach(self)
+        for mon in self.monitors:
+            mon.attach(self)
+        for el in self.lumped:
+            el.attach(self)
+        self._sqsum = kernels.SqSum(K, dev)
+        self._energy = torch.zeros(K, dtype=torch.float64, device=dev)
+        self._setup_done = True
+
+    # ------------------------------------------------------------------
+    def _step(self) -> None:
+        dims, K = self.dims, self.nsets
+        stream = torch.cuda.current_stream(self._dev)
+        sh = int(stream.cuda_stream)
+        ev = self.timing_events.pop() if self.timing_events else None
+        if ev is not None:
+            ev[0].record(stream)
+        kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
+                         kchunk=self.kchunk, cpml=self.cpml, stream=stream)
+        if ev is not None:
+            ev[1].record(stream)
+            self.timed_events.append(ev)
+        if self._dispersion is not None:
+            self._dispersion.update(self)
+        for src in self.sources:
+            src.inject_e(self, stream)
+        if self.dports is not None:
+            code = 4 if self._t_dtype == torch.float32 else 8
+            _lib.call("mgh_port_step", code, K, self.e.data_ptr(), self.e.shape[1], self.h.data_ptr(),
+                      self.h.shape[1], self.dports.desc_ptr, len(self.ports), self.step_ctr.data_ptr(), sh)
+        for el in self.lumped:
+            el.step(self, stream)
+        kernels.update_h(self.e, self.h, self.alpha_h, self.beta_h, dims, kchunk=self.kchunk,
+                         cpml=self.cpml, stream=stream)
+        for src in self.sources:
+            src.inject_h(self, stream)
+        for mon in self.monitors:
+            mon.record(self, stream)
+        _lib.call("mgh_step_advance", self.step_ctr.data_ptr(), 1, sh)
+
+    def energy(self) -> np.ndarray:
+        """0.5 (sum M_eps e^2 + sum M_mu h^2) per field set (float64, syncs)."""
+        self._sqsum(self.e, self.m_eps_dev, self._energy, scale=0.5)
+        self._sqsum(self.h, self.m_mu_dev, self._energy, accumulate=True, scale=0.5)
+        return self._energy.cpu().numpy().copy()
+
+    def run(self, n_steps: int | None = None) -> None:
+        if not self._setup_done:
+            self.setup()
+        total = int(self.total_time_steps) if n_steps is None else self._resume_step + int(n_steps)
+        if total > int(self.total_time_steps):
+            raise ValueError("cannot run past total_time_steps (recorder and DTBC histories are sized for it)")
+        interval = self.energy_check_interval or max(1, min(100, int(self.total_time_steps) // 20))
+        peak_e = getattr(self, "_peak_energy", 0.0)
+        peak_sig = getattr(self, "_peak_signal", 0.0)
+        self._stop_reason = None
+        graph_ok = self.use_graphs
+        for n in range(self._resume_step, total):
+            if graph_ok and self._graph is None and n - self._resume_step >= 2:
+                self._capture()
+            if self._graph is not None:
+                self._graph.replay()
+            else:
+                self._step()
+            self._resume_step = n + 1
+            if (self.energy_stop_db is not None or self.port_signal_stop_db is not None) and n % interval == 0:
+                if self.energy_stop_db is not None:
+                    en = float(self.energy().max())
+                    if en > peak_e:
+                        peak_e = en
+                    elif peak_e > 0 and en < peak_e * 10 ** (-self.energy_stop_db / 10):
+                        self._stop_reason = "energy"
+                        break
+                if self.port_signal_stop_db is not None and self.dports is not None:
+                    env = max(float(a.max()) for a in self.dports.absmax)
+                    for a in self.dports.absmax:
+                        a.zero_()
+                    if env > peak_sig:
+                        peak_sig = env
+                    elif peak_sig > 0 and env < peak_sig * 10 ** (-self.port_signal_stop_db / 20):
+                        self._stop_reason = "port_signal"
+                        break
+        self._peak_energy, self._peak_signal = peak_e, peak_sig
+        if self._stop_reason is None and self._resume_step >= int(self.total_time_steps):
+            self._stop_reason = "steps"
+        torch.cuda.synchronize(self._dev)
+
+    def _capture(self) -> None:
+        """Capture one step as a HIP graph; fall back to eager launches on failure."""
+        try:
+            torch.cuda.synchronize(self._dev)
+            g = torch.cuda.CUDAGraph()
+            s = torch.cuda.Stream(device=self._dev)
+            s.wait_stream(torch.cuda.current_stream(self._dev))
+            with torch.cuda.stream(s):
+                g.capture_begin()
+                self._step()
+                g.capture_end()
+            torch.cuda.current_stream(self._dev).wait_stream(s)
+            torch.cuda.synchronize(self._dev)
+            # the capture recorded (not executed) the step: roll the counter
+            # back is unnecessary because recording launches nothing
+            self._graph = g
+        except Exception as exc:  # pragma: no cover - depends on the runtime
+            import warnings  # noqa: PLC0415
+
+            warnings.warn(f"graph capture failed ({exc}); continuing with eager launches")
+            self.use_graphs = False
+            self._graph = None
+
+    # ------------------------------------------------------------------
+    @property
+    def n_completed(self) -> int:
+        return self._resume_step
+
+    def fields(self, s: int = 0) -> tuple[np.ndarray, np.ndarray]:
+        """(e, h) of field set s in the reference layout (host copies)."""
+        dims = self.dims
+        e = layout.to_reference(self.e[s].cpu().numpy(), *dims, "E")
+        h = layout.to_reference(self.h[s].cpu().numpy(), *dims, "H")
+        return e, h
+
+    def recorded(self, s: int = 0, n: int | None = None) -> dict:
+        """{(port, mode): (V, I)} float64 time series of field set s."""
+        n = self._resume_step if n is None else n
+        out = {}
+        for P, V, I in zip(self.ports, self.dports.V_rec, self.dports.I_rec):
+            v = V[s, :, :n].cpu().numpy()
+            i = I[s, :, :n].cpu().numpy()
+            for m in range(P.n_modes):
+                out[(P.name, m)] = (v[m].copy(), i[m].copy())
+        return out
+
+    def state_dict(self) -> dict:
+        torch.cuda.synchronize(self._dev)
+        sd = {"n_completed": self._resume_step,
+              "peak_energy": getattr(self, "_peak_energy", 0.0),
+              "peak_signal": getattr(self, "_peak_signal", 0.0),
+              "e": layout.to_reference(self.e.cpu().numpy(), *self.dims, "E"),
+              "h": layout.to_reference(self.h.cpu().numpy(), *self.dims, "H")}
+        if self.cpml is not None:
+            sd["cpml"] = self.cpml.state_dict()
+        if self.dports is not None:
+            n = self._resume_step
+            sd["ports"] = {}
+            for P, st, wh, shh, V, I in zip(self.ports, self.dports.state, self.dports.w_hist,
+                                           self.dports.s_hist, self.dports.V_rec, self.dports.I_rec):
+                stn = st.cpu().numpy()
+                sd["ports"][P.name] = {
+                    "u": stn[:, :, 0].copy(), "u_prev": stn[:, :, 1].copy(),
+                    "V_port_prev": stn[:, :, 2].copy(), "V_interior_prev": stn[:, :, 3].copy(),
+                    "w_hist": wh[:, :, :n].cpu().numpy().copy(), "s_hist": shh[:, :, :n].cpu().numpy().copy(),
+                    "V_rec": V[:, :, :n].cpu().numpy().copy(), "I_rec": I[:, :, :n].cpu().numpy().copy(),
+                }
+        for i, mon in enumerate(self.monitors):
+            if hasattr(mon, "state_dict"):
+                sd[f"monitor{i}"] = mon.state_dict()
+        for i, el in enumerate(self.lumped):
+            if hasattr(el, "state_dict"):
+                sd[f"lumped{i}"] = el.state_dict()
+        return sd
+
+    def load_state_dict(self, sd: dict) -> None:
+        if not self._setup_done:
+            self.setup()
+        dims = self.dims
+        n = int(sd["n_completed"])
+        self.e.copy_(torch.from_numpy(layout.to_device(np.asarray(sd["e"]), *dims, "E")).to(self._dev, self._t_dtype))
+        self.h.copy_(torch.from_numpy(layout.to_device(np.asarray(sd["h"]), *dims, "H")).to(self._dev, self._t_dtype))
+        if self.cpml is not None:
+            self.cpml.load_state_dict(sd["cpml"])
+        if self.dports is not None:
+            for P, st, wh, shh, V, I in zip(self.ports, self.dports.state, self.dports.w_hist,
+                                           self.dports.s_hist, self.dports.V_rec, self.dports.I_rec):
+                p = sd["ports"][P.name]
+                stn = np.stack([p["u"], p["u_prev"], p["V_port_prev"], p["V_interior_prev"]], axis=-1)
+                st.copy_(torch.from_numpy(stn))
+                if n and wh.shape[2] >= n:
+                    wh[:, :, :n].copy_(torch.from_numpy(p["w_hist"]))
+                    shh[:, :, :n].copy_(torch.from_numpy(p["s_hist"]))
+                if n:
+                    V[:, :, :n].copy_(torch.from_numpy(p["V_rec"]))
+                    I[:, :, :n].copy_(torch.from_numpy(p["I_rec"]))
+        for i, mon in enumerate(self.monitors):
+            if hasattr(mon, "load_state_dict"):
+                mon.load_state_dict(sd[f"monitor{i}"])
+        for i, el in enumerate(self.lumped):
+            if hasattr(el, "load_state_dict"):
+                el.load_state_dict(sd[f"lumped{i}"])
+        self.step_ctr.fill_(n)
+        self._resume_step = n
+        self._peak_energy = float(sd.get("peak_energy", 0.0))
+        self._peak_signal = float(sd.get("peak_signal", 0.0))
+        self._graph = None
+        torch.cuda.synchronize(self._dev)

@@ -1,0 +1,73 @@
+"""Oracle: the FIT-TD leapfrog loop in the reference's step order (CPU).
+
+Follows src/magnelio/solver/fit_td.py:950-1007 for the parts this build
+covers, with t = n dt:
+
+  E phase  curl update (stencil or Numba order), CPML E corrections in
+           boundary order                                    :826-889
+  sources  TF/SF E injection at t + dt                       :965-967
+  ports    update_e(fields, t + dt) in port order            :975-976
+  record   V/I of every port (e at t^{n+1}, h at t^{n+1/2})  :981-982
+  H phase  curl update, CPML H corrections                   :891-927
+  sources  TF/SF H injection at t + 3dt/2                    :991-993
+  monitors record(fields, n, t, dt)                          :996-1002
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import yee
+
+
+class Oracle:
+    def __init__(self, dims, alpha_e, beta_e, alpha_h, beta_h, dt, *, cpml=(), ports=(),
+                 sources=(), monitors=(), lumped=(), dtype=np.float64, order="stencil",
+                 dispersion=None):
+        self.dims = dims
+        self.dt = dt
+        self.dtype = dtype
+        self.ae, self.be = alpha_e.astype(dtype), beta_e.astype(dtype)
+        self.ah, self.bh = alpha_h.astype(dtype), beta_h.astype(dtype)
+        self.e = np.zeros(yee.n_e(*dims), dtype)
+        self.h = np.zeros(yee.n_h(*dims), dtype)
+        self.cpml = list(cpml)
+        self.ports = list(ports)
+        self.sources = list(sources)
+        self.monitors = list(monitors)
+        self.lumped = list(lumped)
+        self.order = order
+        self.dispersion = dispersion
+        self.n = 0
+
+    def step(self) -> None:
+        n, dt = self.n, self.dt
+        t = n * dt
+        dims = self.dims
+        if self.dispersion is not None:
+            self.dispersion.save_field(self.e)
+        yee.update_e(self.e, self.h, self.ae, self.be, *dims, order=self.order)
+        if self.dispersion is not None:
+            self.dispersion.update_field(self.e)
+        for f in self.cpml:
+            f.update_e(self.e, self.h, self.be)
+        for s in self.sources:
+            s.inject_e(self.e, t + dt)
+        for p in self.ports:
+            p.update_e(self.e, t + dt)
+        for el in self.lumped:
+            el.update_e(self.e, t + dt)
+        for p in self.ports:
+            p.record(self.e, self.h)
+        yee.update_h(self.e, self.h, self.ah, self.bh, *dims)
+        for f in self.cpml:
+            f.update_h(self.e, self.h, self.bh)
+        for s in self.sources:
+            s.inject_h(self.h, t + dt + dt / 2)
+        for m in self.monitors:
+            m.record(self.e, self.h, n, t, dt)
+        self.n = n + 1
+
+    def run(self, n_steps: int) -> None:
+        for _ in range(n_steps):
+            self.step()

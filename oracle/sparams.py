@@ -1,0 +1,52 @@
+"""Oracle: port-signal DFT and power-wave S-parameters (CPU, float64).
+
+Restates src/magnelio/signals/signal_1d.py:60-118 (direct DFT
+sum_n x_n e^{-2 pi j f n dt}, no dt factor) and
+src/magnelio/post/modal_sparameters.py:118-242, :335-557: I is shifted by
+the temporal half step e^{+j w dt/2}; certified DTBC channels use the exact
+discrete wave impedance (TE: z0 s/rad) and the discrete half-cell factor
+lambda^{1/2} for the two-plane de-stagger; S = b / a_excited with
+|a| < 1e-12 peak -> NaN.
+
+Pinned by tests/test_oracle_sparams.py: the DFT of a sine and of DC against
+closed forms, and a synthetic discrete travelling wave whose a/b split must
+give |b| = 0 (reflection-free) and a equal to the launched amplitude.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .dtbc import destagger_theta, dtbc_wave_impedance
+
+
+def dft(x, f, dt):
+    n = np.arange(len(x))
+    return np.exp(-2j * math.pi * np.outer(np.asarray(f, float), n) * dt) @ np.asarray(x, float)
+
+
+def waves(V, I, f, dt, line):
+    """(a, b) at the port plane for a certified TE channel line = (r, q, z0)."""
+    w = 2 * math.pi * np.asarray(f, float)
+    Vf = dft(V, f, dt)
+    If = dft(I, f, dt) * np.exp(1j * w * dt / 2)
+    r, q, z0 = line
+    Z = dtbc_wave_impedance(w * dt, q, z0, "TE")
+    sz = np.sqrt(Z)
+    th = destagger_theta(w * dt, r, q)
+    ep, em = np.exp(th), np.exp(-th)
+    den = ep + em
+    a = (Vf / sz * ep + sz * If) / den
+    b = (Vf / sz * em - sz * If) / den
+    return a, b
+
+
+def s_params(sig: dict, lines: dict, excited, f, dt, thr=1e-12):
+    """sig: {(port, m): (V, I)}, lines: {(port, m): (r, q, z0)}."""
+    ab = {k: waves(V, I, f, dt, lines[k]) for k, (V, I) in sig.items()}
+    a_exc = ab[excited][0]
+    valid = np.abs(a_exc) >= thr * np.max(np.abs(a_exc))
+    safe = np.where(valid, a_exc, 1.0)
+    return {k: np.where(valid, b / safe, np.nan) for k, (_, b) in ab.items()}

@@ -1,0 +1,95 @@
+"""Shared workloads for the parity tests (built independently on both sides).
+
+``c1_*``: the plain one-excitation WR-90 case — empty air-filled PEC guide
+22.86 x 10.16 x 40 mm on a uniform 48 x 24 x 96 grid, DTBC TE ports with
+3 modes on z_min / z_max, modulated Gaussian on port 1 mode 0 from the
+TE10 cut-off up to 25 GHz, dt = courant_dt(grid, "normal").
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from oracle import fdtd as ofd
+from oracle import mesh as om
+from oracle import ports as op
+
+C1_DIMS = (48, 24, 96)
+C1_SIZE = (22.86e-3, 10.16e-3, 40e-3)
+C1_FMAX = 25e9
+C1_MODES = [(1, 0), (2, 0), (0, 1)]  # TE10, TE20, TE01 (ascending cut-off)
+
+
+def c1_axes(dims=C1_DIMS, size=C1_SIZE):
+    return tuple(np.linspace(0.0, L, n + 1) for L, n in zip(size, dims))
+
+
+def modulated_gaussian(t: float, f_max: float, f_min: float) -> float:
+    bw = f_max - f_min
+    sigma = 2.0 / (math.pi * bw)
+    t0 = 4.0 / bw
+    x = (t - t0) / sigma
+    fc = 0.5 * (f_min + f_max)
+    return math.exp(-x * x) * math.cos(2.0 * math.pi * fc * (t - t0))
+
+
+def c1_oracle(dtype=np.float64, dims=C1_DIMS, size=C1_SIZE, excite=("port1", 0), order="stencil",
+              termination="auto", band=None):
+    """Oracle side: coefficients, two ports, excitation; returns (Oracle, ports, dt)."""
+    x, y, z = c1_axes(dims, size)
+    nx, ny, nz = dims
+    mid = np.zeros(dims, np.int32)
+    me = om.mass_e(x, y, z, mid, [[1.0]] * 3, om.EPS0)
+    mm = om.mass_h(x, y, z, mid, [[1.0]] * 3, om.MU0)
+    dt = om.courant_dt(x, y, z, 0.95)
+    pec = om.bbox_walls(np.zeros(me.size, bool), nx, ny, nz, ["xmin", "xmax", "ymin", "ymax"])
+    for f in ("zmin", "zmax"):
+        me = om.flatten_port(me, nx, ny, nz, f, "E")
+        mm = om.flatten_port(mm, nx, ny, nz, f, "H")
+        pec = om.flatten_port(pec, nx, ny, nz, f, "E")
+    ae, be = om.alpha_beta(me, np.zeros_like(me), dt, pec)
+    ah, bh = om.alpha_beta(mm, np.zeros_like(mm), dt)
+    omega_calc = 2 * math.pi * C1_FMAX
+    ports = []
+    for name, face in (("port1", "zmin"), ("port2", "zmax")):
+        pl = op.plane(face, x, y, z)
+        modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES]
+        ports.append(op.Port(name, pl, modes, me, mm, dt, omega_calc, termination=termination))
+    if excite is not None:
+        p = ports[0] if excite[0] == "port1" else ports[1]
+        f_lo, f_hi = band if band is not None else (p.modes[excite[1]]["wc"] / (2 * math.pi), C1_FMAX)
+        p.set_excitation(excite[1], lambda t, lo=f_lo, hi=f_hi: modulated_gaussian(t, hi, lo))
+    orc = ofd.Oracle(dims, ae, be, ah, bh, dt, ports=ports, dtype=dtype, order=order)
+    return orc, ports, dt, (me, mm)
+
+
+def c1_solver(dims=C1_DIMS, size=C1_SIZE, excitations=(("port1", 0),), n_steps=4000,
+              precision="single", use_graphs=False, termination="auto", curl64=False):
+    """Product side: mesh, ports, batched HIP solver (one set per excitation)."""
+    from magnelio_amd import mesh as mm
+    from magnelio_amd import ports as pp
+    from magnelio_amd.boundaries import PECBoundary
+    from magnelio_amd.signals import resolve_excitation
+    from magnelio_amd.solver import ChannelExcitation, FITTimeDomainSolver
+
+    g = mm.GridLines(*c1_axes(dims, size))
+    mesh = mm.Mesh.from_grid(g, pec_faces=("xmin", "xmax", "ymin", "ymax"))
+    dt = mm.courant_dt(g, "normal")
+    me, mu = mm.build_m_eps(mesh), mm.build_m_mu(mesh)
+    ports = [pp.build_port("port1", "zmin", mesh, me, mu, dt=dt, f_calc=C1_FMAX, n_modes=3,
+                           termination=termination),
+             pp.build_port("port2", "zmax", mesh, me, mu, dt=dt, f_calc=C1_FMAX, n_modes=3,
+                           termination=termination)]
+    by = {p.name: p for p in ports}
+    exc = []
+    for name, m in excitations:
+        wf = resolve_excitation(by[name].modes[m].omega_c, 0.0, C1_FMAX, m)
+        exc.append(ChannelExcitation(name, m, wf))
+    bcs = {f: PECBoundary(f) for f in ("xmin", "xmax", "ymin", "ymax")}
+    sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, ports=ports,
+                              total_time_steps=n_steps, dt=dt, precision=precision,
+                              nsets=len(exc), excitations=exc, use_graphs=use_graphs,
+                              curl64=curl64)
+    return sol, ports, dt

@@ -1,0 +1,154 @@
+"""GPU parity of the full stepping path on the plain WR-90 case (C1).
+
+48 x 24 x 96 empty PEC guide, DTBC TE ports (3 modes) on z_min / z_max,
+port 1 mode 0 excited from the TE10 cut-off to 25 GHz.
+
+* FP32 HIP vs the float64 oracle after 4000 steps: field relative L2 and
+  S-parameter error (|dS| <= 1e-4, the reference's single-vs-double gate);
+* FP32 HIP vs the float32 oracle (same operation order): fields agree to
+  float32 round-off (the port reductions associate differently in float64);
+* FP64 HIP vs the float64 oracle: 1e-12;
+* K = 6 excitations in one batched launch == six single runs, bit for bit;
+* HIP-graph replay == eager launches, bit for bit;
+* checkpoint / resume continues bit-exactly;
+* the modal Mur termination (with the excited TF/SF ring buffer) against
+  the oracle.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from magnelio_amd.signals import s_parameters
+from oracle import sparams as osp
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _hip_sparams(sol, ports, dt, f):
+    sig = sol.recorded(0)
+    modes = {p.name: p.modes for p in ports}
+    lines = {(p.name, m): lp for p in ports for m, lp in p.dtbc_line_params.items()}
+    ndx = {p.name: p.plane.normal_dx for p in ports}
+    return s_parameters(sig, modes, ("port1", 0), f, dt, port_normal_dx=ndx, port_line_params=lines)
+
+
+def _oracle_sparams(ports, dt, f):
+    sig = {(p.name, m): (np.array(p.V[m]), np.array(p.I[m])) for p in ports for m in range(p.n_modes)}
+    lines = {(p.name, m): p.line[m] for p in ports for m in range(p.n_modes)}
+    return osp.s_params(sig, lines, ("port1", 0), f, dt)
+
+
+@pytest.fixture(scope="module")
+def c1_oracle64():
+    print("oracle C1 float64, 4000 steps (CPU)", flush=True)
+    orc, ports, dt, _ = cases.c1_oracle(np.float64)
+    orc.run(4000)
+    return orc, ports, dt
+
+
+def test_c1_fp32_vs_fp64_oracle(c1_oracle64, gpu):
+    orc, oports, dt = c1_oracle64
+    print("HIP C1 fp32 4000 steps", flush=True)
+    sol, ports, dt_h = cases.c1_solver(n_steps=4000)
+    assert dt_h == dt
+    sol.run()
+    e, h = sol.fields(0)
+    err_e = _rel_l2(e.astype(np.float64), orc.e)
+    err_h = _rel_l2(h.astype(np.float64), orc.h)
+    print(f"C1 fp32 vs fp64 oracle: rel L2 e {err_e:.3e} h {err_h:.3e}", flush=True)
+    f_c = oports[0].modes[0]["wc"] / (2 * math.pi)
+    f = np.linspace(1.1 * f_c, 24e9, 201)
+    S_h = _hip_sparams(sol, ports, dt, f)
+    S_o = _oracle_sparams(oports, dt, f)
+    dmax = max(float(np.nanmax(np.abs(S_h[k] - S_o[k]))) for k in S_o)
+    print(f"C1 fp32 vs fp64 oracle: max |dS| {dmax:.3e}", flush=True)
+    assert dmax <= 1e-4
+    assert err_e <= 1e-4 and err_h <= 1e-4
+
+
+def test_c1_fp64_vs_oracle(c1_oracle64, gpu):
+    orc, oports, dt = c1_oracle64
+    sol, ports, _ = cases.c1_solver(n_steps=4000, precision="double")
+    sol.run()
+    e, h = sol.fields(0)
+    err_e, err_h = _rel_l2(e, orc.e), _rel_l2(h, orc.h)
+    print(f"C1 fp64 vs fp64 oracle: rel L2 e {err_e:.3e} h {err_h:.3e}", flush=True)
+    assert err_e <= 1e-10 and err_h <= 1e-10
+    sig = sol.recorded(0)
+    for p in oports:
+        for m in range(p.n_modes):
+            V, I = sig[(p.name, m)]
+            ref_v = np.array(p.V[m])
+            assert np.max(np.abs(V - ref_v)) <= 1e-9 * np.max(np.abs(oports[0].V[0]))
+
+
+def test_c1_fp32_vs_fp32_oracle_same_order(gpu):
+    orc, oports, dt, _ = cases.c1_oracle(np.float32)
+    orc.run(600)
+    sol, ports, _ = cases.c1_solver(n_steps=600)
+    sol.run()
+    e, h = sol.fields(0)
+    err_e = _rel_l2(e.astype(np.float64), orc.e.astype(np.float64))
+    print(f"C1 fp32 vs fp32 oracle 600 steps: rel L2 e {err_e:.3e}", flush=True)
+    assert err_e <= 1e-6
+
+
+def test_batched_k6_equals_single_runs(gpu):
+    exc = [("port1", 0), ("port1", 1), ("port1", 2), ("port2", 0), ("port2", 1), ("port2", 2)]
+    sol, _, _ = cases.c1_solver(excitations=exc, n_steps=500)
+    sol.run()
+    for s, ex in enumerate(exc):
+        one, _, _ = cases.c1_solver(excitations=[ex], n_steps=500)
+        one.run()
+        assert torch.equal(one.e[0], sol.e[s]) and torch.equal(one.h[0], sol.h[s]), ex
+        a, b = one.recorded(0), sol.recorded(s)
+        for k in a:
+            assert np.array_equal(a[k][0], b[k][0]) and np.array_equal(a[k][1], b[k][1])
+
+
+def test_graph_replay_equals_eager(gpu):
+    g, _, _ = cases.c1_solver(n_steps=400, use_graphs=True)
+    g.run()
+    assert g._graph is not None
+    eg, _, _ = cases.c1_solver(n_steps=400, use_graphs=False)
+    eg.run()
+    assert torch.equal(g.e, eg.e) and torch.equal(g.h, eg.h)
+    a, b = g.recorded(0), eg.recorded(0)
+    for k in a:
+        assert np.array_equal(a[k][0], b[k][0])
+
+
+def test_resume_is_bit_exact(gpu):
+    full, _, _ = cases.c1_solver(n_steps=600)
+    full.run()
+    part, _, _ = cases.c1_solver(n_steps=600)
+    part.run(300)
+    sd = part.state_dict()
+    fresh, _, _ = cases.c1_solver(n_steps=600)
+    fresh.load_state_dict(sd)
+    fresh.run()
+    assert fresh.n_completed == 600
+    assert torch.equal(fresh.e, full.e) and torch.equal(fresh.h, full.h)
+    a, b = fresh.recorded(0), full.recorded(0)
+    for k in a:
+        assert np.array_equal(a[k][0], b[k][0]) and np.array_equal(a[k][1], b[k][1])
+
+
+def test_mur_termination_vs_oracle(gpu):
+    orc, oports, dt, _ = cases.c1_oracle(np.float64, termination="mur")
+    orc.run(500)
+    sol, ports, _ = cases.c1_solver(n_steps=500, precision="double", termination="mur")
+    assert all(k == "mur" for p in ports for k in p.termination_kinds)
+    sol.run()
+    e, _ = sol.fields(0)
+    err = _rel_l2(e, orc.e)
+    print(f"Mur fp64 vs oracle: rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-10
