@@ -138,15 +138,15 @@ def main(argv=None) -> int:
     setup_s = time.perf_counter() - t_setup
 
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        sol._step()
+    for i in range(args.warmup):
+        sol._step(i)
     torch.cuda.synchronize()
     sol.timing_events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                          for _ in range(args.steps)]
     sol.timed_events = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sol._step()
+    for i in range(args.steps):
+        sol._step(args.warmup + i)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     e_ms = float(np.mean([a.elapsed_time(b) for a, b in sol.timed_events]))

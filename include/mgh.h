@@ -85,6 +85,32 @@ int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e
                      const void* alpha_h, const void* beta_h, int kchunk, const void* cpml_desc,
                      void* stream);
 
+/* ---- (a8)-(a11) modal port step ------------------------------------------
+ * Replaces PortOperatorModal.update_e + project_V/project_I
+ * (src/magnelio/ports/_modal/operator.py:1351-1536, :1209-1321),
+ * DTBCTermination.advance (src/magnelio/ports/_modal/dtbc.py:417-463) and
+ * PortSignalRecorder.record (src/magnelio/ports/recorder.py:216-250), called
+ * after the E phase in FITTimeDomainSolver.run (fit_td.py:975-982).
+ * One work-group per (port, field set); float64 projections, history
+ * convolutions and recursion; the step index n is read from *step_ctr.
+ * port_desc: n_ports records of 29 int64 words (see csrc/mgh_ports.h):
+ *   0 n_u, 1 n_v, 2 M, 3-5 idx_port/idx_int/idx_h (int64 device-layout
+ *   indices), 6-8 me_port/me_int/mh, 9-10 prof_e/prof_h (M x n),
+ *   11 params (M x 8: kind, r^2, q^2, mur_r, record_scale, 2-2r^2-q^2),
+ *   12-14 kern/kern_stride/k_init (packed DTBC kernels), 15 state
+ *   (nsets x M x 4), 16-18 w_hist/s_hist/cap, 19 exc_mode (int32 nsets),
+ *   20-21 src_tab/tab_len, 22 mur_tab, 23-25 V_rec/I_rec/rec_cap,
+ *   26 absmax; 27-28 spare. */
+int mgh_port_step(int dtype, int nsets, void* e, long long e_set_stride, const void* h,
+                  long long h_set_stride, const long long* port_desc, int n_ports,
+                  const long long* step_ctr, void* stream);
+
+/* ---- step counter ----------------------------------------------------------
+ * *step_ctr += by on the device (keeps per-step launches graph-capturable).
+ * No reference counterpart: the reference's loop index n lives on the host
+ * (fit_td.py:950). */
+int mgh_step_advance(long long* step_ctr, long long by, void* stream);
+
 /* ---- (a18) stored-energy sum ---------------------------------------------
  * Replaces the float64-accumulated reductions of the energy check
  * (src/magnelio/solver/fit_td.py:1014-1019).
@@ -95,6 +121,32 @@ long long mgh_weighted_sq_sum_workspace(int nsets);
 int mgh_weighted_sq_sum(int dtype, long long n, int nsets, const void* x, long long set_stride,
                         const void* w, double* out, int accumulate, double scale, double* work,
                         long long work_len, void* stream);
+
+/* ---- (a16) DFT monitors ----------------------------------------------------
+ * Replace DFTAccumulator.accumulate (src/magnelio/monitors/_dft.py:37-52) fed
+ * with cell-centred samples (src/magnelio/monitors/base.py:51-173) from the
+ * record hooks of MonitorFieldFrequency (field_frequency.py:249-294) and
+ * MonitorFarField (far_field.py:230-248); E stamped t = n dt, H t + dt/2.
+ *
+ * mgh_dft_sample: row ((n - n_start) mod B) of X (nsets x 64 x S_pad float32)
+ * receives, for each of S samples, sum_t w[s,t] * field[idx[s,t]] over 8
+ * terms; idx >= 0 indexes E, idx < 0 indexes H at -(idx + 1) (device
+ * layout); weight 0 marks an unused term. n is read from *step_ctr. */
+int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, const void* h,
+                   long long h_set_stride, const int* idx, const float* w, int S, float* X,
+                   int S_pad, int B, const long long* step_ctr, long long n_start, void* stream);
+
+/* mgh_dft_block: bins[set, f, s] += exp(j w_f t0) * sum_{b < nb_valid}
+ * (Pc + j Ps)[f, b] * X[set, b, s], t0 = n0 dt for s < s_h_start and
+ * n0 dt + dt/2 otherwise, n0 the first step of the block holding *step_ctr.
+ * Pc/Ps: F_pad x 64 float32 tables cos/sin(w_f b dt) dt. bins: nsets x F x S
+ * complex float64 (re/im interleaved); lo (same shape, or NULL) holds TwoSum
+ * compensation terms. blk: F x 4 float64 scratch. Contraction on the MFMA
+ * units (v_mfma_f32_16x16x4f32). */
+int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
+                  const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
+                  double dt, double* blk, int s_h_start, double* bins, double* lo,
+                  const long long* step_ctr, long long n_start, void* stream);
 
 #ifdef __cplusplus
 }

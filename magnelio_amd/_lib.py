@@ -51,11 +51,28 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
          _c_void_p, _c_int, _c_void_p, _c_void_p],
     ),
+    "mgh_port_step": (
+        _c_int,
+        [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_int, _c_void_p,
+         _c_void_p],
+    ),
+    "mgh_step_advance": (_c_int, [_c_void_p, _c_ll, _c_void_p]),
     "mgh_weighted_sq_sum_workspace": (_c_ll, [_c_int]),
     "mgh_weighted_sq_sum": (
         _c_int,
         [_c_int, _c_ll, _c_int, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int, _c_double,
          _c_void_p, _c_ll, _c_void_p],
+    ),
+    "mgh_dft_sample": (
+        _c_int,
+        [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int,
+         _c_void_p, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p],
+    ),
+    "mgh_dft_block": (
+        _c_int,
+        [_c_int, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,
+         _c_int, _c_int, _c_double, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_ll,
+         _c_void_p],
     ),
 }
 
@@ -121,5 +138,8 @@ def check(status: int, what: str) -> None:
 
 def call(name: str, *args) -> None:
     """Call a status-returning entry point and raise on failure."""
+    if name not in SIGNATURES:
+        # without argtypes ctypes would pass 64-bit pointers as C ints
+        raise MghError(f"{name} has no declared ctypes signature")
     fn = getattr(load(), name)
     check(fn(*args), name)

@@ -1,0 +1,207 @@
+// On-the-fly DFT monitors: per-step field sampling into a 64-step block, and
+// the block contraction against a phasor matrix on the MFMA units with
+// float64 (optionally compensated) bin accumulation.
+//
+// Reference: DFTAccumulator.accumulate (src/magnelio/monitors/_dft.py:37-52)
+//   bins[f, s] += exp(+j w_f t_n) dt * x_n[s]
+// with cell-centred physical samples (monitors/base.py:51-173) taken after
+// the H update of step n, E stamped t = n dt and H stamped t + dt/2
+// (field_frequency.py:249-294, far_field.py:230-248).
+//
+// Factorisation used here: for a block of B steps starting at n0,
+//   sum_b exp(j w (n0 + b) dt) dt x_b = exp(j w t0) * sum_b P[f, b] x_b,
+//   P[f, b] = exp(j w_f b dt) dt   (float32, host-built),
+//   t0 = n0 dt (E samples) or n0 dt + dt/2 (H samples).
+// The inner sum is a (2F x B) x (B x S) product done with
+// v_mfma_f32_16x16x4f32 (16 frequencies x 16 samples per wave, K = B = 64 in
+// 16 MFMA steps); the outer factor exp(j w t0) is float64 and the update of
+// the float64 bins is fused into the MFMA epilogue, so the partial products
+// never reach HBM.  Each wave keeps its 16-sample x 64-step operand in
+// registers and sweeps every frequency, so x is read once per block and the
+// bins are read and written once per block: the kernel is bound by the bin
+// traffic (16 B complex128 per (f, s), x2 with compensation).
+#include "mgh_common.h"
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int DFT_NT = 8;     // terms per sample (cell-centre averaging)
+constexpr int DFT_BMAX = 64;  // block length (MFMA K)
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft_sample(const T* __restrict__ e, int64_t e_set,
+                                                    const T* __restrict__ h, int64_t h_set,
+                                                    const int* __restrict__ idx,
+                                                    const float* __restrict__ w, int S,
+                                                    float* __restrict__ X, int S_pad, int B,
+                                                    const long long* __restrict__ ctr,
+                                                    long long n_start) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int set = blockIdx.y;
+  if (s >= S) return;
+  const long long n = *ctr;
+  const int b = (int)((n - n_start) % B);
+  const T* es = e + (int64_t)set * e_set;
+  const T* hs = h + (int64_t)set * h_set;
+  float acc = 0.0f;
+#pragma unroll
+  for (int t = 0; t < DFT_NT; ++t) {
+    const int id = idx[(int64_t)s * DFT_NT + t];
+    const float wt = w[(int64_t)s * DFT_NT + t];
+    if (wt != 0.0f) {
+      const float v = id >= 0 ? (float)es[id] : (float)hs[-(id + 1)];
+      acc += wt * v;
+    }
+  }
+  X[((int64_t)set * DFT_BMAX + b) * S_pad + s] = acc;
+}
+
+// Block phase factors exp(j w t0) for E and H stamps: blk[f*4 + 0..3] =
+// cos(w t0E), sin(w t0E), cos(w t0H), sin(w t0H).
+__global__ void k_dft_phase(const double* __restrict__ omega, int F, double dt,
+                            const long long* __restrict__ ctr, long long n_start, int B,
+                            double* __restrict__ blk) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const long long n = *ctr;
+  const long long n0 = n - ((n - n_start) % B);
+  const double te = (double)n0 * dt;
+  const double th = te + 0.5 * dt;
+  double s, c;
+  sincos(omega[f] * te, &s, &c);
+  blk[f * 4 + 0] = c;
+  blk[f * 4 + 1] = s;
+  sincos(omega[f] * th, &s, &c);
+  blk[f * 4 + 2] = c;
+  blk[f * 4 + 3] = s;
+}
+
+__device__ __forceinline__ void two_sum_acc(double* hi, double* lo, double v) {
+  const double a = *hi;
+  const double s = a + v;
+  const double bp = s - a;
+  const double err = (a - (s - bp)) + (v - bp);
+  *hi = s;
+  *lo += err;
+}
+
+__global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, int S, int S_pad,
+                                                   int nb_valid, const float* __restrict__ Pc,
+                                                   const float* __restrict__ Ps, int F, int F_pad,
+                                                   const double* __restrict__ blk, int s_h_start,
+                                                   double* __restrict__ bins,
+                                                   double* __restrict__ lo) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int set = blockIdx.y;
+  const int s0 = wave * 16;
+  if (s0 >= S) return;
+  const int col = lane & 15;   // sample within the tile (B operand column / D column)
+  const int krow = lane >> 4;  // 0..3: k within an MFMA step (A/B), row group of D
+  const float* Xs = X + (int64_t)set * DFT_BMAX * S_pad;
+  float xb[16];
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) {
+    const int b = 4 * kk + krow;
+    xb[kk] = b < nb_valid ? Xs[(int64_t)b * S_pad + s0 + col] : 0.0f;
+  }
+  const int ph = (s0 >= s_h_start) ? 2 : 0;
+  const int s = s0 + col;
+  double* bset = bins + (int64_t)set * F * S * 2;
+  double* lset = lo ? lo + (int64_t)set * F * S * 2 : nullptr;
+  for (int f0 = 0; f0 < F; f0 += 16) {
+    v4f cc = {0.f, 0.f, 0.f, 0.f};
+    v4f cs = {0.f, 0.f, 0.f, 0.f};
+    const int fa = f0 + col;  // A operand row (frequency) for this lane
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      const int b = 4 * kk + krow;
+      const float ac = Pc[(int64_t)fa * DFT_BMAX + b];
+      const float as = Ps[(int64_t)fa * DFT_BMAX + b];
+      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac, xb[kk], cc, 0, 0, 0);
+      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as, xb[kk], cs, 0, 0, 0);
+    }
+    if (s < S) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int f = f0 + 4 * krow + r;
+        if (f < F) {
+          const double pre = (double)cc[r];
+          const double pim = (double)cs[r];
+          const double c = blk[f * 4 + ph];
+          const double sn = blk[f * 4 + ph + 1];
+          const double dre = c * pre - sn * pim;
+          const double dim = sn * pre + c * pim;
+          const int64_t o = ((int64_t)f * S + s) * 2;
+          if (lset) {
+            two_sum_acc(bset + o, lset + o, dre);
+            two_sum_acc(bset + o + 1, lset + o + 1, dim);
+          } else {
+            bset[o] += dre;
+            bset[o + 1] += dim;
+          }
+        }
+      }
+    }
+  }
+  (void)F_pad;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Sample one step into row ((n - n_start) mod B) of the block buffer
+// X (nsets x 64 x S_pad float32).  idx/w: S x 8 terms; idx >= 0 indexes the
+// E vector, idx < 0 the H vector at -(idx + 1) (device layout); weight 0
+// marks an unused term.
+int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, const void* h,
+                   long long h_set_stride, const int* idx, const float* w, int S, float* X,
+                   int S_pad, int B, const long long* step_ctr, long long n_start, void* stream) {
+  MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8");
+  MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts S=%d S_pad=%d", S, S_pad);
+  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..64, got %d", B);
+  MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
+  MGH_REQUIRE(e && h && idx && w && X && step_ctr, "null pointer");
+  dim3 grd((S + 255) / 256, nsets, 1);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_dft_sample<float>, grd, dim3(256), 0, st, (const float*)e, e_set_stride,
+                       (const float*)h, h_set_stride, idx, w, S, X, S_pad, B, step_ctr, n_start);
+  else
+    hipLaunchKernelGGL(k_dft_sample<double>, grd, dim3(256), 0, st, (const double*)e, e_set_stride,
+                       (const double*)h, h_set_stride, idx, w, S, X, S_pad, B, step_ctr, n_start);
+  MGH_CHECK_LAUNCH("k_dft_sample");
+  return MGH_OK;
+}
+
+// Contract the block (first nb_valid rows of X) and accumulate into the
+// float64 bins (nsets x F x S complex, interleaved re/im); lo (same shape or
+// NULL) holds the compensation terms.  Pc/Ps: F_pad x 64 float32 phasor
+// tables cos/sin(w_f b dt) dt, zero beyond B and F.  Samples with index >=
+// s_h_start carry the H time stamp.  blk: F x 4 float64 scratch for the
+// block phase, filled here from *step_ctr (n = last step of the block).
+int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
+                  const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
+                  double dt, double* blk, int s_h_start, double* bins, double* lo,
+                  const long long* step_ctr, long long n_start, void* stream) {
+  MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts");
+  MGH_REQUIRE(F >= 1 && F_pad >= F && F_pad % 16 == 0, "bad frequency counts F=%d F_pad=%d", F,
+              F_pad);
+  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX && nb_valid >= 0 && nb_valid <= B, "bad block length");
+  MGH_REQUIRE(X && Pc && Ps && omega && blk && bins && step_ctr, "null pointer");
+  MGH_REQUIRE(s_h_start % 16 == 0, "s_h_start must be a multiple of 16");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dft_phase, dim3((F + 255) / 256), dim3(256), 0, st, omega, F, dt, step_ctr,
+                     n_start, B, blk);
+  MGH_CHECK_LAUNCH("k_dft_phase");
+  const int waves = (S + 15) / 16;
+  dim3 grd((waves + 3) / 4, nsets, 1);
+  hipLaunchKernelGGL(k_dft_block, grd, dim3(256), 0, st, X, S, S_pad, nb_valid, Pc, Ps, F, F_pad,
+                     (const double*)blk, s_h_start, bins, lo);
+  MGH_CHECK_LAUNCH("k_dft_block");
+  return MGH_OK;
+}
+
+}  // extern "C"

@@ -306,7 +306,8 @@ class FITTimeDomainSolver:
         self._setup_done = True
 
     # ------------------------------------------------------------------
-    def _step(self) -> None:
+    def _step(self, n: int) -> None:
+        """Launch one leapfrog step (step index n) on the current stream."""
         dims, K = self.dims, self.nsets
         stream = torch.cuda.current_stream(self._dev)
         sh = int(stream.cuda_stream)
@@ -319,7 +320,7 @@ class FITTimeDomainSolver:
             ev[1].record(stream)
             self.timed_events.append(ev)
         if self._dispersion is not None:
-            self._dispersion.update(self)
+            self._dispersion.update(self, stream)
         for src in self.sources:
             src.inject_e(self, stream)
         if self.dports is not None:
@@ -333,7 +334,7 @@ class FITTimeDomainSolver:
         for src in self.sources:
             src.inject_h(self, stream)
         for mon in self.monitors:
-            mon.record(self, stream)
+            mon.record(self, stream, n)
         _lib.call("mgh_step_advance", self.step_ctr.data_ptr(), 1, sh)
 
     def energy(self) -> np.ndarray:
@@ -342,6 +343,15 @@ class FITTimeDomainSolver:
         self._sqsum(self.h, self.m_mu_dev, self._energy, accumulate=True, scale=0.5)
         return self._energy.cpu().numpy().copy()
 
+    @property
+    def graph_steps(self) -> int:
+        """Steps per captured graph: one monitor block (the block contraction
+        sits at a fixed position inside it), else one step."""
+        g = 1
+        for mon in self.monitors:
+            g = g * getattr(mon, "block", 1) // math.gcd(g, getattr(mon, "block", 1))
+        return g
+
     def run(self, n_steps: int | None = None) -> None:
         if not self._setup_done:
             self.setup()
@@ -349,19 +359,28 @@ class FITTimeDomainSolver:
         if total > int(self.total_time_steps):
             raise ValueError("cannot run past total_time_steps (recorder and DTBC histories are sized for it)")
         interval = self.energy_check_interval or max(1, min(100, int(self.total_time_steps) // 20))
+        checks = self.energy_stop_db is not None or self.port_signal_stop_db is not None
         peak_e = getattr(self, "_peak_energy", 0.0)
         peak_sig = getattr(self, "_peak_signal", 0.0)
         self._stop_reason = None
-        graph_ok = self.use_graphs
-        for n in range(self._resume_step, total):
-            if graph_ok and self._graph is None and n - self._resume_step >= 2:
-                self._capture()
-            if self._graph is not None:
-                self._graph.replay()
-            else:
-                self._step()
-            self._resume_step = n + 1
-            if (self.energy_stop_db is not None or self.port_signal_stop_db is not None) and n % interval == 0:
+        G = self.graph_steps
+        n = self._resume_step
+        while n < total:
+            # graphs replay G steps that start on a block boundary; the stop
+            # checks need single steps, so a run with checks stays eager
+            if (self.use_graphs and not checks and n + G <= total and n >= 2 and
+                    self._mon_aligned(n, G)):
+                if self._graph is None:
+                    self._capture(n, G)
+                if self._graph is not None:
+                    self._graph.replay()
+                    n += G
+                    self._resume_step = n
+                    continue
+            self._step(n)
+            n += 1
+            self._resume_step = n
+            if checks and (n - 1) % interval == 0:
                 if self.energy_stop_db is not None:
                     en = float(self.energy().max())
                     if en > peak_e:
@@ -381,10 +400,22 @@ class FITTimeDomainSolver:
         self._peak_energy, self._peak_signal = peak_e, peak_sig
         if self._stop_reason is None and self._resume_step >= int(self.total_time_steps):
             self._stop_reason = "steps"
+        if self._stop_reason is not None:
+            for mon in self.monitors:
+                mon.finalize(self)
         torch.cuda.synchronize(self._dev)
 
-    def _capture(self) -> None:
-        """Capture one step as a HIP graph; fall back to eager launches on failure."""
+    def _mon_aligned(self, n: int, G: int) -> bool:
+        return all((n - getattr(m, "n_start", 0)) % getattr(m, "block", 1) == 0 for m in self.monitors)
+
+    def _capture(self, n: int, G: int) -> None:
+        """Capture G steps as one HIP graph; fall back to eager launches on failure.
+
+        Capturing records the launches without executing them, so the step
+        counter and every state buffer are untouched; the graph is replayed
+        from the same step on.  Device pointers baked into the graph belong
+        to buffers this solver keeps alive for its lifetime.
+        """
         try:
             torch.cuda.synchronize(self._dev)
             g = torch.cuda.CUDAGraph()
@@ -392,17 +423,17 @@ class FITTimeDomainSolver:
             s.wait_stream(torch.cuda.current_stream(self._dev))
             with torch.cuda.stream(s):
                 g.capture_begin()
-                self._step()
+                for k in range(G):
+                    self._step(n + k)
                 g.capture_end()
             torch.cuda.current_stream(self._dev).wait_stream(s)
             torch.cuda.synchronize(self._dev)
-            # the capture recorded (not executed) the step: roll the counter
-            # back is unnecessary because recording launches nothing
             self._graph = g
         except Exception as exc:  # pragma: no cover - depends on the runtime
             import warnings  # noqa: PLC0415
 
-            warnings.warn(f"graph capture failed ({exc}); continuing with eager launches")
+            self.graph_error = f"{type(exc).__name__}: {exc}"
+            warnings.warn(f"graph capture failed ({self.graph_error}); continuing with eager launches")
             self.use_graphs = False
             self._graph = None
 

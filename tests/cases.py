@@ -66,7 +66,7 @@ def c1_oracle(dtype=np.float64, dims=C1_DIMS, size=C1_SIZE, excite=("port1", 0),
 
 
 def c1_solver(dims=C1_DIMS, size=C1_SIZE, excitations=(("port1", 0),), n_steps=4000,
-              precision="single", use_graphs=False, termination="auto", curl64=False):
+              precision="single", use_graphs=False, termination="auto", curl64=False, monitors=()):
     """Product side: mesh, ports, batched HIP solver (one set per excitation)."""
     from magnelio_amd import mesh as mm
     from magnelio_amd import ports as pp
@@ -91,5 +91,5 @@ def c1_solver(dims=C1_DIMS, size=C1_SIZE, excitations=(("port1", 0),), n_steps=4
     sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, ports=ports,
                               total_time_steps=n_steps, dt=dt, precision=precision,
                               nsets=len(exc), excitations=exc, use_graphs=use_graphs,
-                              curl64=curl64)
+                              curl64=curl64, monitors=list(monitors))
     return sol, ports, dt

@@ -1,0 +1,134 @@
+"""GPU parity of the DFT monitors (sampling kernel + MFMA block contraction).
+
+* the block contraction against numpy on random inputs: ragged F (37) and S
+  (70, H samples from 48), two field sets, a full and a partial block, plain
+  and compensated bins — a wrong MFMA operand or result lane layout gives
+  O(1) errors, float32 products give ~1e-7;
+* a region monitor inside the plain WR-90 run (C1, 300 steps: four full
+  blocks and a partial one closed by ``finalize``) against the oracle's
+  FieldDFT fed with float64 fields — FP64 and FP32 stepping;
+* the same run replayed through HIP graphs == eager launches, bit for bit.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from magnelio_amd import _lib
+from magnelio_amd.monitors import MonitorFieldFrequency
+from oracle import monitors as omon
+
+pytestmark = pytest.mark.gpu
+
+FREQS = [8e9, 12e9, 16e9, 20e9, 24e9]
+REGION = (slice(10, 30), slice(4, 20), slice(30, 60))
+N_STEPS = 300
+
+
+def test_block_contraction_matches_numpy(gpu):
+    rng = np.random.default_rng(7)
+    K, S, s_h, F = 2, 70, 48, 37
+    S_pad, F_pad = 80, 48
+    dt = 1.3e-12
+    omega = 2 * np.pi * rng.uniform(1e9, 30e9, F)
+    b = np.arange(64)
+    Pc = np.zeros((F_pad, 64), np.float32)
+    Ps = np.zeros((F_pad, 64), np.float32)
+    Pc[:F] = np.cos(np.outer(omega, b * dt)) * dt
+    Ps[:F] = np.sin(np.outer(omega, b * dt)) * dt
+    X = rng.standard_normal((K, 64, S_pad)).astype(np.float32)
+    n_start = 5
+    ctr_val = n_start + 3 * 64 + 40
+    n0 = n_start + 3 * 64
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev)
+    Pcd, Psd = torch.from_numpy(Pc).to(dev), torch.from_numpy(Ps).to(dev)
+    om = torch.from_numpy(omega).to(dev)
+    blk = torch.zeros((F, 4), dtype=torch.float64, device=dev)
+    ctr = torch.tensor([ctr_val], dtype=torch.int64, device=dev)
+    t0 = np.where(np.arange(S) >= s_h, n0 * dt + 0.5 * dt, n0 * dt)
+    rot = np.exp(1j * omega[:, None] * t0[None, :])  # (F, S)
+    P = (Pc[:F].astype(np.float64) + 1j * Ps[:F].astype(np.float64))
+    for nb in (64, 23):
+        for comp in (False, True):
+            print(f"dft block contraction nb={nb} compensated={comp}", flush=True)
+            base = rng.standard_normal((K, F, S, 2))
+            bins = torch.from_numpy(base.copy()).to(dev)
+            lo = torch.zeros_like(bins) if comp else None
+            stream = torch.cuda.current_stream(dev)
+            _lib.call("mgh_dft_block", K, Xd.data_ptr(), S, S_pad, 64, nb, Pcd.data_ptr(),
+                      Psd.data_ptr(), om.data_ptr(), F, F_pad, dt, blk.data_ptr(), s_h,
+                      bins.data_ptr(), lo.data_ptr() if comp else None, ctr.data_ptr(), n_start,
+                      int(stream.cuda_stream))
+            torch.cuda.synchronize(dev)
+            got = bins.cpu().numpy()
+            if comp:
+                got = got + lo.cpu().numpy()
+            for k in range(K):
+                want = (P[:, :nb] @ X[k, :nb, :S].astype(np.float64)) * rot
+                g = (got[k, ..., 0] - base[k, ..., 0]) + 1j * (got[k, ..., 1] - base[k, ..., 1])
+                err = np.abs(g - want).max() / (math.sqrt(nb) * dt)
+                print(f"  set {k}: max err / (sqrt(nb) dt) = {err:.2e}", flush=True)
+                assert err < 1e-5
+
+
+def _oracle_dft(steps):
+    orc, _, dt, _ = cases.c1_oracle(np.float64)
+    x, y, z = cases.c1_axes()
+    mon = omon.FieldDFT(FREQS, x, y, z, REGION)
+    orc.monitors.append(mon)
+    orc.run(steps)
+    return mon.bins
+
+
+@pytest.fixture(scope="module")
+def oracle_bins():
+    print(f"oracle C1 float64 + region DFT, {N_STEPS} steps (CPU)", flush=True)
+    return _oracle_dft(N_STEPS)
+
+
+def _hip_bins(precision, use_graphs=False):
+    mon = MonitorFieldFrequency(FREQS, region=REGION)
+    sol, _, _ = cases.c1_solver(n_steps=N_STEPS, precision=precision, use_graphs=use_graphs,
+                                monitors=[mon])
+    sol.run()
+    if use_graphs:
+        assert sol._graph is not None, getattr(sol, "graph_error", "graph not captured")
+    return mon.result(0)
+
+
+def _errors(got, want):
+    """Per component: relative L2 for the TE10 components (Ey, Hx, Hz);
+    absolute error over the E or H peak for the ones that stay at round-off."""
+    out = {}
+    peak = {f: max(np.abs(want[c]).max() for c in want if c[0] == f) for f in "EH"}
+    for c in want:
+        if c in ("Ey", "Hx", "Hz"):
+            out[c] = float(np.linalg.norm(got[c] - want[c]) / np.linalg.norm(want[c]))
+        else:
+            out[c] = float(np.abs(got[c] - want[c]).max() / peak[c[0]])
+    return out
+
+
+@pytest.mark.parametrize("precision,gate", [("double", 1e-5), ("single", 2e-5)])
+def test_region_monitor_vs_oracle(oracle_bins, precision, gate, gpu):
+    print(f"HIP C1 {precision} + region DFT, {N_STEPS} steps", flush=True)
+    got = _hip_bins(precision)
+    assert set(got) == set(oracle_bins)
+    for c in oracle_bins:
+        assert got[c].shape == oracle_bins[c].shape
+    errs = _errors(got, oracle_bins)
+    print(f"region DFT {precision} vs float64 oracle: " +
+          " ".join(f"{c} {v:.2e}" for c, v in errs.items()), flush=True)
+    assert max(errs.values()) < gate
+
+
+def test_region_monitor_graph_equals_eager(gpu):
+    print("HIP C1 single + region DFT, graphs vs eager", flush=True)
+    eager = _hip_bins("single")
+    graph = _hip_bins("single", use_graphs=True)
+    for c in eager:
+        assert np.array_equal(eager[c], graph[c]), c

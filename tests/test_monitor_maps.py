@@ -1,0 +1,73 @@
+"""CPU checks of the DFT monitors' sampling maps (host integer/weight work).
+
+The product builds each monitor sample as a sparse row of (field index,
+weight) terms in the reference layout; applied to random fields it must
+reproduce the oracle's cell-centre interpolation (base.py:51-173) and the
+Huygens-face weighting of the far-field monitor (far_field.py:230-248).
+"""
+
+import numpy as np
+
+from magnelio_amd import mesh as mm
+from magnelio_amd.boundaries import CPMLBoundary
+from magnelio_amd.monitors import MonitorFarField, MonitorFieldFrequency
+from oracle import monitors as omon
+from oracle import yee
+
+
+def _apply(idx, w, e, h):
+    vals = np.where(idx >= 0, e[np.where(idx >= 0, idx, 0)], h[np.where(idx < 0, -(idx + 1), 0)])
+    return np.sum(np.where(w != 0, w * vals, 0.0), axis=1)
+
+
+def _grid():
+    rng = np.random.default_rng(2)
+    axes = [np.cumsum(np.r_[0, rng.uniform(0.5e-3, 1e-3, n)]) for n in (14, 15, 16)]
+    return mm.GridLines(*axes), axes
+
+
+def test_region_map_matches_oracle_interpolation():
+    g, (x, y, z) = _grid()
+    rng = np.random.default_rng(3)
+    e = rng.standard_normal(yee.n_e(*g.dims))
+    h = rng.standard_normal(yee.n_h(*g.dims))
+    mon = MonitorFieldFrequency([1e9], region=(slice(2, 7), slice(0, 15), slice(4, 9)))
+    idx, w = mon.sample_map(g)
+    vals = _apply(idx, w, e, h)
+    ref = omon.cell_centred(e, h, x, y, z, ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"], mon.slices)
+    for comp, shape, start, n in mon._layout:
+        scale = np.abs(ref[comp]).max()
+        assert np.allclose(vals[start:start + n].reshape(shape), ref[comp], rtol=0, atol=1e-13 * scale)
+    assert mon.s_h_start % 16 == 0
+
+
+def test_huygens_map_weights_two_layers():
+    g, (x, y, z) = _grid()
+    rng = np.random.default_rng(4)
+    e = rng.standard_normal(yee.n_e(*g.dims))
+    h = rng.standard_normal(yee.n_h(*g.dims))
+    bcs = {f: CPMLBoundary(f, g, thickness_cells=3) for f in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")}
+    mon = MonitorFarField([1e9], margin_cells=2)
+    idx, w = mon.sample_map(g, bcs)
+    vals = _apply(idx, w, e, h)
+    nodes = (x, y, z)
+    lo, hi = [5, 5, 5], [14 - 5, 15 - 5, 16 - 5]
+    for face, axis, nn, wgt, shape in mon.faces:
+        t1, t2 = [a for a in range(3) if a != axis]
+        cc = 0.5 * (nodes[axis][:-1] + nodes[axis][1:])
+        assert wgt == (nodes[axis][nn] - cc[nn - 1]) / (cc[nn] - cc[nn - 1])
+        sl = [None] * 3
+        sl[axis] = slice(nn - 1, nn + 1)
+        sl[t1] = slice(lo[t1], hi[t1])
+        sl[t2] = slice(lo[t2], hi[t2])
+        ref = omon.cell_centred(e, h, x, y, z, ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"], sl)
+        for comp, shp, start, n in mon._layout:
+            if shp != shape:
+                continue
+            arr = np.moveaxis(ref[comp], axis, 0)
+            want = (1.0 - wgt) * arr[0] + wgt * arr[1]
+            got = vals[start:start + n].reshape(shp)
+            if np.allclose(got, want, rtol=1e-12):
+                break
+        else:
+            raise AssertionError(f"no sample group matches face {face}")
