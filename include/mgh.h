@@ -62,20 +62,23 @@ int mgh_update_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
                  long long e_set_stride, void* h, long long h_set_stride, const void* alpha_h,
                  const void* beta_h, int kchunk, void* stream);
 
-/* ---- (a3)+(a7) / (a4)+(a7): curl phases with fused CPML -----------------
- * Same as mgh_update_e / mgh_update_h, plus the CPML auxiliary recursion of
- * every listed face fused into the boundary slabs.  Replaces
- * CPMLBoundary.update_E / update_H (src/magnelio/boundaries/cpml.py:343-516),
- * called right after the curl update in e_phase_device / h_phase_device
+/* ---- (a3)+(a7) / (a4)+(a7): curl phases with CPML ------------------------
+ * Same as mgh_update_e / mgh_update_h, followed by the CPML auxiliary
+ * recursion of every listed face over its slab, in record order (one small
+ * launch per face on the same stream).  Replaces CPMLBoundary.update_E /
+ * update_H (src/magnelio/boundaries/cpml.py:343-516), called right after the
+ * curl update in e_phase_device / h_phase_device
  * (src/magnelio/solver/fit_td.py:870-876, :918-921).
- * cpml_desc: host pointer to int64 words: [F, F records of 19 words]
- *   axis, k0, n, b, c, ck (device ptrs, n values in the field dtype),
- *   psi_e0, psi_e1, psi_h0, psi_h1 (device ptrs, field set 0),
- *   stride_e0, stride_e1, stride_h0, stride_h1 (elements between sets),
- *   air_e0, air_e1, air_h0, air_h1 (device uint8 0/1 masks or 0), spare.
+ * cpml_desc: host pointer to int64 words: [F, F records of 9 words]
+ *   axis, k0, n,
+ *   coef   (device ptr: b[n] | c[n] | ck[n] in the field dtype),
+ *   psi_e, psi_h (device ptrs: term 0 of all K sets, then term 1),
+ *   air_e, air_h (device uint8 0/1 masks [term 0 | term 1] or 0),
+ *   K     (field sets held by the psi buffers, >= nsets).
  * Term components per normal axis: x (Ey,Ez | Hy,Hz), y (Ex,Ez | Hx,Hz),
  * z (Ex,Ey | Hx,Hy); psi slabs are x-fastest with the normal extent cut to
- * n layers.  Faces apply in record order.  cpml_desc = NULL: no CPML. */
+ * n layers.  At most two faces per axis, not overlapping.
+ * cpml_desc = NULL: no CPML. */
 int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
                      long long e_set_stride, const void* h, long long h_set_stride,
                      const void* alpha_e, const void* beta_e, int curl64, int kchunk,
@@ -84,6 +87,18 @@ int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e
                      long long e_set_stride, void* h, long long h_set_stride,
                      const void* alpha_h, const void* beta_h, int kchunk, const void* cpml_desc,
                      void* stream);
+
+/* ---- (a7) CPML slab passes alone -------------------------------------------
+ * The CPML part of mgh_update_e_pml / mgh_update_h_pml: for each face in
+ * record order, psi and the tangential components of its slab are updated
+ * from the fields as they stand (call right after mgh_update_e / _h).
+ * Replaces CPMLBoundary.update_E / update_H (cpml.py:343-516). */
+int mgh_cpml_e(int dtype, int nx, int ny, int nz, int nsets, void* e, long long e_set_stride,
+               const void* h, long long h_set_stride, const void* beta_e, const void* cpml_desc,
+               void* stream);
+int mgh_cpml_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+               long long e_set_stride, void* h, long long h_set_stride, const void* beta_h,
+               const void* cpml_desc, void* stream);
 
 /* ---- (a8)-(a11) modal port step ------------------------------------------
  * Replaces PortOperatorModal.update_e + project_V/project_I

@@ -51,6 +51,16 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
          _c_void_p, _c_int, _c_void_p, _c_void_p],
     ),
+    "mgh_cpml_e": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_void_p],
+    ),
+    "mgh_cpml_h": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_void_p],
+    ),
     "mgh_port_step": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_int, _c_void_p,

@@ -167,7 +167,7 @@ class DeviceCPML:
     field set, so the fused kernels index them from the element (i, j, k).
     """
 
-    WORDS = 19
+    WORDS = 9
 
     def __init__(self, faces, dims, nsets: int, dtype, device, pec_e_flat=None, pec_cell=None):
         import torch  # noqa: PLC0415
@@ -186,27 +186,32 @@ class DeviceCPML:
         for fi, f in enumerate(self.faces):
             if not f._initialized:
                 raise RuntimeError(f"CPML face {f.face} is not initialized")
-            coef = [torch.tensor(a, dtype=dtype, device=device) for a in (f.b, f.c, f.ck)]
+            coef = torch.tensor(np.concatenate([f.b, f.c, f.ck]), dtype=dtype, device=device)
             e_sh, h_sh = f.psi_shapes(dims)
-            pe = [torch.zeros((nsets, int(np.prod(s))), dtype=dtype, device=device) for s in e_sh]
-            ph = [torch.zeros((nsets, int(np.prod(s))), dtype=dtype, device=device) for s in h_sh]
+            # one buffer per side: term 0 of every set, then term 1
+            views = []
+            for shapes in (e_sh, h_sh):
+                sizes = [int(np.prod(sh)) for sh in shapes]
+                buf = torch.zeros(nsets * sum(sizes), dtype=dtype, device=device)
+                views.append((buf, [buf[: nsets * sizes[0]].view(nsets, sizes[0]),
+                                    buf[nsets * sizes[0]:].view(nsets, sizes[1])]))
+            (pe_buf, pe), (ph_buf, ph) = views
             masks = f.air_masks(pec_e_flat, pec_cell, dims) if pec_e_flat is not None else None
-            air = [None] * 4
+            air = [None, None]
             if masks is not None:
-                air = [torch.from_numpy(np.ascontiguousarray(
-                    np.moveaxis(m, (0, 1, 2), (2, 1, 0)).ravel())).to(device) for m in masks]
+                flat = [np.ascontiguousarray(np.moveaxis(m, (0, 1, 2), (2, 1, 0)).ravel()) for m in masks]
+                air = [torch.from_numpy(np.concatenate(flat[0:2])).to(device),
+                       torch.from_numpy(np.concatenate(flat[2:4])).to(device)]
             self.psi_e.append(pe)
             self.psi_h.append(ph)
-            self._keep += coef + pe + ph + [a for a in air if a is not None]
+            self._keep += [coef, pe_buf, ph_buf] + [a for a in air if a is not None]
             r = desc[1 + fi * self.WORDS: 1 + (fi + 1) * self.WORDS]
             r[0], r[1], r[2] = f.axis, f.k0, f.n
-            r[3], r[4], r[5] = (t.data_ptr() for t in coef)
-            r[6], r[7] = pe[0].data_ptr(), pe[1].data_ptr()
-            r[8], r[9] = ph[0].data_ptr(), ph[1].data_ptr()
-            r[10], r[11] = pe[0].shape[1], pe[1].shape[1]
-            r[12], r[13] = ph[0].shape[1], ph[1].shape[1]
-            for t in range(4):
-                r[14 + t] = air[t].data_ptr() if air[t] is not None else 0
+            r[3] = coef.data_ptr()
+            r[4], r[5] = pe_buf.data_ptr(), ph_buf.data_ptr()
+            r[6] = air[0].data_ptr() if air[0] is not None else 0
+            r[7] = air[1].data_ptr() if air[1] is not None else 0
+            r[8] = nsets
         self.desc = desc
 
     @property

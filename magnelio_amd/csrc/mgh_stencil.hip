@@ -28,64 +28,68 @@
 // changes no rounding, because the element's updated value is the register
 // the pass would have reloaded and H (resp. E) is not written in the phase.
 //
-// Work decomposition (x-fastest device layout, see mgh_common.h): a 64 x 4
-// thread tile covers (i, j) columns of the node lattice i in [0, Nx],
-// j in [0, Ny]; each thread marches a chunk of z planes and keeps the
-// z-neighbour values (H at k-1 in the E phase, E at k+1 in the H phase) in
-// registers, so every field word is fetched from memory once per phase.  The
-// in-plane neighbours (i-1, j-1 / i+1, j+1) are loads of words that adjacent
-// lanes or the adjacent wave of the same work-group fetch in the same
-// iteration.  One thread owns all three components at its node, so the
-// shared H (or E) words are read once for the three updates, and the
-// material coefficients are read once per node for all NS field sets.
+// Work decomposition.  A thread owns one (i, j) node column of the node
+// lattice i in [0, Nx], j in [0, Ny] and marches a chunk of z planes, keeping
+// the z-neighbour values (H at k-1 in the E phase, E at k+1 in the H phase)
+// in registers; it updates all three components at its node, so shared words
+// are read once for the three updates.  Nodes are numbered x-fastest, 256 to
+// a work-group (four 64-wide waves over consecutive nodes), which keeps every
+// wave's loads on contiguous rows and leaves no partly empty tile at i = Nx.
+// With two or more field sets a thread advances two sets, so the coefficient
+// words serve both, and the work-group index is remapped so that consecutive
+// units — the set groups of one tile first — land on the same XCD
+// (work-groups are dealt to the 8 XCDs round robin) and share its L2.
+//
+// CPML.  The curl launch carries no CPML code (it would cost the whole grid
+// registers and occupancy for a thin shell).  Each CPML face then runs one
+// small launch over its slab, in the configured face order — the structure
+// of the reference (cpml.update_E / update_H per face after the curl pass).
+// A slab launch reads the element's freshly updated value, so the result is
+// the fused recursion above bit for bit.
 #include "mgh_common.h"
 #include "mgh_cpml.h"
 
+#include <algorithm>
+
 namespace {
 
-constexpr int TX = 64;
-constexpr int TY = 4;
+constexpr int WG = 256;
 
-// Slab-local index of element (i, j, k) of a component with shape
-// (n0, n1, n2) whose normal-axis extent is cut to the face's n layers.
-__device__ __forceinline__ int64_t slab_index(const mgh::CpmlFace& f, int i, int j, int k, int n0,
-                                              int n1) {
-  if (f.axis == 0) return ((int64_t)k * n1 + j) * f.n + (i - f.k0);
-  if (f.axis == 1) return ((int64_t)k * f.n + (j - f.k0)) * n0 + i;
-  return ((int64_t)(k - f.k0) * n1 + j) * n0 + i;
+// Units of WG nodes x kchunk planes x one set group.
+struct Plan {
+  int nx, ny, nz;
+  int kchunk, ngroups;
+  int ntp, nkc, units, per_xcd, remap;
+};
+
+__device__ __forceinline__ bool unit_of(const Plan& p, int& grp, int& i, int& j, int& k0,
+                                        int& k1) {
+  const int blk = blockIdx.x;
+  const int u = p.remap ? (blk & 7) * p.per_xcd + (blk >> 3) : blk;
+  if (u >= p.units) return false;
+  grp = u % p.ngroups;
+  const int t = u / p.ngroups;
+  const int tp = t % p.ntp;
+  const int kc = t / p.ntp;
+  const int q = tp * WG + (int)threadIdx.x;
+  const int nx1 = p.nx + 1;
+  if (q >= nx1 * (p.ny + 1)) return false;
+  j = q / nx1;
+  i = q - j * nx1;
+  k0 = kc * p.kchunk;
+  k1 = min(k0 + p.kchunk, p.nz + 1);
+  return true;
 }
 
-template <typename T>
-__device__ __forceinline__ T cpml_apply(const mgh::CpmlFace& f, int term, bool e_side,
-                                        int64_t sidx, int s, int layer, T d, T beta, T val,
-                                        int sign) {
-  T* psi = (T*)(e_side ? f.psi_e[term] : f.psi_h[term]) +
-           (int64_t)s * (e_side ? f.psi_e_stride[term] : f.psi_h_stride[term]);
-  const unsigned char* air = e_side ? f.air_e[term] : f.air_h[term];
-  const T b = ((const T*)f.b)[layer];
-  const T c = ((const T*)f.c)[layer];
-  const T ck = ((const T*)f.ck)[layer];
-  T p = psi[sidx];
-  p = p * b;
-  p = p + (c * d);
-  if (air != nullptr && air[sidx] == 0) p = p * T(0);
-  psi[sidx] = p;
-  const T corr = beta * ((ck * d) + p);
-  return sign > 0 ? (val + corr) : (val - corr);
-}
-
-template <typename T, int NS, bool CURL64, bool PML>
-__global__ __launch_bounds__(TX* TY) void k_update_e(T* __restrict__ e, const T* __restrict__ h,
-                                                      const T* __restrict__ ae,
-                                                      const T* __restrict__ be, int nx, int ny,
-                                                      int nz, int kchunk, int64_t e_set,
-                                                      int64_t h_set, int set0,
-                                                      mgh::CpmlSet pml) {
-  const int i = blockIdx.x * TX + threadIdx.x;
-  const int j = blockIdx.y * TY + threadIdx.y;
-  if (i > nx || j > ny) return;
-  const int k0 = blockIdx.z * kchunk;
-  const int k1 = min(k0 + kchunk, nz + 1);
+template <typename T, int NS, bool CURL64>
+__global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __restrict__ h,
+                                                 const T* __restrict__ ae,
+                                                 const T* __restrict__ be, Plan pl, int nsets,
+                                                 int64_t e_set, int64_t h_set) {
+  int grp, i, j, k0, k1;
+  if (!unit_of(pl, grp, i, j, k0, k1)) return;
+  const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
+  const int s0 = grp * NS;
 
   const mgh::Dims d{nx, ny, nz};
   const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
@@ -95,159 +99,113 @@ __global__ __launch_bounds__(TX* TY) void k_update_e(T* __restrict__ e, const T*
                 pEz = (int64_t)(ny + 1) * (nx + 1);
   const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
                 pHz = (int64_t)ny * nx;
-  // in-plane offsets
-  const int64_t rEx = (int64_t)j * nx + i;        // Ex (i<nx, j<=ny)
-  const int64_t rEy = (int64_t)j * (nx + 1) + i;  // Ey (i<=nx, j<ny)
-  const int64_t rEz = (int64_t)j * (nx + 1) + i;  // Ez (i<=nx, j<=ny)
-  const int64_t rHx = (int64_t)j * (nx + 1) + i;  // Hx (i<=nx, j<ny)
-  const int64_t rHy = (int64_t)j * nx + i;        // Hy (i<nx, j<=ny)
-  const int64_t rHz = (int64_t)j * nx + i;        // Hz (i<nx, j<ny)
-
+  // in-plane offsets: rows of nx (Ex, Hy, Hz) or nx + 1 (Ey, Ez, Hx) words
+  const int64_t rN = (int64_t)j * nx + i;
+  const int64_t rN1 = (int64_t)j * (nx + 1) + i;
   const bool inx = i < nx, iny = j < ny;
 
   T hy_prev[NS], hx_prev[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    hy_prev[s] = T(0);
-    hx_prev[s] = T(0);
-    if (k0 > 0) {
-      const T* hs = h + s * h_set;
-      if (inx) hy_prev[s] = hs[oHy + (int64_t)(k0 - 1) * pHy + rHy];
-      if (iny) hx_prev[s] = hs[(int64_t)(k0 - 1) * pHx + rHx];
+  for (int q = 0; q < NS; ++q) {
+    hy_prev[q] = T(0);
+    hx_prev[q] = T(0);
+    if (s0 + q < nsets && k0 > 0) {
+      const T* hs = h + (int64_t)(s0 + q) * h_set;
+      if (inx) hy_prev[q] = hs[oHy + (int64_t)(k0 - 1) * pHy + rN];
+      if (iny) hx_prev[q] = hs[(int64_t)(k0 - 1) * pHx + rN1];
     }
   }
 
   for (int k = k0; k < k1; ++k) {
     const bool knz = k < nz;
-    const int64_t iEx = (int64_t)k * pEx + rEx;
-    const int64_t iEy = oEy + (int64_t)k * pEy + rEy;
-    const int64_t iEz = oEz + (int64_t)k * pEz + rEz;
+    const int64_t iEx = (int64_t)k * pEx + rN;
+    const int64_t iEy = oEy + (int64_t)k * pEy + rN1;
+    const int64_t iEz = oEz + (int64_t)k * pEz + rN1;
     T a_x = T(0), b_x = T(0), a_y = T(0), b_y = T(0), a_z = T(0), b_z = T(0);
     if (inx) { a_x = ae[iEx]; b_x = be[iEx]; }
     if (iny) { a_y = ae[iEy]; b_y = be[iEy]; }
     if (knz) { a_z = ae[iEz]; b_z = be[iEz]; }
 
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const T* hs = h + s * h_set;
-      T* es = e + s * e_set;
-      // H words at this node (zero where the face does not exist)
-      const T hy = (inx && knz) ? hs[oHy + (int64_t)k * pHy + rHy] : T(0);
-      const T hx = (iny && knz) ? hs[(int64_t)k * pHx + rHx] : T(0);
-      const T hz = (inx && iny) ? hs[oHz + (int64_t)k * pHz + rHz] : T(0);
+    for (int q = 0; q < NS; ++q) {
+      if (s0 + q >= nsets) break;
+      const T* hs = h + (int64_t)(s0 + q) * h_set;
+      T* es = e + (int64_t)(s0 + q) * e_set;
+      // H words around the node (zero where the face does not exist)
+      const T hy = (inx && knz) ? hs[oHy + (int64_t)k * pHy + rN] : T(0);
+      const T hx = (iny && knz) ? hs[(int64_t)k * pHx + rN1] : T(0);
+      const T hz = (inx && iny) ? hs[oHz + (int64_t)k * pHz + rN] : T(0);
 
       if (inx) {  // Ex(i,j,k): dHz/dy - dHy/dz
-        const T hz_jm = (j > 0) ? hs[oHz + (int64_t)k * pHz + rHz - nx] : T(0);
-        T v;
+        const T hz_jm = (j > 0) ? hs[oHz + (int64_t)k * pHz + rN - nx] : T(0);
         if constexpr (CURL64) {
           double c = 0.0;
           if (iny) c += (double)hz;
           if (j > 0) c -= (double)hz_jm;
           if (knz) c -= (double)hy;
-          if (k > 0) c += (double)hy_prev[s];
-          v = (T)((double)(a_x * es[iEx]) + (double)b_x * c);
+          if (k > 0) c += (double)hy_prev[q];
+          es[iEx] = (T)((double)(a_x * es[iEx]) + (double)b_x * c);
         } else {
           T c = T(0);
           if (iny) c += hz;
           if (j > 0) c -= hz_jm;
           if (knz) c -= hy;
-          if (k > 0) c += hy_prev[s];
-          v = (a_x * es[iEx]) + (b_x * c);
+          if (k > 0) c += hy_prev[q];
+          es[iEx] = (a_x * es[iEx]) + (b_x * c);
         }
-        if constexpr (PML) {
-          for (int f = 0; f < pml.nfaces; ++f) {
-            const mgh::CpmlFace& F = pml.f[f];
-            if (F.axis == 2 && k >= F.k0 && k < F.k0 + F.n)
-              v = cpml_apply<T>(F, 0, true, slab_index(F, i, j, k, nx, ny + 1), set0 + s, k - F.k0,
-                                hy - hy_prev[s], b_x, v, +1);
-            else if (F.axis == 1 && j >= F.k0 && j < F.k0 + F.n)
-              v = cpml_apply<T>(F, 0, true, slab_index(F, i, j, k, nx, ny + 1), set0 + s, j - F.k0,
-                                hz - hz_jm, b_x, v, -1);
-          }
-        }
-        es[iEx] = v;
       }
       if (iny) {  // Ey(i,j,k): dHx/dz - dHz/dx
-        const T hz_im = (i > 0) ? hs[oHz + (int64_t)k * pHz + rHz - 1] : T(0);
-        T v;
+        const T hz_im = (i > 0) ? hs[oHz + (int64_t)k * pHz + rN - 1] : T(0);
         if constexpr (CURL64) {
           double c = 0.0;
           if (knz) c += (double)hx;
-          if (k > 0) c -= (double)hx_prev[s];
+          if (k > 0) c -= (double)hx_prev[q];
           if (inx) c -= (double)hz;
           if (i > 0) c += (double)hz_im;
-          v = (T)((double)(a_y * es[iEy]) + (double)b_y * c);
+          es[iEy] = (T)((double)(a_y * es[iEy]) + (double)b_y * c);
         } else {
           T c = T(0);
           if (knz) c += hx;
-          if (k > 0) c -= hx_prev[s];
+          if (k > 0) c -= hx_prev[q];
           if (inx) c -= hz;
           if (i > 0) c += hz_im;
-          v = (a_y * es[iEy]) + (b_y * c);
+          es[iEy] = (a_y * es[iEy]) + (b_y * c);
         }
-        if constexpr (PML) {
-          for (int f = 0; f < pml.nfaces; ++f) {
-            const mgh::CpmlFace& F = pml.f[f];
-            if (F.axis == 2 && k >= F.k0 && k < F.k0 + F.n)
-              v = cpml_apply<T>(F, 1, true, slab_index(F, i, j, k, nx + 1, ny), set0 + s, k - F.k0,
-                                hx - hx_prev[s], b_y, v, -1);
-            else if (F.axis == 0 && i >= F.k0 && i < F.k0 + F.n)
-              v = cpml_apply<T>(F, 0, true, slab_index(F, i, j, k, nx + 1, ny), set0 + s, i - F.k0,
-                                hz - hz_im, b_y, v, +1);
-          }
-        }
-        es[iEy] = v;
       }
       if (knz) {  // Ez(i,j,k): dHy/dx - dHx/dy
-        const T hy_im = (i > 0) ? hs[oHy + (int64_t)k * pHy + rHy - 1] : T(0);
-        const T hx_jm = (j > 0) ? hs[(int64_t)k * pHx + rHx - (nx + 1)] : T(0);
-        T v;
+        const T hy_im = (i > 0) ? hs[oHy + (int64_t)k * pHy + rN - 1] : T(0);
+        const T hx_jm = (j > 0) ? hs[(int64_t)k * pHx + rN1 - (nx + 1)] : T(0);
         if constexpr (CURL64) {
           double c = 0.0;
           if (inx) c += (double)hy;
           if (i > 0) c -= (double)hy_im;
           if (iny) c -= (double)hx;
           if (j > 0) c += (double)hx_jm;
-          v = (T)((double)(a_z * es[iEz]) + (double)b_z * c);
+          es[iEz] = (T)((double)(a_z * es[iEz]) + (double)b_z * c);
         } else {
           T c = T(0);
           if (inx) c += hy;
           if (i > 0) c -= hy_im;
           if (iny) c -= hx;
           if (j > 0) c += hx_jm;
-          v = (a_z * es[iEz]) + (b_z * c);
+          es[iEz] = (a_z * es[iEz]) + (b_z * c);
         }
-        if constexpr (PML) {
-          for (int f = 0; f < pml.nfaces; ++f) {
-            const mgh::CpmlFace& F = pml.f[f];
-            if (F.axis == 0 && i >= F.k0 && i < F.k0 + F.n)
-              v = cpml_apply<T>(F, 1, true, slab_index(F, i, j, k, nx + 1, ny + 1), set0 + s,
-                                i - F.k0, hy - hy_im, b_z, v, -1);
-            else if (F.axis == 1 && j >= F.k0 && j < F.k0 + F.n)
-              v = cpml_apply<T>(F, 1, true, slab_index(F, i, j, k, nx + 1, ny + 1), set0 + s,
-                                j - F.k0, hx - hx_jm, b_z, v, +1);
-          }
-        }
-        es[iEz] = v;
       }
-      hy_prev[s] = hy;
-      hx_prev[s] = hx;
+      hy_prev[q] = hy;
+      hx_prev[q] = hx;
     }
   }
 }
 
-template <typename T, int NS, bool PML>
-__global__ __launch_bounds__(TX* TY) void k_update_h(const T* __restrict__ e, T* __restrict__ h,
-                                                      const T* __restrict__ ah,
-                                                      const T* __restrict__ bh, int nx, int ny,
-                                                      int nz, int kchunk, int64_t e_set,
-                                                      int64_t h_set, int set0,
-                                                      mgh::CpmlSet pml) {
-  const int i = blockIdx.x * TX + threadIdx.x;
-  const int j = blockIdx.y * TY + threadIdx.y;
-  if (i > nx || j > ny) return;
-  const int k0 = blockIdx.z * kchunk;
-  const int k1 = min(k0 + kchunk, nz + 1);
+template <typename T, int NS>
+__global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __restrict__ h,
+                                                 const T* __restrict__ ah,
+                                                 const T* __restrict__ bh, Plan pl, int nsets,
+                                                 int64_t e_set, int64_t h_set) {
+  int grp, i, j, k0, k1;
+  if (!unit_of(pl, grp, i, j, k0, k1)) return;
+  const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
+  const int s0 = grp * NS;
 
   const mgh::Dims d{nx, ny, nz};
   const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
@@ -256,29 +214,28 @@ __global__ __launch_bounds__(TX* TY) void k_update_h(const T* __restrict__ e, T*
                 pEz = (int64_t)(ny + 1) * (nx + 1);
   const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
                 pHz = (int64_t)ny * nx;
-  const int64_t rEx = (int64_t)j * nx + i;
-  const int64_t rEy = (int64_t)j * (nx + 1) + i;
-  const int64_t rEz = (int64_t)j * (nx + 1) + i;
-  const int64_t rHx = (int64_t)j * (nx + 1) + i;
-  const int64_t rHy = (int64_t)j * nx + i;
-  const int64_t rHz = (int64_t)j * nx + i;
-
+  const int64_t rN = (int64_t)j * nx + i;
+  const int64_t rN1 = (int64_t)j * (nx + 1) + i;
   const bool inx = i < nx, iny = j < ny;
 
   // E at plane k (carried from the previous iteration's k+1 load)
   T ey_k[NS], ex_k[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const T* es = e + s * e_set;
-    ey_k[s] = iny ? es[oEy + (int64_t)k0 * pEy + rEy] : T(0);
-    ex_k[s] = inx ? es[(int64_t)k0 * pEx + rEx] : T(0);
+  for (int q = 0; q < NS; ++q) {
+    ey_k[q] = T(0);
+    ex_k[q] = T(0);
+    if (s0 + q < nsets) {
+      const T* es = e + (int64_t)(s0 + q) * e_set;
+      if (iny) ey_k[q] = es[oEy + (int64_t)k0 * pEy + rN1];
+      if (inx) ex_k[q] = es[(int64_t)k0 * pEx + rN];
+    }
   }
 
   for (int k = k0; k < k1; ++k) {
     const bool knz = k < nz;
-    const int64_t iHx = (int64_t)k * pHx + rHx;
-    const int64_t iHy = oHy + (int64_t)k * pHy + rHy;
-    const int64_t iHz = oHz + (int64_t)k * pHz + rHz;
+    const int64_t iHx = (int64_t)k * pHx + rN1;
+    const int64_t iHy = oHy + (int64_t)k * pHy + rN;
+    const int64_t iHz = oHz + (int64_t)k * pHz + rN;
     const bool hasHx = iny && knz, hasHy = inx && knz, hasHz = inx && iny;
     T a_x = T(0), b_x = T(0), a_y = T(0), b_y = T(0), a_z = T(0), b_z = T(0);
     if (hasHx) { a_x = ah[iHx]; b_x = bh[iHx]; }
@@ -286,136 +243,404 @@ __global__ __launch_bounds__(TX* TY) void k_update_h(const T* __restrict__ e, T*
     if (hasHz) { a_z = ah[iHz]; b_z = bh[iHz]; }
 
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const T* es = e + s * e_set;
-      T* hs = h + s * h_set;
+    for (int q = 0; q < NS; ++q) {
+      if (s0 + q >= nsets) break;
+      const T* es = e + (int64_t)(s0 + q) * e_set;
+      T* hs = h + (int64_t)(s0 + q) * h_set;
       // E at plane k+1 (exists for k+1 <= nz)
-      const T ey_kp = (iny && knz) ? es[oEy + (int64_t)(k + 1) * pEy + rEy] : T(0);
-      const T ex_kp = (inx && knz) ? es[(int64_t)(k + 1) * pEx + rEx] : T(0);
-      const T ez = knz ? es[oEz + (int64_t)k * pEz + rEz] : T(0);
+      const T ey_kp = (iny && knz) ? es[oEy + (int64_t)(k + 1) * pEy + rN1] : T(0);
+      const T ex_kp = (inx && knz) ? es[(int64_t)(k + 1) * pEx + rN] : T(0);
+      const T ez = knz ? es[oEz + (int64_t)k * pEz + rN1] : T(0);
       if (hasHx) {  // Hx(i,j,k): dEz/dy - dEy/dz
-        const T ez_jp = es[oEz + (int64_t)k * pEz + rEz + (nx + 1)];
-        const T c = ((ez_jp - ez) - ey_kp) + ey_k[s];
-        T v = (a_x * hs[iHx]) - (b_x * c);
-        if constexpr (PML) {
-          for (int f = 0; f < pml.nfaces; ++f) {
-            const mgh::CpmlFace& F = pml.f[f];
-            if (F.axis == 2 && k >= F.k0 && k < F.k0 + F.n)
-              v = cpml_apply<T>(F, 0, false, slab_index(F, i, j, k, nx + 1, ny), set0 + s, k - F.k0,
-                                ey_kp - ey_k[s], b_x, v, -1);
-            else if (F.axis == 1 && j >= F.k0 && j < F.k0 + F.n)
-              v = cpml_apply<T>(F, 0, false, slab_index(F, i, j, k, nx + 1, ny), set0 + s, j - F.k0,
-                                ez_jp - ez, b_x, v, +1);
-          }
-        }
-        hs[iHx] = v;
+        const T ez_jp = es[oEz + (int64_t)k * pEz + rN1 + (nx + 1)];
+        const T c = ((ez_jp - ez) - ey_kp) + ey_k[q];
+        hs[iHx] = (a_x * hs[iHx]) - (b_x * c);
       }
       if (hasHy) {  // Hy(i,j,k): dEx/dz - dEz/dx
-        const T ez_ip = es[oEz + (int64_t)k * pEz + rEz + 1];
-        const T c = ((ex_kp - ex_k[s]) - ez_ip) + ez;
-        T v = (a_y * hs[iHy]) - (b_y * c);
-        if constexpr (PML) {
-          for (int f = 0; f < pml.nfaces; ++f) {
-            const mgh::CpmlFace& F = pml.f[f];
-            if (F.axis == 2 && k >= F.k0 && k < F.k0 + F.n)
-              v = cpml_apply<T>(F, 1, false, slab_index(F, i, j, k, nx, ny + 1), set0 + s, k - F.k0,
-                                ex_kp - ex_k[s], b_y, v, +1);
-            else if (F.axis == 0 && i >= F.k0 && i < F.k0 + F.n)
-              v = cpml_apply<T>(F, 0, false, slab_index(F, i, j, k, nx, ny + 1), set0 + s, i - F.k0,
-                                ez_ip - ez, b_y, v, -1);
-          }
-        }
-        hs[iHy] = v;
+        const T ez_ip = es[oEz + (int64_t)k * pEz + rN1 + 1];
+        const T c = ((ex_kp - ex_k[q]) - ez_ip) + ez;
+        hs[iHy] = (a_y * hs[iHy]) - (b_y * c);
       }
       if (hasHz) {  // Hz(i,j,k): dEy/dx - dEx/dy
-        const T ey_ip = es[oEy + (int64_t)k * pEy + rEy + 1];
-        const T ex_jp = es[(int64_t)k * pEx + rEx + nx];
-        const T c = ((ey_ip - ey_k[s]) - ex_jp) + ex_k[s];
-        T v = (a_z * hs[iHz]) - (b_z * c);
-        if constexpr (PML) {
-          for (int f = 0; f < pml.nfaces; ++f) {
-            const mgh::CpmlFace& F = pml.f[f];
-            if (F.axis == 0 && i >= F.k0 && i < F.k0 + F.n)
-              v = cpml_apply<T>(F, 1, false, slab_index(F, i, j, k, nx, ny), set0 + s, i - F.k0,
-                                ey_ip - ey_k[s], b_z, v, +1);
-            else if (F.axis == 1 && j >= F.k0 && j < F.k0 + F.n)
-              v = cpml_apply<T>(F, 1, false, slab_index(F, i, j, k, nx, ny), set0 + s, j - F.k0,
-                                ex_jp - ex_k[s], b_z, v, -1);
-          }
-        }
-        hs[iHz] = v;
+        const T ey_ip = es[oEy + (int64_t)k * pEy + rN1 + 1];
+        const T ex_jp = es[(int64_t)k * pEx + rN + nx];
+        const T c = ((ey_ip - ey_k[q]) - ex_jp) + ex_k[q];
+        hs[iHz] = (a_z * hs[iHz]) - (b_z * c);
       }
-      ey_k[s] = ey_kp;
-      ex_k[s] = ex_kp;
+      ey_k[q] = ey_kp;
+      ex_k[q] = ex_kp;
     }
   }
 }
 
-template <typename T, int NS, bool C64, bool PML>
-int launch_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny, int nz,
-             int kchunk, int64_t es, int64_t hs, int set0, const mgh::CpmlSet& pml,
-             hipStream_t st) {
-  dim3 blk(TX, TY, 1);
-  dim3 grd((nx + 1 + TX - 1) / TX, (ny + 1 + TY - 1) / TY, (nz + 1 + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL((k_update_e<T, NS, C64, PML>), grd, blk, 0, st, (T*)e, (const T*)h,
-                     (const T*)ae, (const T*)be, nx, ny, nz, kchunk, es, hs, set0, pml);
+// ---------------------------------------------------------------------------
+// CPML slab corrections, one launch per face.
+//
+// Threads cover the face's slab nodes in the order of the psi layout (the
+// in-plane index fastest: x for y/z faces, the layer for x faces) and every
+// field set; each thread corrects the two tangential components at its node.
+// Terms per normal axis (term 0, term 1), d and sign:
+//   E  z: Ex  d = Hy(k) - Hy(k-1)   +      Ey  d = Hx(k) - Hx(k-1)   -
+//      y: Ex  d = Hz(j) - Hz(j-1)   -      Ez  d = Hx(j) - Hx(j-1)   +
+//      x: Ey  d = Hz(i) - Hz(i-1)   +      Ez  d = Hy(i) - Hy(i-1)   -
+//   H  z: Hx  d = Ey(k+1) - Ey(k)   -      Hy  d = Ex(k+1) - Ex(k)   +
+//      y: Hx  d = Ez(j+1) - Ez(j)   +      Hz  d = Ex(j+1) - Ex(j)   -
+//      x: Hy  d = Ez(i+1) - Ez(i)   -      Hz  d = Ey(i+1) - Ey(i)   +
+// (a neighbour at index -1 is zero, cpml.py:375-379).
+
+template <typename T>
+struct CFace {
+  int k0, n, ksets;
+  const T* coef;  // b[n] | c[n] | ck[n]
+  T* psi;         // [K x S0 | K x S1]
+  const unsigned char* air;  // [S0 | S1] or null
+};
+
+// psi-layout index of (i, j, k) in the slab of a face normal to AX, for a
+// component with extents (n0, n1, n2)
+template <int AX>
+__device__ __forceinline__ int64_t slab_index(int i, int j, int k, int k0, int n, int n0, int n1) {
+  if constexpr (AX == 0) return ((int64_t)k * n1 + j) * n + (i - k0);
+  else if constexpr (AX == 1) return ((int64_t)k * n + (j - k0)) * n0 + i;
+  else return ((int64_t)(k - k0) * n1 + j) * n0 + i;
+}
+
+template <typename T>
+__device__ __forceinline__ void cpml_term(const CFace<T>& F, int term, int64_t size0, int64_t size,
+                                          int64_t sidx, int s, int layer, T d, T beta, T* val,
+                                          int sign) {
+  T* psi = F.psi + (term ? (int64_t)F.ksets * size0 : 0) + (int64_t)s * size + sidx;
+  const T b = F.coef[layer], c = F.coef[F.n + layer], ck = F.coef[2 * F.n + layer];
+  T p = *psi;
+  p = p * b;
+  p = p + (c * d);
+  if (F.air != nullptr && F.air[(term ? size0 : 0) + sidx] == 0) p = p * T(0);
+  *psi = p;
+  const T corr = beta * ((ck * d) + p);
+  *val = sign > 0 ? (*val + corr) : (*val - corr);
+}
+
+// slab node (i, j, k) of thread t, in psi-layout order
+template <int AX>
+__device__ __forceinline__ bool slab_node(int64_t t, int nx, int ny, int nz, int k0, int n, int& i,
+                                          int& j, int& k) {
+  if constexpr (AX == 2) {
+    const int64_t per = (int64_t)(nx + 1) * (ny + 1);
+    if (t >= per * n) return false;
+    const int l = (int)(t / per);
+    const int r = (int)(t - (int64_t)l * per);
+    j = r / (nx + 1);
+    i = r - j * (nx + 1);
+    k = k0 + l;
+  } else if constexpr (AX == 1) {
+    const int64_t per = (int64_t)(nx + 1) * n;
+    if (t >= per * (nz + 1)) return false;
+    k = (int)(t / per);
+    const int r = (int)(t - (int64_t)k * per);
+    const int l = r / (nx + 1);
+    i = r - l * (nx + 1);
+    j = k0 + l;
+  } else {
+    const int64_t per = (int64_t)n * (ny + 1);
+    if (t >= per * (nz + 1)) return false;
+    k = (int)(t / per);
+    const int r = (int)(t - (int64_t)k * per);
+    j = r / n;
+    i = k0 + (r - j * n);
+  }
+  return true;
+}
+
+template <typename T, int AX>
+__global__ __launch_bounds__(WG) void k_cpml_e(T* __restrict__ e, const T* __restrict__ h,
+                                               const T* __restrict__ be, int nx, int ny, int nz,
+                                               int64_t e_set, int64_t h_set, CFace<T> F) {
+  int i, j, k;
+  if (!slab_node<AX>((int64_t)blockIdx.x * WG + threadIdx.x, nx, ny, nz, F.k0, F.n, i, j, k))
+    return;
+  const int s = blockIdx.y;
+  const mgh::Dims d{nx, ny, nz};
+  T* es = e + (int64_t)s * e_set;
+  const T* hs = h + (int64_t)s * h_set;
+  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
+  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
+  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
+                pEz = (int64_t)(ny + 1) * (nx + 1);
+  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
+                pHz = (int64_t)ny * nx;
+  const int64_t rN = (int64_t)j * nx + i, rN1 = (int64_t)j * (nx + 1) + i;
+  auto Hx = [&](int di, int dj, int dk) { return hs[(int64_t)(k + dk) * pHx + rN1 + dj * (nx + 1) + di]; };
+  auto Hy = [&](int di, int dj, int dk) { return hs[oHy + (int64_t)(k + dk) * pHy + rN + dj * nx + di]; };
+  auto Hz = [&](int di, int dj, int dk) { return hs[oHz + (int64_t)(k + dk) * pHz + rN + dj * nx + di]; };
+  if constexpr (AX == 2) {
+    const int layer = k - F.k0;
+    const int64_t s0 = (int64_t)F.n * nx * (ny + 1);  // Ex slab
+    if (i < nx) {  // Ex
+      const int64_t ie = (int64_t)k * pEx + rN;
+      const T dd = Hy(0, 0, 0) - (k > 0 ? Hy(0, 0, -1) : T(0));
+      T v = es[ie];
+      cpml_term(F, 0, s0, s0, slab_index<2>(i, j, k, F.k0, F.n, nx, ny + 1), s, layer, dd, be[ie],
+                &v, +1);
+      es[ie] = v;
+    }
+    if (j < ny) {  // Ey
+      const int64_t ie = oEy + (int64_t)k * pEy + rN1;
+      const T dd = Hx(0, 0, 0) - (k > 0 ? Hx(0, 0, -1) : T(0));
+      T v = es[ie];
+      cpml_term(F, 1, s0, (int64_t)F.n * (nx + 1) * ny,
+                slab_index<2>(i, j, k, F.k0, F.n, nx + 1, ny), s, layer, dd, be[ie], &v, -1);
+      es[ie] = v;
+    }
+  } else if constexpr (AX == 1) {
+    const int layer = j - F.k0;
+    const int64_t s0 = (int64_t)F.n * nx * (nz + 1);  // Ex slab
+    if (i < nx) {  // Ex
+      const int64_t ie = (int64_t)k * pEx + rN;
+      const T dd = Hz(0, 0, 0) - (j > 0 ? Hz(0, -1, 0) : T(0));
+      T v = es[ie];
+      cpml_term(F, 0, s0, s0, slab_index<1>(i, j, k, F.k0, F.n, nx, ny + 1), s, layer, dd, be[ie],
+                &v, -1);
+      es[ie] = v;
+    }
+    if (k < nz) {  // Ez
+      const int64_t ie = oEz + (int64_t)k * pEz + rN1;
+      const T dd = Hx(0, 0, 0) - (j > 0 ? Hx(0, -1, 0) : T(0));
+      T v = es[ie];
+      cpml_term(F, 1, s0, (int64_t)F.n * (nx + 1) * nz,
+                slab_index<1>(i, j, k, F.k0, F.n, nx + 1, ny + 1), s, layer, dd, be[ie], &v, +1);
+      es[ie] = v;
+    }
+  } else {
+    const int layer = i - F.k0;
+    const int64_t s0 = (int64_t)F.n * ny * (nz + 1);  // Ey slab
+    if (j < ny) {  // Ey
+      const int64_t ie = oEy + (int64_t)k * pEy + rN1;
+      const T dd = Hz(0, 0, 0) - (i > 0 ? Hz(-1, 0, 0) : T(0));
+      T v = es[ie];
+      cpml_term(F, 0, s0, s0, slab_index<0>(i, j, k, F.k0, F.n, nx + 1, ny), s, layer, dd, be[ie],
+                &v, +1);
+      es[ie] = v;
+    }
+    if (k < nz) {  // Ez
+      const int64_t ie = oEz + (int64_t)k * pEz + rN1;
+      const T dd = Hy(0, 0, 0) - (i > 0 ? Hy(-1, 0, 0) : T(0));
+      T v = es[ie];
+      cpml_term(F, 1, s0, (int64_t)F.n * (ny + 1) * nz,
+                slab_index<0>(i, j, k, F.k0, F.n, nx + 1, ny + 1), s, layer, dd, be[ie], &v, -1);
+      es[ie] = v;
+    }
+  }
+}
+
+template <typename T, int AX>
+__global__ __launch_bounds__(WG) void k_cpml_h(const T* __restrict__ e, T* __restrict__ h,
+                                               const T* __restrict__ bh, int nx, int ny, int nz,
+                                               int64_t e_set, int64_t h_set, CFace<T> F) {
+  int i, j, k;
+  if (!slab_node<AX>((int64_t)blockIdx.x * WG + threadIdx.x, nx, ny, nz, F.k0, F.n, i, j, k))
+    return;
+  const int s = blockIdx.y;
+  const mgh::Dims d{nx, ny, nz};
+  const T* es = e + (int64_t)s * e_set;
+  T* hs = h + (int64_t)s * h_set;
+  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
+  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
+  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
+                pEz = (int64_t)(ny + 1) * (nx + 1);
+  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
+                pHz = (int64_t)ny * nx;
+  const int64_t rN = (int64_t)j * nx + i, rN1 = (int64_t)j * (nx + 1) + i;
+  auto Ex = [&](int di, int dj, int dk) { return es[(int64_t)(k + dk) * pEx + rN + dj * nx + di]; };
+  auto Ey = [&](int di, int dj, int dk) { return es[oEy + (int64_t)(k + dk) * pEy + rN1 + dj * (nx + 1) + di]; };
+  auto Ez = [&](int di, int dj, int dk) { return es[oEz + (int64_t)(k + dk) * pEz + rN1 + dj * (nx + 1) + di]; };
+  // H components exist for: Hx (j < ny, k < nz), Hy (i < nx, k < nz), Hz (i < nx, j < ny)
+  if constexpr (AX == 2) {
+    const int layer = k - F.k0;
+    const int64_t s0 = (int64_t)F.n * (nx + 1) * ny;  // Hx slab
+    if (j < ny) {  // Hx
+      const int64_t ih = (int64_t)k * pHx + rN1;
+      const T dd = Ey(0, 0, 1) - Ey(0, 0, 0);
+      T v = hs[ih];
+      cpml_term(F, 0, s0, s0, slab_index<2>(i, j, k, F.k0, F.n, nx + 1, ny), s, layer, dd, bh[ih],
+                &v, -1);
+      hs[ih] = v;
+    }
+    if (i < nx) {  // Hy
+      const int64_t ih = oHy + (int64_t)k * pHy + rN;
+      const T dd = Ex(0, 0, 1) - Ex(0, 0, 0);
+      T v = hs[ih];
+      cpml_term(F, 1, s0, (int64_t)F.n * nx * (ny + 1),
+                slab_index<2>(i, j, k, F.k0, F.n, nx, ny + 1), s, layer, dd, bh[ih], &v, +1);
+      hs[ih] = v;
+    }
+  } else if constexpr (AX == 1) {
+    const int layer = j - F.k0;
+    const int64_t s0 = (int64_t)F.n * (nx + 1) * nz;  // Hx slab
+    if (k < nz) {  // Hx
+      const int64_t ih = (int64_t)k * pHx + rN1;
+      const T dd = Ez(0, 1, 0) - Ez(0, 0, 0);
+      T v = hs[ih];
+      cpml_term(F, 0, s0, s0, slab_index<1>(i, j, k, F.k0, F.n, nx + 1, ny), s, layer, dd, bh[ih],
+                &v, +1);
+      hs[ih] = v;
+    }
+    if (i < nx) {  // Hz
+      const int64_t ih = oHz + (int64_t)k * pHz + rN;
+      const T dd = Ex(0, 1, 0) - Ex(0, 0, 0);
+      T v = hs[ih];
+      cpml_term(F, 1, s0, (int64_t)F.n * nx * (nz + 1),
+                slab_index<1>(i, j, k, F.k0, F.n, nx, ny), s, layer, dd, bh[ih], &v, -1);
+      hs[ih] = v;
+    }
+  } else {
+    const int layer = i - F.k0;
+    const int64_t s0 = (int64_t)F.n * (ny + 1) * nz;  // Hy slab
+    if (k < nz) {  // Hy
+      const int64_t ih = oHy + (int64_t)k * pHy + rN;
+      const T dd = Ez(1, 0, 0) - Ez(0, 0, 0);
+      T v = hs[ih];
+      cpml_term(F, 0, s0, s0, slab_index<0>(i, j, k, F.k0, F.n, nx, ny + 1), s, layer, dd, bh[ih],
+                &v, -1);
+      hs[ih] = v;
+    }
+    if (j < ny) {  // Hz
+      const int64_t ih = oHz + (int64_t)k * pHz + rN;
+      const T dd = Ey(1, 0, 0) - Ey(0, 0, 0);
+      T v = hs[ih];
+      cpml_term(F, 1, s0, (int64_t)F.n * ny * (nz + 1),
+                slab_index<0>(i, j, k, F.k0, F.n, nx, ny), s, layer, dd, bh[ih], &v, +1);
+      hs[ih] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+constexpr int kSetsPerThread = 2;
+
+Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns) {
+  Plan p;
+  std::memset(&p, 0, sizeof(p));
+  p.nx = nx;
+  p.ny = ny;
+  p.nz = nz;
+  p.kchunk = kchunk;
+  p.ngroups = (nsets + ns - 1) / ns;
+  p.ntp = (int)(((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG);
+  p.nkc = (nz + 1 + kchunk - 1) / kchunk;
+  p.units = p.ntp * p.nkc * p.ngroups;
+  p.per_xcd = (p.units + 7) / 8;
+  p.remap = p.ngroups > 1;
+  return p;
+}
+
+int plan_blocks(const Plan& p) { return p.remap ? 8 * p.per_xcd : p.units; }
+
+template <typename T>
+CFace<T> face_of(const mgh::CpmlFace& F, bool e_side) {
+  CFace<T> c;
+  c.k0 = F.k0;
+  c.n = F.n;
+  c.ksets = F.nsets;
+  c.coef = (const T*)F.coef;
+  c.psi = (T*)(e_side ? F.psi_e : F.psi_h);
+  c.air = e_side ? F.air_e : F.air_h;
+  return c;
+}
+
+int64_t slab_threads(int axis, int nx, int ny, int nz, int n) {
+  if (axis == 2) return (int64_t)(nx + 1) * (ny + 1) * n;
+  if (axis == 1) return (int64_t)(nx + 1) * n * (nz + 1);
+  return (int64_t)n * (ny + 1) * (nz + 1);
+}
+
+template <typename T, bool C64>
+int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny, int nz,
+          int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st) {
+  const int ns = nsets >= 2 ? kSetsPerThread : 1;
+  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns);
+  if (ns == kSetsPerThread)
+    hipLaunchKernelGGL((k_update_e<T, kSetsPerThread, C64>), dim3(plan_blocks(pl)), dim3(WG), 0, st,
+                       (T*)e, (const T*)h, (const T*)ae, (const T*)be, pl, nsets, es, hs);
+  else
+    hipLaunchKernelGGL((k_update_e<T, 1, C64>), dim3(plan_blocks(pl)), dim3(WG), 0, st, (T*)e,
+                       (const T*)h, (const T*)ae, (const T*)be, pl, nsets, es, hs);
   MGH_CHECK_LAUNCH("k_update_e");
   return MGH_OK;
 }
 
-template <typename T, int NS, bool PML>
-int launch_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny, int nz,
-             int kchunk, int64_t es, int64_t hs, int set0, const mgh::CpmlSet& pml,
-             hipStream_t st) {
-  dim3 blk(TX, TY, 1);
-  dim3 grd((nx + 1 + TX - 1) / TX, (ny + 1 + TY - 1) / TY, (nz + 1 + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL((k_update_h<T, NS, PML>), grd, blk, 0, st, (const T*)e, (T*)h, (const T*)ah,
-                     (const T*)bh, nx, ny, nz, kchunk, es, hs, set0, pml);
+template <typename T>
+int run_cpml_e(void* e, const void* h, const void* be, int nx, int ny, int nz, int nsets,
+               int64_t es, int64_t hs, const mgh::CpmlSet& pml, hipStream_t st) {
+  for (int f = 0; f < pml.nfaces; ++f) {
+    const mgh::CpmlFace& F = pml.f[f];
+    const int64_t nt = slab_threads(F.axis, nx, ny, nz, F.n);
+    const dim3 grd((unsigned)((nt + WG - 1) / WG), (unsigned)nsets, 1);
+    const CFace<T> c = face_of<T>(F, true);
+    if (F.axis == 0)
+      hipLaunchKernelGGL((k_cpml_e<T, 0>), grd, dim3(WG), 0, st, (T*)e, (const T*)h, (const T*)be,
+                         nx, ny, nz, es, hs, c);
+    else if (F.axis == 1)
+      hipLaunchKernelGGL((k_cpml_e<T, 1>), grd, dim3(WG), 0, st, (T*)e, (const T*)h, (const T*)be,
+                         nx, ny, nz, es, hs, c);
+    else
+      hipLaunchKernelGGL((k_cpml_e<T, 2>), grd, dim3(WG), 0, st, (T*)e, (const T*)h, (const T*)be,
+                         nx, ny, nz, es, hs, c);
+    MGH_CHECK_LAUNCH("k_cpml_e");
+  }
+  return MGH_OK;
+}
+
+template <typename T>
+int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny, int nz,
+          int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st) {
+  const int ns = nsets >= 2 ? kSetsPerThread : 1;
+  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns);
+  if (ns == kSetsPerThread)
+    hipLaunchKernelGGL((k_update_h<T, kSetsPerThread>), dim3(plan_blocks(pl)), dim3(WG), 0, st,
+                       (const T*)e, (T*)h, (const T*)ah, (const T*)bh, pl, nsets, es, hs);
+  else
+    hipLaunchKernelGGL((k_update_h<T, 1>), dim3(plan_blocks(pl)), dim3(WG), 0, st, (const T*)e,
+                       (T*)h, (const T*)ah, (const T*)bh, pl, nsets, es, hs);
   MGH_CHECK_LAUNCH("k_update_h");
   return MGH_OK;
 }
 
-#define MGH_NS_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
-
-template <typename T, bool C64, bool PML>
-int dispatch_e(int ns, void* e, const void* h, const void* ae, const void* be, int nx, int ny,
-               int nz, int kc, int64_t es, int64_t hs, int set0, const mgh::CpmlSet& pml,
-               hipStream_t st) {
-  switch (ns) {
-#define X(N) \
-  case N: return launch_e<T, N, C64, PML>(e, h, ae, be, nx, ny, nz, kc, es, hs, set0, pml, st);
-    MGH_NS_CASES(X)
-#undef X
+template <typename T>
+int run_cpml_h(const void* e, void* h, const void* bh, int nx, int ny, int nz, int nsets,
+               int64_t es, int64_t hs, const mgh::CpmlSet& pml, hipStream_t st) {
+  for (int f = 0; f < pml.nfaces; ++f) {
+    const mgh::CpmlFace& F = pml.f[f];
+    const int64_t nt = slab_threads(F.axis, nx, ny, nz, F.n);
+    const dim3 grd((unsigned)((nt + WG - 1) / WG), (unsigned)nsets, 1);
+    const CFace<T> c = face_of<T>(F, false);
+    if (F.axis == 0)
+      hipLaunchKernelGGL((k_cpml_h<T, 0>), grd, dim3(WG), 0, st, (const T*)e, (T*)h, (const T*)bh,
+                         nx, ny, nz, es, hs, c);
+    else if (F.axis == 1)
+      hipLaunchKernelGGL((k_cpml_h<T, 1>), grd, dim3(WG), 0, st, (const T*)e, (T*)h, (const T*)bh,
+                         nx, ny, nz, es, hs, c);
+    else
+      hipLaunchKernelGGL((k_cpml_h<T, 2>), grd, dim3(WG), 0, st, (const T*)e, (T*)h, (const T*)bh,
+                         nx, ny, nz, es, hs, c);
+    MGH_CHECK_LAUNCH("k_cpml_h");
   }
-  mgh::set_error("unsupported field-set group size %d", ns);
-  return MGH_EINVAL;
+  return MGH_OK;
 }
-
-template <typename T, bool PML>
-int dispatch_h(int ns, const void* e, void* h, const void* ah, const void* bh, int nx, int ny,
-               int nz, int kc, int64_t es, int64_t hs, int set0, const mgh::CpmlSet& pml,
-               hipStream_t st) {
-  switch (ns) {
-#define X(N) \
-  case N: return launch_h<T, N, PML>(e, h, ah, bh, nx, ny, nz, kc, es, hs, set0, pml, st);
-    MGH_NS_CASES(X)
-#undef X
-  }
-  mgh::set_error("unsupported field-set group size %d", ns);
-  return MGH_EINVAL;
-}
-
-constexpr int kMaxGroup = 8;
 
 int check_common(int dtype, int nx, int ny, int nz, int nsets, int kchunk) {
   MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 (float32) or 8 (float64), got %d",
               dtype);
   MGH_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "grid must have at least one cell per axis");
-  MGH_REQUIRE(nsets >= 1, "nsets must be >= 1, got %d", nsets);
+  MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "nsets must be in 1..65535, got %d", nsets);
   MGH_REQUIRE(kchunk >= 1, "kchunk must be >= 1, got %d", kchunk);
-  // Work-group grid limits: y dimension carries ceil((ny+1)/4) tiles.
-  MGH_REQUIRE((int64_t)(ny + 1) / TY < 65535 && (int64_t)(nz + 1) / kchunk < 65535,
-              "grid too large for the launch geometry");
+  // node numbers, unit counts and slab thread blocks are 32-bit
+  const int64_t npl = (int64_t)(nx + 1) * (ny + 1);
+  const int64_t units =
+      ((npl + WG - 1) / WG) * ((int64_t)(nz + 1 + kchunk - 1) / kchunk) * (int64_t)nsets;
+  MGH_REQUIRE(npl < (1ll << 31) && units + 8 < (1ll << 31),
+              "grid too large for the launch plan");
   return MGH_OK;
 }
 
@@ -423,99 +648,101 @@ int check_common(int dtype, int nx, int ny, int nz, int nsets, int kchunk) {
 
 extern "C" {
 
-int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
-                     long long e_set_stride, const void* h, long long h_set_stride,
-                     const void* alpha_e, const void* beta_e, int curl64, int kchunk,
-                     const void* cpml, void* stream) {
+int mgh_update_e(int dtype, int nx, int ny, int nz, int nsets, void* e, long long e_set_stride,
+                 const void* h, long long h_set_stride, const void* alpha_e, const void* beta_e,
+                 int curl64, int kchunk, void* stream) {
   int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
   if (rc) return rc;
   MGH_REQUIRE(e && h && alpha_e && beta_e, "null device pointer");
   MGH_REQUIRE(!(curl64 && dtype == 8), "curl64 applies to float32 fields only");
-  mgh::CpmlSet pml{};
-  if (cpml) {
-    rc = mgh::load_cpml_set(cpml, nx, ny, nz, &pml);
-    if (rc) return rc;
-  }
-  const bool has_pml = pml.nfaces > 0;
   hipStream_t st = (hipStream_t)stream;
-  const size_t w = (size_t)dtype;
-  for (int s0 = 0; s0 < nsets; s0 += kMaxGroup) {
-    const int ns = (nsets - s0) < kMaxGroup ? (nsets - s0) : kMaxGroup;
-    char* eg = (char*)e + (size_t)s0 * (size_t)e_set_stride * w;
-    const char* hg = (const char*)h + (size_t)s0 * (size_t)h_set_stride * w;
-    if (dtype == 4) {
-      if (curl64)
-        rc = has_pml ? dispatch_e<float, true, true>(ns, eg, hg, alpha_e, beta_e, nx, ny, nz,
-                                                     kchunk, e_set_stride, h_set_stride, s0, pml, st)
-                     : dispatch_e<float, true, false>(ns, eg, hg, alpha_e, beta_e, nx, ny, nz,
-                                                      kchunk, e_set_stride, h_set_stride, s0, pml,
-                                                      st);
-      else
-        rc = has_pml ? dispatch_e<float, false, true>(ns, eg, hg, alpha_e, beta_e, nx, ny, nz,
-                                                      kchunk, e_set_stride, h_set_stride, s0, pml,
-                                                      st)
-                     : dispatch_e<float, false, false>(ns, eg, hg, alpha_e, beta_e, nx, ny, nz,
-                                                       kchunk, e_set_stride, h_set_stride, s0,
-                                                       pml, st);
-    } else {
-      rc = has_pml ? dispatch_e<double, false, true>(ns, eg, hg, alpha_e, beta_e, nx, ny, nz,
-                                                     kchunk, e_set_stride, h_set_stride, s0, pml,
-                                                     st)
-                   : dispatch_e<double, false, false>(ns, eg, hg, alpha_e, beta_e, nx, ny, nz,
-                                                      kchunk, e_set_stride, h_set_stride, s0, pml,
-                                                      st);
-    }
+  if (dtype == 4)
+    return curl64 ? run_e<float, true>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk,
+                                        e_set_stride, h_set_stride, st)
+                  : run_e<float, false>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk,
+                                         e_set_stride, h_set_stride, st);
+  return run_e<double, false>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk, e_set_stride,
+                              h_set_stride, st);
+}
+
+int mgh_update_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                 long long e_set_stride, void* h, long long h_set_stride, const void* alpha_h,
+                 const void* beta_h, int kchunk, void* stream) {
+  int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && alpha_h && beta_h, "null device pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return run_h<float>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
+                        h_set_stride, st);
+  return run_h<double>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
+                       h_set_stride, st);
+}
+
+int mgh_cpml_e(int dtype, int nx, int ny, int nz, int nsets, void* e, long long e_set_stride,
+               const void* h, long long h_set_stride, const void* beta_e, const void* cpml,
+               void* stream) {
+  int rc = check_common(dtype, nx, ny, nz, nsets, 1);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && beta_e && cpml, "null pointer");
+  mgh::CpmlSet pml{};
+  rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return run_cpml_e<float>(e, h, beta_e, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+  return run_cpml_e<double>(e, h, beta_e, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+}
+
+int mgh_cpml_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+               long long e_set_stride, void* h, long long h_set_stride, const void* beta_h,
+               const void* cpml, void* stream) {
+  int rc = check_common(dtype, nx, ny, nz, nsets, 1);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && beta_h && cpml, "null pointer");
+  mgh::CpmlSet pml{};
+  rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return run_cpml_h<float>(e, h, beta_h, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+  return run_cpml_h<double>(e, h, beta_h, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+}
+
+int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                     long long e_set_stride, const void* h, long long h_set_stride,
+                     const void* alpha_e, const void* beta_e, int curl64, int kchunk,
+                     const void* cpml, void* stream) {
+  if (cpml) {  // validate before launching anything
+    int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
+    if (rc) return rc;
+    mgh::CpmlSet pml{};
+    rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
     if (rc) return rc;
   }
-  return MGH_OK;
+  int rc = mgh_update_e(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_e,
+                        beta_e, curl64, kchunk, stream);
+  if (rc || !cpml) return rc;
+  return mgh_cpml_e(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, beta_e, cpml,
+                    stream);
 }
 
 int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e,
                      long long e_set_stride, void* h, long long h_set_stride,
                      const void* alpha_h, const void* beta_h, int kchunk, const void* cpml,
                      void* stream) {
-  int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
-  if (rc) return rc;
-  MGH_REQUIRE(e && h && alpha_h && beta_h, "null device pointer");
-  mgh::CpmlSet pml{};
   if (cpml) {
-    rc = mgh::load_cpml_set(cpml, nx, ny, nz, &pml);
+    int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
+    if (rc) return rc;
+    mgh::CpmlSet pml{};
+    rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
     if (rc) return rc;
   }
-  const bool has_pml = pml.nfaces > 0;
-  hipStream_t st = (hipStream_t)stream;
-  const size_t w = (size_t)dtype;
-  for (int s0 = 0; s0 < nsets; s0 += kMaxGroup) {
-    const int ns = (nsets - s0) < kMaxGroup ? (nsets - s0) : kMaxGroup;
-    const char* eg = (const char*)e + (size_t)s0 * (size_t)e_set_stride * w;
-    char* hg = (char*)h + (size_t)s0 * (size_t)h_set_stride * w;
-    if (dtype == 4)
-      rc = has_pml ? dispatch_h<float, true>(ns, eg, hg, alpha_h, beta_h, nx, ny, nz, kchunk,
-                                             e_set_stride, h_set_stride, s0, pml, st)
-                   : dispatch_h<float, false>(ns, eg, hg, alpha_h, beta_h, nx, ny, nz, kchunk,
-                                              e_set_stride, h_set_stride, s0, pml, st);
-    else
-      rc = has_pml ? dispatch_h<double, true>(ns, eg, hg, alpha_h, beta_h, nx, ny, nz, kchunk,
-                                              e_set_stride, h_set_stride, s0, pml, st)
-                   : dispatch_h<double, false>(ns, eg, hg, alpha_h, beta_h, nx, ny, nz, kchunk,
-                                               e_set_stride, h_set_stride, s0, pml, st);
-    if (rc) return rc;
-  }
-  return MGH_OK;
-}
-
-int mgh_update_e(int dtype, int nx, int ny, int nz, int nsets, void* e, long long e_set_stride,
-                 const void* h, long long h_set_stride, const void* alpha_e, const void* beta_e,
-                 int curl64, int kchunk, void* stream) {
-  return mgh_update_e_pml(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_e,
-                          beta_e, curl64, kchunk, nullptr, stream);
-}
-
-int mgh_update_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
-                 long long e_set_stride, void* h, long long h_set_stride, const void* alpha_h,
-                 const void* beta_h, int kchunk, void* stream) {
-  return mgh_update_h_pml(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_h,
-                          beta_h, kchunk, nullptr, stream);
+  int rc = mgh_update_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_h,
+                        beta_h, kchunk, stream);
+  if (rc || !cpml) return rc;
+  return mgh_cpml_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, beta_h, cpml,
+                    stream);
 }
 
 }  // extern "C"

@@ -17,7 +17,7 @@ import torch
 
 from . import _lib
 
-DEFAULT_KCHUNK = 16
+DEFAULT_KCHUNK = 2
 
 
 def _dtype_code(t: torch.Tensor) -> int:
@@ -84,6 +84,32 @@ def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: to
     _lib.call("mgh_update_e_pml", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
               alpha_e.data_ptr(), beta_e.data_ptr(), int(bool(curl64)), int(kchunk), desc,
               stream_handle(stream))
+
+
+def cpml_e(e: torch.Tensor, h: torch.Tensor, beta_e: torch.Tensor, dims: tuple[int, int, int],
+           cpml, *, stream: torch.cuda.Stream | None = None) -> None:
+    """CPML slab passes of the E phase alone (after :func:`update_e` without ``cpml``)."""
+    nx, ny, nz = dims
+    ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
+    k = e.view(-1, ne).shape[0]
+    if h.view(-1, nh).shape[0] != k or beta_e.numel() != ne:
+        raise ValueError("field sets or coefficient length do not match the grid")
+    _check_dev(e, h, beta_e)
+    _lib.call("mgh_cpml_e", _dtype_code(e), nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+              beta_e.data_ptr(), _cpml_desc(cpml, k, e.dtype), stream_handle(stream))
+
+
+def cpml_h(e: torch.Tensor, h: torch.Tensor, beta_h: torch.Tensor, dims: tuple[int, int, int],
+           cpml, *, stream: torch.cuda.Stream | None = None) -> None:
+    """CPML slab passes of the H phase alone (after :func:`update_h` without ``cpml``)."""
+    nx, ny, nz = dims
+    ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
+    k = e.view(-1, ne).shape[0]
+    if h.view(-1, nh).shape[0] != k or beta_h.numel() != nh:
+        raise ValueError("field sets or coefficient length do not match the grid")
+    _check_dev(e, h, beta_h)
+    _lib.call("mgh_cpml_h", _dtype_code(h), nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+              beta_h.data_ptr(), _cpml_desc(cpml, k, h.dtype), stream_handle(stream))
 
 
 def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: torch.Tensor,

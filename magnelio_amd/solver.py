@@ -315,10 +315,12 @@ class FITTimeDomainSolver:
         if ev is not None:
             ev[0].record(stream)
         kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
-                         kchunk=self.kchunk, cpml=self.cpml, stream=stream)
+                         kchunk=self.kchunk, stream=stream)
         if ev is not None:
             ev[1].record(stream)
             self.timed_events.append(ev)
+        if self.cpml is not None:
+            kernels.cpml_e(self.e, self.h, self.beta_e, dims, self.cpml, stream=stream)
         if self._dispersion is not None:
             self._dispersion.update(self, stream)
         for src in self.sources:
@@ -330,7 +332,9 @@ class FITTimeDomainSolver:
         for el in self.lumped:
             el.step(self, stream)
         kernels.update_h(self.e, self.h, self.alpha_h, self.beta_h, dims, kchunk=self.kchunk,
-                         cpml=self.cpml, stream=stream)
+                         stream=stream)
+        if self.cpml is not None:
+            kernels.cpml_h(self.e, self.h, self.beta_h, dims, self.cpml, stream=stream)
         for src in self.sources:
             src.inject_h(self, stream)
         for mon in self.monitors:

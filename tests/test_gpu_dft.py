@@ -55,7 +55,8 @@ def test_block_contraction_matches_numpy(gpu):
     for nb in (64, 23):
         for comp in (False, True):
             print(f"dft block contraction nb={nb} compensated={comp}", flush=True)
-            base = rng.standard_normal((K, F, S, 2))
+            # bins already holding values of the size of one block's sum
+            base = rng.standard_normal((K, F, S, 2)) * (8 * dt)
             bins = torch.from_numpy(base.copy()).to(dev)
             lo = torch.zeros_like(bins) if comp else None
             stream = torch.cuda.current_stream(dev)
