@@ -137,6 +137,21 @@ int mgh_weighted_sq_sum(int dtype, long long n, int nsets, const void* x, long l
                         const void* w, double* out, int accumulate, double scale, double* work,
                         long long work_len, void* stream);
 
+/* ---- (a14) TF/SF plane-wave corrections ---------------------------------
+ * Replaces PlaneWaveSource.inject_E / inject_H / _apply
+ * (src/magnelio/sources/plane_wave.py:710-734), called after the E phase and
+ * after the H phase (fit_td.py:965-967, :991-993).  The face patches
+ * (plane_wave.py:318-707, coef = beta * (factor * metric)) arrive grouped by
+ * target element: targets[t] (int64 device-layout index) owns terms
+ * [ptr[t], ptr[t+1]) (int32), applied in order:
+ *   f[set][targets[t]] += coef[q] * wave[n * n_delays + did[q]]
+ * with wave a tab_len x n_delays table of waveform samples in the field
+ * dtype and n = *step_ctr + step_offset (no injection outside the table). */
+int mgh_tfsf_apply(int dtype, int nsets, void* field, long long set_stride, int n_targets,
+                   const long long* targets, const int* ptr, const void* coef, const int* did,
+                   const void* wave, int n_delays, long long tab_len, const long long* step_ctr,
+                   long long step_offset, void* stream);
+
 /* ---- (a16) DFT monitors ----------------------------------------------------
  * Replace DFTAccumulator.accumulate (src/magnelio/monitors/_dft.py:37-52) fed
  * with cell-centred samples (src/magnelio/monitors/base.py:51-173) from the

@@ -73,6 +73,11 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_ll, _c_int, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int, _c_double,
          _c_void_p, _c_ll, _c_void_p],
     ),
+    "mgh_tfsf_apply": (
+        _c_int,
+        [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+         _c_void_p, _c_int, _c_ll, _c_void_p, _c_ll, _c_void_p],
+    ),
     "mgh_dft_sample": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int,

@@ -319,10 +319,11 @@ class FITTimeDomainSolver:
         if ev is not None:
             ev[1].record(stream)
             self.timed_events.append(ev)
-        if self.cpml is not None:
-            kernels.cpml_e(self.e, self.h, self.beta_e, dims, self.cpml, stream=stream)
+        # reference E phase order: curl, dispersion completion, CPML faces
         if self._dispersion is not None:
             self._dispersion.update(self, stream)
+        if self.cpml is not None:
+            kernels.cpml_e(self.e, self.h, self.beta_e, dims, self.cpml, stream=stream)
         for src in self.sources:
             src.inject_e(self, stream)
         if self.dports is not None:
