@@ -1,11 +1,14 @@
 """FIT-TD stepping benchmark on one MI355X.
 
 Prints ONE JSON line: Yee Gcell-updates/s of the stepping engine on the
-single-large-job workload (256^3 seeded non-uniform grid with 40 seeded
-dielectric bricks/spheres, eps_r ~ U[1, 10]), the HBM-roofline fraction of
-the dominant kernel measured live with HIP events, and the CPU baseline (the
-oracle's C/OpenMP restatement of the reference's Numba kernels) timed on the
-host's cores on a bounded sample of the same workload.
+single-large-job workload — 256^3 seeded non-uniform grid with 40 seeded
+dielectric bricks/spheres (eps_r ~ U[1, 10]), 8-layer CPML on all six faces,
+a TF/SF plane wave (+z, x-polarised Gaussian) and a 1001-frequency far-field
+(Huygens box) DFT monitor with compensated float64 bins, every part of the
+step inside the timed region — the HBM-roofline fraction of the dominant
+kernel measured live with HIP events, and the CPU baseline (the oracle's
+C/OpenMP restatement of the reference's Numba curl kernels) timed on the
+host's cores on a bounded sample of the same grid.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -98,7 +101,8 @@ def cpu_baseline(mesh, coeffs, nsamp=(10, 20, 40)):
         "unit": "Gcell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{nx}x{ny}x{nz} same workload, E+H fused Numba-order C/OpenMP, "
+        "sample": (f"{nx}x{ny}x{nz} same grid and coefficients, E+H curl kernels only (Numba-order "
+                   "C/OpenMP; CPML, TF/SF and DFT not included), "
                    f"{'/'.join(map(str, nsamp))} steps, fit t=a+N*c; {threads} threads of {cpu}"),
     }
 
@@ -106,10 +110,13 @@ def cpu_baseline(mesh, coeffs, nsamp=(10, 20, 40)):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=512)
+    ap.add_argument("--warmup", type=int, default=256)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--k", type=int, default=1, help="field sets advanced per launch")
+    ap.add_argument("--freqs", type=int, default=1001, help="DFT monitor frequencies (0: none)")
+    ap.add_argument("--dft-block", type=int, default=256, help="steps per DFT contraction")
+    ap.add_argument("--no-tfsf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -121,20 +128,34 @@ def main(argv=None) -> int:
 
     from magnelio_amd import mesh as mm
     from magnelio_amd.boundaries import CPMLBoundary
+    from magnelio_amd.monitors import MonitorFarField
     from magnelio_amd.solver import FITTimeDomainSolver
+    from magnelio_amd.sources import PlaneWaveSource
 
     t_setup = time.perf_counter()
-    mesh = c2_mesh(args.n)
+    length = 64e-3
+    mesh = c2_mesh(args.n, length=length)
     nx, ny, nz = mesh.dims
     dt = mm.courant_dt(mesh.grid, "normal")
     bcs = {f: CPMLBoundary(f, mesh.grid, thickness_cells=8) for f in mm.FACES}
     total = args.warmup + args.steps
+    sources, monitors = [], []
+    if not args.no_tfsf:
+        sources.append(PlaneWaveSource(direction=(0, 0, 1), polarization=(1, 0, 0),
+                                       corners=((0.15 * length,) * 3, (0.85 * length,) * 3),
+                                       f_max=20e9, waveform="gaussian"))
+    if args.freqs > 0:
+        # Huygens box 10 cells in from every face (8 CPML layers + 2)
+        monitors.append(MonitorFarField(np.linspace(1e9, 20e9, args.freqs), margin_cells=2,
+                                        block=args.dft_block, name="far_field"))
     sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, total_time_steps=total, dt=dt,
-                              precision="single", nsets=args.k, record=False)
+                              precision="single", nsets=args.k, record=False, sources=sources,
+                              monitors=monitors)
     sol.setup()
-    # a seeded initial field so the timed steps do real arithmetic
+    # a seeded initial field so the timed steps do real arithmetic from the start
     g = torch.Generator(device=sol.e.device).manual_seed(7)
     sol.e.normal_(generator=g).mul_(1e-3).mul_((sol.beta_e != 0).float())
+    torch.cuda.synchronize()
     setup_s = time.perf_counter() - t_setup
 
     stream = torch.cuda.current_stream()
@@ -176,9 +197,15 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {
             "workload": (f"single large job {nx}x{ny}x{nz} seeded non-uniform grid (ratio <= 1.5), "
-                         "40 seeded dielectric bricks/spheres eps_r U[1,10], 8-layer CPML on six faces "
-                         "(fused); TF/SF source and DFT monitor not yet in the timed step"),
+                         "40 seeded dielectric bricks/spheres eps_r U[1,10], 8-layer CPML on six faces"
+                         + ("" if args.no_tfsf else
+                            ", TF/SF +z x-polarised Gaussian (f_max 20 GHz) on [0.15L, 0.85L]^3")
+                         + ("" if not monitors else
+                            f", far-field Huygens-box DFT {args.freqs} freqs 1-20 GHz, 10 cells in, "
+                            f"{monitors[0].S} samples/step, block {args.dft_block}, compensated FP64 bins")
+                         + "; all inside the timed steps"),
             "K": K,
+            "dft_bins_gb": round(monitors[0].bytes() / 1e9, 2) if monitors else 0.0,
             "setup_s": round(setup_s, 2),
         },
         "roofline": {

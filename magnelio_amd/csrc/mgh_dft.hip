@@ -13,21 +13,26 @@
 //   P[f, b] = exp(j w_f b dt) dt   (float32, host-built),
 //   t0 = n0 dt (E samples) or n0 dt + dt/2 (H samples).
 // The inner sum is a (2F x B) x (B x S) product done with
-// v_mfma_f32_16x16x4f32 (16 frequencies x 16 samples per wave, K = B = 64 in
-// 16 MFMA steps); the outer factor exp(j w t0) is float64 and the update of
-// the float64 bins is fused into the MFMA epilogue, so the partial products
-// never reach HBM.  Each wave keeps its 16-sample x 64-step operand in
-// registers and sweeps every frequency, so x is read once per block and the
-// bins are read and written once per block: the kernel is bound by the bin
-// traffic (16 B complex128 per (f, s), x2 with compensation).
+// v_mfma_f32_16x16x4f32 (16 frequencies x 16 samples per wave, K = B in B/4
+// MFMA steps, B up to 256); the outer factor exp(j w t0) is float64 and the
+// update of the float64 bins is fused into the MFMA epilogue, so the partial
+// products never reach HBM.  Each wave keeps its 16-sample x B-step operand
+// in registers and sweeps every frequency, so x is read once per block and
+// the bins are read and written once per block: the kernel is bound by the
+// bin traffic (16 B complex128 per (f, s), x2 with compensation), which a
+// longer block divides.
 #include "mgh_common.h"
 
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int DFT_NT = 8;     // terms per sample (cell-centre averaging)
-constexpr int DFT_BMAX = 64;  // block length (MFMA K)
+constexpr int DFT_NT = 8;      // terms per sample (cell-centre averaging)
+constexpr int DFT_BMAX = 256;  // longest block (MFMA K)
+
+// phasor-table row length for a block of B steps (a whole number of 64-step
+// register tiles)
+inline int dft_bpad(int B) { return (B + 63) / 64 * 64; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dft_sample(const T* __restrict__ e, int64_t e_set,
@@ -54,7 +59,7 @@ __global__ __launch_bounds__(256) void k_dft_sample(const T* __restrict__ e, int
       acc += wt * v;
     }
   }
-  X[((int64_t)set * DFT_BMAX + b) * S_pad + s] = acc;
+  X[((int64_t)set * B + b) * S_pad + s] = acc;
 }
 
 // Block phase factors exp(j w t0) for E and H stamps: blk[f*4 + 0..3] =
@@ -86,9 +91,12 @@ __device__ __forceinline__ void two_sum_acc(double* hi, double* lo, double v) {
   *lo += err;
 }
 
+// NB: MFMA k-steps per block (B padded to a multiple of 64, divided by 4).
+template <int NB>
 __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, int S, int S_pad,
-                                                   int nb_valid, const float* __restrict__ Pc,
-                                                   const float* __restrict__ Ps, int F, int F_pad,
+                                                   int B, int nb_valid,
+                                                   const float* __restrict__ Pc,
+                                                   const float* __restrict__ Ps, int F,
                                                    const double* __restrict__ blk, int s_h_start,
                                                    double* __restrict__ bins,
                                                    double* __restrict__ lo) {
@@ -99,10 +107,10 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
   if (s0 >= S) return;
   const int col = lane & 15;   // sample within the tile (B operand column / D column)
   const int krow = lane >> 4;  // 0..3: k within an MFMA step (A/B), row group of D
-  const float* Xs = X + (int64_t)set * DFT_BMAX * S_pad;
-  float xb[16];
+  const float* Xs = X + (int64_t)set * B * S_pad;
+  float xb[NB];
 #pragma unroll
-  for (int kk = 0; kk < 16; ++kk) {
+  for (int kk = 0; kk < NB; ++kk) {
     const int b = 4 * kk + krow;
     xb[kk] = b < nb_valid ? Xs[(int64_t)b * S_pad + s0 + col] : 0.0f;
   }
@@ -113,14 +121,21 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
   for (int f0 = 0; f0 < F; f0 += 16) {
     v4f cc = {0.f, 0.f, 0.f, 0.f};
     v4f cs = {0.f, 0.f, 0.f, 0.f};
-    const int fa = f0 + col;  // A operand row (frequency) for this lane
+    // this lane's A operands of the tile: NB contiguous words (lane order)
+    const float4* pc = (const float4*)(Pc + ((int64_t)(f0 >> 4) * 64 + lane) * NB);
+    const float4* ps = (const float4*)(Ps + ((int64_t)(f0 >> 4) * 64 + lane) * NB);
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      const int b = 4 * kk + krow;
-      const float ac = Pc[(int64_t)fa * DFT_BMAX + b];
-      const float as = Ps[(int64_t)fa * DFT_BMAX + b];
-      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac, xb[kk], cc, 0, 0, 0);
-      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as, xb[kk], cs, 0, 0, 0);
+    for (int q = 0; q < NB / 4; ++q) {
+      const float4 ac = pc[q];
+      const float4 as = ps[q];
+      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.x, xb[4 * q + 0], cc, 0, 0, 0);
+      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, xb[4 * q + 0], cs, 0, 0, 0);
+      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.y, xb[4 * q + 1], cc, 0, 0, 0);
+      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, xb[4 * q + 1], cs, 0, 0, 0);
+      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.z, xb[4 * q + 2], cc, 0, 0, 0);
+      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, xb[4 * q + 2], cs, 0, 0, 0);
+      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.w, xb[4 * q + 3], cc, 0, 0, 0);
+      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, xb[4 * q + 3], cs, 0, 0, 0);
     }
     if (s < S) {
 #pragma unroll
@@ -145,7 +160,6 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
       }
     }
   }
-  (void)F_pad;
 }
 
 }  // namespace
@@ -153,7 +167,7 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
 extern "C" {
 
 // Sample one step into row ((n - n_start) mod B) of the block buffer
-// X (nsets x 64 x S_pad float32).  idx/w: S x 8 terms; idx >= 0 indexes the
+// X (nsets x B x S_pad float32).  idx/w: S x 8 terms; idx >= 0 indexes the
 // E vector, idx < 0 the H vector at -(idx + 1) (device layout); weight 0
 // marks an unused term.
 int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, const void* h,
@@ -161,7 +175,7 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
                    int S_pad, int B, const long long* step_ctr, long long n_start, void* stream) {
   MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8");
   MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts S=%d S_pad=%d", S, S_pad);
-  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..64, got %d", B);
+  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..256, got %d", B);
   MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
   MGH_REQUIRE(e && h && idx && w && X && step_ctr, "null pointer");
   dim3 grd((S + 255) / 256, nsets, 1);
@@ -178,10 +192,13 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
 
 // Contract the block (first nb_valid rows of X) and accumulate into the
 // float64 bins (nsets x F x S complex, interleaved re/im); lo (same shape or
-// NULL) holds the compensation terms.  Pc/Ps: F_pad x 64 float32 phasor
-// tables cos/sin(w_f b dt) dt, zero beyond B and F.  Samples with index >=
-// s_h_start carry the H time stamp.  blk: F x 4 float64 scratch for the
-// block phase, filled here from *step_ctr (n = last step of the block).
+// NULL) holds the compensation terms.  Pc/Ps: the float32 phasor tables
+// P[f][b] = cos/sin(w_f b dt) dt (f < F_pad, b < Bp = B rounded up to a
+// multiple of 64, zero beyond B and F) stored in MFMA lane order:
+// element [t][l][q] (t < F_pad/16, l < 64, q < Bp/4) = P[16t + l%16][4q + l/16],
+// so each lane streams its A operands as contiguous 16-byte loads.  Samples with index >= s_h_start carry the H time stamp.
+// blk: F x 4 float64 scratch for the block phase, filled here from
+// *step_ctr (n = last step of the block, or one past it for a partial one).
 int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
                   const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
                   double dt, double* blk, int s_h_start, double* bins, double* lo,
@@ -192,15 +209,24 @@ int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_val
   MGH_REQUIRE(B >= 1 && B <= DFT_BMAX && nb_valid >= 0 && nb_valid <= B, "bad block length");
   MGH_REQUIRE(X && Pc && Ps && omega && blk && bins && step_ctr, "null pointer");
   MGH_REQUIRE(s_h_start % 16 == 0, "s_h_start must be a multiple of 16");
+  MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_dft_phase, dim3((F + 255) / 256), dim3(256), 0, st, omega, F, dt, step_ctr,
                      n_start, B, blk);
   MGH_CHECK_LAUNCH("k_dft_phase");
   const int waves = (S + 15) / 16;
   dim3 grd((waves + 3) / 4, nsets, 1);
-  hipLaunchKernelGGL(k_dft_block, grd, dim3(256), 0, st, X, S, S_pad, nb_valid, Pc, Ps, F, F_pad,
-                     (const double*)blk, s_h_start, bins, lo);
+  const int bp = dft_bpad(B);
+#define MGH_DFT_LAUNCH(NB)                                                                       \
+  hipLaunchKernelGGL(k_dft_block<NB>, grd, dim3(256), 0, st, X, S, S_pad, B, nb_valid, Pc, Ps, F, \
+                     (const double*)blk, s_h_start, bins, lo)
+  if (bp == 64) MGH_DFT_LAUNCH(16);
+  else if (bp == 128) MGH_DFT_LAUNCH(32);
+  else if (bp == 192) MGH_DFT_LAUNCH(48);
+  else MGH_DFT_LAUNCH(64);
+#undef MGH_DFT_LAUNCH
   MGH_CHECK_LAUNCH("k_dft_block");
+  (void)F_pad;
   return MGH_OK;
 }
 

@@ -30,7 +30,8 @@ import torch
 from . import _lib, layout
 
 NT = 8
-BMAX = 64
+BMAX = 256  # longest block (steps per MFMA contraction)
+BLOCK = 64  # default block
 TANGENTIALS = {0: ("Ey", "Ez", "Hy", "Hz"), 1: ("Ex", "Ez", "Hx", "Hz"), 2: ("Ex", "Ey", "Hx", "Hy")}
 COMPS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
@@ -43,6 +44,23 @@ def _dual(d: np.ndarray) -> np.ndarray:
     if n > 1:
         out[1:n] = 0.5 * (d[:-1] + d[1:])
     return out
+
+
+def natural(vals: np.ndarray, shape) -> np.ndarray:
+    """Samples of one group (last axis, stored with the first grid axis
+    fastest) -> (..., *shape) in natural axis order."""
+    lead = vals.shape[:-1]
+    nd = len(shape)
+    arr = vals.reshape(*lead, *shape[::-1])
+    return arr.transpose(*range(len(lead)), *range(len(lead) + nd - 1, len(lead) - 1, -1))
+
+
+def lane_order(P: np.ndarray) -> np.ndarray:
+    """(F_pad, Bp) phasor table -> the contraction kernel's lane order:
+    element [t][l][q] = P[16 t + l % 16][4 q + l // 16] (contiguous per lane)."""
+    fp, bp = P.shape
+    # (t, col, q, krow) -> (t, krow, col, q); lane l = 16 krow + col
+    return np.ascontiguousarray(P.reshape(fp // 16, 16, bp // 4, 4).transpose(0, 3, 1, 2)).ravel()
 
 
 def snap_range(cc: np.ndarray, lo: float, hi: float) -> slice:
@@ -117,15 +135,15 @@ def _pack(rows: list[list]) -> tuple[np.ndarray, np.ndarray]:
 class DFTMonitorBase:
     """Shared device machinery: sample map, block buffer, phasors, bins."""
 
-    block = BMAX
+    block = BLOCK
     compensated = True
 
-    def __init__(self, freqs, name: str = "", block: int = BMAX, compensated: bool = True):
+    def __init__(self, freqs, name: str = "", block: int = BLOCK, compensated: bool = True):
         self.freqs = np.asarray(freqs, dtype=float).ravel()
         if self.freqs.size == 0:
             raise ValueError("freqs must be a non-empty 1D array")
         if not (1 <= block <= BMAX):
-            raise ValueError("block must be 1..64")
+            raise ValueError(f"block must be 1..{BMAX}")
         self.block = int(block)
         self.compensated = compensated
         self.name = name or f"dft_{id(self):x}"
@@ -194,21 +212,22 @@ class DFTMonitorBase:
         dt = solver.dt
         self.dt = dt
         omega = 2.0 * math.pi * self.freqs
-        b = np.arange(BMAX)
+        bp = (self.block + 63) // 64 * 64
+        b = np.arange(bp)
         ph = np.outer(omega, b * dt)
-        Pc = np.zeros((self.F_pad, BMAX))
-        Ps = np.zeros((self.F_pad, BMAX))
+        Pc = np.zeros((self.F_pad, bp))
+        Ps = np.zeros((self.F_pad, bp))
         valid = b < self.block
         Pc[: self.F, valid] = (np.cos(ph) * dt)[:, valid]
         Ps[: self.F, valid] = (np.sin(ph) * dt)[:, valid]
         f32 = torch.float32
         self.idx_d = torch.from_numpy(dev_idx.astype(np.int32)).to(dev)
         self.w_d = torch.from_numpy(w.astype(np.float32)).to(dev)
-        self.Pc = torch.from_numpy(Pc).to(dev, f32)
-        self.Ps = torch.from_numpy(Ps).to(dev, f32)
+        self.Pc = torch.from_numpy(lane_order(Pc)).to(dev, f32)
+        self.Ps = torch.from_numpy(lane_order(Ps)).to(dev, f32)
         self.omega = torch.from_numpy(omega).to(dev, torch.float64)
         self.blk = torch.zeros((self.F, 4), dtype=torch.float64, device=dev)
-        self.X = torch.zeros((K, BMAX, self.S_pad), dtype=f32, device=dev)
+        self.X = torch.zeros((K, self.block, self.S_pad), dtype=f32, device=dev)
         self.bins = torch.zeros((K, self.F, self.S, 2), dtype=torch.float64, device=dev)
         self.lo = torch.zeros_like(self.bins) if self.compensated else None
         self.nsets = K
@@ -255,7 +274,7 @@ class DFTMonitorBase:
         z = b[..., 0] + 1j * b[..., 1]
         out = {}
         for comp, shape, start, n in self._layout:
-            out.setdefault(comp, []).append(z[:, start:start + n].reshape(self.F, *shape))
+            out.setdefault(comp, []).append(natural(z[:, start:start + n], shape))
         return {k: (v[0] if len(v) == 1 else v) for k, v in out.items()}
 
     def state_dict(self) -> dict:
@@ -278,7 +297,7 @@ class MonitorFieldFrequency(DFTMonitorBase):
     """DFT of cell-centred fields in a box region (field_frequency.py)."""
 
     def __init__(self, freqs, corners=None, fields=("E", "H"), name: str = "", region=None,
-                 block: int = BMAX, compensated: bool = True):
+                 block: int = BLOCK, compensated: bool = True):
         super().__init__(freqs, name, block, compensated)
         self.corners = corners
         self.region = region  # optional explicit (slice, slice, slice) of cells
@@ -306,6 +325,9 @@ class MonitorFieldFrequency(DFTMonitorBase):
         I, J, K = np.meshgrid(np.arange(sl[0].start, sl[0].stop), np.arange(sl[1].start, sl[1].stop),
                               np.arange(sl[2].start, sl[2].stop), indexing="ij")
         shape = I.shape
+        # samples numbered x fastest, like the device field layout, so the
+        # per-step gathers of neighbouring samples coalesce
+        I, J, K = (a.transpose(2, 1, 0) for a in (I, J, K))
         self._groups = [(c, shape, sb.centred(c, I, J, K)) for c in self.components]
 
 
@@ -318,7 +340,7 @@ class MonitorFarField(DFTMonitorBase):
     of the two cell layers around the box node plane.
     """
 
-    def __init__(self, freqs, margin_cells: int = 3, name: str = "", block: int = BMAX,
+    def __init__(self, freqs, margin_cells: int = 3, name: str = "", block: int = BLOCK,
                  compensated: bool = True):
         super().__init__(freqs, name, block, compensated)
         if margin_cells < 1:
@@ -358,6 +380,7 @@ class MonitorFarField(DFTMonitorBase):
             r2 = np.arange(lo_n[t2], hi_n[t2])
             A, Bm = np.meshgrid(r1, r2, indexing="ij")
             shape = A.shape
+            A, Bm = A.T, Bm.T  # samples numbered along the lower axis fastest
             self.faces.append((face, axis, nn, wgt, shape))
             for comp in TANGENTIALS[axis]:
                 terms = []

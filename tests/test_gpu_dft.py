@@ -18,7 +18,7 @@ import torch
 
 import cases
 from magnelio_amd import _lib
-from magnelio_amd.monitors import MonitorFieldFrequency
+from magnelio_amd.monitors import MonitorFieldFrequency, lane_order
 from oracle import monitors as omon
 
 pytestmark = pytest.mark.gpu
@@ -28,39 +28,41 @@ REGION = (slice(10, 30), slice(4, 20), slice(30, 60))
 N_STEPS = 300
 
 
-def test_block_contraction_matches_numpy(gpu):
+@pytest.mark.parametrize("B", [64, 100, 256])
+def test_block_contraction_matches_numpy(B, gpu):
     rng = np.random.default_rng(7)
     K, S, s_h, F = 2, 70, 48, 37
     S_pad, F_pad = 80, 48
     dt = 1.3e-12
     omega = 2 * np.pi * rng.uniform(1e9, 30e9, F)
-    b = np.arange(64)
-    Pc = np.zeros((F_pad, 64), np.float32)
-    Ps = np.zeros((F_pad, 64), np.float32)
-    Pc[:F] = np.cos(np.outer(omega, b * dt)) * dt
-    Ps[:F] = np.sin(np.outer(omega, b * dt)) * dt
-    X = rng.standard_normal((K, 64, S_pad)).astype(np.float32)
+    bp = (B + 63) // 64 * 64
+    b = np.arange(bp)
+    Pc = np.zeros((F_pad, bp), np.float32)
+    Ps = np.zeros((F_pad, bp), np.float32)
+    Pc[:F, :B] = (np.cos(np.outer(omega, b * dt)) * dt)[:, :B]
+    Ps[:F, :B] = (np.sin(np.outer(omega, b * dt)) * dt)[:, :B]
+    X = rng.standard_normal((K, B, S_pad)).astype(np.float32)
     n_start = 5
-    ctr_val = n_start + 3 * 64 + 40
-    n0 = n_start + 3 * 64
+    ctr_val = n_start + 3 * B + 40
+    n0 = n_start + 3 * B
     dev = torch.device("cuda:0")
     Xd = torch.from_numpy(X).to(dev)
-    Pcd, Psd = torch.from_numpy(Pc).to(dev), torch.from_numpy(Ps).to(dev)
+    Pcd, Psd = torch.from_numpy(lane_order(Pc)).to(dev), torch.from_numpy(lane_order(Ps)).to(dev)
     om = torch.from_numpy(omega).to(dev)
     blk = torch.zeros((F, 4), dtype=torch.float64, device=dev)
     ctr = torch.tensor([ctr_val], dtype=torch.int64, device=dev)
     t0 = np.where(np.arange(S) >= s_h, n0 * dt + 0.5 * dt, n0 * dt)
     rot = np.exp(1j * omega[:, None] * t0[None, :])  # (F, S)
     P = (Pc[:F].astype(np.float64) + 1j * Ps[:F].astype(np.float64))
-    for nb in (64, 23):
+    for nb in (B, 23):
         for comp in (False, True):
-            print(f"dft block contraction nb={nb} compensated={comp}", flush=True)
+            print(f"dft block contraction B={B} nb={nb} compensated={comp}", flush=True)
             # bins already holding values of the size of one block's sum
             base = rng.standard_normal((K, F, S, 2)) * (8 * dt)
             bins = torch.from_numpy(base.copy()).to(dev)
             lo = torch.zeros_like(bins) if comp else None
             stream = torch.cuda.current_stream(dev)
-            _lib.call("mgh_dft_block", K, Xd.data_ptr(), S, S_pad, 64, nb, Pcd.data_ptr(),
+            _lib.call("mgh_dft_block", K, Xd.data_ptr(), S, S_pad, B, nb, Pcd.data_ptr(),
                       Psd.data_ptr(), om.data_ptr(), F, F_pad, dt, blk.data_ptr(), s_h,
                       bins.data_ptr(), lo.data_ptr() if comp else None, ctr.data_ptr(), n_start,
                       int(stream.cuda_stream))
@@ -91,8 +93,8 @@ def oracle_bins():
     return _oracle_dft(N_STEPS)
 
 
-def _hip_bins(precision, use_graphs=False):
-    mon = MonitorFieldFrequency(FREQS, region=REGION)
+def _hip_bins(precision, use_graphs=False, block=64):
+    mon = MonitorFieldFrequency(FREQS, region=REGION, block=block)
     sol, _, _ = cases.c1_solver(n_steps=N_STEPS, precision=precision, use_graphs=use_graphs,
                                 monitors=[mon])
     sol.run()
@@ -114,10 +116,11 @@ def _errors(got, want):
     return out
 
 
-@pytest.mark.parametrize("precision,gate", [("double", 1e-5), ("single", 2e-5)])
-def test_region_monitor_vs_oracle(oracle_bins, precision, gate, gpu):
-    print(f"HIP C1 {precision} + region DFT, {N_STEPS} steps", flush=True)
-    got = _hip_bins(precision)
+@pytest.mark.parametrize("precision,gate,block", [("double", 1e-5, 64), ("single", 2e-5, 64),
+                                                  ("double", 1e-5, 256), ("single", 2e-5, 256)])
+def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, gpu):
+    print(f"HIP C1 {precision} + region DFT (block {block}), {N_STEPS} steps", flush=True)
+    got = _hip_bins(precision, block=block)
     assert set(got) == set(oracle_bins)
     for c in oracle_bins:
         assert got[c].shape == oracle_bins[c].shape

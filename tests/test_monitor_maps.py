@@ -10,7 +10,7 @@ import numpy as np
 
 from magnelio_amd import mesh as mm
 from magnelio_amd.boundaries import CPMLBoundary
-from magnelio_amd.monitors import MonitorFarField, MonitorFieldFrequency
+from magnelio_amd.monitors import MonitorFarField, MonitorFieldFrequency, lane_order, natural
 from oracle import monitors as omon
 from oracle import yee
 
@@ -37,7 +37,7 @@ def test_region_map_matches_oracle_interpolation():
     ref = omon.cell_centred(e, h, x, y, z, ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"], mon.slices)
     for comp, shape, start, n in mon._layout:
         scale = np.abs(ref[comp]).max()
-        assert np.allclose(vals[start:start + n].reshape(shape), ref[comp], rtol=0, atol=1e-13 * scale)
+        assert np.allclose(natural(vals[start:start + n], shape), ref[comp], rtol=0, atol=1e-13 * scale)
     assert mon.s_h_start % 16 == 0
 
 
@@ -66,8 +66,25 @@ def test_huygens_map_weights_two_layers():
                 continue
             arr = np.moveaxis(ref[comp], axis, 0)
             want = (1.0 - wgt) * arr[0] + wgt * arr[1]
-            got = vals[start:start + n].reshape(shp)
+            got = natural(vals[start:start + n], shp)
             if np.allclose(got, want, rtol=1e-12):
                 break
         else:
             raise AssertionError(f"no sample group matches face {face}")
+
+
+def test_lane_order_layout():
+    """element [t][l][q] of the lane-ordered table is P[16 t + l % 16][4 q + l // 16]."""
+    rng = np.random.default_rng(0)
+    P = rng.standard_normal((32, 128))
+    L = lane_order(P).reshape(2, 64, 32)
+    for t in range(2):
+        for lane in range(64):
+            for q in range(32):
+                assert L[t, lane, q] == P[16 * t + lane % 16, 4 * q + lane // 16]
+
+
+def test_natural_order_inverts_storage():
+    a = np.arange(2 * 3 * 4 * 5).reshape(2, 3, 4, 5)  # (F, n0, n1, n2) natural
+    stored = a.transpose(0, 3, 2, 1).reshape(2, -1)  # first axis fastest
+    assert np.array_equal(natural(stored, (3, 4, 5)), a)
