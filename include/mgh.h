@@ -137,6 +137,25 @@ int mgh_weighted_sq_sum(int dtype, long long n, int nsets, const void* x, long l
                         const void* w, double* out, int accumulate, double scale, double* work,
                         long long work_len, void* stream);
 
+/* ---- (a12) lumped RLC elements and discrete ports -----------------------
+ * Replaces LumpedElementOperator.update_e (src/magnelio/ports/_lumped/
+ * operator.py:134-166) with SeriesRLC / ParallelRLC companions
+ * (src/magnelio/circuit/companion.py:45-180), run in the port loop after the
+ * E phase (fit_td.py:975-976), and the recorder's project_V / project_I for
+ * lumped ports (recorder.py:216-250).  One work-group per (element, set):
+ *   v_total = sum sign*e (float64, chain order)
+ *   i = ((v_src - v_hist) - v_total) / (r_eq + sum beta)
+ *   e += (sign*beta)*i;  companion advance;  V = sum sign*e, I = i.
+ * desc: host array of n_elems records of 12 int64 words:
+ *   0 n_edges (<= 1024), 1 idx (int64 device-layout E indices), 2 sign (f64),
+ *   3 beta (f64), 4 params (f64[8]: parallel, has_L, has_C, r_eq, sum beta,
+ *   kL, kC, G), 5 state (f64 nsets x 5), 6 exc (int32 per set), 7 src (f64
+ *   nsets x tab_len: v_src of step n), 8 tab_len, 9 V_rec, 10 I_rec (f64
+ *   nsets x cap, or 0), 11 cap.  Edge chains of different elements must be
+ *   disjoint. */
+int mgh_lumped_step(int dtype, int nsets, void* e, long long e_set_stride, const long long* desc,
+                    int n_elems, const long long* step_ctr, void* stream);
+
 /* ---- (a14) TF/SF plane-wave corrections ---------------------------------
  * Replaces PlaneWaveSource.inject_E / inject_H / _apply
  * (src/magnelio/sources/plane_wave.py:710-734), called after the E phase and

@@ -73,6 +73,10 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_ll, _c_int, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int, _c_double,
          _c_void_p, _c_ll, _c_void_p],
     ),
+    "mgh_lumped_step": (
+        _c_int,
+        [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_int, _c_void_p, _c_void_p],
+    ),
     "mgh_tfsf_apply": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,

@@ -137,11 +137,15 @@ def power_waves(V_f, I_f, omega, dt, mode, normal_dx=None, line_params=None):
 
 def s_parameters(signals: dict, port_modes: dict, excited: tuple, f_axis: np.ndarray, dt: float,
                  *, port_normal_dx: dict | None = None, port_line_params: dict | None = None,
-                 a_threshold: float = 1e-12) -> dict:
+                 lumped_z0: dict | None = None, a_threshold: float = 1e-12) -> dict:
     """S(f) per recorded channel for one excitation (compute_s_parameters).
 
-    ``signals``: {(port, mode): (V array, I array)}; I carries the temporal
-    Yee half-step exp(+j w dt/2) correction here.
+    ``signals``: {(port, mode): (V array, I array)}; modal I carries the
+    temporal Yee half-step exp(+j w dt/2) correction here.  Ports named in
+    ``lumped_z0`` ({port: Z0}) are discrete ports: their V and I are
+    co-temporal (the Thevenin current of the same step), so the split is
+    a = (V + Z0 I) / (2 sqrt(Z0)), b = (V - Z0 I) / (2 sqrt(Z0)) without the
+    half-step factor (modal_sparameters.py, test_lumped_port.py:241-285).
     """
     f_axis = np.asarray(f_axis, dtype=float)
     if np.any(f_axis <= 0):
@@ -153,6 +157,13 @@ def s_parameters(signals: dict, port_modes: dict, excited: tuple, f_axis: np.nda
     a_ch, b_ch = {}, {}
     for key, (V, I) in signals.items():
         port, m = key
+        if lumped_z0 is not None and port in lumped_z0:
+            z0 = float(lumped_z0[port])
+            V_f = dft(V, f_axis, dt)
+            I_f = dft(I, f_axis, dt)
+            a_ch[key] = (V_f + z0 * I_f) / (2.0 * math.sqrt(z0))
+            b_ch[key] = (V_f - z0 * I_f) / (2.0 * math.sqrt(z0))
+            continue
         mode = port_modes[port][m]
         V_f = dft(V, f_axis, dt)
         I_f = dft(I, f_axis, dt) * phase

@@ -36,6 +36,7 @@ import torch
 
 from . import _lib, kernels, layout
 from . import mesh as _mesh
+from .lumped import DeviceLumped
 from .boundaries import CPMLBoundary, DeviceCPML, PECBoundary, PMCBoundary
 from .ports import PortModel
 
@@ -299,8 +300,7 @@ class FITTimeDomainSolver:
             src.attach(self)
         for mon in self.monitors:
             mon.attach(self)
-        for el in self.lumped:
-            el.attach(self)
+        self.dlumped = DeviceLumped(self.lumped, self) if self.lumped else None
         self._sqsum = kernels.SqSum(K, dev)
         self._energy = torch.zeros(K, dtype=torch.float64, device=dev)
         self._setup_done = True
@@ -330,8 +330,8 @@ class FITTimeDomainSolver:
             code = 4 if self._t_dtype == torch.float32 else 8
             _lib.call("mgh_port_step", code, K, self.e.data_ptr(), self.e.shape[1], self.h.data_ptr(),
                       self.h.shape[1], self.dports.desc_ptr, len(self.ports), self.step_ctr.data_ptr(), sh)
-        for el in self.lumped:
-            el.step(self, stream)
+        if self.dlumped is not None:
+            self.dlumped.step(self, stream)
         kernels.update_h(self.e, self.h, self.alpha_h, self.beta_h, dims, kchunk=self.kchunk,
                          stream=stream)
         if self.cpml is not None:
@@ -455,14 +455,18 @@ class FITTimeDomainSolver:
         return e, h
 
     def recorded(self, s: int = 0, n: int | None = None) -> dict:
-        """{(port, mode): (V, I)} float64 time series of field set s."""
+        """{(port, mode): (V, I)} float64 time series of field set s
+        (modal ports, then lumped ports)."""
         n = self._resume_step if n is None else n
         out = {}
-        for P, V, I in zip(self.ports, self.dports.V_rec, self.dports.I_rec):
-            v = V[s, :, :n].cpu().numpy()
-            i = I[s, :, :n].cpu().numpy()
-            for m in range(P.n_modes):
-                out[(P.name, m)] = (v[m].copy(), i[m].copy())
+        if self.dports is not None:
+            for P, V, I in zip(self.ports, self.dports.V_rec, self.dports.I_rec):
+                v = V[s, :, :n].cpu().numpy()
+                i = I[s, :, :n].cpu().numpy()
+                for m in range(P.n_modes):
+                    out[(P.name, m)] = (v[m].copy(), i[m].copy())
+        if self.dlumped is not None:
+            out.update(self.dlumped.recorded(s, n))
         return out
 
     def state_dict(self) -> dict:
@@ -489,9 +493,8 @@ class FITTimeDomainSolver:
         for i, mon in enumerate(self.monitors):
             if hasattr(mon, "state_dict"):
                 sd[f"monitor{i}"] = mon.state_dict()
-        for i, el in enumerate(self.lumped):
-            if hasattr(el, "state_dict"):
-                sd[f"lumped{i}"] = el.state_dict()
+        if self.dlumped is not None:
+            sd["lumped"] = self.dlumped.state_dict()
         return sd
 
     def load_state_dict(self, sd: dict) -> None:
@@ -518,9 +521,8 @@ class FITTimeDomainSolver:
         for i, mon in enumerate(self.monitors):
             if hasattr(mon, "load_state_dict"):
                 mon.load_state_dict(sd[f"monitor{i}"])
-        for i, el in enumerate(self.lumped):
-            if hasattr(el, "load_state_dict"):
-                el.load_state_dict(sd[f"lumped{i}"])
+        if self.dlumped is not None:
+            self.dlumped.load_state_dict(sd["lumped"])
         self.step_ctr.fill_(n)
         self._resume_step = n
         self._peak_energy = float(sd.get("peak_energy", 0.0))

@@ -7,6 +7,7 @@ covers, with t = n dt:
            boundary order                                    :826-889
   sources  TF/SF E injection at t + dt                       :965-967
   ports    update_e(fields, t + dt) in port order            :975-976
+           (modal ports, then lumped ports / elements)
   record   V/I of every port (e at t^{n+1}, h at t^{n+1/2})  :981-982
   H phase  curl update, CPML H corrections                   :891-927
   sources  TF/SF H injection at t + 3dt/2                    :991-993
@@ -59,6 +60,9 @@ class Oracle:
             el.update_e(self.e, t + dt)
         for p in self.ports:
             p.record(self.e, self.h)
+        for el in self.lumped:
+            if getattr(el, "recorded", False):
+                el.record(self.e, self.h)
         yee.update_h(self.e, self.h, self.ah, self.bh, *dims)
         for f in self.cpml:
             f.update_h(self.e, self.h, self.bh)
