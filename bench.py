@@ -116,6 +116,7 @@ def main(argv=None) -> int:
     ap.add_argument("--k", type=int, default=1, help="field sets advanced per launch")
     ap.add_argument("--freqs", type=int, default=1001, help="DFT monitor frequencies (0: none)")
     ap.add_argument("--dft-block", type=int, default=256, help="steps per DFT contraction")
+    ap.add_argument("--dft-plain", action="store_true", help="uncompensated float64 bins")
     ap.add_argument("--no-tfsf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args(argv)
@@ -147,7 +148,8 @@ def main(argv=None) -> int:
     if args.freqs > 0:
         # Huygens box 10 cells in from every face (8 CPML layers + 2)
         monitors.append(MonitorFarField(np.linspace(1e9, 20e9, args.freqs), margin_cells=2,
-                                        block=args.dft_block, name="far_field"))
+                                        block=args.dft_block, name="far_field",
+                                        compensated=not args.dft_plain))
     sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, total_time_steps=total, dt=dt,
                               precision="single", nsets=args.k, record=False, sources=sources,
                               monitors=monitors)
@@ -202,7 +204,8 @@ def main(argv=None) -> int:
                             ", TF/SF +z x-polarised Gaussian (f_max 20 GHz) on [0.15L, 0.85L]^3")
                          + ("" if not monitors else
                             f", far-field Huygens-box DFT {args.freqs} freqs 1-20 GHz, 10 cells in, "
-                            f"{monitors[0].S} samples/step, block {args.dft_block}, compensated FP64 bins")
+                            f"{monitors[0].S} samples/step, block {args.dft_block}, "
+                            + ("plain" if args.dft_plain else "compensated") + " FP64 bins")
                          + "; all inside the timed steps"),
             "K": K,
             "dft_bins_gb": round(monitors[0].bytes() / 1e9, 2) if monitors else 0.0,

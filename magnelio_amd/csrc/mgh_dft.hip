@@ -92,7 +92,16 @@ __device__ __forceinline__ void two_sum_acc(double* hi, double* lo, double v) {
 }
 
 // NB: MFMA k-steps per block (B padded to a multiple of 64, divided by 4).
-template <int NB>
+// COMP: TwoSum-compensated bins.
+//
+// The four waves of a work-group take 64 consecutive samples and share each
+// frequency tile's phasor operands: the work-group copies the tile (cos and
+// sin, NB x 64 lanes) from L2 into LDS once, and every wave reads its lanes'
+// A operands from there (16-byte reads, lane-contiguous, no bank
+// conflicts) — a quarter of the L2 traffic of per-wave loads.  The bins of
+// the tile are loaded before the MFMA chain so their HBM latency hides
+// behind the matrix work, and are read and written as complex pairs.
+template <int NB, bool COMP>
 __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, int S, int S_pad,
                                                    int B, int nb_valid,
                                                    const float* __restrict__ Pc,
@@ -100,11 +109,14 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
                                                    const double* __restrict__ blk, int s_h_start,
                                                    double* __restrict__ bins,
                                                    double* __restrict__ lo) {
+  constexpr int NQ = NB / 4;  // float4 groups per lane
+  __shared__ float4 sc[NQ * 64];
+  __shared__ float4 ss[NQ * 64];
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int set = blockIdx.y;
   const int s0 = wave * 16;
-  if (s0 >= S) return;
+  const bool active = s0 < S;  // idle waves still take part in the tile copies
   const int col = lane & 15;   // sample within the tile (B operand column / D column)
   const int krow = lane >> 4;  // 0..3: k within an MFMA step (A/B), row group of D
   const float* Xs = X + (int64_t)set * B * S_pad;
@@ -112,51 +124,70 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
 #pragma unroll
   for (int kk = 0; kk < NB; ++kk) {
     const int b = 4 * kk + krow;
-    xb[kk] = b < nb_valid ? Xs[(int64_t)b * S_pad + s0 + col] : 0.0f;
+    xb[kk] = (active && b < nb_valid) ? Xs[(int64_t)b * S_pad + s0 + col] : 0.0f;
   }
   const int ph = (s0 >= s_h_start) ? 2 : 0;
   const int s = s0 + col;
-  double* bset = bins + (int64_t)set * F * S * 2;
-  double* lset = lo ? lo + (int64_t)set * F * S * 2 : nullptr;
+  const bool own = active && s < S;
+  double2* bset = (double2*)(bins + (int64_t)set * F * S * 2);
+  double2* lset = COMP ? (double2*)(lo + (int64_t)set * F * S * 2) : nullptr;
   for (int f0 = 0; f0 < F; f0 += 16) {
+    double2 bv[4], lv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int f = f0 + 4 * krow + r;
+      if (own && f < F) {
+        const int64_t o = (int64_t)f * S + s;
+        bv[r] = bset[o];
+        if (COMP) lv[r] = lset[o];
+      }
+    }
+    // this tile's phasor operands, [q][lane] float4s, into LDS
+    const float4* gc = (const float4*)Pc + (int64_t)(f0 >> 4) * NQ * 64;
+    const float4* gs = (const float4*)Ps + (int64_t)(f0 >> 4) * NQ * 64;
+    __syncthreads();
+    for (int g = threadIdx.x; g < NQ * 64; g += 256) {
+      sc[g] = gc[g];
+      ss[g] = gs[g];
+    }
+    __syncthreads();
     v4f cc = {0.f, 0.f, 0.f, 0.f};
     v4f cs = {0.f, 0.f, 0.f, 0.f};
-    // this lane's A operands of the tile: NB contiguous words (lane order)
-    const float4* pc = (const float4*)(Pc + ((int64_t)(f0 >> 4) * 64 + lane) * NB);
-    const float4* ps = (const float4*)(Ps + ((int64_t)(f0 >> 4) * 64 + lane) * NB);
+    if (active) {
 #pragma unroll
-    for (int q = 0; q < NB / 4; ++q) {
-      const float4 ac = pc[q];
-      const float4 as = ps[q];
-      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.x, xb[4 * q + 0], cc, 0, 0, 0);
-      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, xb[4 * q + 0], cs, 0, 0, 0);
-      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.y, xb[4 * q + 1], cc, 0, 0, 0);
-      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, xb[4 * q + 1], cs, 0, 0, 0);
-      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.z, xb[4 * q + 2], cc, 0, 0, 0);
-      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, xb[4 * q + 2], cs, 0, 0, 0);
-      cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.w, xb[4 * q + 3], cc, 0, 0, 0);
-      cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, xb[4 * q + 3], cs, 0, 0, 0);
+      for (int q = 0; q < NQ; ++q) {
+        const float4 ac = sc[q * 64 + lane];
+        const float4 as = ss[q * 64 + lane];
+        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.x, xb[4 * q + 0], cc, 0, 0, 0);
+        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, xb[4 * q + 0], cs, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.y, xb[4 * q + 1], cc, 0, 0, 0);
+        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, xb[4 * q + 1], cs, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.z, xb[4 * q + 2], cc, 0, 0, 0);
+        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, xb[4 * q + 2], cs, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.w, xb[4 * q + 3], cc, 0, 0, 0);
+        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, xb[4 * q + 3], cs, 0, 0, 0);
+      }
     }
-    if (s < S) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int f = f0 + 4 * krow + r;
-        if (f < F) {
-          const double pre = (double)cc[r];
-          const double pim = (double)cs[r];
-          const double c = blk[f * 4 + ph];
-          const double sn = blk[f * 4 + ph + 1];
-          const double dre = c * pre - sn * pim;
-          const double dim = sn * pre + c * pim;
-          const int64_t o = ((int64_t)f * S + s) * 2;
-          if (lset) {
-            two_sum_acc(bset + o, lset + o, dre);
-            two_sum_acc(bset + o + 1, lset + o + 1, dim);
-          } else {
-            bset[o] += dre;
-            bset[o + 1] += dim;
-          }
+    for (int r = 0; r < 4; ++r) {
+      const int f = f0 + 4 * krow + r;
+      if (own && f < F) {
+        const double pre = (double)cc[r];
+        const double pim = (double)cs[r];
+        const double c = blk[f * 4 + ph];
+        const double sn = blk[f * 4 + ph + 1];
+        const double dre = c * pre - sn * pim;
+        const double dim = sn * pre + c * pim;
+        const int64_t o = (int64_t)f * S + s;
+        if (COMP) {
+          two_sum_acc(&bv[r].x, &lv[r].x, dre);
+          two_sum_acc(&bv[r].y, &lv[r].y, dim);
+          lset[o] = lv[r];
+        } else {
+          bv[r].x += dre;
+          bv[r].y += dim;
         }
+        bset[o] = bv[r];
       }
     }
   }
@@ -194,9 +225,10 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
 // float64 bins (nsets x F x S complex, interleaved re/im); lo (same shape or
 // NULL) holds the compensation terms.  Pc/Ps: the float32 phasor tables
 // P[f][b] = cos/sin(w_f b dt) dt (f < F_pad, b < Bp = B rounded up to a
-// multiple of 64, zero beyond B and F) stored in MFMA lane order:
-// element [t][l][q] (t < F_pad/16, l < 64, q < Bp/4) = P[16t + l%16][4q + l/16],
-// so each lane streams its A operands as contiguous 16-byte loads.  Samples with index >= s_h_start carry the H time stamp.
+// multiple of 64, zero beyond B and F) stored in MFMA lane order, in float4
+// groups: element [t][q][l][c] (t < F_pad/16, q < Bp/16, l < 64, c < 4) =
+// P[16t + l%16][4(4q + c) + l/16] — a tile is a lane-contiguous block that
+// a work-group copies straight into LDS.  Samples with index >= s_h_start carry the H time stamp.
 // blk: F x 4 float64 scratch for the block phase, filled here from
 // *step_ctr (n = last step of the block, or one past it for a partial one).
 int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
@@ -217,9 +249,15 @@ int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_val
   const int waves = (S + 15) / 16;
   dim3 grd((waves + 3) / 4, nsets, 1);
   const int bp = dft_bpad(B);
-#define MGH_DFT_LAUNCH(NB)                                                                       \
-  hipLaunchKernelGGL(k_dft_block<NB>, grd, dim3(256), 0, st, X, S, S_pad, B, nb_valid, Pc, Ps, F, \
-                     (const double*)blk, s_h_start, bins, lo)
+#define MGH_DFT_LAUNCH(NB)                                                                  \
+  do {                                                                                      \
+    if (lo)                                                                                 \
+      hipLaunchKernelGGL((k_dft_block<NB, true>), grd, dim3(256), 0, st, X, S, S_pad, B,    \
+                         nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, lo);     \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_dft_block<NB, false>), grd, dim3(256), 0, st, X, S, S_pad, B,   \
+                         nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, lo);     \
+  } while (0)
   if (bp == 64) MGH_DFT_LAUNCH(16);
   else if (bp == 128) MGH_DFT_LAUNCH(32);
   else if (bp == 192) MGH_DFT_LAUNCH(48);

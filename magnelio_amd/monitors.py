@@ -56,11 +56,11 @@ def natural(vals: np.ndarray, shape) -> np.ndarray:
 
 
 def lane_order(P: np.ndarray) -> np.ndarray:
-    """(F_pad, Bp) phasor table -> the contraction kernel's lane order:
-    element [t][l][q] = P[16 t + l % 16][4 q + l // 16] (contiguous per lane)."""
+    """(F_pad, Bp) phasor table -> the contraction kernel's lane order, float4
+    groups: element [t][q][l][c] = P[16 t + l % 16][4 (4 q + c) + l // 16]."""
     fp, bp = P.shape
-    # (t, col, q, krow) -> (t, krow, col, q); lane l = 16 krow + col
-    return np.ascontiguousarray(P.reshape(fp // 16, 16, bp // 4, 4).transpose(0, 3, 1, 2)).ravel()
+    # P[16 t + col][16 q + 4 c + krow] -> (t, q, krow, col, c); lane l = 16 krow + col
+    return np.ascontiguousarray(P.reshape(fp // 16, 16, bp // 16, 4, 4).transpose(0, 2, 4, 1, 3)).ravel()
 
 
 def snap_range(cc: np.ndarray, lo: float, hi: float) -> slice:

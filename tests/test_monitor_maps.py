@@ -74,14 +74,15 @@ def test_huygens_map_weights_two_layers():
 
 
 def test_lane_order_layout():
-    """element [t][l][q] of the lane-ordered table is P[16 t + l % 16][4 q + l // 16]."""
+    """element [t][q][l][c] of the lane-ordered table is P[16 t + l % 16][4 (4 q + c) + l // 16]."""
     rng = np.random.default_rng(0)
     P = rng.standard_normal((32, 128))
-    L = lane_order(P).reshape(2, 64, 32)
+    L = lane_order(P).reshape(2, 8, 64, 4)
     for t in range(2):
-        for lane in range(64):
-            for q in range(32):
-                assert L[t, lane, q] == P[16 * t + lane % 16, 4 * q + lane // 16]
+        for q in range(8):
+            for lane in range(64):
+                for c in range(4):
+                    assert L[t, q, lane, c] == P[16 * t + lane % 16, 4 * (4 * q + c) + lane // 16]
 
 
 def test_natural_order_inverts_storage():
