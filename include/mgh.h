@@ -171,6 +171,30 @@ int mgh_tfsf_apply(int dtype, int nsets, void* field, long long set_stride, int 
                    const void* wave, int n_delays, long long tab_len, const long long* step_ctr,
                    long long step_offset, void* stream);
 
+/* ---- (a15) ADE pole-residue dielectrics, E side -------------------------
+ * Replace DispersionOperator.save_field / update_field
+ * (src/magnelio/solver/_dispersion.py:504-569, NumPy branch), called at the
+ * top of the step and right after the E curl (fit_td.py:833-834, :858-859);
+ * the coefficient W is folded into alpha_E / beta_E on the host
+ * (fit_td.py:382-403).  n dispersive E edges idx (int64, device layout,
+ * disjoint), g = EPS0 A_dual/L and beta_E at those edges (field dtype), the
+ * edge's block (material) blk (int32).  fprev: nsets x n.
+ *   save:   fprev = e[idx]
+ *   update: j = sum_q hw_q Re((1 + k_q) J_q) (first term seeds);
+ *           e[idx] -= beta j;  gd = g (e[idx] - fprev);
+ *           J_q = J_q k_q;  J_q += c_q gd
+ * complex a*b rounded as NumPy's SIMD loop on FMA hosts:
+ *   (fma(a.re, b.re, -(a.im b.im)), fma(a.re, b.im, a.im b.re)).
+ * tab: n_blocks x pmax x 8 field-dtype values per pole (kind 0 real /
+ * 1 conjugate pair, hw = weight/2, 1+k, k, c as (re, im) pairs); npoles per
+ * block (int32); J: nsets x pmax x n complex states, (re, im) interleaved. */
+int mgh_ade_save(int dtype, int nsets, const void* e, long long e_set_stride, int n,
+                 const long long* idx, void* fprev, void* stream);
+int mgh_ade_update(int dtype, int nsets, void* e, long long e_set_stride, int n,
+                   const long long* idx, const void* g, const void* beta, const void* fprev,
+                   const int* blk, const int* npoles, const void* tab, int pmax, void* J,
+                   void* stream);
+
 /* ---- (a16) DFT monitors ----------------------------------------------------
  * Replace DFTAccumulator.accumulate (src/magnelio/monitors/_dft.py:37-52) fed
  * with cell-centred samples (src/magnelio/monitors/base.py:51-173) from the

@@ -9,11 +9,14 @@ dt, energy_stop_db, port_signal_stop_db, precision) and the same
 reading the shared material coefficients once for all K.
 
 Step order (fit_td.py:950-1007), t = n dt:
-  E phase    curl update + fused CPML (+ ADE)          mgh_update_e_pml
+  E phase    ADE field stash                           mgh_ade_save
+             curl update                               mgh_update_e
+             ADE history subtraction + pole advance    mgh_ade_update
+             CPML faces                                mgh_cpml_e
   sources    TF/SF E injection at t + dt                mgh_tfsf_apply
   ports      modal DTBC/Mur step, write-back, V/I record mgh_port_step
              lumped RLC elements                       mgh_lumped_step
-  H phase    curl update + fused CPML                  mgh_update_h_pml
+  H phase    curl update, CPML faces                   mgh_update_h, mgh_cpml_h
   sources    TF/SF H injection at t + 3dt/2
   monitors   DFT accumulation                          mgh_dft_*
   counter    n <- n + 1 on the device                  mgh_step_advance
@@ -36,6 +39,7 @@ import torch
 
 from . import _lib, kernels, layout
 from . import mesh as _mesh
+from .dispersion import DeviceDispersion
 from .lumped import DeviceLumped
 from .boundaries import CPMLBoundary, DeviceCPML, PECBoundary, PMCBoundary
 from .ports import PortModel
@@ -243,6 +247,7 @@ class FITTimeDomainSolver:
         m_mu = _mesh.build_m_mu(mesh)
         m_sig = _mesh.build_m_sigma(mesh)
         pec = mesh.pec_mask_edges
+        pec_raw = _mesh.pec_flat(pec, dims)
         port_faces = []
         for P in self.ports:
             port_faces.append(P.face)
@@ -258,9 +263,9 @@ class FITTimeDomainSolver:
         for src in self.sources:
             if hasattr(src, "prepare"):
                 src.prepare(self)
-        disp_builder = getattr(self, "_dispersion_builder", None)
-        if disp_builder is not None:
-            self._dispersion = disp_builder(mesh, dt, m_eps)
+        # ADE: blocks over the unflattened PEC mask (_dispersion.py:296-306)
+        self._dispersion = DeviceDispersion.from_mesh(mesh, dt, pec_raw)
+        if self._dispersion is not None:
             w_e = self._dispersion.w_half
         co = _mesh.update_coefficients(m_eps, m_mu, m_sig, _mesh.build_m_sigma_m(mesh), dt, pec_e, w_e, w_h)
         self._coeffs = co
@@ -276,6 +281,8 @@ class FITTimeDomainSolver:
         K = self.nsets
         self.e = torch.zeros((K, ne), dtype=t_dt, device=dev)
         self.h = torch.zeros((K, nh), dtype=t_dt, device=dev)
+        if self._dispersion is not None:
+            self._dispersion.bind(self)
 
         cpml_faces = []
         for face, bc in self.boundary_conditions.items():
@@ -312,6 +319,8 @@ class FITTimeDomainSolver:
         stream = torch.cuda.current_stream(self._dev)
         sh = int(stream.cuda_stream)
         ev = self.timing_events.pop() if self.timing_events else None
+        if self._dispersion is not None:
+            self._dispersion.save(self, stream)
         if ev is not None:
             ev[0].record(stream)
         kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
@@ -495,6 +504,8 @@ class FITTimeDomainSolver:
                 sd[f"monitor{i}"] = mon.state_dict()
         if self.dlumped is not None:
             sd["lumped"] = self.dlumped.state_dict()
+        if self._dispersion is not None:
+            sd["dispersion"] = self._dispersion.state_dict()
         return sd
 
     def load_state_dict(self, sd: dict) -> None:
@@ -523,6 +534,8 @@ class FITTimeDomainSolver:
                 mon.load_state_dict(sd[f"monitor{i}"])
         if self.dlumped is not None:
             self.dlumped.load_state_dict(sd["lumped"])
+        if self._dispersion is not None:
+            self._dispersion.load_state_dict(sd["dispersion"])
         self.step_ctr.fill_(n)
         self._resume_step = n
         self._peak_energy = float(sd.get("peak_energy", 0.0))
