@@ -19,7 +19,8 @@
 //   22 mur_tab                           float64 (nsets, tab_len, 4) or 0
 //   23 V_rec, 24 I_rec, 25 rec_cap       float64 (nsets, M, rec_cap) or 0
 //   26 absmax                            float64 (nsets) |V| envelope or 0
-//   27, 28 spare
+//   27 work, 28 work_len                 float64 scratch of the split step, at least
+//                                         mgh::port_work_len(n, M, cap, nsets) words
 #pragma once
 
 #include "mgh_common.h"
@@ -27,6 +28,10 @@
 #define MGH_PORT_WORDS 29
 #define MGH_PORT_MAX_PORTS 8
 #define MGH_PORT_MAX_MODES 8
+// points of a port plane (and DTBC history terms) per work-group chunk
+#define MGH_PORT_CHUNK 1024
+// partial words per (set, chunk): 2 per mode (u/v part) then 2 per mode at +16
+#define MGH_PORT_PQ (4 * MGH_PORT_MAX_MODES)
 
 namespace mgh {
 
@@ -55,7 +60,24 @@ struct PortDev {
   double* I_rec;
   long long rec_cap;
   double* absmax;
+  double* work;
+  long long work_len;
 };
+
+// chunks of the projection pass: the plane and the history, whichever is longer
+__host__ __device__ inline int port_chunks(int n_pts, long long cap) {
+  const long long a = (n_pts + MGH_PORT_CHUNK - 1) / MGH_PORT_CHUNK;
+  const long long b = (cap + MGH_PORT_CHUNK - 1) / MGH_PORT_CHUNK;
+  const long long c = a > b ? a : b;
+  return (int)(c > 0 ? c : 1);
+}
+__host__ __device__ inline int port_plane_chunks(int n_pts) {
+  return (n_pts + MGH_PORT_CHUNK - 1) / MGH_PORT_CHUNK;
+}
+// scratch words: projection partials, record partials, corrected voltages
+__host__ __device__ inline long long port_work_len(int n_pts, long long cap, int nsets) {
+  return (long long)nsets * (2LL * port_chunks(n_pts, cap) * MGH_PORT_PQ + MGH_PORT_MAX_MODES);
+}
 
 struct PortSet {
   int n;
@@ -63,7 +85,7 @@ struct PortSet {
   PortDev p[MGH_PORT_MAX_PORTS];
 };
 
-inline int load_port(const long long* w, PortDev* P) {
+inline int load_port(const long long* w, PortDev* P, int nsets) {
   P->n_u = (int)w[0];
   P->n_v = (int)w[1];
   P->M = (int)w[2];
@@ -94,12 +116,17 @@ inline int load_port(const long long* w, PortDev* P) {
   P->I_rec = (double*)w[24];
   P->rec_cap = w[25];
   P->absmax = (double*)w[26];
+  P->work = (double*)w[27];
+  P->work_len = w[28];
   MGH_REQUIRE(P->idx_port && P->idx_int && P->idx_h && P->me_port && P->me_int && P->mh &&
                   P->prof_e && P->prof_h && P->params && P->state,
               "port: null array pointer");
   MGH_REQUIRE(P->k_init >= 2 || P->kern == nullptr, "port: bad kernel length");
   MGH_REQUIRE(P->cap >= 0 && P->tab_len >= 0 && P->rec_cap >= 0, "port: negative length");
   MGH_REQUIRE(!P->V_rec || P->I_rec, "port: V_rec without I_rec");
+  MGH_REQUIRE(P->work && P->work_len >= port_work_len(P->n_u + P->n_v, P->cap, nsets),
+              "port: work area of %lld words, %lld needed", P->work_len,
+              port_work_len(P->n_u + P->n_v, P->cap, nsets));
   return MGH_OK;
 }
 

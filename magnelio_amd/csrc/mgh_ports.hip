@@ -1,6 +1,14 @@
 // Modal waveguide port step on the device: interior projection, exact DTBC
 // recursion (or modal Mur-1), modal write-back of the port plane and the V/I
-// recording — one work-group per (port, field set), float64 throughout.
+// recording, float64 throughout.  Four launches per step, so that a large
+// port plane spreads over many work-groups:
+//   k_port_project  (chunk, port, set)  interior-plane projections and DTBC
+//                   history convolutions, chunk partials to the work area
+//   k_port_advance  (port, set)         fixed-order sum of the partials, the
+//                   scalar termination update, corrected modal voltages
+//   k_port_write    (chunk, port, set)  modal write-back of the plane and the
+//                   V/I recording partials
+//   k_port_record   (port, set)         V/I of the step
 //
 // Reference (src/magnelio/ports/_modal/operator.py):
 //   projections        _V_from_samples / _I_from_samples :1277-1321
@@ -15,176 +23,245 @@
 //                      from the written plane, I at t^{n+1/2}, times
 //                      record_scale)
 // The step index n is read from a device counter, so the launch can sit in a
-// captured graph.  Sums are block reductions in float64; their association
-// differs from the reference's BLAS dots only at float64 round-off.
+// captured graph.  Sums are fixed-order reductions in float64 (thread, wave
+// butterfly, waves, chunks): deterministic, and different from the
+// reference's BLAS dots only at float64 round-off.
+#include <algorithm>
+
 #include "mgh_common.h"
 #include "mgh_ports.h"
 
 namespace {
 
 constexpr int PB = 256;
+constexpr int PR = MGH_PORT_CHUNK / PB;  // points per thread per chunk
+constexpr int PQ = MGH_PORT_PQ;
+constexpr int PH = 2 * MGH_PORT_MAX_MODES;  // offset of the second partial group
 
-__device__ double block_sum(double v, double* sh) {
-  // wave64 tree in LDS (PB = 256 threads -> 4 waves)
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = PB / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
+// Sum the live partial words of a work-group (wave butterfly, then the four
+// waves in order) and store them to out[0..PQ).
+__device__ void store_partials(double (&acc)[PQ], int M, double* __restrict__ out) {
+  __shared__ double sh[PB / 64][PQ];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < PQ; ++q) {
+    if ((q % PH) < 2 * M) {
+      double v = acc[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) sh[w][q] = v;
+    }
   }
-  const double r = sh[0];
   __syncthreads();
-  return r;
+  const int q = threadIdx.x;
+  if (q < PQ) out[q] = ((q % PH) < 2 * M) ? (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]) : 0.0;
+}
+
+// Work area of a port: projection partials [set][chunk][PQ], record
+// partials (same shape), corrected voltages [set][MGH_PORT_MAX_MODES].
+__device__ inline int64_t rec_offset(int nch, int nsets) { return (int64_t)nsets * nch * PQ; }
+__device__ inline int64_t vc_offset(int nch, int nsets) { return 2LL * nsets * nch * PQ; }
+
+// DTBC kernel version in force at step n: smallest k_init * 2^v >= n + 1.
+__device__ inline bool kernel_slice(const mgh::PortDev& P, long long n, long long* off) {
+  long long len = P.k_init, o = 0;
+  while (len < n + 1) { o += len; len *= 2; }
+  *off = o;
+  // history or kernel buffer too short for this step (the host sizes both
+  // for the run; this guards every access)
+  return n <= P.cap && o + len <= P.kern_stride;
 }
 
 template <typename T>
-__global__ __launch_bounds__(PB) void k_port_step(T* __restrict__ e, int64_t e_set,
-                                                  const T* __restrict__ h, int64_t h_set,
-                                                  mgh::PortSet ps, const long long* step_ctr) {
-  const mgh::PortDev& P = ps.p[blockIdx.x];
-  const int s = blockIdx.y;
-  const int n_pts = P.n_u + P.n_v;
-  const int M = P.M;
+__global__ __launch_bounds__(PB) void k_port_project(const T* __restrict__ e, int64_t e_set,
+                                                     mgh::PortSet ps,
+                                                     const long long* __restrict__ step_ctr) {
+  const mgh::PortDev& P = ps.p[blockIdx.y];
+  const int s = blockIdx.z, c = blockIdx.x;
+  const int n_pts = P.n_u + P.n_v, M = P.M;
+  const int nch = mgh::port_chunks(n_pts, P.cap);
+  if (c >= nch) return;  // whole work-group
   const long long n = *step_ctr;
-  T* es = e + (int64_t)s * e_set;
-  const T* hs = h + (int64_t)s * h_set;
-
-  __shared__ double red[PB];
-  __shared__ double vint[MGH_PORT_MAX_MODES];
-  __shared__ double vcorr[MGH_PORT_MAX_MODES];
-  __shared__ double conv_w[MGH_PORT_MAX_MODES];
-  __shared__ double u_inc[MGH_PORT_MAX_MODES];
-
-  // 1. interior-plane projection, u and v parts summed separately
-  double acc[2 * MGH_PORT_MAX_MODES];
+  const T* es = e + (int64_t)s * e_set;
+  double acc[PQ];
 #pragma unroll
-  for (int q = 0; q < 2 * MGH_PORT_MAX_MODES; ++q) acc[q] = 0.0;
-  for (int p = threadIdx.x; p < n_pts; p += PB) {
-    const double ev = (double)es[P.idx_int[p]];
-    const double me = P.me_int[p];
-    const int part = p < P.n_u ? 0 : 1;
+  for (int q = 0; q < PQ; ++q) acc[q] = 0.0;
+  // interior-plane projection, u and v parts kept apart
 #pragma unroll
-    for (int m = 0; m < MGH_PORT_MAX_MODES; ++m)
-      if (m < M) acc[2 * m + part] += me * (P.prof_e[(int64_t)m * n_pts + p] * ev);
-  }
-  for (int m = 0; m < M; ++m) {
-    const double su = block_sum(acc[2 * m], red);
-    const double sv = block_sum(acc[2 * m + 1], red);
-    if (threadIdx.x == 0) vint[m] = su + sv;
-  }
-
-  // 2. DTBC history convolutions sum_{j=1..n} l_j hist[n-j]
-  double* st = P.state + ((int64_t)s * M) * 4;
-  for (int m = 0; m < M; ++m) {
-    const double* prm = P.params + 8 * m;
-    if (prm[0] != 1.0 || n == 0) {
-      if (threadIdx.x == 0) { conv_w[m] = 0.0; u_inc[m] = 0.0; }
-      continue;
-    }
-    // kernel version in force: smallest k_init*2^v >= n+1
-    long long len = P.k_init, off = 0;
-    while (len < n + 1) { off += len; len *= 2; }
-    if (n > P.cap || off + len > P.kern_stride) {
-      // history or kernel buffer too short for this step: poison the channel
-      // (the host sizes both for the run; this guards every access)
-      if (threadIdx.x == 0) { conv_w[m] = __builtin_nan(""); u_inc[m] = 0.0; }
-      continue;
-    }
-    const double* l = P.kern + (int64_t)m * P.kern_stride + off;
-    const double* wh = P.w_hist + ((int64_t)s * M + m) * P.cap;
-    const double* sh = P.s_hist + ((int64_t)s * M + m) * P.cap;
-    double cw = 0.0, ci = 0.0;
-    for (long long j = 1 + threadIdx.x; j <= n; j += PB) {
-      const double lj = l[j];
-      cw += lj * wh[n - j];
-      ci += lj * sh[n - j];
-    }
-    cw = block_sum(cw, red);
-    ci = block_sum(ci, red);
-    if (threadIdx.x == 0) { conv_w[m] = cw; u_inc[m] = ci; }
-  }
-  __syncthreads();
-
-  // 3. per-mode termination (scalar float64, one thread)
-  if (threadIdx.x == 0) {
-    const int exc = P.exc_mode ? P.exc_mode[s] : -1;
-    double env = 0.0;
-    for (int m = 0; m < M; ++m) {
-      const double* prm = P.params + 8 * m;
-      double* sm = st + 4 * m;  // u, u_prev, V_port_prev, V_int_prev
-      double v;
-      if (prm[0] == 1.0) {
-        const double src = (m == exc && n < P.tab_len) ? P.src_tab[(int64_t)s * P.tab_len + n] : 0.0;
-        const double ghost = src + conv_w[m];
-        double* wh = P.w_hist + ((int64_t)s * M + m) * P.cap;
-        double* shh = P.s_hist + ((int64_t)s * M + m) * P.cap;
-        if (n < P.cap) {
-          wh[n] = sm[0] - u_inc[m];
-          shh[n] = src;
-        }
-        const double u_new = (prm[5] * sm[0] - sm[1]) + prm[1] * (ghost + sm[3]);
-        sm[1] = sm[0];
-        sm[0] = u_new;
-        v = u_new;
-      } else if (m == exc && P.mur_tab != nullptr && n < P.tab_len) {
-        const double* t4 = P.mur_tab + ((int64_t)s * P.tab_len + n) * 4;
-        const double sin_now = vint[m] - t4[1];
-        const double sip = sm[3] - t4[2];
-        const double sfp = sm[2] - t4[3];
-        v = (t4[0] + sip) + prm[3] * (sin_now - sfp);
-      } else {
-        v = sm[3] + prm[3] * (vint[m] - sm[2]);
-      }
-      vcorr[m] = v;
-      sm[2] = v;
-      sm[3] = vint[m];
-      const double a = fabs(v) * prm[4];
-      env = a > env ? a : env;
-    }
-    if (P.absmax && env > P.absmax[s]) P.absmax[s] = env;
-  }
-  __syncthreads();
-
-  // 4. modal write-back and 5. recording
-  const bool rec = P.V_rec != nullptr && n < P.rec_cap;
-#pragma unroll
-  for (int q = 0; q < 2 * MGH_PORT_MAX_MODES; ++q) acc[q] = 0.0;
-  double acci[2 * MGH_PORT_MAX_MODES];
-#pragma unroll
-  for (int q = 0; q < 2 * MGH_PORT_MAX_MODES; ++q) acci[q] = 0.0;
-  for (int p = threadIdx.x; p < n_pts; p += PB) {
-    double val = 0.0;
-    for (int m = 0; m < M; ++m) val += vcorr[m] * P.prof_e[(int64_t)m * n_pts + p];
-    const T tv = (T)val;
-    es[P.idx_port[p]] = tv;
-    if (rec) {
-      const double ev = (double)tv;
-      const double hv = (double)hs[P.idx_h[p]];
-      const double me = P.me_port[p], mh = P.mh[p];
+  for (int r = 0; r < PR; ++r) {
+    const int p = c * MGH_PORT_CHUNK + r * PB + threadIdx.x;
+    if (p < n_pts) {
+      const double ev = (double)es[P.idx_int[p]];
+      const double me = P.me_int[p];
       const int part = p < P.n_u ? 0 : 1;
 #pragma unroll
-      for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) {
-        if (m < M) {
-          acc[2 * m + part] += me * (P.prof_e[(int64_t)m * n_pts + p] * ev);
-          acci[2 * m + part] += mh * (P.prof_h[(int64_t)m * n_pts + p] * hv);
+      for (int m = 0; m < MGH_PORT_MAX_MODES; ++m)
+        if (m < M) acc[2 * m + part] += me * (P.prof_e[(int64_t)m * n_pts + p] * ev);
+    }
+  }
+  // DTBC history convolutions sum_{j=1..n} l_j hist[n-j], terms of this chunk
+  if (n >= 1 && (long long)c * MGH_PORT_CHUNK < n) {
+#pragma unroll
+    for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) {
+      if (m >= M || P.params[8 * m] != 1.0) continue;
+      long long off;
+      if (!kernel_slice(P, n, &off)) continue;
+      const double* l = P.kern + (int64_t)m * P.kern_stride + off;
+      const double* wh = P.w_hist + ((int64_t)s * M + m) * P.cap;
+      const double* sh = P.s_hist + ((int64_t)s * M + m) * P.cap;
+#pragma unroll
+      for (int r = 0; r < PR; ++r) {
+        const long long j = 1 + (long long)c * MGH_PORT_CHUNK + r * PB + threadIdx.x;
+        if (j <= n) {
+          const double lj = l[j];
+          acc[PH + 2 * m] += lj * wh[n - j];
+          acc[PH + 2 * m + 1] += lj * sh[n - j];
         }
       }
     }
   }
-  if (rec) {
-    for (int m = 0; m < M; ++m) {
-      const double vu = block_sum(acc[2 * m], red);
-      const double vv = block_sum(acc[2 * m + 1], red);
-      // reference order: the u-edge part (h_u on the v edges) first
-      const double iu = block_sum(acci[2 * m + 1], red);
-      const double iv = block_sum(acci[2 * m], red);
-      if (threadIdx.x == 0) {
-        const double scale = P.params[8 * m + 4];
-        const int64_t o = ((int64_t)s * M + m) * P.rec_cap + n;
-        P.V_rec[o] = (vu + vv) * scale;
-        P.I_rec[o] = (iu + iv) * scale;
+  store_partials(acc, M, P.work + ((int64_t)s * nch + c) * PQ);
+}
+
+__global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps,
+                                                     const long long* __restrict__ step_ctr) {
+  const mgh::PortDev& P = ps.p[blockIdx.x];
+  const int s = blockIdx.y;
+  const int n_pts = P.n_u + P.n_v, M = P.M;
+  const int nch = mgh::port_chunks(n_pts, P.cap);
+  const long long n = *step_ctr;
+  __shared__ double tot[PQ];
+  const int q = threadIdx.x;
+  if (q < PQ) {
+    double v = 0.0;
+    for (int c = 0; c < nch; ++c) v += P.work[((int64_t)s * nch + c) * PQ + q];
+    tot[q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double* st = P.state + ((int64_t)s * M) * 4;
+  double* vcs = P.work + vc_offset(nch, gridDim.y) + (int64_t)s * MGH_PORT_MAX_MODES;
+  const int exc = P.exc_mode ? P.exc_mode[s] : -1;
+  double env = 0.0;
+  for (int m = 0; m < M; ++m) {
+    const double* prm = P.params + 8 * m;
+    double* sm = st + 4 * m;  // u, u_prev, V_port_prev, V_int_prev
+    const double vint = tot[2 * m] + tot[2 * m + 1];
+    double v;
+    if (prm[0] == 1.0) {
+      double conv_w = 0.0, u_inc = 0.0;
+      if (n >= 1) {
+        long long off;
+        if (kernel_slice(P, n, &off)) {
+          conv_w = tot[PH + 2 * m];
+          u_inc = tot[PH + 2 * m + 1];
+        } else {
+          conv_w = __builtin_nan("");
+        }
+      }
+      const double src = (m == exc && n < P.tab_len) ? P.src_tab[(int64_t)s * P.tab_len + n] : 0.0;
+      const double ghost = src + conv_w;
+      double* wh = P.w_hist + ((int64_t)s * M + m) * P.cap;
+      double* shh = P.s_hist + ((int64_t)s * M + m) * P.cap;
+      if (n < P.cap) {
+        wh[n] = sm[0] - u_inc;
+        shh[n] = src;
+      }
+      const double u_new = (prm[5] * sm[0] - sm[1]) + prm[1] * (ghost + sm[3]);
+      sm[1] = sm[0];
+      sm[0] = u_new;
+      v = u_new;
+    } else if (m == exc && P.mur_tab != nullptr && n < P.tab_len) {
+      const double* t4 = P.mur_tab + ((int64_t)s * P.tab_len + n) * 4;
+      const double sin_now = vint - t4[1];
+      const double sip = sm[3] - t4[2];
+      const double sfp = sm[2] - t4[3];
+      v = (t4[0] + sip) + prm[3] * (sin_now - sfp);
+    } else {
+      v = sm[3] + prm[3] * (vint - sm[2]);
+    }
+    vcs[m] = v;
+    sm[2] = v;
+    sm[3] = vint;
+    const double a = fabs(v) * prm[4];
+    env = a > env ? a : env;
+  }
+  if (P.absmax && env > P.absmax[s]) P.absmax[s] = env;
+}
+template <typename T>
+__global__ __launch_bounds__(PB) void k_port_write(T* __restrict__ e, int64_t e_set,
+                                                   const T* __restrict__ h, int64_t h_set,
+                                                   mgh::PortSet ps,
+                                                   const long long* __restrict__ step_ctr) {
+  const mgh::PortDev& P = ps.p[blockIdx.y];
+  const int s = blockIdx.z, c = blockIdx.x;
+  const int n_pts = P.n_u + P.n_v, M = P.M;
+  if (c >= mgh::port_plane_chunks(n_pts)) return;  // whole work-group
+  const int nch = mgh::port_chunks(n_pts, P.cap);
+  const long long n = *step_ctr;
+  const bool rec = P.V_rec != nullptr && n < P.rec_cap;
+  const double* vcs = P.work + vc_offset(nch, gridDim.z) + (int64_t)s * MGH_PORT_MAX_MODES;
+  double vc[MGH_PORT_MAX_MODES];
+#pragma unroll
+  for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) vc[m] = m < M ? vcs[m] : 0.0;
+  T* es = e + (int64_t)s * e_set;
+  const T* hs = h + (int64_t)s * h_set;
+  double acc[PQ];
+#pragma unroll
+  for (int q = 0; q < PQ; ++q) acc[q] = 0.0;
+#pragma unroll
+  for (int r = 0; r < PR; ++r) {
+    const int p = c * MGH_PORT_CHUNK + r * PB + threadIdx.x;
+    if (p < n_pts) {
+      double val = 0.0;
+#pragma unroll
+      for (int m = 0; m < MGH_PORT_MAX_MODES; ++m)
+        if (m < M) val += vc[m] * P.prof_e[(int64_t)m * n_pts + p];
+      const T tv = (T)val;
+      es[P.idx_port[p]] = tv;
+      if (rec) {
+        const double ev = (double)tv;
+        const double hv = (double)hs[P.idx_h[p]];
+        const double me = P.me_port[p], mh = P.mh[p];
+        const int part = p < P.n_u ? 0 : 1;
+#pragma unroll
+        for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) {
+          if (m < M) {
+            acc[2 * m + part] += me * (P.prof_e[(int64_t)m * n_pts + p] * ev);
+            acc[PH + 2 * m + part] += mh * (P.prof_h[(int64_t)m * n_pts + p] * hv);
+          }
+        }
       }
     }
+  }
+  if (rec) store_partials(acc, M, P.work + rec_offset(nch, gridDim.z) + ((int64_t)s * nch + c) * PQ);
+}
+
+__global__ __launch_bounds__(64) void k_port_record(mgh::PortSet ps,
+                                                    const long long* __restrict__ step_ctr) {
+  const mgh::PortDev& P = ps.p[blockIdx.x];
+  const int s = blockIdx.y;
+  const long long n = *step_ctr;
+  if (P.V_rec == nullptr || n >= P.rec_cap) return;  // whole work-group
+  const int n_pts = P.n_u + P.n_v, M = P.M;
+  const int nch = mgh::port_chunks(n_pts, P.cap);
+  const int npc = mgh::port_plane_chunks(n_pts);
+  const double* part = P.work + rec_offset(nch, gridDim.y);
+  __shared__ double tot[PQ];
+  const int q = threadIdx.x;
+  if (q < PQ) {
+    double v = 0.0;
+    for (int c = 0; c < npc; ++c) v += part[((int64_t)s * nch + c) * PQ + q];
+    tot[q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int m = 0; m < M; ++m) {
+    const double scale = P.params[8 * m + 4];
+    const int64_t o = ((int64_t)s * M + m) * P.rec_cap + n;
+    // reference order: the u-edge part (h_u on the v edges) first
+    P.V_rec[o] = (tot[2 * m] + tot[2 * m + 1]) * scale;
+    P.I_rec[o] = (tot[PH + 2 * m + 1] + tot[PH + 2 * m]) * scale;
   }
 }
 
@@ -206,19 +283,34 @@ int mgh_port_step(int dtype, int nsets, void* e, long long e_set_stride, const v
   MGH_REQUIRE(e && h && port_desc && step_ctr, "null pointer");
   mgh::PortSet ps{};
   ps.n = n_ports;
+  int max_ch = 1, max_pch = 1;
   for (int i = 0; i < n_ports; ++i) {
-    int rc = mgh::load_port(port_desc + (int64_t)i * MGH_PORT_WORDS, &ps.p[i]);
+    int rc = mgh::load_port(port_desc + (int64_t)i * MGH_PORT_WORDS, &ps.p[i], nsets);
     if (rc) return rc;
+    const int n_pts = ps.p[i].n_u + ps.p[i].n_v;
+    max_ch = std::max(max_ch, mgh::port_chunks(n_pts, ps.p[i].cap));
+    max_pch = std::max(max_pch, mgh::port_plane_chunks(n_pts));
   }
   hipStream_t st = (hipStream_t)stream;
-  dim3 grd(n_ports, nsets, 1);
+  const dim3 g_proj(max_ch, n_ports, nsets), g_write(max_pch, n_ports, nsets), g_one(n_ports, nsets, 1);
   if (dtype == 4)
-    hipLaunchKernelGGL(k_port_step<float>, grd, dim3(PB), 0, st, (float*)e, e_set_stride,
+    hipLaunchKernelGGL(k_port_project<float>, g_proj, dim3(PB), 0, st, (const float*)e, e_set_stride,
+                       ps, step_ctr);
+  else
+    hipLaunchKernelGGL(k_port_project<double>, g_proj, dim3(PB), 0, st, (const double*)e,
+                       e_set_stride, ps, step_ctr);
+  MGH_CHECK_LAUNCH("k_port_project");
+  hipLaunchKernelGGL(k_port_advance, g_one, dim3(64), 0, st, ps, step_ctr);
+  MGH_CHECK_LAUNCH("k_port_advance");
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_port_write<float>, g_write, dim3(PB), 0, st, (float*)e, e_set_stride,
                        (const float*)h, h_set_stride, ps, step_ctr);
   else
-    hipLaunchKernelGGL(k_port_step<double>, grd, dim3(PB), 0, st, (double*)e, e_set_stride,
+    hipLaunchKernelGGL(k_port_write<double>, g_write, dim3(PB), 0, st, (double*)e, e_set_stride,
                        (const double*)h, h_set_stride, ps, step_ctr);
-  MGH_CHECK_LAUNCH("k_port_step");
+  MGH_CHECK_LAUNCH("k_port_write");
+  hipLaunchKernelGGL(k_port_record, g_one, dim3(64), 0, st, ps, step_ctr);
+  MGH_CHECK_LAUNCH("k_port_record");
   return MGH_OK;
 }
 

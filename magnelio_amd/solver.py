@@ -44,6 +44,10 @@ from .lumped import DeviceLumped
 from .boundaries import CPMLBoundary, DeviceCPML, PECBoundary, PMCBoundary
 from .ports import PortModel
 
+# split port step geometry (csrc/mgh_ports.h)
+PORT_CHUNK, PORT_MAX_MODES = 1024, 8
+PORT_PQ = 4 * PORT_MAX_MODES
+
 PRECISIONS = {"single": (np.float32, torch.float32), "double": (np.float64, torch.float64)}
 
 
@@ -148,6 +152,13 @@ class DevicePorts:
             w[22] = t_mur.data_ptr() if t_mur is not None else 0
             w[23], w[24], w[25] = (V.data_ptr(), I.data_ptr(), rec_cap) if record else (0, 0, 0)
             w[26] = absmax.data_ptr()
+            # scratch of the split port step: projection / record partials per
+            # (set, chunk of 1024 plane points or history terms), voltages
+            n_pts = n_u + n_v
+            nch = max(-(-n_pts // PORT_CHUNK), -(-int(w[18]) // PORT_CHUNK), 1)
+            work_len = nsets * (2 * nch * PORT_PQ + PORT_MAX_MODES)
+            work = dev(np.zeros(work_len))
+            w[27], w[28] = work.data_ptr(), work_len
             self.V_rec.append(V)
             self.I_rec.append(I)
             self.state.append(state)
