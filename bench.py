@@ -93,10 +93,79 @@ def c3_case(dims=(192, 96, 288), seed=3, size=C3_SIZE, n_modes=3, f_max=C3_FMAX)
     mesh = mm.Mesh.from_grid(grid, regions=regions, shapes=shapes,
                              pec_faces=("xmin", "xmax", "ymin", "ymax"))
     dt = mm.courant_dt(grid, "normal")
-    me, mu = mm.build_m_eps(mesh), mm.build_m_mu(mesh)
-    ports = [pp.build_port(name, face, mesh, me, mu, dt=dt, f_calc=f_max, n_modes=n_modes)
+    ports = [pp.build_port_local(name, face, mesh, dt=dt, f_calc=f_max, n_modes=n_modes)
              for name, face in (("port1", "zmin"), ("port2", "zmax"))]
     return mesh, ports, dt
+
+
+C5_SIZE = (22.86e-3, 10.16e-3, 45.72e-3)  # WR-90 cross-section, 2a long
+
+
+def c5_jobs(n_jobs=200, seed=5, rate=1.5, cells=(32 ** 3, 384 ** 3), steps=(2000, 20000),
+            step_scale=1.0, f_max=25e9, n_freq=101):
+    """Serving workload: Poisson arrivals (rate jobs/s), cell counts log-uniform
+    in `cells`, WR-90 guides with 1-4 seeded dielectric bricks/spheres
+    (eps_r U[1, 10]) clear of 1-2 modal ports (z faces) x 1-3 modes, steps
+    U{steps} x step_scale.  Port models depend only on the (homogeneous) feed
+    cross-section, so jobs of equal grid share them (built once, on demand)."""
+    import threading
+
+    from magnelio_amd import mesh as mm
+    from magnelio_amd import ports as pp
+    from magnelio_amd.serve import SweepJob
+
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    t -= t[0]
+    cache, lock = {}, threading.Lock()
+    jobs = []
+    for i in range(n_jobs):
+        c = math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1])))
+        n = max(16, int(round((c / 0.889) ** (1 / 3) / 8.0)) * 8)
+        dims = (n, max(8, int(round(n * C5_SIZE[1] / C5_SIZE[0]))), 2 * n)
+        n_ports = int(rng.integers(1, 3))
+        n_modes = int(rng.integers(1, 4))
+        n_steps = max(64, int(round(int(rng.integers(steps[0], steps[1] + 1)) * step_scale)))
+        objs = []
+        for _ in range(int(rng.integers(1, 5))):
+            ctr = rng.uniform([0.2, 0.2, 0.25], [0.8, 0.8, 0.75]) * np.array(C5_SIZE)
+            half = rng.uniform(0.05, 0.15, 3) * np.array(C5_SIZE)
+            objs.append((float(rng.uniform(1.0, 10.0)), bool(rng.random() < 0.5), ctr, half))
+
+        def build(dims=dims, n_ports=n_ports, n_modes=n_modes, objs=objs):
+            axes = [np.linspace(0.0, L, k + 1) for L, k in zip(C5_SIZE, dims)]
+            grid = mm.GridLines(*axes)
+            boxes, shapes = [], []
+            zlo, zhi = 0.2 * C5_SIZE[2], 0.8 * C5_SIZE[2]  # keep the port feeds homogeneous
+            for eps, sphere, ctr, half in objs:
+                mat = mm.Material.dielectric(eps)
+                if sphere:
+                    r = float(half.min())
+                    shapes.append((mat, lambda X, Y, Z, c=ctr, r=r: ((X - c[0]) ** 2 + (Y - c[1]) ** 2
+                                   + (Z - c[2]) ** 2 < r * r) & (Z > zlo) & (Z < zhi)))
+                else:
+                    lo, hi = ctr - half, ctr + half
+                    boxes.append((mat, (lo[0], lo[1], max(lo[2], zlo), hi[0], hi[1], min(hi[2], zhi))))
+            faces = ("zmin", "zmax")[:n_ports]
+            walls = ("xmin", "xmax", "ymin", "ymax") + (("zmax",) if n_ports == 1 else ())
+            mesh = mm.Mesh.from_grid(grid, regions=boxes, shapes=shapes, pec_faces=walls)
+            dt = mm.courant_dt(grid, "normal")
+            ports = []
+            for name, face in zip(("port1", "port2"), faces):
+                key = (dims, face, n_modes)
+                with lock:
+                    P = cache.get(key)
+                if P is None:
+                    P = pp.build_port_local(name, face, mesh, dt=dt, f_calc=f_max, n_modes=n_modes)
+                    with lock:
+                        cache[key] = P
+                ports.append(P)
+            return mesh, ports, dt
+
+        f_c = 3e8 / (2 * C5_SIZE[0])
+        jobs.append(SweepJob(i, float(t[i]), build, n_steps, np.linspace(1.1 * f_c, 0.96 * f_max, n_freq),
+                             f_max))
+    return jobs
 
 
 def cpu_baseline(mesh, coeffs, nsamp=(10, 20, 40)):
@@ -158,9 +227,14 @@ def main(argv=None) -> int:
     ap.add_argument("--dft-plain", action="store_true", help="uncompensated float64 bins")
     ap.add_argument("--no-tfsf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c5"), default="c2",
                     help="c2: single large job (default line); c3: batched K=6 sweep vs one "
-                         "excitation per launch")
+                         "excitation per launch; c5: serving loop vs one job at a time")
+    ap.add_argument("--jobs", type=int, default=200, help="c5: number of sweep jobs")
+    ap.add_argument("--rate", type=float, default=1.5, help="c5: Poisson arrival rate [jobs/s]")
+    ap.add_argument("--step-scale", type=float, default=0.05,
+                    help="c5: fraction of the 2000-20000 steps each job runs")
+    ap.add_argument("--max-active", type=int, default=8, help="c5: resident jobs at once")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine advances one job per device "
@@ -168,6 +242,8 @@ def main(argv=None) -> int:
         return 2
     if args.workload == "c3":
         return main_c3(args)
+    if args.workload == "c5":
+        return main_c5(args)
 
     import torch
 
@@ -362,6 +438,64 @@ def main_c3(args) -> int:
             "ms_per_step_all_K": t_single / args.steps * 1e3,
             "speedup_batched": rate_b / rate_s,
         },
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def main_c5(args) -> int:
+    """Serving: seeded Poisson stream of sweep jobs through the continuous-
+    batching loop, then the same stream one job at a time."""
+    from magnelio_amd.serve import SweepServer
+
+    def stream_of_jobs():
+        return c5_jobs(n_jobs=args.jobs, rate=args.rate, step_scale=args.step_scale)
+
+    # warm-up: one small job loads the library and the runtime
+    warm = c5_jobs(n_jobs=1, seed=99, cells=(24 ** 3, 24 ** 3), steps=(200, 200))
+    SweepServer().run(warm)
+    reports = {}
+    for mode in ("continuous", "sequential"):
+        print(f"c5 {mode}: {args.jobs} jobs", file=sys.stderr, flush=True)
+        srv = SweepServer(max_active=args.max_active, build_workers=8)
+        reports[mode] = srv.run(stream_of_jobs(), mode=mode)
+        r = reports[mode]
+        print(f"c5 {mode}: {r.jobs_per_s:.3f} jobs/s p50 {r.p50:.2f} s p99 {r.p99:.2f} s",
+              file=sys.stderr, flush=True)
+    c, q = reports["continuous"], reports["sequential"]
+    same = all(np.array_equal(a.result.matrix, b.result.matrix, equal_nan=True)
+               for a, b in zip(c.records, q.records))
+    recs = c.records
+    out = {
+        "metric": "serving sweep jobs/s with p50/p99 job latency (continuous batching)",
+        "value": c.jobs_per_s,
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": int(sum(r.n_steps for r in recs)),
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.jobs} sweep jobs, Poisson arrivals {args.rate}/s (seed 5), cells "
+                         "log-uniform 32^3-384^3 (WR-90 guides, 1-4 dielectric bricks/spheres eps_r "
+                         "U[1,10]), 1-2 modal DTBC ports x 1-3 modes, steps U{2000..20000} x "
+                         f"{args.step_scale}; admission by HBM footprint, up to {args.max_active} "
+                         "resident jobs on separate streams, graph-replayed 256-step chunks"),
+            "cells_min": int(min(r.cells for r in recs)),
+            "cells_max": int(max(r.cells for r in recs)),
+            "cell_updates": float(sum(r.cell_updates for r in recs)),
+        },
+        "latency_s": {"p50": c.p50, "p99": c.p99},
+        "gcell_per_s": c.gcell_per_s,
+        "peak_resident": c.peak_resident,
+        "peak_footprint_gb": round(c.peak_bytes / 1e9, 2),
+        "one_job_at_a_time": {"jobs_per_s": q.jobs_per_s, "p50": q.p50, "p99": q.p99,
+                              "gcell_per_s": q.gcell_per_s},
+        "results_identical": bool(same),
     }
     print(json.dumps(out))
     return 0

@@ -126,6 +126,26 @@ int mgh_port_step(int dtype, int nsets, void* e, long long e_set_stride, const v
  * (fit_td.py:950). */
 int mgh_step_advance(long long* step_ctr, long long by, void* stream);
 
+/* ---- (a2)+(a6) material matrices and coefficients on the device ----------
+ * Replaces build_M_eps / build_M_mu / build_M_sigma / build_M_sigma_m
+ * (src/magnelio/_operators/material_matrices.py:334-757, category-0 cells),
+ * the port-plane flattening (:106-331), the PEC OR rule with bbox walls
+ * (src/magnelio/mesh/indexing.py:168-257) and the alpha/beta construction
+ * of FITTimeDomainSolver.setup (src/magnelio/solver/fit_td.py:375-513),
+ * in float64 in the same operation order, rounded once to dtype.
+ * mat_id: device int32 (nx, ny, nz), z fastest (numpy C order); tab: device
+ * float64 n_mat x 16 (eps xyz, mu xyz, sigma xyz, sigma_m xyz with 0 for PEC
+ * materials, is_pec, 3 unused); geo: device float64 dx[nx] dy[ny] dz[nz]
+ * then dual widths ax[nx+1] ay[ny+1] az[nz+1]; face bits xmin 1, xmax 2,
+ * ymin 4, ymax 8, zmin 16, zmax 32 for PEC walls and port faces (port faces
+ * on one axis).  Outputs in the device layout: alpha/beta/M_eps on the E
+ * side, alpha/beta/M_mu on the H side (m_eps / m_mu may be NULL).  No ADE
+ * W term (dispersive meshes use the host construction). */
+int mgh_build_coefficients(int dtype, int nx, int ny, int nz, const int* mat_id, int n_mat,
+                           const double* tab, const double* geo, double dt, double eps0, double mu0,
+                           int wall_faces, int port_faces, void* alpha_e, void* beta_e, void* m_eps,
+                           void* alpha_h, void* beta_h, void* m_mu, void* stream);
+
 /* ---- (a18) stored-energy sum ---------------------------------------------
  * Replaces the float64-accumulated reductions of the energy check
  * (src/magnelio/solver/fit_td.py:1014-1019).

@@ -77,6 +77,12 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_int, _c_void_p, _c_void_p],
     ),
+    "mgh_build_coefficients": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_double,
+         _c_double, _c_double, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+         _c_void_p, _c_void_p, _c_void_p],
+    ),
     "mgh_ade_save": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p],

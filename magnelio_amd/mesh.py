@@ -135,6 +135,10 @@ class Mesh:
     material_library: dict[int, Material]
     pec_mask_edges: np.ndarray  # bool (3, n_max)
     pec_faces: tuple[str, ...] = field(default_factory=tuple)
+    # pec_mask_edges is exactly the OR rule over PEC cells plus the pec_faces
+    # walls (what the device coefficient builder recomputes); clear it when
+    # the mask is edited by hand
+    pec_rule: bool = True
 
     @property
     def dims(self) -> tuple[int, int, int]:

@@ -595,3 +595,48 @@ def build_port(name: str, face: str, mesh, m_eps: np.ndarray, m_mu: np.ndarray, 
                      mh_v=mh_v, record_scale=rec, source_scale=src, state_scale=st, dtbc_r=rs,
                      dtbc_q=qs, dtbc_z0=z0s, pair_spread=spreads, mur_r=mur_r, tau=tau,
                      slab_defect=defect)
+
+
+def build_port_local(name: str, face: str, mesh, *, dt: float, f_calc: float, n_modes: int = 1,
+                     eps_r: float = 1.0, termination: str = "auto", feed_cells: int = 8) -> PortModel:
+    """``build_port`` evaluated on the feed slab only.
+
+    Everything a port model holds — plane masses, PEC mask, modes,
+    calibration, chain parameters, the slab defect of the first feed cells —
+    depends on the ``feed_cells`` cell layers next to the face alone: each
+    mass entry on those layers comes from its owning cell and the local
+    widths.  The model is built on a sub-mesh of those layers (same grid
+    lines, material ids and walls; the cut face is left open) and its plane
+    is re-targeted to the full grid's flat indices, so it equals the
+    full-grid build bit for bit without the full-grid mass matrices.
+    A mesh whose PEC mask was edited by hand uses the full-grid build.
+    """
+    axis, is_max = FACES[face][0], FACES[face][1]
+    N = mesh.dims[axis]
+    if feed_cells < 4:
+        raise ValueError("feed_cells must be at least 4")
+    if feed_cells >= N or not getattr(mesh, "pec_rule", False):
+        return build_port(name, face, mesh, _mesh.build_m_eps(mesh), _mesh.build_m_mu(mesh), dt=dt,
+                          f_calc=f_calc, n_modes=n_modes, eps_r=eps_r, termination=termination)
+    g = mesh.grid
+    lines = [g.x, g.y, g.z]
+    lines[axis] = lines[axis][N - feed_cells:] if is_max else lines[axis][: feed_cells + 1]
+    sub_grid = _mesh.GridLines(*lines)
+    sl = [slice(None)] * 3
+    sl[axis] = slice(N - feed_cells, N) if is_max else slice(0, feed_cells)
+    mid = np.ascontiguousarray(mesh.material_id[tuple(sl)])
+    cut = ("xmin", "ymin", "zmin")[axis] if is_max else ("xmax", "ymax", "zmax")[axis]
+    walls = tuple(f for f in mesh.pec_faces if f != cut)
+    pec = _mesh.build_pec_mask(sub_grid, mid, mesh.material_library)
+    _mesh.or_bbox_pec_walls(pec, *sub_grid.dims, walls)
+    sub = _mesh.Mesh(grid=sub_grid, material_id=mid, material_library=mesh.material_library,
+                     pec_mask_edges=pec, pec_faces=walls)
+    P = build_port(name, face, sub, _mesh.build_m_eps(sub), _mesh.build_m_mu(sub), dt=dt,
+                   f_calc=f_calc, n_modes=n_modes, eps_r=eps_r, termination=termination)
+    full = PortPlane.from_mesh(face, g)
+    local = P.plane
+    if not (np.array_equal(full.u_edge_uv, local.u_edge_uv) and np.array_equal(full.v_edge_uv, local.v_edge_uv)
+            and full.coordinate == local.coordinate and full.normal_dx == local.normal_dx):
+        raise RuntimeError(f"port {name}: feed sub-mesh plane does not match the full grid")
+    P.plane = full
+    return P

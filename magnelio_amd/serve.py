@@ -1,0 +1,255 @@
+"""Serving loop for S-parameter sweep jobs on one device.
+
+Jobs (a mesh with modal ports, swept over all its (port, mode) channels)
+arrive over time.  The loop keeps every admitted job resident on the device
+— fields, port states, DTBC histories, V/I records — and advances all of
+them together: each resident job owns a stream and a captured graph of its
+steps, and the host enqueues one chunk of steps per resident job per round,
+so the jobs' kernels overlap on the device.  A job that arrives while others
+run is admitted as soon as a slot and its HBM footprint are free (first fit
+in arrival order); a finished job's S-matrix is assembled on the host and
+its memory returned.  ``mode="sequential"`` runs the same jobs one at a
+time, in arrival order, for comparison.
+
+The reference has no serving tier; its unit of work is one
+``AnalysisScatteringTD.run`` (src/magnelio/analysis/scattering_td.py:1252-1330),
+which a job here reproduces as one batched sweep (``sweep.ScatteringSweep``).
+Results do not depend on the mix: every job has its own buffers and every
+kernel is deterministic, so a job's S-matrix in the mix equals its S-matrix
+alone bit for bit.
+"""
+
+from __future__ import annotations
+
+import heapq
+import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import kernels
+from .sweep import SParameterResult, ScatteringSweep
+
+
+@dataclass
+class SweepJob:
+    """One sweep request.  ``build()`` returns (mesh, ports, dt) on the host
+    (mode solving included); it runs in a worker thread before admission."""
+
+    job_id: int
+    arrival: float  # seconds after the start of the run
+    build: object
+    n_steps: int
+    f_axis: np.ndarray
+    f_max: float
+    precision: str = "single"
+    waveforms: dict | None = None
+
+
+def footprint_bytes(dims, n_channels: int, K: int, n_steps: int, precision: str = "single") -> int:
+    """Device bytes of a resident sweep: K field sets plus the shared alpha/beta
+    and masses, the DTBC histories (w, s) and V/I records of every channel of
+    every set in float64, and the port work areas."""
+    nx, ny, nz = dims
+    ne, nh = kernels.n_e(nx, ny, nz), kernels.n_h(nx, ny, nz)
+    w = 4 if precision == "single" else 8
+    fields = w * (ne + nh) * K
+    coeffs = w * (ne + nh) * 3  # alpha, beta and the mass vector of each side
+    per_channel = 8 * 4 * max(n_steps, 1)  # w_hist, s_hist, V_rec, I_rec
+    return int(fields + coeffs + per_channel * n_channels * K + (64 << 20))
+
+
+@dataclass
+class JobRecord:
+    job_id: int
+    arrival: float
+    admitted: float = 0.0
+    finished: float = 0.0
+    cells: int = 0
+    K: int = 0
+    n_steps: int = 0
+    footprint: int = 0
+    result: SParameterResult | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+    @property
+    def cell_updates(self) -> float:
+        return float(self.cells) * self.K * self.n_steps
+
+
+@dataclass
+class ServeReport:
+    mode: str
+    records: list
+    wall: float
+    peak_resident: int = 0
+    peak_bytes: int = 0
+
+    def latencies(self) -> np.ndarray:
+        return np.array([r.latency for r in self.records])
+
+    @property
+    def p50(self) -> float:
+        return float(np.percentile(self.latencies(), 50))
+
+    @property
+    def p99(self) -> float:
+        return float(np.percentile(self.latencies(), 99))
+
+    @property
+    def jobs_per_s(self) -> float:
+        return len(self.records) / self.wall
+
+    @property
+    def gcell_per_s(self) -> float:
+        return sum(r.cell_updates for r in self.records) / self.wall / 1e9
+
+
+@dataclass
+class _Active:
+    rec: JobRecord
+    sweep: ScatteringSweep
+    solver: object
+    stream: torch.cuda.Stream
+    events: deque = field(default_factory=deque)
+
+
+class SweepServer:
+    """Continuous-batching server (see module docstring).
+
+    ``hbm_budget``: bytes the resident jobs may hold (default 85 % of the
+    device memory free at start); ``max_active``: resident jobs at once;
+    ``chunk``: steps enqueued per job per round; ``inflight``: chunks a job
+    may have queued ahead of the device.
+    """
+
+    def __init__(self, device: str = "cuda:0", hbm_budget: int | None = None, max_active: int = 8,
+                 chunk: int = 256, inflight: int = 2, build_workers: int = 2,
+                 use_graphs: bool = True):
+        self.device = torch.device(device)
+        self.hbm_budget = hbm_budget
+        self.max_active = int(max_active)
+        self.chunk = int(chunk)
+        self.inflight = int(inflight)
+        self.build_workers = int(build_workers)
+        self.use_graphs = use_graphs
+
+    # -- helpers ---------------------------------------------------------
+    def _budget(self) -> int:
+        if self.hbm_budget is not None:
+            return int(self.hbm_budget)
+        free, _ = torch.cuda.mem_get_info(self.device)
+        return int(0.85 * free)
+
+    def _prepare(self, job: SweepJob, rec: JobRecord):
+        """Worker-thread half of a job: mesh, ports, material matrices and
+        coefficients on the host (no device access)."""
+        mesh, ports, dt = job.build()
+        sw = ScatteringSweep(mesh, ports, dt=dt, n_steps=job.n_steps, f_axis=job.f_axis,
+                             f_max=job.f_max, waveforms=job.waveforms, precision=job.precision,
+                             use_graphs=self.use_graphs, device=str(self.device))
+        solver = sw.make_solver(sw.excited)
+        solver.prepare_host()
+        rec.cells = int(np.prod(mesh.dims))
+        rec.K = len(sw.excited)
+        rec.n_steps = job.n_steps
+        rec.footprint = footprint_bytes(mesh.dims, len(sw.channels), rec.K, job.n_steps, job.precision)
+        return sw, solver
+
+    @staticmethod
+    def _finish(a: _Active) -> SParameterResult:
+        cols = [a.sweep.s_column(a.solver.recorded(k), ch) for k, ch in enumerate(a.sweep.excited)]
+        return SParameterResult.merge(cols)
+
+    # -- main loop -------------------------------------------------------
+    def run(self, jobs: list[SweepJob], mode: str = "continuous") -> ServeReport:
+        if mode not in ("continuous", "sequential"):
+            raise ValueError("mode must be 'continuous' or 'sequential'")
+        max_active = self.max_active if mode == "continuous" else 1
+        budget = self._budget()
+        order = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
+        recs = {j.job_id: JobRecord(j.job_id, j.arrival) for j in order}
+        pool = ThreadPoolExecutor(max_workers=max(1, self.build_workers))
+        futures = {}
+        arrivals = [(j.arrival, j.job_id, j) for j in order]
+        heapq.heapify(arrivals)
+        waiting: list[tuple] = []  # (job, sweep, solver prepared on the host)
+        active: list[_Active] = []
+        used = 0
+        peak_resident = peak_bytes = 0
+        done = 0
+        t0 = time.perf_counter()
+        try:
+            while done < len(order):
+                now = time.perf_counter() - t0
+                # arrivals start their host-side build
+                while arrivals and arrivals[0][0] <= now:
+                    _, jid, job = heapq.heappop(arrivals)
+                    futures[jid] = (job, pool.submit(self._prepare, job, recs[jid]))
+                # prepared jobs join the admission queue in arrival order
+                for jid in sorted(futures, key=lambda i: (recs[i].arrival, i)):
+                    job, fut = futures[jid]
+                    if fut.done():
+                        waiting.append((job, *fut.result()))
+                        del futures[jid]
+                waiting.sort(key=lambda x: (x[0].arrival, x[0].job_id))
+                # admission: first fit in arrival order
+                for item in list(waiting):
+                    job, sw, solver = item
+                    rec = recs[job.job_id]
+                    if len(active) >= max_active:
+                        break
+                    if rec.footprint > budget:
+                        raise RuntimeError(f"job {job.job_id} needs {rec.footprint / 1e9:.1f} GB, "
+                                           f"more than the {budget / 1e9:.1f} GB budget")
+                    if used + rec.footprint > budget:
+                        if mode == "sequential":
+                            break
+                        continue
+                    # buffers come from the current stream's pool, so a retired
+                    # job's memory is reused by the next one; the job's stream
+                    # waits for the set-up copies before its first step
+                    solver.setup()
+                    stream = torch.cuda.Stream(device=self.device)
+                    stream.wait_stream(torch.cuda.current_stream(self.device))
+                    rec.admitted = time.perf_counter() - t0
+                    active.append(_Active(rec, sw, solver, stream))
+                    used += rec.footprint
+                    waiting.remove(item)
+                peak_resident = max(peak_resident, len(active))
+                peak_bytes = max(peak_bytes, used)
+                # one chunk per resident job per round, up to `inflight` queued
+                progressed = False
+                for a in active:
+                    while a.events and a.events[0].query():
+                        a.events.popleft()
+                    if len(a.events) < self.inflight and a.solver.n_completed < a.rec.n_steps:
+                        with torch.cuda.stream(a.stream):
+                            a.solver.launch(self.chunk)
+                            ev = torch.cuda.Event()
+                            ev.record(a.stream)
+                        a.events.append(ev)
+                        progressed = True
+                # retire finished jobs
+                for a in list(active):
+                    if a.solver.n_completed >= a.rec.n_steps and all(e.query() for e in a.events):
+                        a.stream.synchronize()
+                        a.rec.result = self._finish(a)
+                        a.rec.finished = time.perf_counter() - t0
+                        used -= a.rec.footprint
+                        active.remove(a)
+                        del a
+                        done += 1
+                        progressed = True
+                if not progressed:
+                    time.sleep(2e-4)
+        finally:
+            pool.shutdown(wait=True)
+        wall = time.perf_counter() - t0 - min(j.arrival for j in order)
+        return ServeReport(mode, [recs[j.job_id] for j in order], wall, peak_resident, peak_bytes)

@@ -48,6 +48,14 @@ from .ports import PortModel
 PORT_CHUNK, PORT_MAX_MODES = 1024, 8
 PORT_PQ = 4 * PORT_MAX_MODES
 
+_PORT_AXIS = {"xmin": 0, "xmax": 0, "ymin": 1, "ymax": 1, "zmin": 2, "zmax": 2}
+
+
+def _face_bits(faces) -> int:
+    """xmin 1, xmax 2, ymin 4, ymax 8, zmin 16, zmax 32 (mgh_build_coefficients)."""
+    return sum(1 << _mesh.FACES.index(f) for f in set(faces))
+
+
 PRECISIONS = {"single": (np.float32, torch.float32), "double": (np.float64, torch.float64)}
 
 
@@ -224,6 +232,8 @@ class FITTimeDomainSolver:
     curl64: bool = False
     kchunk: int = kernels.DEFAULT_KCHUNK
     use_graphs: bool = False
+    graph_block: int = 1  # minimum steps per captured graph
+    coefficients: str = "auto"  # alpha/beta built on the "device", the "host", or "auto"
     device: str = "cuda:0"
 
     def __post_init__(self):
@@ -243,17 +253,21 @@ class FITTimeDomainSolver:
         self._graph = None
 
     # ------------------------------------------------------------------
-    def setup(self) -> None:
-        _lib.load()
+    def prepare_host(self) -> None:
+        """Host half of ``setup`` (material matrices, PEC masks, ADE blocks,
+        alpha/beta in float64, device-layout host copies).  Touches no device
+        and may run in a worker thread; ``setup`` calls it when not done."""
+        if getattr(self, "_host", None) is not None:
+            return
         mesh = self.mesh
         dims = mesh.dims
         self.dims = dims
         dt = self.dt
         np_dt, t_dt = PRECISIONS[self.precision]
         self._np_dtype, self._t_dtype = np_dt, t_dt
-        dev = torch.device(self.device)
-        self._dev = dev
-
+        if self._device_coefficients():
+            self._prepare_device_coefficients()
+            return
         m_eps = _mesh.build_m_eps(mesh)
         m_mu = _mesh.build_m_mu(mesh)
         m_sig = _mesh.build_m_sigma(mesh)
@@ -282,13 +296,97 @@ class FITTimeDomainSolver:
         self._coeffs = co
         self._m_eps_host, self._m_mu_host = m_eps, m_mu
 
-        def up(a, side, dtype=t_dt):
-            return torch.from_numpy(np.ascontiguousarray(layout.to_device(np.asarray(a).astype(np_dt), *dims, side))).to(dev, dtype)
+        def lay(a, side):
+            return np.ascontiguousarray(layout.to_device(np.asarray(a).astype(np_dt), *dims, side))
 
-        self.alpha_e, self.beta_e = up(co.alpha_e, "E"), up(co.beta_e, "E")
-        self.alpha_h, self.beta_h = up(co.alpha_h, "H"), up(co.beta_h, "H")
-        self.m_eps_dev, self.m_mu_dev = up(m_eps, "E"), up(m_mu, "H")
+        self._host = {"alpha_e": lay(co.alpha_e, "E"), "beta_e": lay(co.beta_e, "E"),
+                      "alpha_h": lay(co.alpha_h, "H"), "beta_h": lay(co.beta_h, "H"),
+                      "m_eps": lay(m_eps, "E"), "m_mu": lay(m_mu, "H")}
+
+    def _device_coefficients(self) -> bool:
+        """Whether alpha/beta and the masses are built on the device: not for
+        ADE materials or TF/SF sources (their host set-up reads the float64
+        coefficients) nor for port faces on more than one axis."""
+        if self.coefficients not in ("auto", "host", "device"):
+            raise ValueError(f"coefficients must be 'auto', 'host' or 'device', got {self.coefficients!r}")
+        if self.coefficients == "host":
+            return False
+        why = None
+        if any(getattr(m, "poles", ()) for m in self.mesh.material_library.values()):
+            why = "dispersive materials"
+        elif self.sources:
+            why = "sources"
+        elif not getattr(self.mesh, "pec_rule", False):
+            why = "a hand-edited PEC mask"
+        elif len({_PORT_AXIS[P.face] for P in self.ports}) > 1:
+            why = "port faces on more than one axis"
+        if why is not None and self.coefficients == "device":
+            raise ValueError(f"device-built coefficients do not cover {why}")
+        return why is None
+
+    def _prepare_device_coefficients(self) -> None:
+        """Host inputs of mgh_build_coefficients: material table, 1-D widths,
+        face bits; the PEC mask on the host only where the CPML needs it."""
+        mesh, dims = self.mesh, self.mesh.dims
+        lib = mesh.material_library
+        tab = np.zeros((max(lib) + 1, 16))
+        for m, mat in lib.items():
+            tab[m, 0:3] = mat.epsilon
+            tab[m, 3:6] = mat.mu
+            tab[m, 6:9] = (0.0, 0.0, 0.0) if mat.is_pec else mat.sigma
+            tab[m, 9:12] = (0.0, 0.0, 0.0) if mat.is_pec else mat.sigma_m
+            tab[m, 12] = 1.0 if mat.is_pec else 0.0
+        g = mesh.grid
+        geo = np.concatenate([g.dx, g.dy, g.dz, _mesh._avg_d(g.dx), _mesh._avg_d(g.dy), _mesh._avg_d(g.dz)])
+        self._port_faces = {P.face for P in self.ports}
+        self._dispersion = None
+        self._coeffs = None
+        self._m_eps_host = self._m_mu_host = None
+        if any(isinstance(bc, CPMLBoundary) for bc in self.boundary_conditions.values()):
+            pec = mesh.pec_mask_edges
+            for f in self._port_faces:
+                pec = _mesh.flatten_port_pec(pec, dims, f)
+            self._pec_e = _mesh.pec_flat(pec, dims)
+        else:
+            self._pec_e = None
+        self._host = {"device_coefficients": True, "tab": tab, "geo": geo,
+                      "walls": _face_bits(mesh.pec_faces), "ports": _face_bits(self._port_faces)}
+
+    def setup(self) -> None:
+        _lib.load()
+        self.prepare_host()
+        mesh = self.mesh
+        dims = self.dims
+        dt = self.dt
+        t_dt = self._t_dtype
+        pec_e = self._pec_e
+        dev = torch.device(self.device)
+        self._dev = dev
+        hst = self._host
         ne, nh = kernels.n_e(*dims), kernels.n_h(*dims)
+        if hst.get("device_coefficients"):
+            mid = torch.from_numpy(np.ascontiguousarray(mesh.material_id, dtype=np.int32)).to(dev)
+            tab = torch.from_numpy(hst["tab"]).to(dev)
+            geo = torch.from_numpy(hst["geo"]).to(dev)
+            self.alpha_e, self.beta_e, self.m_eps_dev = (torch.empty(ne, dtype=t_dt, device=dev)
+                                                         for _ in range(3))
+            self.alpha_h, self.beta_h, self.m_mu_dev = (torch.empty(nh, dtype=t_dt, device=dev)
+                                                        for _ in range(3))
+            sh = int(torch.cuda.current_stream(dev).cuda_stream)
+            _lib.call("mgh_build_coefficients", 4 if t_dt == torch.float32 else 8, *dims,
+                      mid.data_ptr(), tab.shape[0], tab.data_ptr(), geo.data_ptr(), float(dt),
+                      float(_mesh.EPS0), float(_mesh.MU0), hst["walls"], hst["ports"],
+                      self.alpha_e.data_ptr(), self.beta_e.data_ptr(), self.m_eps_dev.data_ptr(),
+                      self.alpha_h.data_ptr(), self.beta_h.data_ptr(), self.m_mu_dev.data_ptr(), sh)
+            torch.cuda.current_stream(dev).synchronize()  # inputs are freed on return
+        else:
+            def up(key):
+                return torch.from_numpy(hst[key]).to(dev, t_dt)
+
+            self.alpha_e, self.beta_e = up("alpha_e"), up("beta_e")
+            self.alpha_h, self.beta_h = up("alpha_h"), up("beta_h")
+            self.m_eps_dev, self.m_mu_dev = up("m_eps"), up("m_mu")
+        self._host = {}  # uploaded: drop the host copies, keep prepare_host done
         K = self.nsets
         self.e = torch.zeros((K, ne), dtype=t_dt, device=dev)
         self.h = torch.zeros((K, nh), dtype=t_dt, device=dev)
@@ -370,9 +468,10 @@ class FITTimeDomainSolver:
 
     @property
     def graph_steps(self) -> int:
-        """Steps per captured graph: one monitor block (the block contraction
-        sits at a fixed position inside it), else one step."""
-        g = 1
+        """Steps per captured graph: a multiple of every monitor block (the
+        block contraction sits at a fixed position inside it) and of
+        ``graph_block``."""
+        g = max(1, int(self.graph_block))
         for mon in self.monitors:
             g = g * getattr(mon, "block", 1) // math.gcd(g, getattr(mon, "block", 1))
         return g
@@ -430,6 +529,34 @@ class FITTimeDomainSolver:
                 mon.finalize(self)
         torch.cuda.synchronize(self._dev)
 
+    def launch(self, n_steps: int) -> int:
+        """Enqueue up to ``n_steps`` steps on the current stream and return
+        without waiting for them (no stop checks; graph replays where a whole
+        graph fits).  Returns the steps enqueued; the caller synchronises.
+        A capture (first use of graphs) waits for the device once."""
+        if not self._setup_done:
+            self.setup()
+        total = min(int(self.total_time_steps), self._resume_step + int(n_steps))
+        G = self.graph_steps
+        n = self._resume_step
+        while n < total:
+            if self.use_graphs and n + G <= total and n >= 2 and self._mon_aligned(n, G):
+                if self._graph is None:
+                    self._capture(n, G)
+                if self._graph is not None:
+                    self._graph.replay()
+                    n += G
+                    continue
+            self._step(n)
+            n += 1
+        done = n - self._resume_step
+        self._resume_step = n
+        if n >= int(self.total_time_steps):
+            self._stop_reason = "steps"
+            for mon in self.monitors:
+                mon.finalize(self)
+        return done
+
     def _mon_aligned(self, n: int, G: int) -> bool:
         return all((n - getattr(m, "n_start", 0)) % getattr(m, "block", 1) == 0 for m in self.monitors)
 
@@ -442,17 +569,19 @@ class FITTimeDomainSolver:
         to buffers this solver keeps alive for its lifetime.
         """
         try:
-            torch.cuda.synchronize(self._dev)
+            # stream-level waits only: other solvers' streams keep running
+            cur = torch.cuda.current_stream(self._dev)
+            cur.synchronize()
             g = torch.cuda.CUDAGraph()
             s = torch.cuda.Stream(device=self._dev)
-            s.wait_stream(torch.cuda.current_stream(self._dev))
+            s.wait_stream(cur)
             with torch.cuda.stream(s):
                 g.capture_begin()
                 for k in range(G):
                     self._step(n + k)
                 g.capture_end()
-            torch.cuda.current_stream(self._dev).wait_stream(s)
-            torch.cuda.synchronize(self._dev)
+            cur.wait_stream(s)
+            s.synchronize()
             self._graph = g
         except Exception as exc:  # pragma: no cover - depends on the runtime
             import warnings  # noqa: PLC0415

@@ -133,7 +133,7 @@ class ScatteringSweep:
     def __init__(self, mesh, ports, *, dt: float, n_steps: int, f_axis, f_max: float,
                  f_min: float = 0.0, excited=None, waveforms=None, boundary_conditions=None,
                  precision: str = "single", batch: int | None = None, use_graphs: bool = True,
-                 monitors_factory=None, device: str = "cuda:0"):
+                 monitors_factory=None, device: str = "cuda:0", graph_block: int = 32):
         self.mesh, self.ports = mesh, list(ports)
         self.dt, self.n_steps = float(dt), int(n_steps)
         self.f_axis = np.asarray(f_axis, dtype=float)
@@ -156,6 +156,7 @@ class ScatteringSweep:
         self.use_graphs = use_graphs
         self.monitors_factory = monitors_factory
         self.device = device
+        self.graph_block = int(graph_block)
         self.solvers: list[FITTimeDomainSolver] = []
 
     def waveform(self, ch):
@@ -170,7 +171,8 @@ class ScatteringSweep:
         return FITTimeDomainSolver(mesh=self.mesh, boundary_conditions=self.boundary_conditions,
                                    ports=self.ports, total_time_steps=self.n_steps, dt=self.dt,
                                    precision=self.precision, nsets=len(chans), excitations=exc,
-                                   use_graphs=self.use_graphs, monitors=mons, device=self.device)
+                                   use_graphs=self.use_graphs, monitors=mons, device=self.device,
+                                   graph_block=self.graph_block)
 
     def s_column(self, signals: dict, ch) -> SParameterResult:
         modes = {p.name: p.modes for p in self.ports}

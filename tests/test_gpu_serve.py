@@ -1,0 +1,74 @@
+"""GPU serving loop: continuous batching of sweep jobs.
+
+Small seeded jobs of the serving workload (WR-90 guides with dielectric
+bricks/spheres, 1-2 modal ports x 1-3 modes) arrive close together:
+
+* in the mix (several jobs resident, their chunks overlapping on separate
+  streams) every job's S-matrix equals the same job run alone, bit for bit;
+* with an HBM budget that holds only part of the jobs, admission keeps the
+  resident footprint within the budget and every job still completes with
+  the same S-matrix;
+* the one-job-at-a-time mode gives the same S-matrices;
+* latency percentiles and throughput are reported.
+"""
+
+import numpy as np
+import pytest
+
+import bench
+from magnelio_amd.serve import SweepServer
+from magnelio_amd.sweep import ScatteringSweep
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def jobs():
+    return bench.c5_jobs(n_jobs=5, seed=11, rate=20.0, cells=(16 ** 3, 40 ** 3), steps=(300, 700))
+
+
+@pytest.fixture(scope="module")
+def alone(jobs, gpu):
+    out = {}
+    for j in jobs:
+        mesh, ports, dt = j.build()
+        print(f"job {j.job_id} alone: dims {mesh.dims}, {sum(p.n_modes for p in ports)} channels, "
+              f"{j.n_steps} steps", flush=True)
+        sw = ScatteringSweep(mesh, ports, dt=dt, n_steps=j.n_steps, f_axis=j.f_axis, f_max=j.f_max)
+        out[j.job_id] = sw.run().s_params
+    return out
+
+
+def _same(a, b):
+    return (a.channels == b.channels and a.excitations == b.excitations
+            and np.array_equal(a.matrix, b.matrix, equal_nan=True))
+
+
+def test_mix_equals_alone(jobs, alone, gpu):
+    print("continuous mix", flush=True)
+    rep = SweepServer(max_active=4, chunk=64).run(jobs, mode="continuous")
+    assert rep.peak_resident >= 2
+    for r in rep.records:
+        assert _same(r.result, alone[r.job_id]), r.job_id
+    lat = rep.latencies()
+    assert (lat > 0).all() and rep.p99 >= rep.p50 > 0 and rep.jobs_per_s > 0
+    print(f"p50 {rep.p50:.3f} s  p99 {rep.p99:.3f} s  {rep.jobs_per_s:.2f} jobs/s  "
+          f"peak resident {rep.peak_resident}")
+
+
+def test_budget_admission(jobs, alone, gpu):
+    fps = sorted(r.footprint for r in SweepServer(max_active=8).run(jobs[:2]).records)
+    budget = int(1.2 * max(fps))
+    print(f"budget admission at {budget / 1e6:.0f} MB", flush=True)
+    rep = SweepServer(hbm_budget=budget, max_active=8, chunk=64).run(jobs)
+    assert rep.peak_bytes <= budget
+    for r in rep.records:
+        assert r.footprint <= budget
+        assert _same(r.result, alone[r.job_id]), r.job_id
+
+
+def test_sequential_mode(jobs, alone, gpu):
+    rep = SweepServer(chunk=128).run(jobs, mode="sequential")
+    assert rep.peak_resident == 1
+    for r in rep.records:
+        assert _same(r.result, alone[r.job_id]), r.job_id

@@ -1,0 +1,86 @@
+"""Device-built material matrices and coefficients equal the host
+construction bit for bit (mgh_build_coefficients vs mesh.build_m_* +
+update_coefficients), FP32 and FP64:
+
+* seeded non-uniform grid with a PEC brick (OR rule), a lossy dielectric
+  (sigma), a magnetic lossy material (mu, sigma_m) and PEC walls on four
+  faces, modal ports on zmin and zmax (port-plane flattening of M_eps, M_mu
+  and the PEC mask);
+* the same materials with ports on xmin / xmax and walls on y and z;
+* no ports, walls on all six faces.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from magnelio_amd import mesh as mm
+from magnelio_amd import ports as pp
+from magnelio_amd.solver import FITTimeDomainSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _axes(n, seed, port_axis):
+    """Seeded non-uniform spacings; along the port axis the three cells at
+    each end are equal (the port feed rule)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for a, k in enumerate(n):
+        d = rng.uniform(0.8e-3, 1.2e-3, k)
+        if a == port_axis:
+            d[:3] = d[-3:] = 1e-3
+        out.append(np.concatenate([[0.0], np.cumsum(d)]))
+    return out
+
+
+def _mesh(port_axis, walls):
+    n = (22, 14, 18) if port_axis == 2 else (18, 14, 16)
+    x, y, z = _axes(n, 3, port_axis)
+    L = [a[-1] for a in (x, y, z)]
+
+    def box(f0, f1):
+        return tuple(f * l for f, l in zip(f0, L)) + tuple(f * l for f, l in zip(f1, L))
+
+    regions = [
+        (mm.Material.pec(), box((0.35, 0.3, 0.4), (0.55, 0.6, 0.55))),
+        (mm.Material.dielectric(4.2, sigma=0.3), box((0.2, 0.1, 0.3), (0.5, 0.5, 0.7))),
+        (mm.Material(name="ferrite", epsilon=(2.0, 3.0, 4.0), mu=(3.0, 2.5, 2.0), sigma=(0.0, 0.1, 0.2),
+                     sigma_m=(50.0, 20.0, 10.0)), box((0.6, 0.5, 0.35), (0.85, 0.9, 0.65))),
+    ]
+    return mm.Mesh.from_grid(mm.GridLines(x, y, z), regions=regions, pec_faces=walls)
+
+
+CASES = {
+    "z_ports": (2, ("xmin", "xmax", "ymin", "ymax"), ("zmin", "zmax")),
+    "x_ports": (0, ("ymin", "ymax", "zmin", "zmax"), ("xmin", "xmax")),
+    "no_ports": (2, mm.FACES, ()),
+}
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_device_coefficients_equal_host(case, precision, gpu):
+    axis, walls, faces = CASES[case]
+    mesh = _mesh(axis, walls)
+    dt = mm.courant_dt(mesh.grid, "normal")
+    me, mu = mm.build_m_eps(mesh), mm.build_m_mu(mesh)
+    ports = [pp.build_port(f"p{i}", f, mesh, me, mu, dt=dt, f_calc=60e9, n_modes=1)
+             for i, f in enumerate(faces)]
+    print(f"coefficients {case} {precision}", flush=True)
+    sols = {}
+    for how in ("host", "device"):
+        s = FITTimeDomainSolver(mesh=mesh, ports=ports, total_time_steps=4, dt=dt, precision=precision,
+                                coefficients=how)
+        s.setup()
+        sols[how] = s
+    assert sols["device"]._coeffs is None and sols["host"]._coeffs is not None
+    for name in ("alpha_e", "beta_e", "alpha_h", "beta_h", "m_eps_dev", "m_mu_dev"):
+        a, b = getattr(sols["host"], name), getattr(sols["device"], name)
+        assert a.dtype == b.dtype and a.shape == b.shape
+        bad = int((a != b).sum())
+        assert bad == 0, f"{name}: {bad} entries differ"
+    # the case exercises what it claims: PEC edges, lossy and frozen entries
+    be = sols["device"].beta_e
+    assert int((be == 0).sum()) > 0 and int((sols["device"].alpha_h < 1).sum()) > 0
+    torch.cuda.synchronize()

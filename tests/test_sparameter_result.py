@@ -35,3 +35,33 @@ def test_validation():
     other = SParameterResult(F * 2, CH, (("p1", 1),), np.zeros((3, 3, 1)))
     with pytest.raises(ValueError):
         SParameterResult.merge([_col(("p1", 0), 1.0), other])
+
+
+def test_footprint_estimate_scales():
+    from magnelio_amd.serve import footprint_bytes
+
+    a = footprint_bytes((32, 16, 64), 3, 3, 1000)
+    b = footprint_bytes((32, 16, 64), 3, 6, 1000)
+    c = footprint_bytes((64, 32, 128), 3, 3, 1000)
+    assert a < b < c
+    # fields dominate at size: K = 6 sets of E and H in float32
+    n = 64 * 33 * 129 * 3 * 2
+    assert c > 4 * n * 3
+
+
+def test_device_coefficient_eligibility():
+    from magnelio_amd import mesh as mm
+    from magnelio_amd.solver import FITTimeDomainSolver, _face_bits
+
+    assert _face_bits(("xmin", "zmax")) == 1 + 32
+    g = mm.GridLines(np.linspace(0, 1, 5), np.linspace(0, 1, 4), np.linspace(0, 1, 6))
+    plain = mm.Mesh.from_grid(g)
+    assert FITTimeDomainSolver(mesh=plain)._device_coefficients()
+    assert not FITTimeDomainSolver(mesh=plain, coefficients="host")._device_coefficients()
+    disp = mm.Mesh.from_grid(g, regions=[(mm.Material(poles=((-1e9, 1e9),)), (0, 0, 0, 1, 1, 1))])
+    assert not FITTimeDomainSolver(mesh=disp)._device_coefficients()
+    with pytest.raises(ValueError):
+        FITTimeDomainSolver(mesh=disp, coefficients="device")._device_coefficients()
+    edited = mm.Mesh.from_grid(g)
+    edited.pec_rule = False
+    assert not FITTimeDomainSolver(mesh=edited)._device_coefficients()
