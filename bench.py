@@ -489,12 +489,12 @@ def main_c5(args) -> int:
             "cells_max": int(max(r.cells for r in recs)),
             "cell_updates": float(sum(r.cell_updates for r in recs)),
         },
-        "latency_s": {"p50": c.p50, "p99": c.p99},
+        "latency_s": {"p50": c.p50, "p99": c.p99, **c.breakdown()},
         "gcell_per_s": c.gcell_per_s,
         "peak_resident": c.peak_resident,
         "peak_footprint_gb": round(c.peak_bytes / 1e9, 2),
         "one_job_at_a_time": {"jobs_per_s": q.jobs_per_s, "p50": q.p50, "p99": q.p99,
-                              "gcell_per_s": q.gcell_per_s},
+                              "gcell_per_s": q.gcell_per_s, **q.breakdown()},
         "results_identical": bool(same),
     }
     print(json.dumps(out))

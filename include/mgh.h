@@ -62,6 +62,26 @@ int mgh_update_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
                  long long e_set_stride, void* h, long long h_set_stride, const void* alpha_h,
                  const void* beta_h, int kchunk, void* stream);
 
+/* ---- (a5) listed launches of the curl phases -------------------------------
+ * Replace the live-block launches of update_E_fused_cuda / update_H_fused_cuda
+ * (src/magnelio/_operators/numba_kernels.py:249-262, :459-474) driven by the
+ * dead-tile plan (src/magnelio/solver/_tile_skip.py:187-256).  Same update as
+ * mgh_update_e / mgh_update_h over the listed launch units only: a unit is
+ * 256 consecutive nodes (i, j) of the x-fastest node plane (tile
+ * tp = (j (nx+1) + i) / 256) times kchunk z planes (kc = k / kchunk), id
+ * kc * ceil((nx+1)(ny+1)/256) + tp; live: device int32 ids (ascending),
+ * n_live of them (0 launches nothing; live = NULL with n_live = 0 runs every
+ * unit).  Skipped units must hold only elements whose update leaves them
+ * unchanged (dead E edges already zero, no-op or lossless curl-dead H). */
+int mgh_update_e_tiles(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                       long long e_set_stride, const void* h, long long h_set_stride,
+                       const void* alpha_e, const void* beta_e, int curl64, int kchunk,
+                       const int* live, int n_live, void* stream);
+int mgh_update_h_tiles(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                       long long e_set_stride, void* h, long long h_set_stride,
+                       const void* alpha_h, const void* beta_h, int kchunk, const int* live,
+                       int n_live, void* stream);
+
 /* ---- (a3)+(a7) / (a4)+(a7): curl phases with CPML ------------------------
  * Same as mgh_update_e / mgh_update_h, followed by the CPML auxiliary
  * recursion of every listed face over its slab, in record order (one small

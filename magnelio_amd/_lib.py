@@ -83,6 +83,16 @@ SIGNATURES: dict[str, tuple] = {
          _c_double, _c_double, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
          _c_void_p, _c_void_p, _c_void_p],
     ),
+    "mgh_update_e_tiles": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p],
+    ),
+    "mgh_update_h_tiles": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p],
+    ),
     "mgh_ade_save": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p],

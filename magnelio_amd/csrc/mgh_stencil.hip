@@ -60,6 +60,7 @@ struct Plan {
   int nx, ny, nz;
   int kchunk, ngroups;
   int ntp, nkc, units, per_xcd, remap;
+  const int* live;  // live spatial units (kc * ntp + tp), or null for all
 };
 
 __device__ __forceinline__ bool unit_of(const Plan& p, int& grp, int& i, int& j, int& k0,
@@ -68,7 +69,9 @@ __device__ __forceinline__ bool unit_of(const Plan& p, int& grp, int& i, int& j,
   const int u = p.remap ? (blk & 7) * p.per_xcd + (blk >> 3) : blk;
   if (u >= p.units) return false;
   grp = u % p.ngroups;
-  const int t = u / p.ngroups;
+  const int t0 = u / p.ngroups;
+  const int t = p.live ? p.live[t0] : t0;
+  if (t < 0 || t >= p.ntp * p.nkc) return false;
   const int tp = t % p.ntp;
   const int kc = t / p.ntp;
   const int q = tp * WG + (int)threadIdx.x;
@@ -519,7 +522,8 @@ __global__ __launch_bounds__(WG) void k_cpml_h(const T* __restrict__ e, T* __res
 
 constexpr int kSetsPerThread = 2;
 
-Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns) {
+Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, const int* live = nullptr,
+               int nlive = 0) {
   Plan p;
   std::memset(&p, 0, sizeof(p));
   p.nx = nx;
@@ -529,7 +533,8 @@ Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns) {
   p.ngroups = (nsets + ns - 1) / ns;
   p.ntp = (int)(((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG);
   p.nkc = (nz + 1 + kchunk - 1) / kchunk;
-  p.units = p.ntp * p.nkc * p.ngroups;
+  p.live = live;
+  p.units = (live ? nlive : p.ntp * p.nkc) * p.ngroups;
   p.per_xcd = (p.units + 7) / 8;
   p.remap = p.ngroups > 1;
   return p;
@@ -557,9 +562,11 @@ int64_t slab_threads(int axis, int nx, int ny, int nz, int n) {
 
 template <typename T, bool C64>
 int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny, int nz,
-          int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st) {
+          int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
+          int nlive = 0) {
   const int ns = nsets >= 2 ? kSetsPerThread : 1;
-  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns);
+  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
+  if (pl.units == 0) return MGH_OK;
   if (ns == kSetsPerThread)
     hipLaunchKernelGGL((k_update_e<T, kSetsPerThread, C64>), dim3(plan_blocks(pl)), dim3(WG), 0, st,
                        (T*)e, (const T*)h, (const T*)ae, (const T*)be, pl, nsets, es, hs);
@@ -594,9 +601,11 @@ int run_cpml_e(void* e, const void* h, const void* be, int nx, int ny, int nz, i
 
 template <typename T>
 int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny, int nz,
-          int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st) {
+          int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
+          int nlive = 0) {
   const int ns = nsets >= 2 ? kSetsPerThread : 1;
-  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns);
+  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
+  if (pl.units == 0) return MGH_OK;
   if (ns == kSetsPerThread)
     hipLaunchKernelGGL((k_update_h<T, kSetsPerThread>), dim3(plan_blocks(pl)), dim3(WG), 0, st,
                        (const T*)e, (T*)h, (const T*)ah, (const T*)bh, pl, nsets, es, hs);
@@ -677,6 +686,43 @@ int mgh_update_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
                         h_set_stride, st);
   return run_h<double>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
                        h_set_stride, st);
+}
+
+int mgh_update_e_tiles(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                       long long e_set_stride, const void* h, long long h_set_stride,
+                       const void* alpha_e, const void* beta_e, int curl64, int kchunk,
+                       const int* live, int n_live, void* stream) {
+  int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && alpha_e && beta_e && (live || n_live == 0), "null device pointer");
+  MGH_REQUIRE(!(curl64 && dtype == 8), "curl64 applies to float32 fields only");
+  const int64_t all = (((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG) * ((nz + kchunk) / kchunk);
+  MGH_REQUIRE(n_live >= 0 && n_live <= all, "n_live %d outside 0..%lld", n_live, (long long)all);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return curl64 ? run_e<float, true>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk,
+                                        e_set_stride, h_set_stride, st, live, n_live)
+                  : run_e<float, false>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk,
+                                         e_set_stride, h_set_stride, st, live, n_live);
+  return run_e<double, false>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk, e_set_stride,
+                              h_set_stride, st, live, n_live);
+}
+
+int mgh_update_h_tiles(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                       long long e_set_stride, void* h, long long h_set_stride,
+                       const void* alpha_h, const void* beta_h, int kchunk, const int* live,
+                       int n_live, void* stream) {
+  int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && alpha_h && beta_h && (live || n_live == 0), "null device pointer");
+  const int64_t all = (((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG) * ((nz + kchunk) / kchunk);
+  MGH_REQUIRE(n_live >= 0 && n_live <= all, "n_live %d outside 0..%lld", n_live, (long long)all);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return run_h<float>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
+                        h_set_stride, st, live, n_live);
+  return run_h<double>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
+                       h_set_stride, st, live, n_live);
 }
 
 int mgh_cpml_e(int dtype, int nx, int ny, int nz, int nsets, void* e, long long e_set_stride,

@@ -61,11 +61,13 @@ def _cpml_desc(cpml, nsets: int, dtype) -> int | None:
 
 def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: torch.Tensor,
              dims: tuple[int, int, int], *, curl64: bool = False, kchunk: int = DEFAULT_KCHUNK,
-             cpml=None, stream: torch.cuda.Stream | None = None) -> None:
+             cpml=None, live: torch.Tensor | None = None,
+             stream: torch.cuda.Stream | None = None) -> None:
     """E phase on K field sets: ``e`` is (K, n_E) or (n_E,), ``h`` likewise.
 
     ``cpml`` is a :class:`magnelio_amd.boundaries.DeviceCPML` (psi slabs for
-    the same K sets) or None."""
+    the same K sets) or None; ``live``: int32 device list of the launch
+    units to run (``tileskip.build_tile_plan``), None for all."""
     nx, ny, nz = dims
     ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
     e2 = e.view(-1, ne)
@@ -80,6 +82,14 @@ def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: to
         if t.dtype != e.dtype:
             raise TypeError("fields and coefficients must share one dtype")
     _check_dev(e, h, alpha_e, beta_e)
+    if live is not None:
+        if cpml is not None:
+            raise ValueError("listed launches take the CPML passes separately (cpml_e)")
+        _check_live(live, e)
+        _lib.call("mgh_update_e_tiles", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+                  alpha_e.data_ptr(), beta_e.data_ptr(), int(bool(curl64)), int(kchunk),
+                  live.data_ptr(), live.numel(), stream_handle(stream))
+        return
     desc = _cpml_desc(cpml, k, e.dtype)
     _lib.call("mgh_update_e_pml", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
               alpha_e.data_ptr(), beta_e.data_ptr(), int(bool(curl64)), int(kchunk), desc,
@@ -114,7 +124,7 @@ def cpml_h(e: torch.Tensor, h: torch.Tensor, beta_h: torch.Tensor, dims: tuple[i
 
 def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: torch.Tensor,
              dims: tuple[int, int, int], *, kchunk: int = DEFAULT_KCHUNK, cpml=None,
-             stream: torch.cuda.Stream | None = None) -> None:
+             live: torch.Tensor | None = None, stream: torch.cuda.Stream | None = None) -> None:
     """H phase on K field sets (same conventions as :func:`update_e`)."""
     nx, ny, nz = dims
     ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
@@ -130,9 +140,24 @@ def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: to
         if t.dtype != h.dtype:
             raise TypeError("fields and coefficients must share one dtype")
     _check_dev(e, h, alpha_h, beta_h)
+    if live is not None:
+        if cpml is not None:
+            raise ValueError("listed launches take the CPML passes separately (cpml_h)")
+        _check_live(live, h)
+        _lib.call("mgh_update_h_tiles", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+                  alpha_h.data_ptr(), beta_h.data_ptr(), int(kchunk), live.data_ptr(), live.numel(),
+                  stream_handle(stream))
+        return
     desc = _cpml_desc(cpml, k, h.dtype)
     _lib.call("mgh_update_h_pml", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
               alpha_h.data_ptr(), beta_h.data_ptr(), int(kchunk), desc, stream_handle(stream))
+
+
+def _check_live(live: torch.Tensor, like: torch.Tensor) -> None:
+    if live.dtype != torch.int32 or live.dim() != 1 or not live.is_contiguous():
+        raise TypeError("live must be a contiguous 1-D int32 tensor")
+    if live.device != like.device:
+        raise ValueError("live list and fields must be on the same device")
 
 
 class SqSum:

@@ -201,8 +201,10 @@ class DeviceLumped:
         self._keep = []
         self.state, self.V_rec, self.I_rec = [], [], []
         t = np.arange(n_steps, dtype=np.float64) * dt + dt  # update_e(fields, t + dt)
+        edge_lists = []
         for q, op in enumerate(self.ops):
             idx = layout.index_to_device(op.flat, *dims, "E")
+            edge_lists.append(idx)
             exc = np.zeros(K, dtype=np.int32)
             src = np.zeros((K, n_steps), dtype=np.float64)
             for s, ch in enumerate(solver.excitations):
@@ -249,6 +251,8 @@ class DeviceLumped:
             w[10] = ir.data_ptr() if ir is not None else 0
             w[11] = n_steps if vr is not None else 0
         self.dtype_code = 4 if solver.e.dtype == torch.float32 else 8
+        # device-layout E edges the elements write (kept live by dead-unit skipping)
+        self.edges = np.concatenate(edge_lists) if edge_lists else np.zeros(0, np.int64)
 
     def step(self, solver, stream) -> None:
         if not self.ops:

@@ -66,6 +66,7 @@ def footprint_bytes(dims, n_channels: int, K: int, n_steps: int, precision: str 
 class JobRecord:
     job_id: int
     arrival: float
+    prepared: float = 0.0  # host preparation done (worker thread)
     admitted: float = 0.0
     finished: float = 0.0
     cells: int = 0
@@ -93,6 +94,15 @@ class ServeReport:
 
     def latencies(self) -> np.ndarray:
         return np.array([r.latency for r in self.records])
+
+    def breakdown(self) -> dict:
+        """Median seconds per phase: host preparation, wait for admission,
+        device run (admission to S-matrix)."""
+        prep = np.array([r.prepared - r.arrival for r in self.records])
+        wait = np.array([r.admitted - r.prepared for r in self.records])
+        run = np.array([r.finished - r.admitted for r in self.records])
+        return {"prep_p50": float(np.median(prep)), "wait_p50": float(np.median(wait)),
+                "run_p50": float(np.median(run)), "run_p99": float(np.percentile(run, 99))}
 
     @property
     def p50(self) -> float:
@@ -147,7 +157,7 @@ class SweepServer:
         free, _ = torch.cuda.mem_get_info(self.device)
         return int(0.85 * free)
 
-    def _prepare(self, job: SweepJob, rec: JobRecord):
+    def _prepare(self, job: SweepJob, rec: JobRecord, t0: float):
         """Worker-thread half of a job: mesh, ports, material matrices and
         coefficients on the host (no device access)."""
         mesh, ports, dt = job.build()
@@ -160,6 +170,7 @@ class SweepServer:
         rec.K = len(sw.excited)
         rec.n_steps = job.n_steps
         rec.footprint = footprint_bytes(mesh.dims, len(sw.channels), rec.K, job.n_steps, job.precision)
+        rec.prepared = time.perf_counter() - t0
         return sw, solver
 
     @staticmethod
@@ -191,7 +202,7 @@ class SweepServer:
                 # arrivals start their host-side build
                 while arrivals and arrivals[0][0] <= now:
                     _, jid, job = heapq.heappop(arrivals)
-                    futures[jid] = (job, pool.submit(self._prepare, job, recs[jid]))
+                    futures[jid] = (job, pool.submit(self._prepare, job, recs[jid], t0))
                 # prepared jobs join the admission queue in arrival order
                 for jid in sorted(futures, key=lambda i: (recs[i].arrival, i)):
                     job, fut = futures[jid]

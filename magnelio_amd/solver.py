@@ -41,6 +41,7 @@ from . import _lib, kernels, layout
 from . import mesh as _mesh
 from .dispersion import DeviceDispersion
 from .lumped import DeviceLumped
+from .tileskip import build_tile_plan
 from .boundaries import CPMLBoundary, DeviceCPML, PECBoundary, PMCBoundary
 from .ports import PortModel
 
@@ -234,6 +235,7 @@ class FITTimeDomainSolver:
     use_graphs: bool = False
     graph_block: int = 1  # minimum steps per captured graph
     coefficients: str = "auto"  # alpha/beta built on the "device", the "host", or "auto"
+    tile_skip: str = "auto"  # dead-unit skipping: "auto" (when it skips >= 2 %), "on", "off"
     device: str = "cuda:0"
 
     def __post_init__(self):
@@ -419,7 +421,23 @@ class FITTimeDomainSolver:
         self.dlumped = DeviceLumped(self.lumped, self) if self.lumped else None
         self._sqsum = kernels.SqSum(K, dev)
         self._energy = torch.zeros(K, dtype=torch.float64, device=dev)
+        self.tile_plan = self._build_tile_plan()
         self._setup_done = True
+
+    def _build_tile_plan(self):
+        """Live-unit lists of the curl launches (tileskip), or None for dense
+        launches: off on request and with TF/SF sources (their H injection
+        reaches faces inside metal)."""
+        if self.tile_skip not in ("auto", "on", "off"):
+            raise ValueError(f"tile_skip must be 'auto', 'on' or 'off', got {self.tile_skip!r}")
+        if self.tile_skip == "off" or self.sources:
+            return None
+        force = self.dlumped.edges if self.dlumped is not None else None
+        plan = build_tile_plan(self.alpha_e, self.beta_e, self.alpha_h, self.beta_h, self.dims,
+                               self.kchunk, force_live_e=force)
+        if self.tile_skip == "auto" and max(plan.skipped_e, plan.skipped_h) < 0.02:
+            return None
+        return plan
 
     # ------------------------------------------------------------------
     def _step(self, n: int) -> None:
@@ -432,8 +450,9 @@ class FITTimeDomainSolver:
             self._dispersion.save(self, stream)
         if ev is not None:
             ev[0].record(stream)
+        tp = self.tile_plan
         kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
-                         kchunk=self.kchunk, stream=stream)
+                         kchunk=self.kchunk, live=tp.live_e if tp is not None else None, stream=stream)
         if ev is not None:
             ev[1].record(stream)
             self.timed_events.append(ev)
@@ -451,7 +470,7 @@ class FITTimeDomainSolver:
         if self.dlumped is not None:
             self.dlumped.step(self, stream)
         kernels.update_h(self.e, self.h, self.alpha_h, self.beta_h, dims, kchunk=self.kchunk,
-                         stream=stream)
+                         live=tp.live_h if tp is not None else None, stream=stream)
         if self.cpml is not None:
             kernels.cpml_h(self.e, self.h, self.beta_h, dims, self.cpml, stream=stream)
         for src in self.sources:
@@ -676,6 +695,9 @@ class FITTimeDomainSolver:
             self.dlumped.load_state_dict(sd["lumped"])
         if self._dispersion is not None:
             self._dispersion.load_state_dict(sd["dispersion"])
+        if self.tile_plan is not None:
+            # dead E edges: zero, as the dense launch leaves them at its next E update
+            self.e.masked_fill_(self.tile_plan.zero_e.unsqueeze(0), 0)
         self.step_ctr.fill_(n)
         self._resume_step = n
         self._peak_energy = float(sd.get("peak_energy", 0.0))
