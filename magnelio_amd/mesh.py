@@ -294,13 +294,73 @@ def _face_mass(mesh: Mesh, prop: str, scale: float, pec_value=None) -> np.ndarra
 
 
 def build_m_eps(mesh: Mesh) -> np.ndarray:
-    """M_eps = EPS0 eps_r A_dual / L_primal (material_matrices.py:334-464, cat 0)."""
-    return _edge_mass(mesh, "epsilon", EPS0)
+    """M_eps = EPS0 eps_r A_dual / L_primal (material_matrices.py:334-464, cat 0),
+    with the thin-wire category-1 edges of the mesh (thinwire.py)."""
+    m = _edge_mass(mesh, "epsilon", EPS0)
+    wc = getattr(mesh, "wire_corrections", None)
+    if wc is not None and wc.edge_idx.size:
+        from .thinwire import correct_m_eps  # noqa: PLC0415
+
+        geom = np.zeros(m.size)
+        geom[wc.edge_idx] = _edge_geom_at(mesh.grid, wc.edge_idx)
+        correct_m_eps(m, mesh, geom)
+    return m
 
 
 def build_m_mu(mesh: Mesh) -> np.ndarray:
-    """M_mu = MU0 mu_r A_primal / L_dual (material_matrices.py:494-613, cat 0)."""
-    return _face_mass(mesh, "mu", MU0)
+    """M_mu = MU0 mu_r A_primal / L_dual (material_matrices.py:494-613, cat 0),
+    with the thin-wire category-2 faces of the mesh (thinwire.py)."""
+    m = _face_mass(mesh, "mu", MU0)
+    wc = getattr(mesh, "wire_corrections", None)
+    if wc is not None and wc.face_idx.size:
+        from .thinwire import correct_m_mu  # noqa: PLC0415
+
+        mu_face, l_dual = np.ones(m.size), np.ones(m.size)
+        mu_face[wc.face_idx], l_dual[wc.face_idx] = _face_mu_ldual_at(mesh, wc.face_idx)
+        correct_m_mu(m, mesh, mu_face, l_dual)
+    return m
+
+
+def _edge_geom_at(grid: GridLines, idx: np.ndarray) -> np.ndarray:
+    """A_dual / L_primal of E edges (reference flat indices), as _edge_mass forms it."""
+    nx, ny, nz = grid.dims
+    ax, ay, az = _avg_d(grid.dx), _avg_d(grid.dy), _avg_d(grid.dz)
+    shp = [(nx, ny + 1, nz + 1), (nx + 1, ny, nz + 1), (nx + 1, ny + 1, nz)]
+    off = np.cumsum([0] + [int(np.prod(s)) for s in shp])
+    out = np.empty(idx.size)
+    for c in range(3):
+        sel = (idx >= off[c]) & (idx < off[c + 1])
+        i, j, k = np.unravel_index(idx[sel] - off[c], shp[c])
+        if c == 0:
+            out[sel] = ay[j] * az[k] / grid.dx[i]
+        elif c == 1:
+            out[sel] = ax[i] * az[k] / grid.dy[j]
+        else:
+            out[sel] = ax[i] * ay[j] / grid.dz[k]
+    return out
+
+
+def _face_mu_ldual_at(mesh: Mesh, idx: np.ndarray):
+    """(staircase mu_r of the owning cell, dual length) of H faces."""
+    g = mesh.grid
+    nx, ny, nz = g.dims
+    ax, ay, az = _avg_d(g.dx), _avg_d(g.dy), _avg_d(g.dz)
+    il, jl, kl = _clamp_idx(nx), _clamp_idx(ny), _clamp_idx(nz)
+    shp = [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+    off = np.cumsum([0] + [int(np.prod(s)) for s in shp])
+    mu, ld = np.empty(idx.size), np.empty(idx.size)
+    mid = mesh.material_id
+    for c in range(3):
+        sel = (idx >= off[c]) & (idx < off[c + 1])
+        i, j, k = np.unravel_index(idx[sel] - off[c], shp[c])
+        t = _table(mesh.material_library, "mu", c)
+        if c == 0:
+            mu[sel], ld[sel] = t[mid[il[i], j, k]], ax[i]
+        elif c == 1:
+            mu[sel], ld[sel] = t[mid[i, jl[j], k]], ay[j]
+        else:
+            mu[sel], ld[sel] = t[mid[i, j, kl[k]]], az[k]
+    return mu, ld
 
 
 def build_m_sigma(mesh: Mesh) -> np.ndarray:
