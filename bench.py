@@ -29,6 +29,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_update_e launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (bench.py --pmc-probe), FETCH scaled x2.0 and WRITE x1.0 from a 1 GiB device copy:
+# {(n, K): bytes}
+K_UPDATE_E_TRAFFIC = {(256, 1): 1.300e9}
 
 
 def c2_mesh(n: int = 256, seed: int = 2024, n_objects: int = 40, length: float = 64e-3):
@@ -168,6 +172,150 @@ def c5_jobs(n_jobs=200, seed=5, rate=1.5, cells=(32 ** 3, 384 ** 3), steps=(2000
     return jobs
 
 
+C4_SIZE = (32e-3, 20e-3, 48e-3)
+
+
+def lorentz_pole(d_eps, f0, gamma):
+    """Pole-residue pair (stored once) of d_eps w0^2 / (w0^2 - w^2 + j gamma w)."""
+    w0 = 2 * math.pi * f0
+    wd = math.sqrt(w0 * w0 - 0.25 * gamma * gamma)
+    return complex(-0.5 * gamma, wd), complex(0.0, -d_eps * w0 * w0 / (2.0 * wd))
+
+
+def c4_case(n=320, seed=4, f_max=20e9, n_modes=2):
+    """Precision study: a 32 x 20 x 48 mm PEC guide on an n^3 grid with a
+    seeded slab (z in [0.4, 0.6] L) of two-pole Lorentz material (eps_inf
+    U[2, 4], pole frequencies U[5, 20] GHz, seeded residues, damping f0/10)
+    with bulk conductivity 0.05 S/m, modal DTBC ports with n_modes modes on
+    zmin and zmax.  Every open end carries a port, so the DTBC is the
+    absorber and no CPML is placed."""
+    from magnelio_amd import mesh as mm
+    from magnelio_amd import ports as pp
+
+    rng = np.random.default_rng(seed)
+    W, H, L = C4_SIZE
+    grid = mm.GridLines(*(np.linspace(0.0, s, n + 1) for s in C4_SIZE))
+    poles = tuple(lorentz_pole(float(rng.uniform(0.5, 2.0)), f0, 2 * math.pi * f0 / 10)
+                  for f0 in rng.uniform(5e9, 20e9, 2))
+    eps_inf = float(rng.uniform(2.0, 4.0))
+    slab = mm.Material(name="lorentz", epsilon=(eps_inf,) * 3, sigma=(0.05,) * 3, poles=poles)
+    mesh = mm.Mesh.from_grid(grid, regions=[(slab, (0.0, 0.0, 0.4 * L, W, H, 0.6 * L))],
+                             pec_faces=("xmin", "xmax", "ymin", "ymax"))
+    dt = mm.courant_dt(grid, "normal")
+    ports = [pp.build_port_local(name, face, mesh, dt=dt, f_calc=f_max, n_modes=n_modes)
+             for name, face in (("port1", "zmin"), ("port2", "zmax"))]
+    return mesh, ports, dt, {"eps_inf": eps_inf, "poles_hz": [p[0].imag / (2 * math.pi) for p in poles]}
+
+
+def main_c4(args) -> int:
+    """FP32 fields and curls (FP64 DTBC history, port accumulators and
+    compensated DFT) against a full FP64 run of the same kernels."""
+    import torch
+
+    from magnelio_amd.monitors import MonitorFieldFrequency
+    from magnelio_amd.sweep import ScatteringSweep
+
+    t_setup = time.perf_counter()
+    mesh, ports, dt, info = c4_case(args.n if args.n != 256 else 320)
+    nx, ny, nz = mesh.dims
+    steps = args.steps if args.steps != 512 else 3000
+    f_axis = np.linspace(6e9, 19e9, 131)
+    kz = round(0.75 * nz)
+
+    def monitors():
+        return [MonitorFieldFrequency(np.linspace(6e9, 19e9, 64),
+                                      region=(slice(0, nx), slice(0, ny), slice(kz, kz + 1)),
+                                      fields=("E",), block=64, name="plane")]
+
+    excited = [("port1", 0), ("port1", 1)]
+    out_runs = {}
+    for prec in ("single", "double"):
+        sw = ScatteringSweep(mesh, ports, dt=dt, n_steps=steps, f_axis=f_axis, f_max=20e9,
+                             excited=excited, precision=prec, monitors_factory=monitors)
+        res = sw.run()
+        sol = sw.solvers[0]
+        out_runs[prec] = {"S": res.s_params.matrix, "seconds": res.seconds,
+                          "e": sol.e.double().cpu(), "h": sol.h.double().cpu(),
+                          "dft": [sol.monitors[0].result(k) for k in range(len(excited))]}
+        del sw, sol, res
+        torch.cuda.empty_cache()
+    a, b = out_runs["single"], out_runs["double"]
+
+    def rel(x, y):
+        return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
+
+    s_err = np.abs(a["S"] - b["S"])
+    finite = np.isfinite(s_err)
+    def stack(r):
+        return np.concatenate([np.asarray(r[c]).ravel() for c in sorted(r)])
+
+    dft_err = max(float(np.linalg.norm(stack(x) - stack(y)) / np.linalg.norm(stack(y)))
+                  for x, y in zip(a["dft"], b["dft"]))
+    cells = nx * ny * nz
+    out = {
+        "metric": "precision study: FP32 engine vs FP64 run of the same kernels (S-parameter and field error)",
+        "value": float(s_err[finite].max()),
+        "unit": "max |S_fp32 - S_fp64|",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": a["seconds"] / steps * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{nx}x{ny}x{nz} PEC guide 32x20x48 mm, seeded two-pole Lorentz slab with "
+                         "sigma 0.05 S/m (eps_inf %.2f, poles %.1f/%.1f GHz), DTBC ports on zmin/zmax "
+                         "x 2 modes, 2 excitations, %d steps, field-plane DFT 64 freqs compensated"
+                         % (info["eps_inf"], info["poles_hz"][0] / 1e9, info["poles_hz"][1] / 1e9, steps)),
+            "setup_s": round(time.perf_counter() - t_setup - a["seconds"] - b["seconds"], 1),
+        },
+        "s_err_db": float(20 * np.log10(s_err[finite].max())),
+        "field_rel_l2": {"e": rel(a["e"], b["e"]), "h": rel(a["h"], b["h"])},
+        "dft_rel_l2": dft_err,
+        "gcell_per_s": {"fp32": len(excited) * cells * steps / a["seconds"] / 1e9,
+                        "fp64": len(excited) * cells * steps / b["seconds"] / 1e9},
+    }
+    print(json.dumps(out))
+    return 0
+
+
+COPY_BYTES = 1 << 30
+
+
+def pmc_probe(args) -> int:
+    """Launches whose counters calibrate and measure HBM traffic: a device
+    copy of COPY_BYTES (1 GiB read + 1 GiB written) and `--steps` E/H curl
+    launches on the c2 grid (CPML, sources and monitors off)."""
+    import torch
+
+    from magnelio_amd import mesh as mm
+    from magnelio_amd.solver import FITTimeDomainSolver
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    mesh = c2_mesh(args.n)
+    dt = mm.courant_dt(mesh.grid, "normal")
+    sol = FITTimeDomainSolver(mesh=mesh, total_time_steps=args.steps + 2, dt=dt, record=False,
+                              tile_skip="off")
+    sol.setup()
+    sol.e.normal_().mul_((sol.beta_e != 0).float())
+    for i in range(args.steps):
+        sol._step(i)
+    torch.cuda.synchronize()
+    cells = int(np.prod(mesh.dims))
+    print(json.dumps({"copy_bytes": COPY_BYTES, "cells": cells,
+                      "alg_bytes_update_e": 4.0 * cells * (6 + 9),
+                      "alg_bytes_update_h": 4.0 * cells * (6 + 9), "launches": args.steps}))
+    return 0
+
+
 def cpu_baseline(mesh, coeffs, nsamp=(10, 20, 40)):
     """Oracle C/OpenMP restatement of the Numba kernels on the same arrays."""
     from oracle import build as ob
@@ -227,7 +375,10 @@ def main(argv=None) -> int:
     ap.add_argument("--dft-plain", action="store_true", help="uncompensated float64 bins")
     ap.add_argument("--no-tfsf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c3", "c5"), default="c2",
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="a short stencil-only run of the c2 grid plus a 1 GiB device copy, "
+                         "for HBM traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: single large job (default line); c3: batched K=6 sweep vs one "
                          "excitation per launch; c5: serving loop vs one job at a time")
     ap.add_argument("--jobs", type=int, default=200, help="c5: number of sweep jobs")
@@ -240,10 +391,14 @@ def main(argv=None) -> int:
         print("bench.py measures one GPU only: this engine advances one job per device "
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.pmc_probe:
+        return pmc_probe(args)
     if args.workload == "c3":
         return main_c3(args)
     if args.workload == "c5":
         return main_c5(args)
+    if args.workload == "c4":
+        return main_c4(args)
 
     import torch
 
@@ -337,7 +492,7 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": K_UPDATE_E_TRAFFIC.get((args.n, K)),
             "kernel": "k_update_e",
             "kernel_ms": e_ms,
             "alg_bytes_per_launch": bytes_e,
