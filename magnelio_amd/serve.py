@@ -22,6 +22,7 @@ alone bit for bit.
 from __future__ import annotations
 
 import heapq
+import sys
 import time
 from collections import deque
 from concurrent.futures import ThreadPoolExecutor
@@ -35,9 +36,27 @@ from .sweep import SParameterResult, ScatteringSweep
 
 
 @dataclass
+class BuildPlan:
+    """A job's host build split into tasks that a pool can run apart.
+
+    ``tasks``: {name: (cache_key or None, callable)}; tasks with equal
+    non-None keys are run once per server and their result shared (a port
+    model depends only on its feed cross-section, not on the job).
+    ``assemble({name: result})`` returns (mesh, ports, dt).  The callables
+    must be picklable for a process pool."""
+
+    tasks: dict
+    assemble: object
+
+    def __call__(self):
+        return self.assemble({name: fn() for name, (_, fn) in self.tasks.items()})
+
+
+@dataclass
 class SweepJob:
-    """One sweep request.  ``build()`` returns (mesh, ports, dt) on the host
-    (mode solving included); it runs in a worker thread before admission."""
+    """One sweep request.  ``build()`` (a callable or a :class:`BuildPlan`)
+    returns (mesh, ports, dt) on the host (mode solving included); it runs in
+    worker threads or processes before admission."""
 
     job_id: int
     arrival: float  # seconds after the start of the run
@@ -141,7 +160,7 @@ class SweepServer:
 
     def __init__(self, device: str = "cuda:0", hbm_budget: int | None = None, max_active: int = 8,
                  chunk: int = 256, inflight: int = 2, build_workers: int = 2,
-                 use_graphs: bool = True):
+                 use_graphs: bool = True, build_pool=None, progress: int = 0):
         self.device = torch.device(device)
         self.hbm_budget = hbm_budget
         self.max_active = int(max_active)
@@ -149,6 +168,10 @@ class SweepServer:
         self.inflight = int(inflight)
         self.build_workers = int(build_workers)
         self.use_graphs = use_graphs
+        # an executor for job.build (e.g. a process pool: the builds are CPU
+        # bound and hold the GIL); None = a thread pool of build_workers
+        self.build_pool = build_pool
+        self.progress = int(progress)  # report every `progress` finished jobs (0: never)
 
     # -- helpers ---------------------------------------------------------
     def _budget(self) -> int:
@@ -157,10 +180,11 @@ class SweepServer:
         free, _ = torch.cuda.mem_get_info(self.device)
         return int(0.85 * free)
 
-    def _prepare(self, job: SweepJob, rec: JobRecord, t0: float):
-        """Worker-thread half of a job: mesh, ports, material matrices and
-        coefficients on the host (no device access)."""
-        mesh, ports, dt = job.build()
+    def _prepare(self, job: SweepJob, rec: JobRecord, t0: float, built=None):
+        """Host half of a job: mesh, ports, material matrices and
+        coefficients on the host (no device access).  ``built``: the result
+        of ``job.build()`` when it already ran elsewhere."""
+        mesh, ports, dt = built if built is not None else job.build()
         sw = ScatteringSweep(mesh, ports, dt=dt, n_steps=job.n_steps, f_axis=job.f_axis,
                              f_max=job.f_max, waveforms=job.waveforms, precision=job.precision,
                              use_graphs=self.use_graphs, device=str(self.device))
@@ -172,6 +196,27 @@ class SweepServer:
         rec.footprint = footprint_bytes(mesh.dims, len(sw.channels), rec.K, job.n_steps, job.precision)
         rec.prepared = time.perf_counter() - t0
         return sw, solver
+
+    @staticmethod
+    def _submit_build(executor, build, shared: dict) -> dict:
+        """Futures of a job's build tasks; keyed tasks are shared server-wide."""
+        if not isinstance(build, BuildPlan):
+            return {None: executor.submit(build)}
+        futs = {}
+        for name, (key, fn) in build.tasks.items():
+            if key is None:
+                futs[name] = executor.submit(fn)
+            else:
+                if key not in shared:
+                    shared[key] = executor.submit(fn)
+                futs[name] = shared[key]
+        return futs
+
+    @staticmethod
+    def _assemble(build, futs: dict):
+        if not isinstance(build, BuildPlan):
+            return futs[None].result()
+        return build.assemble({name: f.result() for name, f in futs.items()})
 
     @staticmethod
     def _finish(a: _Active) -> SParameterResult:
@@ -187,7 +232,9 @@ class SweepServer:
         order = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
         recs = {j.job_id: JobRecord(j.job_id, j.arrival) for j in order}
         pool = ThreadPoolExecutor(max_workers=max(1, self.build_workers))
+        external = self.build_pool
         futures = {}
+        shared: dict = {}  # cache key -> future of a shared build task
         arrivals = [(j.arrival, j.job_id, j) for j in order]
         heapq.heapify(arrivals)
         waiting: list[tuple] = []  # (job, sweep, solver prepared on the host)
@@ -202,11 +249,19 @@ class SweepServer:
                 # arrivals start their host-side build
                 while arrivals and arrivals[0][0] <= now:
                     _, jid, job = heapq.heappop(arrivals)
-                    futures[jid] = (job, pool.submit(self._prepare, job, recs[jid], t0))
+                    if external is not None:
+                        futures[jid] = (job, self._submit_build(external, job.build, shared))
+                    else:
+                        futures[jid] = (job, pool.submit(self._prepare, job, recs[jid], t0))
                 # prepared jobs join the admission queue in arrival order
                 for jid in sorted(futures, key=lambda i: (recs[i].arrival, i)):
                     job, fut = futures[jid]
-                    if fut.done():
+                    if external is not None:
+                        if all(f.done() for f in fut.values()):
+                            built = self._assemble(job.build, fut)
+                            waiting.append((job, *self._prepare(job, recs[jid], t0, built)))
+                            del futures[jid]
+                    elif fut.done():
                         waiting.append((job, *fut.result()))
                         del futures[jid]
                 waiting.sort(key=lambda x: (x[0].arrival, x[0].job_id))
@@ -258,6 +313,9 @@ class SweepServer:
                         del a
                         done += 1
                         progressed = True
+                        if self.progress and done % self.progress == 0:
+                            print(f"serve[{mode}]: {done}/{len(order)} jobs done at "
+                                  f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
                 if not progressed:
                     time.sleep(2e-4)
         finally:
