@@ -31,8 +31,8 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_update_e launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (bench.py --pmc-probe), FETCH scaled x2.0 and WRITE x1.0 from a 1 GiB device copy:
-# {(n, K): bytes}
-K_UPDATE_E_TRAFFIC = {(256, 1): 1.300e9}
+# {(n, K, CPML fused into the launch): bytes}
+K_UPDATE_E_TRAFFIC = {(256, 1, False): 1.300e9}
 
 
 def c2_mesh(n: int = 256, seed: int = 2024, n_objects: int = 40, length: float = 64e-3):
@@ -315,7 +315,8 @@ COPY_BYTES = 1 << 30
 def pmc_probe(args) -> int:
     """Launches whose counters calibrate and measure HBM traffic: a device
     copy of COPY_BYTES (1 GiB read + 1 GiB written) and `--steps` E/H curl
-    launches on the c2 grid (CPML, sources and monitors off)."""
+    launches on the c2 grid with its 8-layer CPML on six faces inside them
+    (sources and monitors off)."""
     import torch
 
     from magnelio_amd import mesh as mm
@@ -327,10 +328,13 @@ def pmc_probe(args) -> int:
         dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
+    from magnelio_amd.boundaries import CPMLBoundary
+
     mesh = c2_mesh(args.n)
     dt = mm.courant_dt(mesh.grid, "normal")
-    sol = FITTimeDomainSolver(mesh=mesh, total_time_steps=args.steps + 2, dt=dt, record=False,
-                              tile_skip="off")
+    bcs = {f: CPMLBoundary(f, mesh.grid, thickness_cells=8) for f in mm.FACES}
+    sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, total_time_steps=args.steps + 2,
+                              dt=dt, record=False, tile_skip="off")
     sol.setup()
     sol.e.normal_().mul_((sol.beta_e != 0).float())
     for i in range(args.steps):
@@ -338,8 +342,9 @@ def pmc_probe(args) -> int:
     torch.cuda.synchronize()
     cells = int(np.prod(mesh.dims))
     print(json.dumps({"copy_bytes": COPY_BYTES, "cells": cells,
-                      "alg_bytes_update_e": 4.0 * cells * (6 + 9),
-                      "alg_bytes_update_h": 4.0 * cells * (6 + 9), "launches": args.steps}))
+                      "alg_bytes_update_e": 4.0 * cells * (6 + 9) + sol.cpml.slab_bytes("E"),
+                      "alg_bytes_update_h": 4.0 * cells * (6 + 9) + sol.cpml.slab_bytes("H"),
+                      "launches": args.steps}))
     return 0
 
 
@@ -483,8 +488,10 @@ def main(argv=None) -> int:
     cells = nx * ny * nz
     K = args.k
     rate = K * cells * args.steps / wall / 1e9
-    # E phase: 6 coefficient words shared by the K sets + K x (read E, read H, write E)
-    bytes_e = 4.0 * cells * (6 + 9 * K)
+    # E phase: 6 coefficient words shared by the K sets + K x (read E, read H, write E),
+    # plus the psi words (read + written) of the CPML slabs the launch carries
+    fused = sol.cpml is not None and sol.tile_plan is None and sol._dispersion is None
+    bytes_e = 4.0 * cells * (6 + 9 * K) + (sol.cpml.slab_bytes("E") if fused else 0)
     achieved = bytes_e / (e_ms * 1e-3) / 1e9
     out = {
         "metric": "Yee Gcell-updates/s & % HBM roofline",
@@ -519,7 +526,7 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": K_UPDATE_E_TRAFFIC.get((args.n, K)),
+            "traffic": K_UPDATE_E_TRAFFIC.get((args.n, K, fused)),
             "kernel": "k_update_e",
             "kernel_ms": e_ms,
             "alg_bytes_per_launch": bytes_e,

@@ -84,8 +84,10 @@ int mgh_update_h_tiles(int dtype, int nx, int ny, int nz, int nsets, const void*
 
 /* ---- (a3)+(a7) / (a4)+(a7): curl phases with CPML ------------------------
  * Same as mgh_update_e / mgh_update_h, followed by the CPML auxiliary
- * recursion of every listed face over its slab, in record order (one small
- * launch per face on the same stream).  Replaces CPMLBoundary.update_E /
+ * recursion of every listed face over its slab, in record order — inside the
+ * same launch: a slab element applies its face terms to its updated value
+ * before the store (same values as mgh_update_* then mgh_cpml_*, bit for
+ * bit).  Replaces CPMLBoundary.update_E /
  * update_H (src/magnelio/boundaries/cpml.py:343-516), called right after the
  * curl update in e_phase_device / h_phase_device
  * (src/magnelio/solver/fit_td.py:870-876, :918-921).
@@ -248,6 +250,20 @@ int mgh_ade_update(int dtype, int nsets, void* e, long long e_set_stride, int n,
 int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, const void* h,
                    long long h_set_stride, const int* idx, const float* w, int S, float* X,
                    int S_pad, int B, const long long* step_ctr, long long n_start, void* stream);
+
+/* mgh_dft_sample_groups: the same row of X from structured groups — the
+ * samples of one monitor component on one face or box, whose terms are affine
+ * in the sample's natural coordinates (c0 fastest): term t reads
+ * field[base_t + c0 s0 + c1 s1 + c2 s2] with weight wtab[woff_t + c_{waxis_t}]
+ * (waxis_t < 0: a constant).  grp: G descriptors of 40 int32 words sorted by
+ * start — [0] start [1] count [2] n0 [3] n1 [4] field (0 E, 1 H) [5] terms
+ * [6..8] strides [9..16] bases [17..24] weight coordinates [25..32] weight
+ * offsets.  Built by the host from the mgh_dft_sample map and checked
+ * against it; the values are those of mgh_dft_sample bit for bit. */
+int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_stride,
+                          const void* h, long long h_set_stride, const int* grp, int G,
+                          const float* wtab, int S, float* X, int S_pad, int B,
+                          const long long* step_ctr, long long n_start, void* stream);
 
 /* mgh_dft_block: bins[set, f, s] += exp(j w_f t0) * sum_{b < nb_valid}
  * (Pc + j Ps)[f, b] * X[set, b, s], t0 = n0 dt for s < s_h_start and

@@ -112,6 +112,11 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int,
          _c_void_p, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p],
     ),
+    "mgh_dft_sample_groups": (
+        _c_int,
+        [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_int, _c_void_p, _c_int,
+         _c_void_p, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p],
+    ),
     "mgh_dft_block": (
         _c_int,
         [_c_int, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,

@@ -180,6 +180,7 @@ class DeviceCPML:
         self.dtype = dtype
         self.psi_e: list[list] = []
         self.psi_h: list[list] = []
+        self._air_bytes = [0, 0]  # uint8 air-mask words of the E / H terms
         self._keep = []
         desc = np.zeros(1 + self.WORDS * len(self.faces), dtype=np.int64)
         desc[0] = len(self.faces)
@@ -204,6 +205,8 @@ class DeviceCPML:
                        torch.from_numpy(np.concatenate(flat[2:4])).to(device)]
             self.psi_e.append(pe)
             self.psi_h.append(ph)
+            self._air_bytes[0] += air[0].numel() if air[0] is not None else 0
+            self._air_bytes[1] += air[1].numel() if air[1] is not None else 0
             self._keep += [coef, pe_buf, ph_buf] + [a for a in air if a is not None]
             r = desc[1 + fi * self.WORDS: 1 + (fi + 1) * self.WORDS]
             r[0], r[1], r[2] = f.axis, f.k0, f.n
@@ -217,6 +220,16 @@ class DeviceCPML:
     @property
     def desc_ptr(self) -> int:
         return int(self.desc.ctypes.data)
+
+    def slab_bytes(self, side: str) -> int:
+        """Bytes the CPML adds to one fused curl launch of ``side`` ("E" or
+        "H"): every psi word of every set read and written, the air masks
+        read once (the field, coefficient and neighbour words are the curl's
+        own; the layer coefficients are a few words)."""
+        lst = self.psi_e if side == "E" else self.psi_h
+        words = sum(t.numel() for terms in lst for t in terms)
+        item = lst[0][0].element_size() if lst else 0
+        return int(2 * item * words + self._air_bytes[0 if side == "E" else 1])
 
     def zero_(self) -> None:
         for lst in self.psi_e + self.psi_h:

@@ -62,6 +62,54 @@ __global__ __launch_bounds__(256) void k_dft_sample(const T* __restrict__ e, int
   X[((int64_t)set * B + b) * S_pad + s] = acc;
 }
 
+// Structured sampling: the samples of one monitor component on one face or
+// box form a group whose terms are affine in the sample's natural
+// coordinates (c0 fastest): term t reads field[base_t + c0 s0 + c1 s1 +
+// c2 s2] with weight wtab[woff_t + c_{waxis_t}] (one coordinate, or a
+// constant when waxis_t < 0).  The weights and indices are those of the
+// generic map (the host derives the groups from it and checks them), so the
+// sum — taps in map order, zero weights skipped — is the same float32 value;
+// only the 64 B of per-sample index/weight words per step are gone.
+constexpr int DG_W = 40;  // int32 words per group descriptor
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft_sample_groups(
+    const T* __restrict__ e, int64_t e_set, const T* __restrict__ h, int64_t h_set,
+    const int* __restrict__ grp, int G, const float* __restrict__ wtab, int S,
+    float* __restrict__ X, int S_pad, int B, const long long* __restrict__ ctr,
+    long long n_start) {
+  const int s_first = blockIdx.x * blockDim.x;
+  const int s = s_first + threadIdx.x;
+  const int set = blockIdx.y;
+  // group of the work-group's first sample (uniform), then per lane
+  int g = 0;
+  while (g + 1 < G && grp[(g + 1) * DG_W] <= s_first) ++g;
+  while (g + 1 < G && grp[(g + 1) * DG_W] <= s) ++g;
+  if (s >= S) return;
+  const long long n = *ctr;
+  const int b = (int)((n - n_start) % B);
+  const int* d = grp + g * DG_W;
+  const int loc = s - d[0];
+  float acc = 0.0f;
+  if (loc >= 0 && loc < d[1]) {
+    const int n0 = d[2], n1 = d[3];
+    const int c0 = loc % n0;
+    const int r = loc / n0;
+    const int c1 = r % n1;
+    const int c2 = r / n1;
+    const int64_t off = (int64_t)c0 * d[6] + (int64_t)c1 * d[7] + (int64_t)c2 * d[8];
+    const T* f = d[4] ? h + (int64_t)set * h_set : e + (int64_t)set * e_set;
+    const int nt = d[5];
+    for (int t = 0; t < nt; ++t) {
+      const int ax = d[17 + t];
+      const int c = ax == 0 ? c0 : (ax == 1 ? c1 : (ax == 2 ? c2 : 0));
+      const float wt = wtab[d[25 + t] + c];
+      if (wt != 0.0f) acc += wt * (float)f[(int64_t)d[9 + t] + off];
+    }
+  }
+  X[((int64_t)set * B + b) * S_pad + s] = acc;
+}
+
 // Block phase factors exp(j w t0) for E and H stamps: blk[f*4 + 0..3] =
 // cos(w t0E), sin(w t0E), cos(w t0H), sin(w t0H).
 __global__ void k_dft_phase(const double* __restrict__ omega, int F, double dt,
@@ -218,6 +266,37 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
     hipLaunchKernelGGL(k_dft_sample<double>, grd, dim3(256), 0, st, (const double*)e, e_set_stride,
                        (const double*)h, h_set_stride, idx, w, S, X, S_pad, B, step_ctr, n_start);
   MGH_CHECK_LAUNCH("k_dft_sample");
+  return MGH_OK;
+}
+
+// Structured form of mgh_dft_sample (same X row, same values): grp holds G
+// descriptors of DG_W int32 words, sorted by start:
+//   [0] start  [1] count  [2] n0  [3] n1  [4] field (0 E, 1 H)  [5] terms
+//   [6..8] index strides of c0, c1, c2  [9..16] base index per term
+//   [17..24] weight coordinate per term (0..2, or -1: constant)
+//   [25..32] offset of the term's weight table in wtab
+// Samples outside every group's [start, start + count) are written as 0.
+int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_stride,
+                          const void* h, long long h_set_stride, const int* grp, int G,
+                          const float* wtab, int S, float* X, int S_pad, int B,
+                          const long long* step_ctr, long long n_start, void* stream) {
+  MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8");
+  MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts S=%d S_pad=%d", S, S_pad);
+  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..256, got %d", B);
+  MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
+  MGH_REQUIRE(G >= 1 && G <= 4096, "bad group count %d", G);
+  MGH_REQUIRE(e && h && grp && wtab && X && step_ctr, "null pointer");
+  dim3 grd((S + 255) / 256, nsets, 1);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_dft_sample_groups<float>, grd, dim3(256), 0, st, (const float*)e,
+                       e_set_stride, (const float*)h, h_set_stride, grp, G, wtab, S, X, S_pad, B,
+                       step_ctr, n_start);
+  else
+    hipLaunchKernelGGL(k_dft_sample_groups<double>, grd, dim3(256), 0, st, (const double*)e,
+                       e_set_stride, (const double*)h, h_set_stride, grp, G, wtab, S, X, S_pad,
+                       B, step_ctr, n_start);
+  MGH_CHECK_LAUNCH("k_dft_sample_groups");
   return MGH_OK;
 }
 

@@ -40,12 +40,19 @@
 // units — the set groups of one tile first — land on the same XCD
 // (work-groups are dealt to the 8 XCDs round robin) and share its L2.
 //
-// CPML.  The curl launch carries no CPML code (it would cost the whole grid
-// registers and occupancy for a thin shell).  Each CPML face then runs one
-// small launch over its slab, in the configured face order — the structure
-// of the reference (cpml.update_E / update_H per face after the curl pass).
-// A slab launch reads the element's freshly updated value, so the result is
-// the fused recursion above bit for bit.
+// CPML, two forms with the same values.  Fused (PML = true, the
+// mgh_update_*_pml entry points): every d of every face is a difference of
+// neighbour words the curl already holds in registers, so an element in a
+// slab applies its face terms, in descriptor order, to its updated value
+// before the single store; only psi, the layer coefficients and the air mask
+// add traffic.  One "node in some slab" test per element keeps the interior
+// on the plain path, and the face loop is not unrolled (its per-face values
+// are uniform scalar loads), which holds the register count within one
+// occupancy step of the plain kernel.  Separate (mgh_cpml_e / mgh_cpml_h):
+// one small launch per face over its slab, in descriptor order — the
+// structure of the reference (cpml.update_E / update_H per face after the
+// curl pass) — for listed launches and for dispersive meshes, whose ADE
+// completion sits between the curl and the CPML pass.
 #include "mgh_common.h"
 #include "mgh_cpml.h"
 
@@ -82,198 +89,6 @@ __device__ __forceinline__ bool unit_of(const Plan& p, int& grp, int& i, int& j,
   k0 = kc * p.kchunk;
   k1 = min(k0 + p.kchunk, p.nz + 1);
   return true;
-}
-
-template <typename T, int NS, bool CURL64>
-__global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __restrict__ h,
-                                                 const T* __restrict__ ae,
-                                                 const T* __restrict__ be, Plan pl, int nsets,
-                                                 int64_t e_set, int64_t h_set) {
-  int grp, i, j, k0, k1;
-  if (!unit_of(pl, grp, i, j, k0, k1)) return;
-  const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
-  const int s0 = grp * NS;
-
-  const mgh::Dims d{nx, ny, nz};
-  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
-  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
-  // per-plane sizes (x-fastest: plane = k)
-  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
-                pEz = (int64_t)(ny + 1) * (nx + 1);
-  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
-                pHz = (int64_t)ny * nx;
-  // in-plane offsets: rows of nx (Ex, Hy, Hz) or nx + 1 (Ey, Ez, Hx) words
-  const int64_t rN = (int64_t)j * nx + i;
-  const int64_t rN1 = (int64_t)j * (nx + 1) + i;
-  const bool inx = i < nx, iny = j < ny;
-
-  T hy_prev[NS], hx_prev[NS];
-#pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    hy_prev[q] = T(0);
-    hx_prev[q] = T(0);
-    if (s0 + q < nsets && k0 > 0) {
-      const T* hs = h + (int64_t)(s0 + q) * h_set;
-      if (inx) hy_prev[q] = hs[oHy + (int64_t)(k0 - 1) * pHy + rN];
-      if (iny) hx_prev[q] = hs[(int64_t)(k0 - 1) * pHx + rN1];
-    }
-  }
-
-  for (int k = k0; k < k1; ++k) {
-    const bool knz = k < nz;
-    const int64_t iEx = (int64_t)k * pEx + rN;
-    const int64_t iEy = oEy + (int64_t)k * pEy + rN1;
-    const int64_t iEz = oEz + (int64_t)k * pEz + rN1;
-    T a_x = T(0), b_x = T(0), a_y = T(0), b_y = T(0), a_z = T(0), b_z = T(0);
-    if (inx) { a_x = ae[iEx]; b_x = be[iEx]; }
-    if (iny) { a_y = ae[iEy]; b_y = be[iEy]; }
-    if (knz) { a_z = ae[iEz]; b_z = be[iEz]; }
-
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      if (s0 + q >= nsets) break;
-      const T* hs = h + (int64_t)(s0 + q) * h_set;
-      T* es = e + (int64_t)(s0 + q) * e_set;
-      // H words around the node (zero where the face does not exist)
-      const T hy = (inx && knz) ? hs[oHy + (int64_t)k * pHy + rN] : T(0);
-      const T hx = (iny && knz) ? hs[(int64_t)k * pHx + rN1] : T(0);
-      const T hz = (inx && iny) ? hs[oHz + (int64_t)k * pHz + rN] : T(0);
-
-      if (inx) {  // Ex(i,j,k): dHz/dy - dHy/dz
-        const T hz_jm = (j > 0) ? hs[oHz + (int64_t)k * pHz + rN - nx] : T(0);
-        if constexpr (CURL64) {
-          double c = 0.0;
-          if (iny) c += (double)hz;
-          if (j > 0) c -= (double)hz_jm;
-          if (knz) c -= (double)hy;
-          if (k > 0) c += (double)hy_prev[q];
-          es[iEx] = (T)((double)(a_x * es[iEx]) + (double)b_x * c);
-        } else {
-          T c = T(0);
-          if (iny) c += hz;
-          if (j > 0) c -= hz_jm;
-          if (knz) c -= hy;
-          if (k > 0) c += hy_prev[q];
-          es[iEx] = (a_x * es[iEx]) + (b_x * c);
-        }
-      }
-      if (iny) {  // Ey(i,j,k): dHx/dz - dHz/dx
-        const T hz_im = (i > 0) ? hs[oHz + (int64_t)k * pHz + rN - 1] : T(0);
-        if constexpr (CURL64) {
-          double c = 0.0;
-          if (knz) c += (double)hx;
-          if (k > 0) c -= (double)hx_prev[q];
-          if (inx) c -= (double)hz;
-          if (i > 0) c += (double)hz_im;
-          es[iEy] = (T)((double)(a_y * es[iEy]) + (double)b_y * c);
-        } else {
-          T c = T(0);
-          if (knz) c += hx;
-          if (k > 0) c -= hx_prev[q];
-          if (inx) c -= hz;
-          if (i > 0) c += hz_im;
-          es[iEy] = (a_y * es[iEy]) + (b_y * c);
-        }
-      }
-      if (knz) {  // Ez(i,j,k): dHy/dx - dHx/dy
-        const T hy_im = (i > 0) ? hs[oHy + (int64_t)k * pHy + rN - 1] : T(0);
-        const T hx_jm = (j > 0) ? hs[(int64_t)k * pHx + rN1 - (nx + 1)] : T(0);
-        if constexpr (CURL64) {
-          double c = 0.0;
-          if (inx) c += (double)hy;
-          if (i > 0) c -= (double)hy_im;
-          if (iny) c -= (double)hx;
-          if (j > 0) c += (double)hx_jm;
-          es[iEz] = (T)((double)(a_z * es[iEz]) + (double)b_z * c);
-        } else {
-          T c = T(0);
-          if (inx) c += hy;
-          if (i > 0) c -= hy_im;
-          if (iny) c -= hx;
-          if (j > 0) c += hx_jm;
-          es[iEz] = (a_z * es[iEz]) + (b_z * c);
-        }
-      }
-      hy_prev[q] = hy;
-      hx_prev[q] = hx;
-    }
-  }
-}
-
-template <typename T, int NS>
-__global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __restrict__ h,
-                                                 const T* __restrict__ ah,
-                                                 const T* __restrict__ bh, Plan pl, int nsets,
-                                                 int64_t e_set, int64_t h_set) {
-  int grp, i, j, k0, k1;
-  if (!unit_of(pl, grp, i, j, k0, k1)) return;
-  const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
-  const int s0 = grp * NS;
-
-  const mgh::Dims d{nx, ny, nz};
-  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
-  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
-  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
-                pEz = (int64_t)(ny + 1) * (nx + 1);
-  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
-                pHz = (int64_t)ny * nx;
-  const int64_t rN = (int64_t)j * nx + i;
-  const int64_t rN1 = (int64_t)j * (nx + 1) + i;
-  const bool inx = i < nx, iny = j < ny;
-
-  // E at plane k (carried from the previous iteration's k+1 load)
-  T ey_k[NS], ex_k[NS];
-#pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    ey_k[q] = T(0);
-    ex_k[q] = T(0);
-    if (s0 + q < nsets) {
-      const T* es = e + (int64_t)(s0 + q) * e_set;
-      if (iny) ey_k[q] = es[oEy + (int64_t)k0 * pEy + rN1];
-      if (inx) ex_k[q] = es[(int64_t)k0 * pEx + rN];
-    }
-  }
-
-  for (int k = k0; k < k1; ++k) {
-    const bool knz = k < nz;
-    const int64_t iHx = (int64_t)k * pHx + rN1;
-    const int64_t iHy = oHy + (int64_t)k * pHy + rN;
-    const int64_t iHz = oHz + (int64_t)k * pHz + rN;
-    const bool hasHx = iny && knz, hasHy = inx && knz, hasHz = inx && iny;
-    T a_x = T(0), b_x = T(0), a_y = T(0), b_y = T(0), a_z = T(0), b_z = T(0);
-    if (hasHx) { a_x = ah[iHx]; b_x = bh[iHx]; }
-    if (hasHy) { a_y = ah[iHy]; b_y = bh[iHy]; }
-    if (hasHz) { a_z = ah[iHz]; b_z = bh[iHz]; }
-
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      if (s0 + q >= nsets) break;
-      const T* es = e + (int64_t)(s0 + q) * e_set;
-      T* hs = h + (int64_t)(s0 + q) * h_set;
-      // E at plane k+1 (exists for k+1 <= nz)
-      const T ey_kp = (iny && knz) ? es[oEy + (int64_t)(k + 1) * pEy + rN1] : T(0);
-      const T ex_kp = (inx && knz) ? es[(int64_t)(k + 1) * pEx + rN] : T(0);
-      const T ez = knz ? es[oEz + (int64_t)k * pEz + rN1] : T(0);
-      if (hasHx) {  // Hx(i,j,k): dEz/dy - dEy/dz
-        const T ez_jp = es[oEz + (int64_t)k * pEz + rN1 + (nx + 1)];
-        const T c = ((ez_jp - ez) - ey_kp) + ey_k[q];
-        hs[iHx] = (a_x * hs[iHx]) - (b_x * c);
-      }
-      if (hasHy) {  // Hy(i,j,k): dEx/dz - dEz/dx
-        const T ez_ip = es[oEz + (int64_t)k * pEz + rN1 + 1];
-        const T c = ((ex_kp - ex_k[q]) - ez_ip) + ez;
-        hs[iHy] = (a_y * hs[iHy]) - (b_y * c);
-      }
-      if (hasHz) {  // Hz(i,j,k): dEy/dx - dEx/dy
-        const T ey_ip = es[oEy + (int64_t)k * pEy + rN1 + 1];
-        const T ex_jp = es[(int64_t)k * pEx + rN + nx];
-        const T c = ((ey_ip - ey_k[q]) - ex_jp) + ex_k[q];
-        hs[iHz] = (a_z * hs[iHz]) - (b_z * c);
-      }
-      ey_k[q] = ey_kp;
-      ex_k[q] = ex_kp;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -322,6 +137,314 @@ __device__ __forceinline__ void cpml_term(const CFace<T>& F, int term, int64_t s
   const T corr = beta * ((ck * d) + p);
   *val = sign > 0 ? (*val + corr) : (*val - corr);
 }
+
+// A CPML face of a fused launch (curl and CPML in one pass): its slab, the
+// slab sizes of its two terms on the launch's side, its coefficient, psi and
+// air buffers.
+template <typename T>
+struct PFace {
+  int axis, k0, n, ksets;
+  int64_t sz0, sz1;
+  const T* coef;
+  T* psi;
+  const unsigned char* air;
+};
+
+// Faces in descriptor order, and per axis the (up to two) slab intervals of
+// the node index, [lo, lo + len), for the quick "node in some slab" test.
+template <typename T>
+struct PmlArgs {
+  int nf;
+  int lo[3][2], len[3][2];
+  PFace<T> f[MGH_CPML_MAX_FACES];
+};
+
+template <typename T>
+__device__ __forceinline__ bool pml_hit(const PmlArgs<T>& pa, int i, int j, int k) {
+  return (unsigned)(i - pa.lo[0][0]) < (unsigned)pa.len[0][0] ||
+         (unsigned)(i - pa.lo[0][1]) < (unsigned)pa.len[0][1] ||
+         (unsigned)(j - pa.lo[1][0]) < (unsigned)pa.len[1][0] ||
+         (unsigned)(j - pa.lo[1][1]) < (unsigned)pa.len[1][1] ||
+         (unsigned)(k - pa.lo[2][0]) < (unsigned)pa.len[2][0] ||
+         (unsigned)(k - pa.lo[2][1]) < (unsigned)pa.len[2][1];
+}
+
+// Term `term` of face F on a component whose psi slab has x/y extents
+// (n0, n1), at node (i, j, k) of set s, if the node lies in the slab: the
+// recursion of cpml_term with the same operation order.
+template <typename T>
+__device__ __forceinline__ void pml_term(const PFace<T>& F, int term, int n0, int n1, int i, int j,
+                                         int k, int s, T d, T beta, T* v, int sign) {
+  const int c = F.axis == 0 ? i : (F.axis == 1 ? j : k);
+  const int layer = c - F.k0;
+  if ((unsigned)layer >= (unsigned)F.n) return;
+  int64_t sidx;
+  if (F.axis == 0) sidx = ((int64_t)k * n1 + j) * F.n + layer;
+  else if (F.axis == 1) sidx = ((int64_t)k * F.n + layer) * n0 + i;
+  else sidx = ((int64_t)layer * n1 + j) * n0 + i;
+  T* psi = F.psi + (term ? (int64_t)F.ksets * F.sz0 : 0) + (int64_t)s * (term ? F.sz1 : F.sz0) +
+           sidx;
+  const T b = F.coef[layer], cc = F.coef[F.n + layer], ck = F.coef[2 * F.n + layer];
+  T p = *psi;
+  p = p * b;
+  p = p + (cc * d);
+  if (F.air != nullptr && F.air[(term ? F.sz0 : 0) + sidx] == 0) p = p * T(0);
+  *psi = p;
+  const T corr = beta * ((ck * d) + p);
+  *v = sign > 0 ? (*v + corr) : (*v - corr);
+}
+
+// The CPML terms of one component, faces in descriptor order.  AXA/AXB: the
+// component's two normal axes; TA/TB: its term index on those faces; SA/SB:
+// the signs; dA/dB: the one-sided differences along them.
+// The CPML terms of one component, faces in descriptor order.  AXA/AXB: the
+// component's two normal axes; TA/TB: its term index on those faces; SA/SB:
+// the signs; dA/dB: the one-sided differences along them.  The loop is kept
+// rolled, its face records read as uniform scalar loads: unrolled, the
+// compiler holds every face's record in scalar registers, spills them and
+// costs the whole launch two waves of occupancy per SIMD.
+template <typename T, int AXA, int TA, int SA, int AXB, int TB, int SB>
+__device__ __forceinline__ void pml_comp(const PmlArgs<T>& pa, int n0, int n1, int i, int j, int k,
+                                         int s, T dA, T dB, T beta, T* v) {
+#pragma unroll 1
+  for (int fi = 0; fi < pa.nf; ++fi) {
+    const int ax = pa.f[fi].axis;
+    if (ax == AXA) pml_term(pa.f[fi], TA, n0, n1, i, j, k, s, dA, beta, v, SA);
+    else if (ax == AXB) pml_term(pa.f[fi], TB, n0, n1, i, j, k, s, dB, beta, v, SB);
+  }
+}
+
+template <typename T, int NS, bool CURL64, bool PML>
+__global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __restrict__ h,
+                                                 const T* __restrict__ ae,
+                                                 const T* __restrict__ be, Plan pl, int nsets,
+                                                 int64_t e_set, int64_t h_set, PmlArgs<T> pa) {
+  int grp, i, j, k0, k1;
+  if (!unit_of(pl, grp, i, j, k0, k1)) return;
+  const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
+  const int s0 = grp * NS;
+
+  const mgh::Dims d{nx, ny, nz};
+  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
+  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
+  // per-plane sizes (x-fastest: plane = k)
+  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
+                pEz = (int64_t)(ny + 1) * (nx + 1);
+  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
+                pHz = (int64_t)ny * nx;
+  // in-plane offsets: rows of nx (Ex, Hy, Hz) or nx + 1 (Ey, Ez, Hx) words
+  const int64_t rN = (int64_t)j * nx + i;
+  const int64_t rN1 = (int64_t)j * (nx + 1) + i;
+  const bool inx = i < nx, iny = j < ny;
+
+  T hy_prev[NS], hx_prev[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    hy_prev[q] = T(0);
+    hx_prev[q] = T(0);
+    if (s0 + q < nsets && k0 > 0) {
+      const T* hs = h + (int64_t)(s0 + q) * h_set;
+      if (inx) hy_prev[q] = hs[oHy + (int64_t)(k0 - 1) * pHy + rN];
+      if (iny) hx_prev[q] = hs[(int64_t)(k0 - 1) * pHx + rN1];
+    }
+  }
+
+  for (int k = k0; k < k1; ++k) {
+    const bool knz = k < nz;
+    bool hit = false;
+    if constexpr (PML) hit = pml_hit(pa, i, j, k);
+    const int64_t iEx = (int64_t)k * pEx + rN;
+    const int64_t iEy = oEy + (int64_t)k * pEy + rN1;
+    const int64_t iEz = oEz + (int64_t)k * pEz + rN1;
+    T a_x = T(0), b_x = T(0), a_y = T(0), b_y = T(0), a_z = T(0), b_z = T(0);
+    if (inx) { a_x = ae[iEx]; b_x = be[iEx]; }
+    if (iny) { a_y = ae[iEy]; b_y = be[iEy]; }
+    if (knz) { a_z = ae[iEz]; b_z = be[iEz]; }
+
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      if (s0 + q >= nsets) break;
+      const T* hs = h + (int64_t)(s0 + q) * h_set;
+      T* es = e + (int64_t)(s0 + q) * e_set;
+      // H words around the node (zero where the face does not exist)
+      const T hy = (inx && knz) ? hs[oHy + (int64_t)k * pHy + rN] : T(0);
+      const T hx = (iny && knz) ? hs[(int64_t)k * pHx + rN1] : T(0);
+      const T hz = (inx && iny) ? hs[oHz + (int64_t)k * pHz + rN] : T(0);
+
+      if (inx) {  // Ex(i,j,k): dHz/dy - dHy/dz
+        const T hz_jm = (j > 0) ? hs[oHz + (int64_t)k * pHz + rN - nx] : T(0);
+        T v;
+        if constexpr (CURL64) {
+          double c = 0.0;
+          if (iny) c += (double)hz;
+          if (j > 0) c -= (double)hz_jm;
+          if (knz) c -= (double)hy;
+          if (k > 0) c += (double)hy_prev[q];
+          v = (T)((double)(a_x * es[iEx]) + (double)b_x * c);
+        } else {
+          T c = T(0);
+          if (iny) c += hz;
+          if (j > 0) c -= hz_jm;
+          if (knz) c -= hy;
+          if (k > 0) c += hy_prev[q];
+          v = (a_x * es[iEx]) + (b_x * c);
+        }
+        if constexpr (PML) {  // Ex: z faces d = Hy(k) - Hy(k-1) (+), y faces Hz(j) - Hz(j-1) (-)
+          if (hit) pml_comp<T, 2, 0, +1, 1, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q, hy - hy_prev[q],
+                                                   hz - hz_jm, b_x, &v);
+        }
+        es[iEx] = v;
+      }
+      if (iny) {  // Ey(i,j,k): dHx/dz - dHz/dx
+        const T hz_im = (i > 0) ? hs[oHz + (int64_t)k * pHz + rN - 1] : T(0);
+        T v;
+        if constexpr (CURL64) {
+          double c = 0.0;
+          if (knz) c += (double)hx;
+          if (k > 0) c -= (double)hx_prev[q];
+          if (inx) c -= (double)hz;
+          if (i > 0) c += (double)hz_im;
+          v = (T)((double)(a_y * es[iEy]) + (double)b_y * c);
+        } else {
+          T c = T(0);
+          if (knz) c += hx;
+          if (k > 0) c -= hx_prev[q];
+          if (inx) c -= hz;
+          if (i > 0) c += hz_im;
+          v = (a_y * es[iEy]) + (b_y * c);
+        }
+        if constexpr (PML) {  // Ey: z faces d = Hx(k) - Hx(k-1) (-), x faces Hz(i) - Hz(i-1) (+)
+          if (hit) pml_comp<T, 2, 1, -1, 0, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q, hx - hx_prev[q],
+                                                   hz - hz_im, b_y, &v);
+        }
+        es[iEy] = v;
+      }
+      if (knz) {  // Ez(i,j,k): dHy/dx - dHx/dy
+        const T hy_im = (i > 0) ? hs[oHy + (int64_t)k * pHy + rN - 1] : T(0);
+        const T hx_jm = (j > 0) ? hs[(int64_t)k * pHx + rN1 - (nx + 1)] : T(0);
+        T v;
+        if constexpr (CURL64) {
+          double c = 0.0;
+          if (inx) c += (double)hy;
+          if (i > 0) c -= (double)hy_im;
+          if (iny) c -= (double)hx;
+          if (j > 0) c += (double)hx_jm;
+          v = (T)((double)(a_z * es[iEz]) + (double)b_z * c);
+        } else {
+          T c = T(0);
+          if (inx) c += hy;
+          if (i > 0) c -= hy_im;
+          if (iny) c -= hx;
+          if (j > 0) c += hx_jm;
+          v = (a_z * es[iEz]) + (b_z * c);
+        }
+        if constexpr (PML) {  // Ez: y faces d = Hx(j) - Hx(j-1) (+), x faces Hy(i) - Hy(i-1) (-)
+          if (hit) pml_comp<T, 1, 1, +1, 0, 1, -1>(pa, nx + 1, ny + 1, i, j, k, s0 + q, hx - hx_jm,
+                                                   hy - hy_im, b_z, &v);
+        }
+        es[iEz] = v;
+      }
+      hy_prev[q] = hy;
+      hx_prev[q] = hx;
+    }
+  }
+}
+
+template <typename T, int NS, bool PML>
+__global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __restrict__ h,
+                                                 const T* __restrict__ ah,
+                                                 const T* __restrict__ bh, Plan pl, int nsets,
+                                                 int64_t e_set, int64_t h_set, PmlArgs<T> pa) {
+  int grp, i, j, k0, k1;
+  if (!unit_of(pl, grp, i, j, k0, k1)) return;
+  const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
+  const int s0 = grp * NS;
+
+  const mgh::Dims d{nx, ny, nz};
+  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
+  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
+  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
+                pEz = (int64_t)(ny + 1) * (nx + 1);
+  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
+                pHz = (int64_t)ny * nx;
+  const int64_t rN = (int64_t)j * nx + i;
+  const int64_t rN1 = (int64_t)j * (nx + 1) + i;
+  const bool inx = i < nx, iny = j < ny;
+
+  // E at plane k (carried from the previous iteration's k+1 load)
+  T ey_k[NS], ex_k[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    ey_k[q] = T(0);
+    ex_k[q] = T(0);
+    if (s0 + q < nsets) {
+      const T* es = e + (int64_t)(s0 + q) * e_set;
+      if (iny) ey_k[q] = es[oEy + (int64_t)k0 * pEy + rN1];
+      if (inx) ex_k[q] = es[(int64_t)k0 * pEx + rN];
+    }
+  }
+
+  for (int k = k0; k < k1; ++k) {
+    const bool knz = k < nz;
+    bool hit = false;
+    if constexpr (PML) hit = pml_hit(pa, i, j, k);
+    const int64_t iHx = (int64_t)k * pHx + rN1;
+    const int64_t iHy = oHy + (int64_t)k * pHy + rN;
+    const int64_t iHz = oHz + (int64_t)k * pHz + rN;
+    const bool hasHx = iny && knz, hasHy = inx && knz, hasHz = inx && iny;
+    T a_x = T(0), b_x = T(0), a_y = T(0), b_y = T(0), a_z = T(0), b_z = T(0);
+    if (hasHx) { a_x = ah[iHx]; b_x = bh[iHx]; }
+    if (hasHy) { a_y = ah[iHy]; b_y = bh[iHy]; }
+    if (hasHz) { a_z = ah[iHz]; b_z = bh[iHz]; }
+
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      if (s0 + q >= nsets) break;
+      const T* es = e + (int64_t)(s0 + q) * e_set;
+      T* hs = h + (int64_t)(s0 + q) * h_set;
+      // E at plane k+1 (exists for k+1 <= nz)
+      const T ey_kp = (iny && knz) ? es[oEy + (int64_t)(k + 1) * pEy + rN1] : T(0);
+      const T ex_kp = (inx && knz) ? es[(int64_t)(k + 1) * pEx + rN] : T(0);
+      const T ez = knz ? es[oEz + (int64_t)k * pEz + rN1] : T(0);
+      if (hasHx) {  // Hx(i,j,k): dEz/dy - dEy/dz
+        const T ez_jp = es[oEz + (int64_t)k * pEz + rN1 + (nx + 1)];
+        const T c = ((ez_jp - ez) - ey_kp) + ey_k[q];
+        T v = (a_x * hs[iHx]) - (b_x * c);
+        if constexpr (PML) {  // Hx: z faces d = Ey(k+1) - Ey(k) (-), y faces Ez(j+1) - Ez(j) (+)
+          if (hit) pml_comp<T, 2, 0, -1, 1, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q, ey_kp - ey_k[q],
+                                                   ez_jp - ez, b_x, &v);
+        }
+        hs[iHx] = v;
+      }
+      if (hasHy) {  // Hy(i,j,k): dEx/dz - dEz/dx
+        const T ez_ip = es[oEz + (int64_t)k * pEz + rN1 + 1];
+        const T c = ((ex_kp - ex_k[q]) - ez_ip) + ez;
+        T v = (a_y * hs[iHy]) - (b_y * c);
+        if constexpr (PML) {  // Hy: z faces d = Ex(k+1) - Ex(k) (+), x faces Ez(i+1) - Ez(i) (-)
+          if (hit) pml_comp<T, 2, 1, +1, 0, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q, ex_kp - ex_k[q],
+                                                   ez_ip - ez, b_y, &v);
+        }
+        hs[iHy] = v;
+      }
+      if (hasHz) {  // Hz(i,j,k): dEy/dx - dEx/dy
+        const T ey_ip = es[oEy + (int64_t)k * pEy + rN1 + 1];
+        const T ex_jp = es[(int64_t)k * pEx + rN + nx];
+        const T c = ((ey_ip - ey_k[q]) - ex_jp) + ex_k[q];
+        T v = (a_z * hs[iHz]) - (b_z * c);
+        if constexpr (PML) {  // Hz: y faces d = Ex(j+1) - Ex(j) (-), x faces Ey(i+1) - Ey(i) (+)
+          if (hit) pml_comp<T, 1, 1, -1, 0, 1, +1>(pa, nx, ny, i, j, k, s0 + q, ex_jp - ex_k[q],
+                                                   ey_ip - ey_k[q], b_z, &v);
+        }
+        hs[iHz] = v;
+      }
+      ey_k[q] = ey_kp;
+      ex_k[q] = ex_kp;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CPML slab passes (separate launches: listed launches, dispersive meshes).
 
 // slab node (i, j, k) of thread t, in psi-layout order
 template <int AX>
@@ -560,19 +683,63 @@ int64_t slab_threads(int axis, int nx, int ny, int nz, int n) {
   return (int64_t)n * (ny + 1) * (nz + 1);
 }
 
+// faces of a CPML set for a fused launch (E or H side)
+template <typename T>
+PmlArgs<T> pml_args(const mgh::CpmlSet* pml, bool e_side, int nx, int ny, int nz) {
+  PmlArgs<T> a;
+  std::memset(&a, 0, sizeof(a));  // empty intervals: len 0
+  if (!pml) return a;
+  a.nf = pml->nfaces;
+  int used[3] = {0, 0, 0};
+  for (int f = 0; f < pml->nfaces; ++f) {
+    const mgh::CpmlFace& F = pml->f[f];
+    PFace<T>& P = a.f[f];
+    P.axis = F.axis;
+    P.k0 = F.k0;
+    P.n = F.n;
+    P.ksets = F.nsets;
+    P.coef = (const T*)F.coef;
+    P.psi = (T*)(e_side ? F.psi_e : F.psi_h);
+    P.air = e_side ? F.air_e : F.air_h;
+    const int64_t n = F.n;
+    // term slab sizes: n times the component's extents along the other axes
+    if (e_side) {
+      if (F.axis == 2) { P.sz0 = n * nx * (ny + 1); P.sz1 = n * (nx + 1) * ny; }
+      else if (F.axis == 1) { P.sz0 = n * nx * (nz + 1); P.sz1 = n * (nx + 1) * nz; }
+      else { P.sz0 = n * ny * (nz + 1); P.sz1 = n * (ny + 1) * nz; }
+    } else {
+      if (F.axis == 2) { P.sz0 = n * (nx + 1) * ny; P.sz1 = n * nx * (ny + 1); }
+      else if (F.axis == 1) { P.sz0 = n * (nx + 1) * nz; P.sz1 = n * nx * (nz + 1); }
+      else { P.sz0 = n * (ny + 1) * nz; P.sz1 = n * ny * (nz + 1); }
+    }
+    const int u = used[F.axis]++;  // at most two faces per axis (load_cpml_set)
+    a.lo[F.axis][u] = F.k0;
+    a.len[F.axis][u] = F.n;
+  }
+  return a;
+}
+
+// pml: CPML faces fused into the curl launch (null: curl only)
 template <typename T, bool C64>
 int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny, int nz,
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
-          int nlive = 0) {
+          int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
   const int ns = nsets >= 2 ? kSetsPerThread : 1;
   const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
   if (pl.units == 0) return MGH_OK;
-  if (ns == kSetsPerThread)
-    hipLaunchKernelGGL((k_update_e<T, kSetsPerThread, C64>), dim3(plan_blocks(pl)), dim3(WG), 0, st,
-                       (T*)e, (const T*)h, (const T*)ae, (const T*)be, pl, nsets, es, hs);
-  else
-    hipLaunchKernelGGL((k_update_e<T, 1, C64>), dim3(plan_blocks(pl)), dim3(WG), 0, st, (T*)e,
-                       (const T*)h, (const T*)ae, (const T*)be, pl, nsets, es, hs);
+  const PmlArgs<T> pa = pml_args<T>(pml, true, nx, ny, nz);
+  const dim3 grd(plan_blocks(pl));
+#define MGH_LAUNCH_E(NS_, PML_)                                                               \
+  hipLaunchKernelGGL((k_update_e<T, NS_, C64, PML_>), grd, dim3(WG), 0, st, (T*)e, (const T*)h, \
+                     (const T*)ae, (const T*)be, pl, nsets, es, hs, pa)
+  if (ns == kSetsPerThread) {
+    if (pml) MGH_LAUNCH_E(kSetsPerThread, true);
+    else MGH_LAUNCH_E(kSetsPerThread, false);
+  } else {
+    if (pml) MGH_LAUNCH_E(1, true);
+    else MGH_LAUNCH_E(1, false);
+  }
+#undef MGH_LAUNCH_E
   MGH_CHECK_LAUNCH("k_update_e");
   return MGH_OK;
 }
@@ -602,16 +769,23 @@ int run_cpml_e(void* e, const void* h, const void* be, int nx, int ny, int nz, i
 template <typename T>
 int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny, int nz,
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
-          int nlive = 0) {
+          int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
   const int ns = nsets >= 2 ? kSetsPerThread : 1;
   const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
   if (pl.units == 0) return MGH_OK;
-  if (ns == kSetsPerThread)
-    hipLaunchKernelGGL((k_update_h<T, kSetsPerThread>), dim3(plan_blocks(pl)), dim3(WG), 0, st,
-                       (const T*)e, (T*)h, (const T*)ah, (const T*)bh, pl, nsets, es, hs);
-  else
-    hipLaunchKernelGGL((k_update_h<T, 1>), dim3(plan_blocks(pl)), dim3(WG), 0, st, (const T*)e,
-                       (T*)h, (const T*)ah, (const T*)bh, pl, nsets, es, hs);
+  const PmlArgs<T> pa = pml_args<T>(pml, false, nx, ny, nz);
+  const dim3 grd(plan_blocks(pl));
+#define MGH_LAUNCH_H(NS_, PML_)                                                              \
+  hipLaunchKernelGGL((k_update_h<T, NS_, PML_>), grd, dim3(WG), 0, st, (const T*)e, (T*)h,     \
+                     (const T*)ah, (const T*)bh, pl, nsets, es, hs, pa)
+  if (ns == kSetsPerThread) {
+    if (pml) MGH_LAUNCH_H(kSetsPerThread, true);
+    else MGH_LAUNCH_H(kSetsPerThread, false);
+  } else {
+    if (pml) MGH_LAUNCH_H(1, true);
+    else MGH_LAUNCH_H(1, false);
+  }
+#undef MGH_LAUNCH_H
   MGH_CHECK_LAUNCH("k_update_h");
   return MGH_OK;
 }
@@ -759,36 +933,45 @@ int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
                      long long e_set_stride, const void* h, long long h_set_stride,
                      const void* alpha_e, const void* beta_e, int curl64, int kchunk,
                      const void* cpml, void* stream) {
-  if (cpml) {  // validate before launching anything
-    int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
-    if (rc) return rc;
-    mgh::CpmlSet pml{};
-    rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
-    if (rc) return rc;
-  }
-  int rc = mgh_update_e(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_e,
+  if (!cpml)
+    return mgh_update_e(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_e,
                         beta_e, curl64, kchunk, stream);
-  if (rc || !cpml) return rc;
-  return mgh_cpml_e(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, beta_e, cpml,
-                    stream);
+  int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && alpha_e && beta_e, "null device pointer");
+  MGH_REQUIRE(!(curl64 && dtype == 8), "curl64 applies to float32 fields only");
+  mgh::CpmlSet pml{};
+  rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return curl64 ? run_e<float, true>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk,
+                                        e_set_stride, h_set_stride, st, nullptr, 0, &pml)
+                  : run_e<float, false>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk,
+                                         e_set_stride, h_set_stride, st, nullptr, 0, &pml);
+  return run_e<double, false>(e, h, alpha_e, beta_e, nx, ny, nz, nsets, kchunk, e_set_stride,
+                              h_set_stride, st, nullptr, 0, &pml);
 }
 
 int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e,
                      long long e_set_stride, void* h, long long h_set_stride,
                      const void* alpha_h, const void* beta_h, int kchunk, const void* cpml,
                      void* stream) {
-  if (cpml) {
-    int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
-    if (rc) return rc;
-    mgh::CpmlSet pml{};
-    rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
-    if (rc) return rc;
-  }
-  int rc = mgh_update_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_h,
+  if (!cpml)
+    return mgh_update_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_h,
                         beta_h, kchunk, stream);
-  if (rc || !cpml) return rc;
-  return mgh_cpml_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, beta_h, cpml,
-                    stream);
+  int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && alpha_h && beta_h, "null device pointer");
+  mgh::CpmlSet pml{};
+  rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return run_h<float>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
+                        h_set_stride, st, nullptr, 0, &pml);
+  return run_h<double>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
+                       h_set_stride, st, nullptr, 0, &pml);
 }
 
 }  // extern "C"

@@ -132,11 +132,115 @@ def _pack(rows: list[list]) -> tuple[np.ndarray, np.ndarray]:
     return np.concatenate(idx_blocks), np.concatenate(w_blocks)
 
 
+DG_W = 40  # int32 words per structured group descriptor (csrc/mgh_dft.hip)
+
+
+def structured_groups(dev_idx: np.ndarray, w32: np.ndarray, groups: list):
+    """Factor a device-layout sample map into affine groups.
+
+    ``dev_idx`` (S, 8) int64 (E >= 0, H at -(i + 1)), ``w32`` (S, 8) float32,
+    ``groups``: [(comp, shape, start, count)] with samples numbered c0
+    fastest.  Returns (descriptors (G, DG_W) int32, weight tables float32)
+    whose expansion (``expand_groups``) equals the map exactly, or None when
+    some group does not factor (the generic gather then serves the map)."""
+    desc, tabs, n_tab = [], [], 0
+    for comp, shape, start, count in sorted(groups, key=lambda g: g[2]):
+        if count == 0:
+            continue
+        shp = tuple(int(v) for v in shape) + (1,) * (3 - len(shape))
+        if len(shape) > 3 or int(np.prod(shp)) != count:
+            return None
+        n0, n1, n2 = shp
+        rows = slice(start, start + count)
+        w = w32[rows]
+        used = [t for t in range(w.shape[1]) if np.any(w[:, t] != 0)]
+        if not used or len(used) > 8:
+            return None
+        is_h = comp.startswith("H")
+        field = 1 if is_h else 0
+        d = np.zeros(DG_W, dtype=np.int64)
+        d[0:6] = (start, count, n0, n1, field, len(used))
+        c2, c1, c0 = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+        c0, c1, c2 = c0.ravel(), c1.ravel(), c2.ravel()
+        strides = None
+        for q, t in enumerate(used):
+            wt = w[:, t]
+            live = wt != 0
+            ix = dev_idx[rows, t].astype(np.int64)
+            if is_h:
+                if np.any(ix[live] >= 0):
+                    return None
+                ix = -(ix + 1)
+            elif np.any(ix[live] < 0):
+                return None
+            base = int(ix[0]) if live[0] else None
+            if strides is None or base is None:
+                # strides from the first sample and its neighbours along each axis
+                st = []
+                for step, n_ax in ((1, n0), (n0, n1), (n0 * n1, n2)):
+                    st.append(int(ix[step] - ix[0]) if n_ax > 1 else 0)
+                if strides is None:
+                    strides = st
+            if base is None:
+                return None
+            pred = base + c0 * strides[0] + c1 * strides[1] + c2 * strides[2]
+            if not np.array_equal(pred[live], ix[live]):
+                return None
+            # weight: constant, or a function of one natural coordinate
+            grid = wt.reshape(n2, n1, n0)
+            ax, tab = None, None
+            if np.all(wt == wt[0]):
+                ax, tab = -1, wt[:1]
+            else:
+                for a, line in ((0, grid[0, 0, :]), (1, grid[0, :, 0]), (2, grid[:, 0, 0])):
+                    cand = (line[c0] if a == 0 else line[c1] if a == 1 else line[c2])
+                    if np.array_equal(cand, wt):
+                        ax, tab = a, line
+                        break
+            if ax is None:
+                return None
+            d[9 + q] = base
+            d[17 + q] = ax
+            d[25 + q] = n_tab
+            tabs.append(np.asarray(tab, dtype=np.float32))
+            n_tab += tab.size
+        d[6:9] = strides
+        desc.append(d)
+    if not desc:
+        return None
+    arr = np.stack(desc)
+    if np.abs(arr).max() >= 2**31 - 1:
+        return None
+    return arr.astype(np.int32), np.concatenate(tabs).astype(np.float32)
+
+
+def expand_groups(desc: np.ndarray, wtab: np.ndarray, S: int):
+    """(idx, w) (S, 8) of structured groups, zero-weight terms dropped and
+    the rest packed first — the rows the sampling kernel sums."""
+    idx = np.zeros((S, 8), dtype=np.int64)
+    w = np.zeros((S, 8), dtype=np.float32)
+    for d in desc.astype(np.int64):
+        start, count, n0, n1, field, nt = (int(v) for v in d[:6])
+        loc = np.arange(count)
+        c = (loc % n0, (loc // n0) % n1, loc // (n0 * n1))
+        off = c[0] * d[6] + c[1] * d[7] + c[2] * d[8]
+        for t in range(nt):
+            ax, wo = int(d[17 + t]), int(d[25 + t])
+            wt = wtab[wo + (c[ax] if ax >= 0 else 0)]
+            ix = d[9 + t] + off
+            idx[start:start + count, t] = -(ix + 1) if field else ix
+            w[start:start + count, t] = wt
+    return idx, w
+
+
 class DFTMonitorBase:
     """Shared device machinery: sample map, block buffer, phasors, bins."""
 
     block = BLOCK
     compensated = True
+    # sample through the structured groups when the map factors into them
+    # (False: always the generic per-sample (index, weight) gather)
+    structured = True
 
     def __init__(self, freqs, name: str = "", block: int = BLOCK, compensated: bool = True):
         self.freqs = np.asarray(freqs, dtype=float).ravel()
@@ -192,10 +296,11 @@ class DFTMonitorBase:
             w[self.s_h_start: self.s_h_start + ih.shape[0]] = wh
         return idx, w
 
-    def attach(self, solver) -> None:
-        idx, w = self.sample_map(solver.mesh.grid, solver.boundary_conditions)
-        dims = solver.dims
-        # reference flat indices -> device layout, E (>= 0) and H (< 0) separately
+    def device_map(self, grid, boundary_conditions=None):
+        """``sample_map`` with the indices in the device layout (E >= 0, H at
+        -(i + 1))."""
+        idx, w = self.sample_map(grid, boundary_conditions)
+        dims = grid.dims
         pos = idx >= 0
         dev_idx = np.zeros_like(idx)
         if pos.any():
@@ -205,6 +310,10 @@ class DFTMonitorBase:
             dev_idx[neg] = -(layout.index_to_device(-(idx[neg] + 1), *dims, "H")) - 1
         if np.abs(dev_idx).max() >= 2**31 - 1:
             raise ValueError("field vectors too large for 32-bit sample indices")
+        return dev_idx, w
+
+    def attach(self, solver) -> None:
+        dev_idx, w = self.device_map(solver.mesh.grid, solver.boundary_conditions)
         dev = solver.e.device
         K = solver.nsets
         self.F = self.freqs.size
@@ -221,8 +330,15 @@ class DFTMonitorBase:
         Pc[: self.F, valid] = (np.cos(ph) * dt)[:, valid]
         Ps[: self.F, valid] = (np.sin(ph) * dt)[:, valid]
         f32 = torch.float32
+        w32 = w.astype(np.float32)
         self.idx_d = torch.from_numpy(dev_idx.astype(np.int32)).to(dev)
-        self.w_d = torch.from_numpy(w.astype(np.float32)).to(dev)
+        self.w_d = torch.from_numpy(w32).to(dev)
+        self.grp_d = self.wtab_d = None
+        if self.structured:
+            st = structured_groups(dev_idx, w32, self._layout)
+            if st is not None:
+                self.grp_d = torch.from_numpy(st[0]).to(dev)
+                self.wtab_d = torch.from_numpy(st[1]).to(dev)
         self.Pc = torch.from_numpy(lane_order(Pc)).to(dev, f32)
         self.Ps = torch.from_numpy(lane_order(Ps)).to(dev, f32)
         self.omega = torch.from_numpy(omega).to(dev, torch.float64)
@@ -242,10 +358,16 @@ class DFTMonitorBase:
     def record(self, solver, stream, n: int) -> None:
         sh = int(stream.cuda_stream)
         code = 4 if solver.e.dtype == torch.float32 else 8
-        _lib.call("mgh_dft_sample", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
-                  solver.h.data_ptr(), solver.h.shape[1], self.idx_d.data_ptr(), self.w_d.data_ptr(),
-                  self.S, self.X.data_ptr(), self.S_pad, self.block, solver.step_ctr.data_ptr(),
-                  self.n_start, sh)
+        if self.grp_d is not None:
+            _lib.call("mgh_dft_sample_groups", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
+                      solver.h.data_ptr(), solver.h.shape[1], self.grp_d.data_ptr(),
+                      self.grp_d.shape[0], self.wtab_d.data_ptr(), self.S, self.X.data_ptr(),
+                      self.S_pad, self.block, solver.step_ctr.data_ptr(), self.n_start, sh)
+        else:
+            _lib.call("mgh_dft_sample", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
+                      solver.h.data_ptr(), solver.h.shape[1], self.idx_d.data_ptr(),
+                      self.w_d.data_ptr(), self.S, self.X.data_ptr(), self.S_pad, self.block,
+                      solver.step_ctr.data_ptr(), self.n_start, sh)
         if (n - self.n_start) % self.block == self.block - 1:
             self._contract(solver, sh, self.block)
             self._pending = 0

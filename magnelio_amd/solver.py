@@ -17,6 +17,10 @@ Step order (fit_td.py:950-1007), t = n dt:
   ports      modal DTBC/Mur step, write-back, V/I record mgh_port_step
              lumped RLC elements                       mgh_lumped_step
   H phase    curl update, CPML faces                   mgh_update_h, mgh_cpml_h
+Without dispersion and listed (tile-skipping) launches, the CPML faces run
+inside the curl launches (mgh_update_e_pml / mgh_update_h_pml): each slab
+element applies its face terms, in face order, to its freshly updated value
+before the store — the same values as the separate slab passes.
   sources    TF/SF H injection at t + 3dt/2
   monitors   DFT accumulation                          mgh_dft_*
   counter    n <- n + 1 on the device                  mgh_step_advance
@@ -451,15 +455,19 @@ class FITTimeDomainSolver:
         if ev is not None:
             ev[0].record(stream)
         tp = self.tile_plan
+        # CPML rides in the curl launches unless a listed launch or the
+        # dispersion completion (which sits between curl and CPML) forbids it
+        fused = self.cpml if (self.cpml is not None and tp is None and self._dispersion is None) else None
         kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
-                         kchunk=self.kchunk, live=tp.live_e if tp is not None else None, stream=stream)
+                         kchunk=self.kchunk, live=tp.live_e if tp is not None else None, cpml=fused,
+                         stream=stream)
         if ev is not None:
             ev[1].record(stream)
             self.timed_events.append(ev)
         # reference E phase order: curl, dispersion completion, CPML faces
         if self._dispersion is not None:
             self._dispersion.update(self, stream)
-        if self.cpml is not None:
+        if self.cpml is not None and fused is None:
             kernels.cpml_e(self.e, self.h, self.beta_e, dims, self.cpml, stream=stream)
         for src in self.sources:
             src.inject_e(self, stream)
@@ -470,8 +478,8 @@ class FITTimeDomainSolver:
         if self.dlumped is not None:
             self.dlumped.step(self, stream)
         kernels.update_h(self.e, self.h, self.alpha_h, self.beta_h, dims, kchunk=self.kchunk,
-                         live=tp.live_h if tp is not None else None, stream=stream)
-        if self.cpml is not None:
+                         live=tp.live_h if tp is not None else None, cpml=fused, stream=stream)
+        if self.cpml is not None and fused is None:
             kernels.cpml_h(self.e, self.h, self.beta_h, dims, self.cpml, stream=stream)
         for src in self.sources:
             src.inject_h(self, stream)

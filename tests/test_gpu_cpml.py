@@ -1,9 +1,10 @@
 """GPU parity of the CPML recursion fused into the curl kernels.
 
 All six faces at once (corner and edge overlaps exercise the face order),
-PEC cells inside the slabs (air masks), K field sets, float32 and float64;
-compared bit-exactly with the oracle's separate slab passes after several
-leapfrog steps.
+PEC cells inside the slabs (air masks), K field sets (one and two per
+thread), float32 and float64, CPML inside the curl launches and as separate
+slab launches; compared bit-exactly with the oracle's separate slab passes
+after several leapfrog steps.
 """
 
 import numpy as np
@@ -39,11 +40,11 @@ def _setup(rng, dims, dtype, with_pec):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("with_pec", [False, True])
-def test_fused_cpml_bit_exact(dtype, with_pec, gpu):
-    print(f"cpml dtype={np.dtype(dtype).name} pec={with_pec}", flush=True)
+@pytest.mark.parametrize("fused,nsets", [(True, 3), (True, 1), (False, 3)])
+def test_fused_cpml_bit_exact(dtype, with_pec, fused, nsets, gpu):
+    print(f"cpml dtype={np.dtype(dtype).name} pec={with_pec} fused={fused} K={nsets}", flush=True)
     rng = np.random.default_rng(12)
     dims = (13, 11, 15)
-    nsets = 3
     axes, pec, pec_cell, dt, (ae, be, ah, bh) = _setup(rng, dims, dtype, with_pec)
     nl = 4
     ofaces_sets = [[oc.CPMLFace(f, *axes, dt, dtype, pec_e_flat=pec if with_pec else None,
@@ -68,8 +69,14 @@ def test_fused_cpml_bit_exact(dtype, with_pec, gpu):
     de, dh = d(e, "E"), d(h, "H")
     dae, dbe, dah, dbh = d(ae, "E"), d(be, "E"), d(ah, "H"), d(bh, "H")
     for _ in range(6):
-        kernels.update_e(de, dh, dae, dbe, dims, cpml=dev_pml, kchunk=4)
-        kernels.update_h(de, dh, dah, dbh, dims, cpml=dev_pml, kchunk=4)
+        if fused:  # CPML inside the curl launches
+            kernels.update_e(de, dh, dae, dbe, dims, cpml=dev_pml, kchunk=4)
+            kernels.update_h(de, dh, dah, dbh, dims, cpml=dev_pml, kchunk=4)
+        else:  # curl launch, then one slab launch per face
+            kernels.update_e(de, dh, dae, dbe, dims, kchunk=4)
+            kernels.cpml_e(de, dh, dbe, dims, dev_pml)
+            kernels.update_h(de, dh, dah, dbh, dims, kchunk=4)
+            kernels.cpml_h(de, dh, dbh, dims, dev_pml)
     torch.cuda.synchronize()
     for s in range(nsets):
         for _ in range(6):

@@ -10,7 +10,8 @@ import numpy as np
 
 from magnelio_amd import mesh as mm
 from magnelio_amd.boundaries import CPMLBoundary
-from magnelio_amd.monitors import MonitorFarField, MonitorFieldFrequency, lane_order, natural
+from magnelio_amd.monitors import (DG_W, MonitorFarField, MonitorFieldFrequency, expand_groups,
+                                   lane_order, natural, structured_groups)
 from oracle import monitors as omon
 from oracle import yee
 
@@ -89,3 +90,58 @@ def test_natural_order_inverts_storage():
     a = np.arange(2 * 3 * 4 * 5).reshape(2, 3, 4, 5)  # (F, n0, n1, n2) natural
     stored = a.transpose(0, 3, 2, 1).reshape(2, -1)  # first axis fastest
     assert np.array_equal(natural(stored, (3, 4, 5)), a)
+
+
+def _rows(idx, w):
+    """Per sample, the (index, weight) terms with non-zero weight, in order."""
+    return [[(int(i), float(v)) for i, v in zip(ri, rw) if v != 0] for ri, rw in zip(idx, w)]
+
+
+def _structured_roundtrip(mon, g, bcs=None):
+    dev_idx, w = mon.device_map(g, bcs)
+    w32 = w.astype(np.float32)
+    st = structured_groups(dev_idx, w32, mon._layout)
+    assert st is not None
+    desc, wtab = st
+    assert desc.dtype == np.int32 and desc.shape[1] == DG_W
+    idx2, w2 = expand_groups(desc, wtab, mon.S)
+    assert _rows(idx2, w2) == _rows(dev_idx, w32)
+    return desc
+
+
+def test_structured_groups_region_monitor():
+    """The region map (4 E terms, 2 H terms, non-uniform weights along the
+    component axis) factors into one affine group per component, and the
+    groups expand to the generic map term for term."""
+    g, _ = _grid()
+    mon = MonitorFieldFrequency([1e9], region=(slice(2, 7), slice(0, 15), slice(4, 9)))
+    desc = _structured_roundtrip(mon, g)
+    assert desc.shape[0] == 6
+    assert list(desc[:, 5]) == [4, 4, 4, 2, 2, 2]
+
+
+def test_structured_groups_huygens_box():
+    """Far-field map: 6 faces x 4 tangential components, 8 E terms (two
+    layers x four edges) and 4 H terms per sample."""
+    g, _ = _grid()
+    bcs = {f: CPMLBoundary(f, g, thickness_cells=3) for f in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")}
+    mon = MonitorFarField([1e9], margin_cells=2)
+    desc = _structured_roundtrip(mon, g, bcs)
+    assert desc.shape[0] == 24
+    assert sorted(set(desc[:, 5])) == [4, 8]
+
+
+def test_structured_groups_reject_non_affine():
+    """A map whose indices are not affine in the sample coordinates (one
+    sample's term moved) or whose weights vary along two axes is refused,
+    so the generic gather serves it."""
+    g, _ = _grid()
+    mon = MonitorFieldFrequency([1e9], region=(slice(2, 7), slice(0, 15), slice(4, 9)))
+    dev_idx, w = mon.device_map(g)
+    w32 = w.astype(np.float32)
+    bad = dev_idx.copy()
+    bad[7, 1] += 1
+    assert structured_groups(bad, w32, mon._layout) is None
+    w_bad = w32.copy()
+    w_bad[5, 0] *= np.float32(1.5)
+    assert structured_groups(dev_idx, w_bad, mon._layout) is None
