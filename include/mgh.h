@@ -243,7 +243,8 @@ int mgh_ade_update(int dtype, int nsets, void* e, long long e_set_stride, int n,
  * record hooks of MonitorFieldFrequency (field_frequency.py:249-294) and
  * MonitorFarField (far_field.py:230-248); E stamped t = n dt, H t + dt/2.
  *
- * mgh_dft_sample: row ((n - n_start) mod B) of X (nsets x 64 x S_pad float32)
+ * mgh_dft_sample: row ((n - n_start) mod B) of X (nsets x B x S_pad float32,
+ * B <= 512)
  * receives, for each of S samples, sum_t w[s,t] * field[idx[s,t]] over 8
  * terms; idx >= 0 indexes E, idx < 0 indexes H at -(idx + 1) (device
  * layout); weight 0 marks an unused term. n is read from *step_ctr. */
@@ -258,17 +259,21 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
  * (waxis_t < 0: a constant).  grp: G descriptors of 40 int32 words sorted by
  * start — [0] start [1] count [2] n0 [3] n1 [4] field (0 E, 1 H) [5] terms
  * [6..8] strides [9..16] bases [17..24] weight coordinates [25..32] weight
- * offsets.  Built by the host from the mgh_dft_sample map and checked
- * against it; the values are those of mgh_dft_sample bit for bit. */
+ * offsets [33] first work-group [34] mapping (0: 256 consecutive samples per
+ * work-group, 1: 16 x 16 patches of (c0, c1)) [35] work-groups; n_blocks is
+ * their total.  Built by the host from the mgh_dft_sample map and checked
+ * against it; the values are those of mgh_dft_sample bit for bit.  Samples
+ * outside every group (padding) are not written. */
 int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_stride,
                           const void* h, long long h_set_stride, const int* grp, int G,
-                          const float* wtab, int S, float* X, int S_pad, int B,
+                          int n_blocks, const float* wtab, int S, float* X, int S_pad, int B,
                           const long long* step_ctr, long long n_start, void* stream);
 
 /* mgh_dft_block: bins[set, f, s] += exp(j w_f t0) * sum_{b < nb_valid}
  * (Pc + j Ps)[f, b] * X[set, b, s], t0 = n0 dt for s < s_h_start and
  * n0 dt + dt/2 otherwise, n0 the first step of the block holding *step_ctr.
- * Pc/Ps: F_pad x 64 float32 tables cos/sin(w_f b dt) dt. bins: nsets x F x S
+ * Pc/Ps: F_pad x Bp float32 tables cos/sin(w_f b dt) dt in the kernel's lane
+ * order (Bp: B rounded up to 64 steps up to 256, 512 beyond). bins: nsets x F x S
  * complex float64 (re/im interleaved); lo (same shape, or NULL) holds TwoSum
  * compensation terms. blk: F x 4 float64 scratch. Contraction on the MFMA
  * units (v_mfma_f32_16x16x4f32). */

@@ -114,8 +114,8 @@ SIGNATURES: dict[str, tuple] = {
     ),
     "mgh_dft_sample_groups": (
         _c_int,
-        [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_int, _c_void_p, _c_int,
-         _c_void_p, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p],
+        [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_int, _c_int, _c_void_p,
+         _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p],
     ),
     "mgh_dft_block": (
         _c_int,

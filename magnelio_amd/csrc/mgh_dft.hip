@@ -14,7 +14,7 @@
 //   t0 = n0 dt (E samples) or n0 dt + dt/2 (H samples).
 // The inner sum is a (2F x B) x (B x S) product done with
 // v_mfma_f32_16x16x4f32 (16 frequencies x 16 samples per wave, K = B in B/4
-// MFMA steps, B up to 256); the outer factor exp(j w t0) is float64 and the
+// MFMA steps, B up to 512); the outer factor exp(j w t0) is float64 and the
 // update of the float64 bins is fused into the MFMA epilogue, so the partial
 // products never reach HBM.  Each wave keeps its 16-sample x B-step operand
 // in registers and sweeps every frequency, so x is read once per block and
@@ -28,11 +28,11 @@ namespace {
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int DFT_NT = 8;      // terms per sample (cell-centre averaging)
-constexpr int DFT_BMAX = 256;  // longest block (MFMA K)
+constexpr int DFT_BMAX = 512;  // longest block (MFMA K)
 
-// phasor-table row length for a block of B steps (a whole number of 64-step
-// register tiles)
-inline int dft_bpad(int B) { return (B + 63) / 64 * 64; }
+// phasor-table row length for a block of B steps: a whole number of 64-step
+// register tiles up to 256, 512 beyond
+inline int dft_bpad(int B) { return B <= 256 ? (B + 63) / 64 * 64 : 512; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dft_sample(const T* __restrict__ e, int64_t e_set,
@@ -70,43 +70,61 @@ __global__ __launch_bounds__(256) void k_dft_sample(const T* __restrict__ e, int
 // generic map (the host derives the groups from it and checks them), so the
 // sum — taps in map order, zero weights skipped — is the same float32 value;
 // only the 64 B of per-sample index/weight words per step are gone.
+//
+// Each group owns a range of work-groups.  A group whose c0 is unit-stride
+// in memory (faces normal to y or z, boxes) maps 256 consecutive samples to
+// a work-group, so every tap load is one contiguous row.  A group whose
+// fastest coordinate is strided (faces normal to x: c0 = y, c1 = z) maps a
+// 16 x 16 patch of (c0, c1) to a work-group instead: each sample's taps sit
+// in the rows of its own and the next c0 / c1 neighbours, which then belong
+// to the same work-group and are fetched once, not once per neighbour.
 constexpr int DG_W = 40;  // int32 words per group descriptor
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dft_sample_groups(
     const T* __restrict__ e, int64_t e_set, const T* __restrict__ h, int64_t h_set,
-    const int* __restrict__ grp, int G, const float* __restrict__ wtab, int S,
-    float* __restrict__ X, int S_pad, int B, const long long* __restrict__ ctr,
-    long long n_start) {
-  const int s_first = blockIdx.x * blockDim.x;
-  const int s = s_first + threadIdx.x;
+    const int* __restrict__ grp, int G, const float* __restrict__ wtab, float* __restrict__ X,
+    int S_pad, int B, const long long* __restrict__ ctr, long long n_start) {
+  const int blk = blockIdx.x;
   const int set = blockIdx.y;
-  // group of the work-group's first sample (uniform), then per lane
+  // group of this work-group: the last whose first block is <= blk (uniform)
   int g = 0;
-  while (g + 1 < G && grp[(g + 1) * DG_W] <= s_first) ++g;
-  while (g + 1 < G && grp[(g + 1) * DG_W] <= s) ++g;
-  if (s >= S) return;
+  while (g + 1 < G && grp[(g + 1) * DG_W + 33] <= blk) ++g;
+  const int* d = grp + g * DG_W;
+  const int n0 = d[2], n1 = d[3];
+  const int n2 = d[1] / (n0 * n1);
+  const int wb = blk - d[33];  // work-group within the group
+  int c0, c1, c2;
+  if (d[34] == 0) {  // linear: 256 consecutive samples
+    const int loc = wb * 256 + (int)threadIdx.x;
+    if (loc >= d[1]) return;
+    c0 = loc % n0;
+    const int r = loc / n0;
+    c1 = r % n1;
+    c2 = r / n1;
+  } else {  // 16 x 16 patches of (c0, c1)
+    const int p0 = (n0 + 15) >> 4, p1 = (n1 + 15) >> 4;
+    const int tc0 = wb % p0;
+    const int r = wb / p0;
+    const int tc1 = r % p1;
+    c2 = r / p1;
+    c0 = tc0 * 16 + (int)(threadIdx.x & 15);
+    c1 = tc1 * 16 + (int)(threadIdx.x >> 4);
+    if (c0 >= n0 || c1 >= n1 || c2 >= n2) return;
+  }
   const long long n = *ctr;
   const int b = (int)((n - n_start) % B);
-  const int* d = grp + g * DG_W;
-  const int loc = s - d[0];
+  const int64_t off = (int64_t)c0 * d[6] + (int64_t)c1 * d[7] + (int64_t)c2 * d[8];
+  const T* f = d[4] ? h + (int64_t)set * h_set : e + (int64_t)set * e_set;
+  const int nt = d[5];
   float acc = 0.0f;
-  if (loc >= 0 && loc < d[1]) {
-    const int n0 = d[2], n1 = d[3];
-    const int c0 = loc % n0;
-    const int r = loc / n0;
-    const int c1 = r % n1;
-    const int c2 = r / n1;
-    const int64_t off = (int64_t)c0 * d[6] + (int64_t)c1 * d[7] + (int64_t)c2 * d[8];
-    const T* f = d[4] ? h + (int64_t)set * h_set : e + (int64_t)set * e_set;
-    const int nt = d[5];
-    for (int t = 0; t < nt; ++t) {
-      const int ax = d[17 + t];
-      const int c = ax == 0 ? c0 : (ax == 1 ? c1 : (ax == 2 ? c2 : 0));
-      const float wt = wtab[d[25 + t] + c];
-      if (wt != 0.0f) acc += wt * (float)f[(int64_t)d[9 + t] + off];
-    }
+  for (int t = 0; t < nt; ++t) {
+    const int ax = d[17 + t];
+    const int c = ax == 0 ? c0 : (ax == 1 ? c1 : (ax == 2 ? c2 : 0));
+    const float wt = wtab[d[25 + t] + c];
+    if (wt != 0.0f) acc += wt * (float)f[(int64_t)d[9 + t] + off];
   }
+  const int s = d[0] + c0 + n0 * (c1 + n1 * c2);
   X[((int64_t)set * B + b) * S_pad + s] = acc;
 }
 
@@ -139,14 +157,16 @@ __device__ __forceinline__ void two_sum_acc(double* hi, double* lo, double v) {
   *lo += err;
 }
 
-// NB: MFMA k-steps per block (B padded to a multiple of 64, divided by 4).
+// NB: MFMA k-steps per block (the padded block length divided by 4).
 // COMP: TwoSum-compensated bins.
 //
 // The four waves of a work-group take 64 consecutive samples and share each
 // frequency tile's phasor operands: the work-group copies the tile (cos and
-// sin, NB x 64 lanes) from L2 into LDS once, and every wave reads its lanes'
-// A operands from there (16-byte reads, lane-contiguous, no bank
-// conflicts) — a quarter of the L2 traffic of per-wave loads.  The bins of
+// sin, NB x 64 lanes) from L2 into LDS, at most 256 steps at a time, and
+// every wave reads its lanes' A operands from there (16-byte reads,
+// lane-contiguous, no bank conflicts) — a quarter of the L2 traffic of
+// per-wave loads.  A 512-step block takes the tile in two passes into the
+// same float32 accumulators; it halves the bin traffic per step.  The bins of
 // the tile are loaded before the MFMA chain so their HBM latency hides
 // behind the matrix work, and are read and written as complex pairs.
 template <int NB, bool COMP>
@@ -157,9 +177,12 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
                                                    const double* __restrict__ blk, int s_h_start,
                                                    double* __restrict__ bins,
                                                    double* __restrict__ lo) {
-  constexpr int NQ = NB / 4;  // float4 groups per lane
-  __shared__ float4 sc[NQ * 64];
-  __shared__ float4 ss[NQ * 64];
+  constexpr int NQ = NB / 4;                // float4 groups per lane
+  constexpr int NQP = NQ < 16 ? NQ : 16;    // groups per LDS pass (256 steps)
+  constexpr int NP = NQ / NQP;              // passes
+  static_assert(NQ % NQP == 0, "block tile must be whole passes");
+  __shared__ float4 sc[NQP * 64];
+  __shared__ float4 ss[NQP * 64];
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int set = blockIdx.y;
@@ -190,30 +213,34 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
         if (COMP) lv[r] = lset[o];
       }
     }
-    // this tile's phasor operands, [q][lane] float4s, into LDS
+    // this tile's phasor operands, [q][lane] float4s, into LDS (pass by pass)
     const float4* gc = (const float4*)Pc + (int64_t)(f0 >> 4) * NQ * 64;
     const float4* gs = (const float4*)Ps + (int64_t)(f0 >> 4) * NQ * 64;
-    __syncthreads();
-    for (int g = threadIdx.x; g < NQ * 64; g += 256) {
-      sc[g] = gc[g];
-      ss[g] = gs[g];
-    }
-    __syncthreads();
     v4f cc = {0.f, 0.f, 0.f, 0.f};
     v4f cs = {0.f, 0.f, 0.f, 0.f};
-    if (active) {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const float4 ac = sc[q * 64 + lane];
-        const float4 as = ss[q * 64 + lane];
-        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.x, xb[4 * q + 0], cc, 0, 0, 0);
-        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, xb[4 * q + 0], cs, 0, 0, 0);
-        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.y, xb[4 * q + 1], cc, 0, 0, 0);
-        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, xb[4 * q + 1], cs, 0, 0, 0);
-        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.z, xb[4 * q + 2], cc, 0, 0, 0);
-        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, xb[4 * q + 2], cs, 0, 0, 0);
-        cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.w, xb[4 * q + 3], cc, 0, 0, 0);
-        cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, xb[4 * q + 3], cs, 0, 0, 0);
+    for (int pass = 0; pass < NP; ++pass) {
+      __syncthreads();
+      for (int g = threadIdx.x; g < NQP * 64; g += 256) {
+        sc[g] = gc[pass * NQP * 64 + g];
+        ss[g] = gs[pass * NQP * 64 + g];
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < NQP; ++q) {
+          const int qq = pass * NQP + q;
+          const float4 ac = sc[q * 64 + lane];
+          const float4 as = ss[q * 64 + lane];
+          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.x, xb[4 * qq + 0], cc, 0, 0, 0);
+          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, xb[4 * qq + 0], cs, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.y, xb[4 * qq + 1], cc, 0, 0, 0);
+          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, xb[4 * qq + 1], cs, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.z, xb[4 * qq + 2], cc, 0, 0, 0);
+          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, xb[4 * qq + 2], cs, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.w, xb[4 * qq + 3], cc, 0, 0, 0);
+          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, xb[4 * qq + 3], cs, 0, 0, 0);
+        }
       }
     }
 #pragma unroll
@@ -254,7 +281,7 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
                    int S_pad, int B, const long long* step_ctr, long long n_start, void* stream) {
   MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8");
   MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts S=%d S_pad=%d", S, S_pad);
-  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..256, got %d", B);
+  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..512, got %d", B);
   MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
   MGH_REQUIRE(e && h && idx && w && X && step_ctr, "null pointer");
   dim3 grd((S + 255) / 256, nsets, 1);
@@ -275,27 +302,31 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
 //   [6..8] index strides of c0, c1, c2  [9..16] base index per term
 //   [17..24] weight coordinate per term (0..2, or -1: constant)
 //   [25..32] offset of the term's weight table in wtab
-// Samples outside every group's [start, start + count) are written as 0.
+//   [33] first work-group of the group  [34] mapping (0 linear, 1 16 x 16
+//   patches of (c0, c1))  [35] work-groups of the group
+// n_blocks: the work-groups of all groups.  Samples outside every group are
+// not written (the X buffer holds zeros there from its allocation).
 int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_stride,
                           const void* h, long long h_set_stride, const int* grp, int G,
-                          const float* wtab, int S, float* X, int S_pad, int B,
+                          int n_blocks, const float* wtab, int S, float* X, int S_pad, int B,
                           const long long* step_ctr, long long n_start, void* stream) {
   MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8");
   MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts S=%d S_pad=%d", S, S_pad);
-  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..256, got %d", B);
+  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..512, got %d", B);
   MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
   MGH_REQUIRE(G >= 1 && G <= 4096, "bad group count %d", G);
+  MGH_REQUIRE(n_blocks >= 1, "bad work-group count %d", n_blocks);
   MGH_REQUIRE(e && h && grp && wtab && X && step_ctr, "null pointer");
-  dim3 grd((S + 255) / 256, nsets, 1);
+  dim3 grd((unsigned)n_blocks, nsets, 1);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == 4)
     hipLaunchKernelGGL(k_dft_sample_groups<float>, grd, dim3(256), 0, st, (const float*)e,
-                       e_set_stride, (const float*)h, h_set_stride, grp, G, wtab, S, X, S_pad, B,
+                       e_set_stride, (const float*)h, h_set_stride, grp, G, wtab, X, S_pad, B,
                        step_ctr, n_start);
   else
     hipLaunchKernelGGL(k_dft_sample_groups<double>, grd, dim3(256), 0, st, (const double*)e,
-                       e_set_stride, (const double*)h, h_set_stride, grp, G, wtab, S, X, S_pad,
-                       B, step_ctr, n_start);
+                       e_set_stride, (const double*)h, h_set_stride, grp, G, wtab, X, S_pad, B,
+                       step_ctr, n_start);
   MGH_CHECK_LAUNCH("k_dft_sample_groups");
   return MGH_OK;
 }
@@ -340,7 +371,8 @@ int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_val
   if (bp == 64) MGH_DFT_LAUNCH(16);
   else if (bp == 128) MGH_DFT_LAUNCH(32);
   else if (bp == 192) MGH_DFT_LAUNCH(48);
-  else MGH_DFT_LAUNCH(64);
+  else if (bp == 256) MGH_DFT_LAUNCH(64);
+  else MGH_DFT_LAUNCH(128);
 #undef MGH_DFT_LAUNCH
   MGH_CHECK_LAUNCH("k_dft_block");
   (void)F_pad;

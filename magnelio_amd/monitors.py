@@ -30,7 +30,7 @@ import torch
 from . import _lib, layout
 
 NT = 8
-BMAX = 256  # longest block (steps per MFMA contraction)
+BMAX = 512  # longest block (steps per MFMA contraction)
 BLOCK = 64  # default block
 TANGENTIALS = {0: ("Ey", "Ez", "Hy", "Hz"), 1: ("Ex", "Ez", "Hx", "Hz"), 2: ("Ex", "Ey", "Hx", "Hy")}
 COMPS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
@@ -53,6 +53,12 @@ def natural(vals: np.ndarray, shape) -> np.ndarray:
     nd = len(shape)
     arr = vals.reshape(*lead, *shape[::-1])
     return arr.transpose(*range(len(lead)), *range(len(lead) + nd - 1, len(lead) - 1, -1))
+
+
+def block_pad(block: int) -> int:
+    """Phasor-table row length of a block: whole 64-step register tiles up to
+    256 steps, 512 beyond (the contraction kernel's variants)."""
+    return (block + 63) // 64 * 64 if block <= 256 else 512
 
 
 def lane_order(P: np.ndarray) -> np.ndarray:
@@ -209,6 +215,16 @@ def structured_groups(dev_idx: np.ndarray, w32: np.ndarray, groups: list):
     if not desc:
         return None
     arr = np.stack(desc)
+    # work-groups of each group: 256 consecutive samples where c0 is
+    # unit-stride in memory, 16 x 16 patches of (c0, c1) where it is strided
+    first = 0
+    for d in arr:
+        n0, n1 = int(d[2]), int(d[3])
+        n2 = int(d[1]) // (n0 * n1)
+        tiled = abs(int(d[6])) > 1
+        nblk = (-(-n0 // 16)) * (-(-n1 // 16)) * n2 if tiled else -(-int(d[1]) // 256)
+        d[33], d[34], d[35] = first, int(tiled), nblk
+        first += nblk
     if np.abs(arr).max() >= 2**31 - 1:
         return None
     return arr.astype(np.int32), np.concatenate(tabs).astype(np.float32)
@@ -321,7 +337,7 @@ class DFTMonitorBase:
         dt = solver.dt
         self.dt = dt
         omega = 2.0 * math.pi * self.freqs
-        bp = (self.block + 63) // 64 * 64
+        bp = block_pad(self.block)
         b = np.arange(bp)
         ph = np.outer(omega, b * dt)
         Pc = np.zeros((self.F_pad, bp))
@@ -339,6 +355,7 @@ class DFTMonitorBase:
             if st is not None:
                 self.grp_d = torch.from_numpy(st[0]).to(dev)
                 self.wtab_d = torch.from_numpy(st[1]).to(dev)
+                self.grp_blocks = int(st[0][:, 35].astype(np.int64).sum())
         self.Pc = torch.from_numpy(lane_order(Pc)).to(dev, f32)
         self.Ps = torch.from_numpy(lane_order(Ps)).to(dev, f32)
         self.omega = torch.from_numpy(omega).to(dev, torch.float64)
@@ -361,8 +378,9 @@ class DFTMonitorBase:
         if self.grp_d is not None:
             _lib.call("mgh_dft_sample_groups", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
                       solver.h.data_ptr(), solver.h.shape[1], self.grp_d.data_ptr(),
-                      self.grp_d.shape[0], self.wtab_d.data_ptr(), self.S, self.X.data_ptr(),
-                      self.S_pad, self.block, solver.step_ctr.data_ptr(), self.n_start, sh)
+                      self.grp_d.shape[0], self.grp_blocks, self.wtab_d.data_ptr(), self.S,
+                      self.X.data_ptr(), self.S_pad, self.block, solver.step_ctr.data_ptr(),
+                      self.n_start, sh)
         else:
             _lib.call("mgh_dft_sample", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
                       solver.h.data_ptr(), solver.h.shape[1], self.idx_d.data_ptr(),

@@ -2,8 +2,9 @@
 
 * the block contraction against numpy on random inputs: ragged F (37) and S
   (70, H samples from 48), two field sets, a full and a partial block, plain
-  and compensated bins — a wrong MFMA operand or result lane layout gives
-  O(1) errors, float32 products give ~1e-7;
+  and compensated bins, blocks of 64-512 steps (one and two LDS passes) — a
+  wrong MFMA operand or result lane layout gives O(1) errors, float32
+  products give ~1e-7;
 * a region monitor inside the plain WR-90 run (C1, 300 steps: four full
   blocks and a partial one closed by ``finalize``) against the oracle's
   FieldDFT fed with float64 fields — FP64 and FP32 stepping;
@@ -18,7 +19,7 @@ import torch
 
 import cases
 from magnelio_amd import _lib
-from magnelio_amd.monitors import MonitorFieldFrequency, lane_order
+from magnelio_amd.monitors import MonitorFieldFrequency, block_pad, lane_order
 from oracle import monitors as omon
 
 pytestmark = pytest.mark.gpu
@@ -28,14 +29,14 @@ REGION = (slice(10, 30), slice(4, 20), slice(30, 60))
 N_STEPS = 300
 
 
-@pytest.mark.parametrize("B", [64, 100, 256])
+@pytest.mark.parametrize("B", [64, 100, 256, 300, 512])
 def test_block_contraction_matches_numpy(B, gpu):
     rng = np.random.default_rng(7)
     K, S, s_h, F = 2, 70, 48, 37
     S_pad, F_pad = 80, 48
     dt = 1.3e-12
     omega = 2 * np.pi * rng.uniform(1e9, 30e9, F)
-    bp = (B + 63) // 64 * 64
+    bp = block_pad(B)
     b = np.arange(bp)
     Pc = np.zeros((F_pad, bp), np.float32)
     Ps = np.zeros((F_pad, bp), np.float32)
@@ -117,7 +118,8 @@ def _errors(got, want):
 
 
 @pytest.mark.parametrize("precision,gate,block", [("double", 1e-5, 64), ("single", 2e-5, 64),
-                                                  ("double", 1e-5, 256), ("single", 2e-5, 256)])
+                                                  ("double", 1e-5, 256), ("single", 2e-5, 256),
+                                                  ("single", 2e-5, 512)])
 def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, gpu):
     print(f"HIP C1 {precision} + region DFT (block {block}), {N_STEPS} steps", flush=True)
     got = _hip_bins(precision, block=block)

@@ -145,3 +145,50 @@ def test_structured_groups_reject_non_affine():
     w_bad = w32.copy()
     w_bad[5, 0] *= np.float32(1.5)
     assert structured_groups(dev_idx, w_bad, mon._layout) is None
+
+
+def _kernel_cover(desc, S):
+    """Samples written by each (work-group, thread) of k_dft_sample_groups,
+    emulated on the host from the launch layout words [33..35]."""
+    hits = np.zeros(S, dtype=np.int64)
+    n_blocks = int(desc[:, 35].astype(np.int64).sum())
+    tid = np.arange(256)
+    for blk in range(n_blocks):
+        g = int(np.nonzero(desc[:, 33] <= blk)[0].max())
+        d = desc[g].astype(np.int64)
+        n0, n1 = int(d[2]), int(d[3])
+        n2 = int(d[1]) // (n0 * n1)
+        wb = blk - int(d[33])
+        if d[34] == 0:
+            loc = wb * 256 + tid
+            ok = loc < d[1]
+            c0, c1, c2 = loc % n0, (loc // n0) % n1, loc // (n0 * n1)
+        else:
+            p0, p1 = -(-n0 // 16), -(-n1 // 16)
+            c0 = (wb % p0) * 16 + (tid & 15)
+            c1 = ((wb // p0) % p1) * 16 + (tid >> 4)
+            c2 = np.full(256, wb // (p0 * p1))
+            ok = (c0 < n0) & (c1 < n1) & (c2 < n2)
+        s = d[0] + c0 + n0 * (c1 + n1 * c2)
+        np.add.at(hits, s[ok], 1)
+    return hits
+
+
+def test_structured_launch_covers_each_sample_once():
+    """The work-group layout (256 consecutive samples, or 16 x 16 patches for
+    the x-normal faces whose fastest coordinate is strided) writes every
+    sample of every group exactly once and nothing else (the E/H padding is
+    left to the buffer's zeros)."""
+    g, _ = _grid()
+    bcs = {f: CPMLBoundary(f, g, thickness_cells=3) for f in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")}
+    for mon, b in ((MonitorFarField([1e9], margin_cells=2), bcs),
+                   (MonitorFieldFrequency([1e9], region=(slice(2, 7), slice(0, 15), slice(4, 9))), None)):
+        dev_idx, w = mon.device_map(g, b)
+        desc, _ = structured_groups(dev_idx, w.astype(np.float32), mon._layout)
+        if isinstance(mon, MonitorFarField):
+            assert set(desc[:, 34]) == {0, 1}  # x faces tiled, y/z faces linear
+        hits = _kernel_cover(desc, mon.S)
+        want = np.zeros(mon.S, dtype=np.int64)
+        for d in desc:
+            want[d[0]:d[0] + d[1]] = 1
+        assert np.array_equal(hits, want)
