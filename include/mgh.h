@@ -84,10 +84,12 @@ int mgh_update_h_tiles(int dtype, int nx, int ny, int nz, int nsets, const void*
 
 /* ---- (a3)+(a7) / (a4)+(a7): curl phases with CPML ------------------------
  * Same as mgh_update_e / mgh_update_h, followed by the CPML auxiliary
- * recursion of every listed face over its slab, in record order — inside the
- * same launch: a slab element applies its face terms to its updated value
- * before the store (same values as mgh_update_* then mgh_cpml_*, bit for
- * bit).  Replaces CPMLBoundary.update_E /
+ * recursion of every listed face over its slab, in record order (same values
+ * as mgh_update_* then mgh_cpml_*, bit for bit), as two launches: the curl
+ * launch applies the y and z faces' terms to every element outside the x
+ * slabs before its store (mgh_update_*_pml_core), then a launch over the
+ * x-slab columns applies every face's terms there (mgh_cpml_xcols_*).
+ * Replaces CPMLBoundary.update_E /
  * update_H (src/magnelio/boundaries/cpml.py:343-516), called right after the
  * curl update in e_phase_device / h_phase_device
  * (src/magnelio/solver/fit_td.py:870-876, :918-921).
@@ -109,6 +111,23 @@ int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e
                      long long e_set_stride, void* h, long long h_set_stride,
                      const void* alpha_h, const void* beta_h, int kchunk, const void* cpml_desc,
                      void* stream);
+/* The two launches of mgh_update_e_pml / mgh_update_h_pml, callable apart
+ * (to time the curl launch alone): _pml_core, then _xcols on the same
+ * stream, equals the composed call. */
+int mgh_update_e_pml_core(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                          long long e_set_stride, const void* h, long long h_set_stride,
+                          const void* alpha_e, const void* beta_e, int curl64, int kchunk,
+                          const void* cpml_desc, void* stream);
+int mgh_update_h_pml_core(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                          long long e_set_stride, void* h, long long h_set_stride,
+                          const void* alpha_h, const void* beta_h, int kchunk,
+                          const void* cpml_desc, void* stream);
+int mgh_cpml_xcols_e(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                     long long e_set_stride, const void* h, long long h_set_stride,
+                     const void* beta_e, const void* cpml_desc, void* stream);
+int mgh_cpml_xcols_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                     long long e_set_stride, void* h, long long h_set_stride,
+                     const void* beta_h, const void* cpml_desc, void* stream);
 
 /* ---- (a7) CPML slab passes alone -------------------------------------------
  * The CPML part of mgh_update_e_pml / mgh_update_h_pml: for each face in

@@ -51,6 +51,26 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
          _c_void_p, _c_int, _c_void_p, _c_void_p],
     ),
+    "mgh_update_e_pml_core": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p],
+    ),
+    "mgh_update_h_pml_core": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_int, _c_void_p, _c_void_p],
+    ),
+    "mgh_cpml_xcols_e": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_void_p],
+    ),
+    "mgh_cpml_xcols_h": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,
+         _c_void_p, _c_void_p],
+    ),
     "mgh_cpml_e": (
         _c_int,
         [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,

@@ -221,15 +221,30 @@ class DeviceCPML:
     def desc_ptr(self) -> int:
         return int(self.desc.ctypes.data)
 
-    def slab_bytes(self, side: str) -> int:
-        """Bytes the CPML adds to one fused curl launch of ``side`` ("E" or
-        "H"): every psi word of every set read and written, the air masks
-        read once (the field, coefficient and neighbour words are the curl's
-        own; the layer coefficients are a few words)."""
+    def slab_bytes(self, side: str, in_curl: bool = False) -> int:
+        """Algorithmic bytes of the CPML of ``side`` ("E" or "H"): every psi
+        word of every set read and written and the air masks read once (the
+        field, coefficient and neighbour words are the curl's own when the
+        recursion rides in the curl launch; the layer coefficients are a few
+        words).  ``in_curl``: only the words the curl launch itself updates —
+        the y and z faces' elements outside every x slab (the x-slab columns
+        take a pass of their own)."""
         lst = self.psi_e if side == "E" else self.psi_h
-        words = sum(t.numel() for terms in lst for t in terms)
-        item = lst[0][0].element_size() if lst else 0
-        return int(2 * item * words + self._air_bytes[0 if side == "E" else 1])
+        if not lst:
+            return 0
+        item = lst[0][0].element_size()
+        if not in_curl:
+            return int(2 * item * sum(t.numel() for terms in lst for t in terms)
+                       + self._air_bytes[0 if side == "E" else 1])
+        x_layers = sum(f.n for f in self.faces if f.axis == 0)
+        words = 0.0
+        for f, terms in zip(self.faces, lst):
+            if f.axis == 0:
+                continue
+            shapes = f.psi_shapes(self.dims)[0 if side == "E" else 1]
+            for sh, t in zip(shapes, terms):
+                words += t.numel() * (sh[0] - x_layers) / sh[0]
+        return int(2 * item * words)
 
     def zero_(self) -> None:
         for lst in self.psi_e + self.psi_h:

@@ -40,15 +40,19 @@
 // units — the set groups of one tile first — land on the same XCD
 // (work-groups are dealt to the 8 XCDs round robin) and share its L2.
 //
-// CPML, two forms with the same values.  Fused (PML = true, the
+// CPML, two forms with the same values.  With the curl (the
 // mgh_update_*_pml entry points): every d of every face is a difference of
-// neighbour words the curl already holds in registers, so an element in a
-// slab applies its face terms, in descriptor order, to its updated value
-// before the single store; only psi, the layer coefficients and the air mask
-// add traffic.  One "node in some slab" test per element keeps the interior
-// on the plain path, and the face loop is not unrolled (its per-face values
-// are uniform scalar loads), which holds the register count within one
-// occupancy step of the plain kernel.  Separate (mgh_cpml_e / mgh_cpml_h):
+// neighbour words the curl already holds in registers, so an element in a y
+// or z slab applies those faces' terms, in descriptor order, to its updated
+// value before the single store; only psi, the layer coefficients and the
+// air mask add traffic.  One "node in some slab" test per element keeps the
+// interior on the plain path, and the face loop is not unrolled (its
+// per-face values are uniform scalar loads), which holds the register count
+// of the plain kernel.  Elements of the x slabs are left out of the curl
+// launch: they are a few lanes of most waves, and running their recursion
+// there stalls the whole wave (measured +25 us per phase on 256^3); a launch
+// over the x-slab columns (k_cpml_xcols_e/h) applies all their faces in
+// order after the curl.  Separate (mgh_cpml_e / mgh_cpml_h):
 // one small launch per face over its slab, in descriptor order — the
 // structure of the reference (cpml.update_E / update_H per face after the
 // curl pass) — for listed launches and for dispersive meshes, whose ADE
@@ -150,8 +154,10 @@ struct PFace {
   const unsigned char* air;
 };
 
-// Faces in descriptor order, and per axis the (up to two) slab intervals of
-// the node index, [lo, lo + len), for the quick "node in some slab" test.
+// The y and z faces in descriptor order, and per axis the (up to two) slab
+// intervals of the node index, [lo, lo + len).  The curl launch applies
+// these faces to the elements outside every x slab; the x-slab columns are
+// left entirely (all their faces, in order) to k_cpml_xcols_e/h.
 template <typename T>
 struct PmlArgs {
   int nf;
@@ -161,12 +167,13 @@ struct PmlArgs {
 
 template <typename T>
 __device__ __forceinline__ bool pml_hit(const PmlArgs<T>& pa, int i, int j, int k) {
-  return (unsigned)(i - pa.lo[0][0]) < (unsigned)pa.len[0][0] ||
-         (unsigned)(i - pa.lo[0][1]) < (unsigned)pa.len[0][1] ||
-         (unsigned)(j - pa.lo[1][0]) < (unsigned)pa.len[1][0] ||
-         (unsigned)(j - pa.lo[1][1]) < (unsigned)pa.len[1][1] ||
-         (unsigned)(k - pa.lo[2][0]) < (unsigned)pa.len[2][0] ||
-         (unsigned)(k - pa.lo[2][1]) < (unsigned)pa.len[2][1];
+  const bool xs = (unsigned)(i - pa.lo[0][0]) < (unsigned)pa.len[0][0] ||
+                  (unsigned)(i - pa.lo[0][1]) < (unsigned)pa.len[0][1];
+  const bool yz = (unsigned)(j - pa.lo[1][0]) < (unsigned)pa.len[1][0] ||
+                  (unsigned)(j - pa.lo[1][1]) < (unsigned)pa.len[1][1] ||
+                  (unsigned)(k - pa.lo[2][0]) < (unsigned)pa.len[2][0] ||
+                  (unsigned)(k - pa.lo[2][1]) < (unsigned)pa.len[2][1];
+  return yz && !xs;
 }
 
 // Term `term` of face F on a component whose psi slab has x/y extents
@@ -640,6 +647,245 @@ __global__ __launch_bounds__(WG) void k_cpml_h(const T* __restrict__ e, T* __res
   }
 }
 
+// CPML of the x-slab columns: the elements of every x-normal face's slab,
+// which the curl launch leaves alone when it carries the CPML.  Each element
+// applies the terms of every face whose slab holds it (x, y and z faces), in
+// descriptor order, to its freshly updated value — the d of each term
+// computed as the separate slab passes above compute it — so the result is
+// theirs bit for bit.  Threads cover each x face's slab layer-fastest (the
+// psi order): one thread per node (l, j, k), all faces' x slabs back to back.
+template <typename T>
+struct XCols {
+  int nf;  // every face, descriptor order
+  int axis[MGH_CPML_MAX_FACES];
+  CFace<T> f[MGH_CPML_MAX_FACES];
+  int nx_faces;
+  int xk0[2], xn[2];
+  int64_t xthreads0;  // threads of the first x face
+};
+
+template <typename T>
+__device__ __forceinline__ bool xcols_node(const XCols<T>& X, int64_t t, int ny, int nz, int& i,
+                                           int& j, int& k) {
+  int q = 0;
+  if (t >= X.xthreads0) {
+    t -= X.xthreads0;
+    q = 1;
+    if (q >= X.nx_faces) return false;
+  }
+  const int n = X.xn[q];
+  const int64_t per = (int64_t)n * (ny + 1);
+  if (t >= per * (nz + 1)) return false;
+  k = (int)(t / per);
+  const int r = (int)(t - (int64_t)k * per);
+  j = r / n;
+  i = X.xk0[q] + (r - j * n);
+  return true;
+}
+
+// layer of coordinate c in face F's slab, or -1
+template <typename T>
+__device__ __forceinline__ int in_slab(const CFace<T>& F, int c) {
+  const int l = c - F.k0;
+  return (unsigned)l < (unsigned)F.n ? l : -1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_cpml_xcols_e(T* __restrict__ e, const T* __restrict__ h,
+                                                     const T* __restrict__ be, int nx, int ny,
+                                                     int nz, int64_t e_set, int64_t h_set,
+                                                     XCols<T> X) {
+  int i, j, k;
+  if (!xcols_node(X, (int64_t)blockIdx.x * WG + threadIdx.x, ny, nz, i, j, k)) return;
+  const int s = blockIdx.y;
+  const mgh::Dims d{nx, ny, nz};
+  T* es = e + (int64_t)s * e_set;
+  const T* hs = h + (int64_t)s * h_set;
+  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
+  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
+  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
+                pEz = (int64_t)(ny + 1) * (nx + 1);
+  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
+                pHz = (int64_t)ny * nx;
+  const int64_t rN = (int64_t)j * nx + i, rN1 = (int64_t)j * (nx + 1) + i;
+  auto Hx = [&](int di, int dj, int dk) { return hs[(int64_t)(k + dk) * pHx + rN1 + dj * (nx + 1) + di]; };
+  auto Hy = [&](int di, int dj, int dk) { return hs[oHy + (int64_t)(k + dk) * pHy + rN + dj * nx + di]; };
+  auto Hz = [&](int di, int dj, int dk) { return hs[oHz + (int64_t)(k + dk) * pHz + rN + dj * nx + di]; };
+  if (i < nx) {  // Ex: z faces (term 0, +), y faces (term 0, -)
+    const int64_t ie = (int64_t)k * pEx + rN;
+    T v = es[ie];
+    bool any = false;
+#pragma unroll 1
+    for (int f = 0; f < X.nf; ++f) {
+      const CFace<T>& F = X.f[f];
+      if (X.axis[f] == 2) {
+        const int l = in_slab(F, k);
+        if (l < 0) continue;
+        const int64_t s0 = (int64_t)F.n * nx * (ny + 1);
+        const T dd = Hy(0, 0, 0) - (k > 0 ? Hy(0, 0, -1) : T(0));
+        cpml_term(F, 0, s0, s0, slab_index<2>(i, j, k, F.k0, F.n, nx, ny + 1), s, l, dd, be[ie], &v, +1);
+        any = true;
+      } else if (X.axis[f] == 1) {
+        const int l = in_slab(F, j);
+        if (l < 0) continue;
+        const int64_t s0 = (int64_t)F.n * nx * (nz + 1);
+        const T dd = Hz(0, 0, 0) - (j > 0 ? Hz(0, -1, 0) : T(0));
+        cpml_term(F, 0, s0, s0, slab_index<1>(i, j, k, F.k0, F.n, nx, ny + 1), s, l, dd, be[ie], &v, -1);
+        any = true;
+      }
+    }
+    if (any) es[ie] = v;
+  }
+  if (j < ny) {  // Ey: z faces (term 1, -), x faces (term 0, +)
+    const int64_t ie = oEy + (int64_t)k * pEy + rN1;
+    T v = es[ie];
+    bool any = false;
+#pragma unroll 1
+    for (int f = 0; f < X.nf; ++f) {
+      const CFace<T>& F = X.f[f];
+      if (X.axis[f] == 2) {
+        const int l = in_slab(F, k);
+        if (l < 0) continue;
+        const T dd = Hx(0, 0, 0) - (k > 0 ? Hx(0, 0, -1) : T(0));
+        cpml_term(F, 1, (int64_t)F.n * nx * (ny + 1), (int64_t)F.n * (nx + 1) * ny,
+                  slab_index<2>(i, j, k, F.k0, F.n, nx + 1, ny), s, l, dd, be[ie], &v, -1);
+        any = true;
+      } else if (X.axis[f] == 0) {
+        const int l = in_slab(F, i);
+        if (l < 0) continue;
+        const int64_t s0 = (int64_t)F.n * ny * (nz + 1);
+        const T dd = Hz(0, 0, 0) - (i > 0 ? Hz(-1, 0, 0) : T(0));
+        cpml_term(F, 0, s0, s0, slab_index<0>(i, j, k, F.k0, F.n, nx + 1, ny), s, l, dd, be[ie], &v, +1);
+        any = true;
+      }
+    }
+    if (any) es[ie] = v;
+  }
+  if (k < nz) {  // Ez: y faces (term 1, +), x faces (term 1, -)
+    const int64_t ie = oEz + (int64_t)k * pEz + rN1;
+    T v = es[ie];
+    bool any = false;
+#pragma unroll 1
+    for (int f = 0; f < X.nf; ++f) {
+      const CFace<T>& F = X.f[f];
+      if (X.axis[f] == 1) {
+        const int l = in_slab(F, j);
+        if (l < 0) continue;
+        const T dd = Hx(0, 0, 0) - (j > 0 ? Hx(0, -1, 0) : T(0));
+        cpml_term(F, 1, (int64_t)F.n * nx * (nz + 1), (int64_t)F.n * (nx + 1) * nz,
+                  slab_index<1>(i, j, k, F.k0, F.n, nx + 1, ny + 1), s, l, dd, be[ie], &v, +1);
+        any = true;
+      } else if (X.axis[f] == 0) {
+        const int l = in_slab(F, i);
+        if (l < 0) continue;
+        const T dd = Hy(0, 0, 0) - (i > 0 ? Hy(-1, 0, 0) : T(0));
+        cpml_term(F, 1, (int64_t)F.n * ny * (nz + 1), (int64_t)F.n * (ny + 1) * nz,
+                  slab_index<0>(i, j, k, F.k0, F.n, nx + 1, ny + 1), s, l, dd, be[ie], &v, -1);
+        any = true;
+      }
+    }
+    if (any) es[ie] = v;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_cpml_xcols_h(const T* __restrict__ e, T* __restrict__ h,
+                                                     const T* __restrict__ bh, int nx, int ny,
+                                                     int nz, int64_t e_set, int64_t h_set,
+                                                     XCols<T> X) {
+  int i, j, k;
+  if (!xcols_node(X, (int64_t)blockIdx.x * WG + threadIdx.x, ny, nz, i, j, k)) return;
+  const int s = blockIdx.y;
+  const mgh::Dims d{nx, ny, nz};
+  const T* es = e + (int64_t)s * e_set;
+  T* hs = h + (int64_t)s * h_set;
+  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
+  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
+  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
+                pEz = (int64_t)(ny + 1) * (nx + 1);
+  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
+                pHz = (int64_t)ny * nx;
+  const int64_t rN = (int64_t)j * nx + i, rN1 = (int64_t)j * (nx + 1) + i;
+  auto Ex = [&](int di, int dj, int dk) { return es[(int64_t)(k + dk) * pEx + rN + dj * nx + di]; };
+  auto Ey = [&](int di, int dj, int dk) { return es[oEy + (int64_t)(k + dk) * pEy + rN1 + dj * (nx + 1) + di]; };
+  auto Ez = [&](int di, int dj, int dk) { return es[oEz + (int64_t)(k + dk) * pEz + rN1 + dj * (nx + 1) + di]; };
+  if (j < ny && k < nz) {  // Hx: z faces (term 0, -), y faces (term 0, +)
+    const int64_t ih = (int64_t)k * pHx + rN1;
+    T v = hs[ih];
+    bool any = false;
+#pragma unroll 1
+    for (int f = 0; f < X.nf; ++f) {
+      const CFace<T>& F = X.f[f];
+      if (X.axis[f] == 2) {
+        const int l = in_slab(F, k);
+        if (l < 0) continue;
+        const int64_t s0 = (int64_t)F.n * (nx + 1) * ny;
+        const T dd = Ey(0, 0, 1) - Ey(0, 0, 0);
+        cpml_term(F, 0, s0, s0, slab_index<2>(i, j, k, F.k0, F.n, nx + 1, ny), s, l, dd, bh[ih], &v, -1);
+        any = true;
+      } else if (X.axis[f] == 1) {
+        const int l = in_slab(F, j);
+        if (l < 0) continue;
+        const int64_t s0 = (int64_t)F.n * (nx + 1) * nz;
+        const T dd = Ez(0, 1, 0) - Ez(0, 0, 0);
+        cpml_term(F, 0, s0, s0, slab_index<1>(i, j, k, F.k0, F.n, nx + 1, ny), s, l, dd, bh[ih], &v, +1);
+        any = true;
+      }
+    }
+    if (any) hs[ih] = v;
+  }
+  if (k < nz) {  // Hy: z faces (term 1, +), x faces (term 0, -)
+    const int64_t ih = oHy + (int64_t)k * pHy + rN;
+    T v = hs[ih];
+    bool any = false;
+#pragma unroll 1
+    for (int f = 0; f < X.nf; ++f) {
+      const CFace<T>& F = X.f[f];
+      if (X.axis[f] == 2) {
+        const int l = in_slab(F, k);
+        if (l < 0) continue;
+        const T dd = Ex(0, 0, 1) - Ex(0, 0, 0);
+        cpml_term(F, 1, (int64_t)F.n * (nx + 1) * ny, (int64_t)F.n * nx * (ny + 1),
+                  slab_index<2>(i, j, k, F.k0, F.n, nx, ny + 1), s, l, dd, bh[ih], &v, +1);
+        any = true;
+      } else if (X.axis[f] == 0) {
+        const int l = in_slab(F, i);
+        if (l < 0) continue;
+        const int64_t s0 = (int64_t)F.n * (ny + 1) * nz;
+        const T dd = Ez(1, 0, 0) - Ez(0, 0, 0);
+        cpml_term(F, 0, s0, s0, slab_index<0>(i, j, k, F.k0, F.n, nx, ny + 1), s, l, dd, bh[ih], &v, -1);
+        any = true;
+      }
+    }
+    if (any) hs[ih] = v;
+  }
+  if (j < ny) {  // Hz: y faces (term 1, -), x faces (term 1, +)
+    const int64_t ih = oHz + (int64_t)k * pHz + rN;
+    T v = hs[ih];
+    bool any = false;
+#pragma unroll 1
+    for (int f = 0; f < X.nf; ++f) {
+      const CFace<T>& F = X.f[f];
+      if (X.axis[f] == 1) {
+        const int l = in_slab(F, j);
+        if (l < 0) continue;
+        const T dd = Ex(0, 1, 0) - Ex(0, 0, 0);
+        cpml_term(F, 1, (int64_t)F.n * (nx + 1) * nz, (int64_t)F.n * nx * (nz + 1),
+                  slab_index<1>(i, j, k, F.k0, F.n, nx, ny), s, l, dd, bh[ih], &v, -1);
+        any = true;
+      } else if (X.axis[f] == 0) {
+        const int l = in_slab(F, i);
+        if (l < 0) continue;
+        const T dd = Ey(1, 0, 0) - Ey(0, 0, 0);
+        cpml_term(F, 1, (int64_t)F.n * (ny + 1) * nz, (int64_t)F.n * ny * (nz + 1),
+                  slab_index<0>(i, j, k, F.k0, F.n, nx, ny), s, l, dd, bh[ih], &v, +1);
+        any = true;
+      }
+    }
+    if (any) hs[ih] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -683,17 +929,21 @@ int64_t slab_threads(int axis, int nx, int ny, int nz, int n) {
   return (int64_t)n * (ny + 1) * (nz + 1);
 }
 
-// faces of a CPML set for a fused launch (E or H side)
+// the y and z faces of a CPML set for a fused launch (E or H side), and the
+// slab intervals of every face (x ones to exclude the x-slab columns)
 template <typename T>
 PmlArgs<T> pml_args(const mgh::CpmlSet* pml, bool e_side, int nx, int ny, int nz) {
   PmlArgs<T> a;
   std::memset(&a, 0, sizeof(a));  // empty intervals: len 0
   if (!pml) return a;
-  a.nf = pml->nfaces;
   int used[3] = {0, 0, 0};
   for (int f = 0; f < pml->nfaces; ++f) {
     const mgh::CpmlFace& F = pml->f[f];
-    PFace<T>& P = a.f[f];
+    const int u = used[F.axis]++;  // at most two faces per axis (load_cpml_set)
+    a.lo[F.axis][u] = F.k0;
+    a.len[F.axis][u] = F.n;
+    if (F.axis == 0) continue;
+    PFace<T>& P = a.f[a.nf++];
     P.axis = F.axis;
     P.k0 = F.k0;
     P.n = F.n;
@@ -705,18 +955,34 @@ PmlArgs<T> pml_args(const mgh::CpmlSet* pml, bool e_side, int nx, int ny, int nz
     // term slab sizes: n times the component's extents along the other axes
     if (e_side) {
       if (F.axis == 2) { P.sz0 = n * nx * (ny + 1); P.sz1 = n * (nx + 1) * ny; }
-      else if (F.axis == 1) { P.sz0 = n * nx * (nz + 1); P.sz1 = n * (nx + 1) * nz; }
-      else { P.sz0 = n * ny * (nz + 1); P.sz1 = n * (ny + 1) * nz; }
+      else { P.sz0 = n * nx * (nz + 1); P.sz1 = n * (nx + 1) * nz; }
     } else {
       if (F.axis == 2) { P.sz0 = n * (nx + 1) * ny; P.sz1 = n * nx * (ny + 1); }
-      else if (F.axis == 1) { P.sz0 = n * (nx + 1) * nz; P.sz1 = n * nx * (nz + 1); }
-      else { P.sz0 = n * (ny + 1) * nz; P.sz1 = n * ny * (nz + 1); }
+      else { P.sz0 = n * (nx + 1) * nz; P.sz1 = n * nx * (nz + 1); }
     }
-    const int u = used[F.axis]++;  // at most two faces per axis (load_cpml_set)
-    a.lo[F.axis][u] = F.k0;
-    a.len[F.axis][u] = F.n;
   }
   return a;
+}
+
+// every face of a CPML set and its x faces' slabs, for the x-column pass
+template <typename T>
+XCols<T> xcols_args(const mgh::CpmlSet& pml, bool e_side, int ny, int nz, int64_t* threads) {
+  XCols<T> x;
+  std::memset(&x, 0, sizeof(x));
+  x.nf = pml.nfaces;
+  for (int f = 0; f < pml.nfaces; ++f) {
+    x.axis[f] = pml.f[f].axis;
+    x.f[f] = face_of<T>(pml.f[f], e_side);
+    if (pml.f[f].axis == 0) {
+      x.xk0[x.nx_faces] = pml.f[f].k0;
+      x.xn[x.nx_faces] = pml.f[f].n;
+      x.nx_faces++;
+    }
+  }
+  const int64_t per = (int64_t)(ny + 1) * (nz + 1);
+  x.xthreads0 = x.nx_faces ? (int64_t)x.xn[0] * per : 0;
+  *threads = x.nx_faces ? x.xthreads0 + (x.nx_faces > 1 ? (int64_t)x.xn[1] * per : 0) : 0;
+  return x;
 }
 
 // pml: CPML faces fused into the curl launch (null: curl only)
@@ -728,15 +994,16 @@ int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny
   const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
   if (pl.units == 0) return MGH_OK;
   const PmlArgs<T> pa = pml_args<T>(pml, true, nx, ny, nz);
+  const bool fused = pml && pa.nf > 0;
   const dim3 grd(plan_blocks(pl));
 #define MGH_LAUNCH_E(NS_, PML_)                                                               \
   hipLaunchKernelGGL((k_update_e<T, NS_, C64, PML_>), grd, dim3(WG), 0, st, (T*)e, (const T*)h, \
                      (const T*)ae, (const T*)be, pl, nsets, es, hs, pa)
   if (ns == kSetsPerThread) {
-    if (pml) MGH_LAUNCH_E(kSetsPerThread, true);
+    if (fused) MGH_LAUNCH_E(kSetsPerThread, true);
     else MGH_LAUNCH_E(kSetsPerThread, false);
   } else {
-    if (pml) MGH_LAUNCH_E(1, true);
+    if (fused) MGH_LAUNCH_E(1, true);
     else MGH_LAUNCH_E(1, false);
   }
 #undef MGH_LAUNCH_E
@@ -774,15 +1041,16 @@ int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny
   const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
   if (pl.units == 0) return MGH_OK;
   const PmlArgs<T> pa = pml_args<T>(pml, false, nx, ny, nz);
+  const bool fused = pml && pa.nf > 0;
   const dim3 grd(plan_blocks(pl));
 #define MGH_LAUNCH_H(NS_, PML_)                                                              \
   hipLaunchKernelGGL((k_update_h<T, NS_, PML_>), grd, dim3(WG), 0, st, (const T*)e, (T*)h,     \
                      (const T*)ah, (const T*)bh, pl, nsets, es, hs, pa)
   if (ns == kSetsPerThread) {
-    if (pml) MGH_LAUNCH_H(kSetsPerThread, true);
+    if (fused) MGH_LAUNCH_H(kSetsPerThread, true);
     else MGH_LAUNCH_H(kSetsPerThread, false);
   } else {
-    if (pml) MGH_LAUNCH_H(1, true);
+    if (fused) MGH_LAUNCH_H(1, true);
     else MGH_LAUNCH_H(1, false);
   }
 #undef MGH_LAUNCH_H
@@ -809,6 +1077,32 @@ int run_cpml_h(const void* e, void* h, const void* bh, int nx, int ny, int nz, i
                          nx, ny, nz, es, hs, c);
     MGH_CHECK_LAUNCH("k_cpml_h");
   }
+  return MGH_OK;
+}
+
+// the x-slab columns of a CPML set: every face, in order (after the curl
+// launch that carries the rest of the CPML)
+template <typename T>
+int run_xcols_e(void* e, const void* h, const void* be, int nx, int ny, int nz, int nsets,
+                int64_t es, int64_t hs, const mgh::CpmlSet& pml, hipStream_t st) {
+  int64_t nt = 0;
+  const XCols<T> X = xcols_args<T>(pml, true, ny, nz, &nt);
+  if (nt == 0) return MGH_OK;
+  hipLaunchKernelGGL((k_cpml_xcols_e<T>), dim3((unsigned)((nt + WG - 1) / WG), nsets), dim3(WG), 0,
+                     st, (T*)e, (const T*)h, (const T*)be, nx, ny, nz, es, hs, X);
+  MGH_CHECK_LAUNCH("k_cpml_xcols_e");
+  return MGH_OK;
+}
+
+template <typename T>
+int run_xcols_h(const void* e, void* h, const void* bh, int nx, int ny, int nz, int nsets,
+                int64_t es, int64_t hs, const mgh::CpmlSet& pml, hipStream_t st) {
+  int64_t nt = 0;
+  const XCols<T> X = xcols_args<T>(pml, false, ny, nz, &nt);
+  if (nt == 0) return MGH_OK;
+  hipLaunchKernelGGL((k_cpml_xcols_h<T>), dim3((unsigned)((nt + WG - 1) / WG), nsets), dim3(WG), 0,
+                     st, (const T*)e, (T*)h, (const T*)bh, nx, ny, nz, es, hs, X);
+  MGH_CHECK_LAUNCH("k_cpml_xcols_h");
   return MGH_OK;
 }
 
@@ -929,10 +1223,10 @@ int mgh_cpml_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
   return run_cpml_h<double>(e, h, beta_h, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
 }
 
-int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
-                     long long e_set_stride, const void* h, long long h_set_stride,
-                     const void* alpha_e, const void* beta_e, int curl64, int kchunk,
-                     const void* cpml, void* stream) {
+int mgh_update_e_pml_core(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                          long long e_set_stride, const void* h, long long h_set_stride,
+                          const void* alpha_e, const void* beta_e, int curl64, int kchunk,
+                          const void* cpml, void* stream) {
   if (!cpml)
     return mgh_update_e(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_e,
                         beta_e, curl64, kchunk, stream);
@@ -953,10 +1247,36 @@ int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
                               h_set_stride, st, nullptr, 0, &pml);
 }
 
-int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e,
-                     long long e_set_stride, void* h, long long h_set_stride,
-                     const void* alpha_h, const void* beta_h, int kchunk, const void* cpml,
-                     void* stream) {
+int mgh_cpml_xcols_e(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                     long long e_set_stride, const void* h, long long h_set_stride,
+                     const void* beta_e, const void* cpml, void* stream) {
+  int rc = check_common(dtype, nx, ny, nz, nsets, 1);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && beta_e && cpml, "null pointer");
+  mgh::CpmlSet pml{};
+  rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return run_xcols_e<float>(e, h, beta_e, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+  return run_xcols_e<double>(e, h, beta_e, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+}
+
+int mgh_update_e_pml(int dtype, int nx, int ny, int nz, int nsets, void* e,
+                     long long e_set_stride, const void* h, long long h_set_stride,
+                     const void* alpha_e, const void* beta_e, int curl64, int kchunk,
+                     const void* cpml, void* stream) {
+  int rc = mgh_update_e_pml_core(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride,
+                                 alpha_e, beta_e, curl64, kchunk, cpml, stream);
+  if (rc || !cpml) return rc;
+  return mgh_cpml_xcols_e(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, beta_e,
+                          cpml, stream);
+}
+
+int mgh_update_h_pml_core(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                          long long e_set_stride, void* h, long long h_set_stride,
+                          const void* alpha_h, const void* beta_h, int kchunk, const void* cpml,
+                          void* stream) {
   if (!cpml)
     return mgh_update_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, alpha_h,
                         beta_h, kchunk, stream);
@@ -972,6 +1292,32 @@ int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e
                         h_set_stride, st, nullptr, 0, &pml);
   return run_h<double>(e, h, alpha_h, beta_h, nx, ny, nz, nsets, kchunk, e_set_stride,
                        h_set_stride, st, nullptr, 0, &pml);
+}
+
+int mgh_cpml_xcols_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                     long long e_set_stride, void* h, long long h_set_stride, const void* beta_h,
+                     const void* cpml, void* stream) {
+  int rc = check_common(dtype, nx, ny, nz, nsets, 1);
+  if (rc) return rc;
+  MGH_REQUIRE(e && h && beta_h && cpml, "null pointer");
+  mgh::CpmlSet pml{};
+  rc = mgh::load_cpml_set(cpml, nx, ny, nz, nsets, &pml);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 4)
+    return run_xcols_h<float>(e, h, beta_h, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+  return run_xcols_h<double>(e, h, beta_h, nx, ny, nz, nsets, e_set_stride, h_set_stride, pml, st);
+}
+
+int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e,
+                     long long e_set_stride, void* h, long long h_set_stride,
+                     const void* alpha_h, const void* beta_h, int kchunk, const void* cpml,
+                     void* stream) {
+  int rc = mgh_update_h_pml_core(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride,
+                                 alpha_h, beta_h, kchunk, cpml, stream);
+  if (rc || !cpml) return rc;
+  return mgh_cpml_xcols_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, beta_h,
+                          cpml, stream);
 }
 
 }  // extern "C"

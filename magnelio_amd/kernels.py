@@ -61,13 +61,15 @@ def _cpml_desc(cpml, nsets: int, dtype) -> int | None:
 
 def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: torch.Tensor,
              dims: tuple[int, int, int], *, curl64: bool = False, kchunk: int = DEFAULT_KCHUNK,
-             cpml=None, live: torch.Tensor | None = None,
+             cpml=None, live: torch.Tensor | None = None, xcols: bool = True,
              stream: torch.cuda.Stream | None = None) -> None:
     """E phase on K field sets: ``e`` is (K, n_E) or (n_E,), ``h`` likewise.
 
     ``cpml`` is a :class:`magnelio_amd.boundaries.DeviceCPML` (psi slabs for
-    the same K sets) or None; ``live``: int32 device list of the launch
-    units to run (``tileskip.build_tile_plan``), None for all."""
+    the same K sets) or None: its y/z faces ride in the curl launch, the
+    x-slab columns follow in a launch of their own (``xcols=False`` leaves
+    that one to :func:`cpml_xcols_e`); ``live``: int32 device list of the
+    launch units to run (``tileskip.build_tile_plan``), None for all."""
     nx, ny, nz = dims
     ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
     e2 = e.view(-1, ne)
@@ -91,7 +93,8 @@ def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: to
                   live.data_ptr(), live.numel(), stream_handle(stream))
         return
     desc = _cpml_desc(cpml, k, e.dtype)
-    _lib.call("mgh_update_e_pml", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+    _lib.call("mgh_update_e_pml" if xcols else "mgh_update_e_pml_core", code, nx, ny, nz, k,
+              e.data_ptr(), ne, h.data_ptr(), nh,
               alpha_e.data_ptr(), beta_e.data_ptr(), int(bool(curl64)), int(kchunk), desc,
               stream_handle(stream))
 
@@ -122,9 +125,37 @@ def cpml_h(e: torch.Tensor, h: torch.Tensor, beta_h: torch.Tensor, dims: tuple[i
               beta_h.data_ptr(), _cpml_desc(cpml, k, h.dtype), stream_handle(stream))
 
 
+def cpml_xcols_e(e: torch.Tensor, h: torch.Tensor, beta_e: torch.Tensor,
+                 dims: tuple[int, int, int], cpml, *,
+                 stream: torch.cuda.Stream | None = None) -> None:
+    """CPML of the x-slab columns, E phase (after :func:`update_e` with
+    ``cpml`` and ``xcols=False``): every face, in order, on those elements."""
+    _xcols("mgh_cpml_xcols_e", e, h, beta_e, dims, cpml, stream, e)
+
+
+def cpml_xcols_h(e: torch.Tensor, h: torch.Tensor, beta_h: torch.Tensor,
+                 dims: tuple[int, int, int], cpml, *,
+                 stream: torch.cuda.Stream | None = None) -> None:
+    """CPML of the x-slab columns, H phase (after :func:`update_h` with
+    ``cpml`` and ``xcols=False``)."""
+    _xcols("mgh_cpml_xcols_h", e, h, beta_h, dims, cpml, stream, h)
+
+
+def _xcols(name, e, h, beta, dims, cpml, stream, side) -> None:
+    nx, ny, nz = dims
+    ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
+    k = e.view(-1, ne).shape[0]
+    if h.view(-1, nh).shape[0] != k or beta.numel() != side.view(k, -1).shape[1]:
+        raise ValueError("field sets or coefficient length do not match the grid")
+    _check_dev(e, h, beta)
+    _lib.call(name, _dtype_code(side), nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+              beta.data_ptr(), _cpml_desc(cpml, k, side.dtype), stream_handle(stream))
+
+
 def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: torch.Tensor,
              dims: tuple[int, int, int], *, kchunk: int = DEFAULT_KCHUNK, cpml=None,
-             live: torch.Tensor | None = None, stream: torch.cuda.Stream | None = None) -> None:
+             live: torch.Tensor | None = None, xcols: bool = True,
+             stream: torch.cuda.Stream | None = None) -> None:
     """H phase on K field sets (same conventions as :func:`update_e`)."""
     nx, ny, nz = dims
     ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
@@ -149,7 +180,8 @@ def update_h(e: torch.Tensor, h: torch.Tensor, alpha_h: torch.Tensor, beta_h: to
                   stream_handle(stream))
         return
     desc = _cpml_desc(cpml, k, h.dtype)
-    _lib.call("mgh_update_h_pml", code, nx, ny, nz, k, e.data_ptr(), ne, h.data_ptr(), nh,
+    _lib.call("mgh_update_h_pml" if xcols else "mgh_update_h_pml_core", code, nx, ny, nz, k,
+              e.data_ptr(), ne, h.data_ptr(), nh,
               alpha_h.data_ptr(), beta_h.data_ptr(), int(kchunk), desc, stream_handle(stream))
 
 

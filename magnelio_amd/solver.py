@@ -17,10 +17,12 @@ Step order (fit_td.py:950-1007), t = n dt:
   ports      modal DTBC/Mur step, write-back, V/I record mgh_port_step
              lumped RLC elements                       mgh_lumped_step
   H phase    curl update, CPML faces                   mgh_update_h, mgh_cpml_h
-Without dispersion and listed (tile-skipping) launches, the CPML faces run
-inside the curl launches (mgh_update_e_pml / mgh_update_h_pml): each slab
-element applies its face terms, in face order, to its freshly updated value
-before the store — the same values as the separate slab passes.
+Without dispersion and listed (tile-skipping) launches, the CPML runs with
+the curl: the curl launch (mgh_update_e_pml_core / _h_pml_core) applies the
+y and z faces' terms, in face order, to every element outside the x slabs
+before its store, and a launch over the x-slab columns (mgh_cpml_xcols_e/h)
+applies every face's terms there — the same values as the separate slab
+passes.
   sources    TF/SF H injection at t + 3dt/2
   monitors   DFT accumulation                          mgh_dft_*
   counter    n <- n + 1 on the device                  mgh_step_advance
@@ -460,10 +462,12 @@ class FITTimeDomainSolver:
         fused = self.cpml if (self.cpml is not None and tp is None and self._dispersion is None) else None
         kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
                          kchunk=self.kchunk, live=tp.live_e if tp is not None else None, cpml=fused,
-                         stream=stream)
+                         xcols=False, stream=stream)
         if ev is not None:
             ev[1].record(stream)
             self.timed_events.append(ev)
+        if fused is not None:
+            kernels.cpml_xcols_e(self.e, self.h, self.beta_e, dims, fused, stream=stream)
         # reference E phase order: curl, dispersion completion, CPML faces
         if self._dispersion is not None:
             self._dispersion.update(self, stream)
@@ -478,7 +482,10 @@ class FITTimeDomainSolver:
         if self.dlumped is not None:
             self.dlumped.step(self, stream)
         kernels.update_h(self.e, self.h, self.alpha_h, self.beta_h, dims, kchunk=self.kchunk,
-                         live=tp.live_h if tp is not None else None, cpml=fused, stream=stream)
+                         live=tp.live_h if tp is not None else None, cpml=fused, xcols=False,
+                         stream=stream)
+        if fused is not None:
+            kernels.cpml_xcols_h(self.e, self.h, self.beta_h, dims, fused, stream=stream)
         if self.cpml is not None and fused is None:
             kernels.cpml_h(self.e, self.h, self.beta_h, dims, self.cpml, stream=stream)
         for src in self.sources:

@@ -2,9 +2,10 @@
 
 All six faces at once (corner and edge overlaps exercise the face order),
 PEC cells inside the slabs (air masks), K field sets (one and two per
-thread), float32 and float64, CPML inside the curl launches and as separate
-slab launches; compared bit-exactly with the oracle's separate slab passes
-after several leapfrog steps.
+thread), float32 and float64, CPML with the curl (y/z faces in the curl
+launch, the x-slab columns after it, composed or through the two entry
+points) and as separate slab launches; compared bit-exactly with the
+oracle's separate slab passes after several leapfrog steps.
 """
 
 import numpy as np
@@ -40,9 +41,10 @@ def _setup(rng, dims, dtype, with_pec):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("with_pec", [False, True])
-@pytest.mark.parametrize("fused,nsets", [(True, 3), (True, 1), (False, 3)])
-def test_fused_cpml_bit_exact(dtype, with_pec, fused, nsets, gpu):
-    print(f"cpml dtype={np.dtype(dtype).name} pec={with_pec} fused={fused} K={nsets}", flush=True)
+@pytest.mark.parametrize("mode,nsets", [("with_curl", 3), ("with_curl", 1), ("split", 3),
+                                        ("separate", 3)])
+def test_fused_cpml_bit_exact(dtype, with_pec, mode, nsets, gpu):
+    print(f"cpml dtype={np.dtype(dtype).name} pec={with_pec} mode={mode} K={nsets}", flush=True)
     rng = np.random.default_rng(12)
     dims = (13, 11, 15)
     axes, pec, pec_cell, dt, (ae, be, ah, bh) = _setup(rng, dims, dtype, with_pec)
@@ -69,9 +71,14 @@ def test_fused_cpml_bit_exact(dtype, with_pec, fused, nsets, gpu):
     de, dh = d(e, "E"), d(h, "H")
     dae, dbe, dah, dbh = d(ae, "E"), d(be, "E"), d(ah, "H"), d(bh, "H")
     for _ in range(6):
-        if fused:  # CPML inside the curl launches
+        if mode == "with_curl":  # y/z faces in the curl launch, then the x-slab columns
             kernels.update_e(de, dh, dae, dbe, dims, cpml=dev_pml, kchunk=4)
             kernels.update_h(de, dh, dah, dbh, dims, cpml=dev_pml, kchunk=4)
+        elif mode == "split":  # the same two launches through their own entry points
+            kernels.update_e(de, dh, dae, dbe, dims, cpml=dev_pml, kchunk=4, xcols=False)
+            kernels.cpml_xcols_e(de, dh, dbe, dims, dev_pml)
+            kernels.update_h(de, dh, dah, dbh, dims, cpml=dev_pml, kchunk=4, xcols=False)
+            kernels.cpml_xcols_h(de, dh, dbh, dims, dev_pml)
         else:  # curl launch, then one slab launch per face
             kernels.update_e(de, dh, dae, dbe, dims, kchunk=4)
             kernels.cpml_e(de, dh, dbe, dims, dev_pml)
