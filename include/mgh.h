@@ -293,12 +293,12 @@ int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_s
  * n0 dt + dt/2 otherwise, n0 the first step of the block holding *step_ctr.
  * Pc/Ps: F_pad x Bp float32 tables cos/sin(w_f b dt) dt in the kernel's lane
  * order (Bp: B rounded up to 64 steps up to 256, 512 beyond). bins: nsets x F x S
- * complex float64 (re/im interleaved); lo (same shape, or NULL) holds TwoSum
- * compensation terms. blk: F x 4 float64 scratch. Contraction on the MFMA
+ * complex float64 (re/im interleaved); lo (same shape in float32, or NULL)
+ * holds the TwoSum compensation terms. blk: F x 4 float64 scratch. Contraction on the MFMA
  * units (v_mfma_f32_16x16x4f32). */
 int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
                   const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
-                  double dt, double* blk, int s_h_start, double* bins, double* lo,
+                  double dt, double* blk, int s_h_start, double* bins, float* lo,
                   const long long* step_ctr, long long n_start, void* stream);
 
 #ifdef __cplusplus

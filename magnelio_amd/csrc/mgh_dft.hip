@@ -19,7 +19,7 @@
 // products never reach HBM.  Each wave keeps its 16-sample x B-step operand
 // in registers and sweeps every frequency, so x is read once per block and
 // the bins are read and written once per block: the kernel is bound by the
-// bin traffic (16 B complex128 per (f, s), x2 with compensation), which a
+// bin traffic (16 B complex128 per (f, s), + 8 B of float32 compensation), which a
 // longer block divides.
 #include "mgh_common.h"
 
@@ -148,13 +148,17 @@ __global__ void k_dft_phase(const double* __restrict__ omega, int F, double dt,
   blk[f * 4 + 3] = s;
 }
 
-__device__ __forceinline__ void two_sum_acc(double* hi, double* lo, double v) {
+// TwoSum of the float64 bin and the block's contribution; the rounding error
+// (at most half an ulp of the bin) joins the float32 compensation word.
+// Float32 holds it with 24 bits to spare over what the float32 block partials
+// carry, at 4 of the 8 bytes a float64 word would add to the bin traffic.
+__device__ __forceinline__ void two_sum_acc(double* hi, float* lo, double v) {
   const double a = *hi;
   const double s = a + v;
   const double bp = s - a;
   const double err = (a - (s - bp)) + (v - bp);
   *hi = s;
-  *lo += err;
+  *lo = (float)((double)*lo + err);
 }
 
 // NB: MFMA k-steps per block (the padded block length divided by 4).
@@ -176,7 +180,7 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
                                                    const float* __restrict__ Ps, int F,
                                                    const double* __restrict__ blk, int s_h_start,
                                                    double* __restrict__ bins,
-                                                   double* __restrict__ lo) {
+                                                   float* __restrict__ lo) {
   constexpr int NQ = NB / 4;                // float4 groups per lane
   constexpr int NQP = NQ < 16 ? NQ : 16;    // groups per LDS pass (256 steps)
   constexpr int NP = NQ / NQP;              // passes
@@ -201,9 +205,10 @@ __global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, 
   const int s = s0 + col;
   const bool own = active && s < S;
   double2* bset = (double2*)(bins + (int64_t)set * F * S * 2);
-  double2* lset = COMP ? (double2*)(lo + (int64_t)set * F * S * 2) : nullptr;
+  float2* lset = COMP ? (float2*)(lo + (int64_t)set * F * S * 2) : nullptr;
   for (int f0 = 0; f0 < F; f0 += 16) {
-    double2 bv[4], lv[4];
+    double2 bv[4];
+    float2 lv[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int f = f0 + 4 * krow + r;
@@ -333,7 +338,7 @@ int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_s
 
 // Contract the block (first nb_valid rows of X) and accumulate into the
 // float64 bins (nsets x F x S complex, interleaved re/im); lo (same shape or
-// NULL) holds the compensation terms.  Pc/Ps: the float32 phasor tables
+// NULL; float32) holds the compensation terms.  Pc/Ps: the float32 phasor tables
 // P[f][b] = cos/sin(w_f b dt) dt (f < F_pad, b < Bp = B rounded up to a
 // multiple of 64, zero beyond B and F) stored in MFMA lane order, in float4
 // groups: element [t][q][l][c] (t < F_pad/16, q < Bp/16, l < 64, c < 4) =
@@ -343,7 +348,7 @@ int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_s
 // *step_ctr (n = last step of the block, or one past it for a partial one).
 int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
                   const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
-                  double dt, double* blk, int s_h_start, double* bins, double* lo,
+                  double dt, double* blk, int s_h_start, double* bins, float* lo,
                   const long long* step_ctr, long long n_start, void* stream) {
   MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts");
   MGH_REQUIRE(F >= 1 && F_pad >= F && F_pad % 16 == 0, "bad frequency counts F=%d F_pad=%d", F,

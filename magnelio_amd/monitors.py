@@ -362,14 +362,15 @@ class DFTMonitorBase:
         self.blk = torch.zeros((self.F, 4), dtype=torch.float64, device=dev)
         self.X = torch.zeros((K, self.block, self.S_pad), dtype=f32, device=dev)
         self.bins = torch.zeros((K, self.F, self.S, 2), dtype=torch.float64, device=dev)
-        self.lo = torch.zeros_like(self.bins) if self.compensated else None
+        # TwoSum compensation terms of the bins, float32 (csrc/mgh_dft.hip two_sum_acc)
+        self.lo = torch.zeros(self.bins.shape, dtype=f32, device=dev) if self.compensated else None
         self.nsets = K
         self.n_start = solver.n_completed
         self._pending = 0
         self._attached = True
 
     def bytes(self) -> int:
-        n = self.bins.numel() * 8 * (2 if self.compensated else 1)
+        n = self.bins.numel() * (12 if self.compensated else 8)
         return int(n + self.X.numel() * 4)
 
     def record(self, solver, stream, n: int) -> None:
@@ -410,7 +411,7 @@ class DFTMonitorBase:
         """{component: complex128 (F, *shape)} of field set s (host)."""
         b = self.bins[s].cpu().numpy()
         if self.lo is not None:
-            b = b + self.lo[s].cpu().numpy()
+            b = b + self.lo[s].cpu().numpy().astype(np.float64)
         z = b[..., 0] + 1j * b[..., 1]
         out = {}
         for comp, shape, start, n in self._layout:

@@ -61,7 +61,7 @@ def test_block_contraction_matches_numpy(B, gpu):
             # bins already holding values of the size of one block's sum
             base = rng.standard_normal((K, F, S, 2)) * (8 * dt)
             bins = torch.from_numpy(base.copy()).to(dev)
-            lo = torch.zeros_like(bins) if comp else None
+            lo = torch.zeros(bins.shape, dtype=torch.float32, device=dev) if comp else None
             stream = torch.cuda.current_stream(dev)
             _lib.call("mgh_dft_block", K, Xd.data_ptr(), S, S_pad, B, nb, Pcd.data_ptr(),
                       Psd.data_ptr(), om.data_ptr(), F, F_pad, dt, blk.data_ptr(), s_h,
@@ -70,7 +70,7 @@ def test_block_contraction_matches_numpy(B, gpu):
             torch.cuda.synchronize(dev)
             got = bins.cpu().numpy()
             if comp:
-                got = got + lo.cpu().numpy()
+                got = got + lo.cpu().numpy().astype(np.float64)
             for k in range(K):
                 want = (P[:, :nb] @ X[k, :nb, :S].astype(np.float64)) * rot
                 g = (got[k, ..., 0] - base[k, ..., 0]) + 1j * (got[k, ..., 1] - base[k, ..., 1])
