@@ -35,8 +35,9 @@
 // are read once for the three updates.  Nodes are numbered x-fastest, 256 to
 // a work-group (four 64-wide waves over consecutive nodes), which keeps every
 // wave's loads on contiguous rows and leaves no partly empty tile at i = Nx.
-// With two or more field sets a thread advances two sets, so the coefficient
-// words serve both, and the work-group index is remapped so that consecutive
+// With two or more field sets a thread advances two sets (three when K is an
+// odd multiple of 3), so the coefficient words serve all of them, and the
+// work-group index is remapped so that consecutive
 // units — the set groups of one tile first — land on the same XCD
 // (work-groups are dealt to the 8 XCDs round robin) and share its L2.
 //
@@ -889,7 +890,17 @@ __global__ __launch_bounds__(WG) void k_cpml_xcols_h(const T* __restrict__ e, T*
 // ---------------------------------------------------------------------------
 // host side
 
+// field sets per thread: each coefficient word a thread loads serves them
+// all.  Two for K >= 2; three when K is an odd multiple of 3, where pairs
+// would leave one group half empty (K = 3 on 192x96x288: 158 against
+// 181 us per E launch).  Three for every K, or six, measured slower at
+// K = 6 (303 / 471 against 297 us): the extra registers cost occupancy.
 constexpr int kSetsPerThread = 2;
+
+inline int sets_per_thread(int nsets) {
+  if (nsets < 2) return 1;
+  return (nsets % 3 == 0 && nsets % 2 == 1) ? 3 : kSetsPerThread;
+}
 
 Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, const int* live = nullptr,
                int nlive = 0) {
@@ -990,7 +1001,7 @@ template <typename T, bool C64>
 int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny, int nz,
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
           int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
-  const int ns = nsets >= 2 ? kSetsPerThread : 1;
+  const int ns = sets_per_thread(nsets);
   const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
   if (pl.units == 0) return MGH_OK;
   const PmlArgs<T> pa = pml_args<T>(pml, true, nx, ny, nz);
@@ -999,7 +1010,10 @@ int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny
 #define MGH_LAUNCH_E(NS_, PML_)                                                               \
   hipLaunchKernelGGL((k_update_e<T, NS_, C64, PML_>), grd, dim3(WG), 0, st, (T*)e, (const T*)h, \
                      (const T*)ae, (const T*)be, pl, nsets, es, hs, pa)
-  if (ns == kSetsPerThread) {
+  if (ns == 3) {
+    if (fused) MGH_LAUNCH_E(3, true);
+    else MGH_LAUNCH_E(3, false);
+  } else if (ns == kSetsPerThread) {
     if (fused) MGH_LAUNCH_E(kSetsPerThread, true);
     else MGH_LAUNCH_E(kSetsPerThread, false);
   } else {
@@ -1037,7 +1051,7 @@ template <typename T>
 int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny, int nz,
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
           int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
-  const int ns = nsets >= 2 ? kSetsPerThread : 1;
+  const int ns = sets_per_thread(nsets);
   const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
   if (pl.units == 0) return MGH_OK;
   const PmlArgs<T> pa = pml_args<T>(pml, false, nx, ny, nz);
@@ -1046,7 +1060,10 @@ int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny
 #define MGH_LAUNCH_H(NS_, PML_)                                                              \
   hipLaunchKernelGGL((k_update_h<T, NS_, PML_>), grd, dim3(WG), 0, st, (const T*)e, (T*)h,     \
                      (const T*)ah, (const T*)bh, pl, nsets, es, hs, pa)
-  if (ns == kSetsPerThread) {
+  if (ns == 3) {
+    if (fused) MGH_LAUNCH_H(3, true);
+    else MGH_LAUNCH_H(3, false);
+  } else if (ns == kSetsPerThread) {
     if (fused) MGH_LAUNCH_H(kSetsPerThread, true);
     else MGH_LAUNCH_H(kSetsPerThread, false);
   } else {

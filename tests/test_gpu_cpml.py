@@ -1,7 +1,7 @@
 """GPU parity of the CPML recursion fused into the curl kernels.
 
 All six faces at once (corner and edge overlaps exercise the face order),
-PEC cells inside the slabs (air masks), K field sets (one and two per
+PEC cells inside the slabs (air masks), K field sets (one, two and three per
 thread), float32 and float64, CPML with the curl (y/z faces in the curl
 launch, the x-slab columns after it, composed or through the two entry
 points) and as separate slab launches; compared bit-exactly with the
@@ -41,8 +41,8 @@ def _setup(rng, dims, dtype, with_pec):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("with_pec", [False, True])
-@pytest.mark.parametrize("mode,nsets", [("with_curl", 3), ("with_curl", 1), ("split", 3),
-                                        ("separate", 3)])
+@pytest.mark.parametrize("mode,nsets", [("with_curl", 3), ("with_curl", 2), ("with_curl", 1),
+                                        ("split", 3), ("separate", 3)])
 def test_fused_cpml_bit_exact(dtype, with_pec, mode, nsets, gpu):
     print(f"cpml dtype={np.dtype(dtype).name} pec={with_pec} mode={mode} K={nsets}", flush=True)
     rng = np.random.default_rng(12)

@@ -74,7 +74,7 @@ def test_numba_order_float32_bit_exact(gpu):
     assert np.array_equal(gh, oh)
 
 
-@pytest.mark.parametrize("nsets", [2, 6, 11])
+@pytest.mark.parametrize("nsets", [2, 3, 6, 9, 11])
 def test_k_field_sets_bit_exact(nsets, gpu):
     print(f"K sets nsets={nsets}", flush=True)
     dims = (20, 7, 9)
