@@ -36,10 +36,9 @@
 // a work-group (four 64-wide waves over consecutive nodes), which keeps every
 // wave's loads on contiguous rows and leaves no partly empty tile at i = Nx.
 // With two or more field sets a thread advances two sets (three when K is an
-// odd multiple of 3), so the coefficient words serve all of them, and the
-// work-group index is remapped so that consecutive
-// units — the set groups of one tile first — land on the same XCD
-// (work-groups are dealt to the 8 XCDs round robin) and share its L2.
+// odd multiple of 3), so the coefficient words serve all of them, and bands of
+// consecutive units — the set groups of one tile first, then the next tiles —
+// run on one XCD and share its L2 (BAND below).
 //
 // CPML, two forms with the same values.  With the curl (the
 // mgh_update_*_pml entry points): every d of every face is a difference of
@@ -67,18 +66,31 @@ namespace {
 
 constexpr int WG = 256;
 
-// Units of WG nodes x kchunk planes x one set group.
+// Units of WG nodes x kchunk planes x one set group, ordered set group
+// fastest, then node tile (one row of the node plane, about), then z chunk.
 struct Plan {
   int nx, ny, nz;
   int kchunk, ngroups;
-  int ntp, nkc, units, per_xcd, remap;
+  int ntp, nkc, units;
   const int* live;  // live spatial units (kc * ntp + tp), or null for all
 };
+
+// Work-groups are dealt to the 8 XCDs round robin (work-group b to XCD
+// b mod 8).  Units go to XCDs in bands of 8 consecutive ones: XCD x runs
+// units [64 s + 8 x, 64 s + 8 x + 8) for s = 0, 1, ..., so consecutive node
+// tiles (and the set groups of one tile) share an XCD's L2 — the previous
+// row a tile reads as its y neighbour was just fetched there — while all
+// XCDs sweep the same z region together.  On 256^3 (K = 1) the curl
+// launches ran 2-4 % faster than with units dealt one by one, and faster
+// than a contiguous unit range per XCD, which spreads the XCDs over eight
+// z regions.
+constexpr int BAND = 8;
 
 __device__ __forceinline__ bool unit_of(const Plan& p, int& grp, int& i, int& j, int& k0,
                                         int& k1) {
   const int blk = blockIdx.x;
-  const int u = p.remap ? (blk & 7) * p.per_xcd + (blk >> 3) : blk;
+  const int sq = blk >> 3;  // this work-group's place in its XCD's sequence
+  const int u = (sq / BAND) * (8 * BAND) + (blk & 7) * BAND + (sq % BAND);
   if (u >= p.units) return false;
   grp = u % p.ngroups;
   const int t0 = u / p.ngroups;
@@ -915,12 +927,11 @@ Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, const int*
   p.nkc = (nz + 1 + kchunk - 1) / kchunk;
   p.live = live;
   p.units = (live ? nlive : p.ntp * p.nkc) * p.ngroups;
-  p.per_xcd = (p.units + 7) / 8;
-  p.remap = p.ngroups > 1;
   return p;
 }
 
-int plan_blocks(const Plan& p) { return p.remap ? 8 * p.per_xcd : p.units; }
+// work-groups of a launch: units rounded up to whole bands on every XCD
+int plan_blocks(const Plan& p) { return (p.units + 8 * BAND - 1) / (8 * BAND) * (8 * BAND); }
 
 template <typename T>
 CFace<T> face_of(const mgh::CpmlFace& F, bool e_side) {
