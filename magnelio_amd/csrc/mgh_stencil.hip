@@ -666,33 +666,36 @@ __global__ __launch_bounds__(WG) void k_cpml_h(const T* __restrict__ e, T* __res
 // descriptor order, to its freshly updated value — the d of each term
 // computed as the separate slab passes above compute it — so the result is
 // theirs bit for bit.  Threads cover each x face's slab layer-fastest (the
-// psi order): one thread per node (l, j, k), all faces' x slabs back to back.
+// psi order) by seams: the upper x slab of row j - 1 and the lower x slab of
+// row j are adjacent in memory (a row's end meets the next row's start), so
+// the threads of one seam, upper layers first, touch one or two cache lines
+// per field array where the two slabs walked apart touch each line twice.
 template <typename T>
 struct XCols {
   int nf;  // every face, descriptor order
   int axis[MGH_CPML_MAX_FACES];
   CFace<T> f[MGH_CPML_MAX_FACES];
-  int nx_faces;
-  int xk0[2], xn[2];
-  int64_t xthreads0;  // threads of the first x face
+  int lo_k0, lo_n, hi_k0, hi_n;  // the lower / upper x slab (n = 0: absent)
 };
 
 template <typename T>
 __device__ __forceinline__ bool xcols_node(const XCols<T>& X, int64_t t, int ny, int nz, int& i,
                                            int& j, int& k) {
-  int q = 0;
-  if (t >= X.xthreads0) {
-    t -= X.xthreads0;
-    q = 1;
-    if (q >= X.nx_faces) return false;
+  const int w = X.hi_n + X.lo_n;  // threads per seam
+  const int64_t seam = t / w;
+  const int l = (int)(t - seam * w);
+  k = (int)(seam / (ny + 2));
+  const int jj = (int)(seam - (int64_t)k * (ny + 2));  // seam between rows jj - 1 and jj
+  if (k > nz) return false;
+  if (l < X.hi_n) {  // upper slab of row jj - 1
+    if (jj == 0) return false;
+    j = jj - 1;
+    i = X.hi_k0 + l;
+  } else {  // lower slab of row jj
+    if (jj > ny) return false;
+    j = jj;
+    i = X.lo_k0 + (l - X.hi_n);
   }
-  const int n = X.xn[q];
-  const int64_t per = (int64_t)n * (ny + 1);
-  if (t >= per * (nz + 1)) return false;
-  k = (int)(t / per);
-  const int r = (int)(t - (int64_t)k * per);
-  j = r / n;
-  i = X.xk0[q] + (r - j * n);
   return true;
 }
 
@@ -992,18 +995,30 @@ XCols<T> xcols_args(const mgh::CpmlSet& pml, bool e_side, int ny, int nz, int64_
   XCols<T> x;
   std::memset(&x, 0, sizeof(x));
   x.nf = pml.nfaces;
+  int xk0[2] = {0, 0}, xn[2] = {0, 0}, nxf = 0;
   for (int f = 0; f < pml.nfaces; ++f) {
     x.axis[f] = pml.f[f].axis;
     x.f[f] = face_of<T>(pml.f[f], e_side);
     if (pml.f[f].axis == 0) {
-      x.xk0[x.nx_faces] = pml.f[f].k0;
-      x.xn[x.nx_faces] = pml.f[f].n;
-      x.nx_faces++;
+      xk0[nxf] = pml.f[f].k0;
+      xn[nxf] = pml.f[f].n;
+      nxf++;
     }
   }
-  const int64_t per = (int64_t)(ny + 1) * (nz + 1);
-  x.xthreads0 = x.nx_faces ? (int64_t)x.xn[0] * per : 0;
-  *threads = x.nx_faces ? x.xthreads0 + (x.nx_faces > 1 ? (int64_t)x.xn[1] * per : 0) : 0;
+  if (nxf == 2 && xk0[1] < xk0[0]) {
+    std::swap(xk0[0], xk0[1]);
+    std::swap(xn[0], xn[1]);
+  }
+  if (nxf == 1) {  // a lone x slab: lower slot, no upper
+    x.lo_k0 = xk0[0];
+    x.lo_n = xn[0];
+  } else if (nxf == 2) {
+    x.lo_k0 = xk0[0];
+    x.lo_n = xn[0];
+    x.hi_k0 = xk0[1];
+    x.hi_n = xn[1];
+  }
+  *threads = nxf ? (int64_t)(x.lo_n + x.hi_n) * (ny + 2) * (nz + 1) : 0;
   return x;
 }
 

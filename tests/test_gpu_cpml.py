@@ -41,20 +41,22 @@ def _setup(rng, dims, dtype, with_pec):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("with_pec", [False, True])
+@pytest.mark.parametrize("faces", [FACES, ["zmin", "xmax", "ymin"]], ids=["six", "lone_x"])
 @pytest.mark.parametrize("mode,nsets", [("with_curl", 3), ("with_curl", 2), ("with_curl", 1),
                                         ("split", 3), ("separate", 3)])
-def test_fused_cpml_bit_exact(dtype, with_pec, mode, nsets, gpu):
-    print(f"cpml dtype={np.dtype(dtype).name} pec={with_pec} mode={mode} K={nsets}", flush=True)
+def test_fused_cpml_bit_exact(dtype, with_pec, faces, mode, nsets, gpu):
+    print(f"cpml dtype={np.dtype(dtype).name} pec={with_pec} faces={faces} mode={mode} K={nsets}",
+          flush=True)
     rng = np.random.default_rng(12)
     dims = (13, 11, 15)
     axes, pec, pec_cell, dt, (ae, be, ah, bh) = _setup(rng, dims, dtype, with_pec)
     nl = 4
     ofaces_sets = [[oc.CPMLFace(f, *axes, dt, dtype, pec_e_flat=pec if with_pec else None,
                                 pec_cell=pec_cell if with_pec else None, n_layers=nl)
-                    for f in FACES] for _ in range(nsets)]
+                    for f in faces] for _ in range(nsets)]
     g = GridLines(*axes)
     hfaces = []
-    for f in FACES:
+    for f in faces:
         b = boundaries.CPMLBoundary(f, g, thickness_cells=nl)
         b.initialize(dt)
         hfaces.append(b)
