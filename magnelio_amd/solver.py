@@ -43,7 +43,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import _lib, kernels, layout
+from . import _lib, autotune, kernels, layout
 from . import mesh as _mesh
 from .dispersion import DeviceDispersion
 from .lumped import DeviceLumped
@@ -237,7 +237,7 @@ class FITTimeDomainSolver:
     excitations: list | None = None  # per set: ChannelExcitation | None
     record: bool = True
     curl64: bool = False
-    kchunk: int = kernels.DEFAULT_KCHUNK
+    kchunk: int | str = kernels.DEFAULT_KCHUNK  # z planes per thread, or "auto" (autotune.py)
     use_graphs: bool = False
     graph_block: int = 1  # minimum steps per captured graph
     coefficients: str = "auto"  # alpha/beta built on the "device", the "host", or "auto"
@@ -247,6 +247,8 @@ class FITTimeDomainSolver:
     def __post_init__(self):
         if self.precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {list(PRECISIONS)}, got {self.precision!r}")
+        if self.kchunk != "auto" and not (isinstance(self.kchunk, int) and self.kchunk >= 1):
+            raise ValueError(f"kchunk must be a positive int or 'auto', got {self.kchunk!r}")
         if self.excitations is None:
             self.excitations = [None] * self.nsets
         if len(self.excitations) != self.nsets:
@@ -427,6 +429,8 @@ class FITTimeDomainSolver:
         self.dlumped = DeviceLumped(self.lumped, self) if self.lumped else None
         self._sqsum = kernels.SqSum(K, dev)
         self._energy = torch.zeros(K, dtype=torch.float64, device=dev)
+        if self.kchunk == "auto":  # timed on the zeroed buffers, before any state is loaded
+            self.kchunk = autotune.kchunk_for(self)
         self.tile_plan = self._build_tile_plan()
         self._setup_done = True
 
