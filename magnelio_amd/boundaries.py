@@ -221,19 +221,30 @@ class DeviceCPML:
     def desc_ptr(self) -> int:
         return int(self.desc.ctypes.data)
 
+    def x_split(self) -> bool:
+        """Both x slabs at the domain ends with columns between them: the
+        curl launches split their planes and carry the x slabs themselves."""
+        nx = self.dims[0]
+        xs = [f for f in self.faces if f.axis == 0]
+        lo = [f for f in xs if f.k0 == 0]
+        hi = [f for f in xs if f.k0 + f.n == nx]
+        return bool(lo) and bool(hi) and lo[0].n < hi[0].k0
+
     def slab_bytes(self, side: str, in_curl: bool = False) -> int:
         """Algorithmic bytes of the CPML of ``side`` ("E" or "H"): every psi
         word of every set read and written and the air masks read once (the
         field, coefficient and neighbour words are the curl's own when the
         recursion rides in the curl launch; the layer coefficients are a few
         words).  ``in_curl``: only the words the curl launch itself updates —
-        the y and z faces' elements outside every x slab (the x-slab columns
-        take a pass of their own)."""
+        all of them when the two x slabs sit at the domain ends (the launch
+        splits its planes and takes the x-slab columns), else the y and z
+        faces' elements outside every x slab (the x columns take a pass of
+        their own)."""
         lst = self.psi_e if side == "E" else self.psi_h
         if not lst:
             return 0
         item = lst[0][0].element_size()
-        if not in_curl:
+        if not in_curl or self.x_split():
             return int(2 * item * sum(t.numel() for terms in lst for t in terms)
                        + self._air_bytes[0 if side == "E" else 1])
         x_layers = sum(f.n for f in self.faces if f.axis == 0)

@@ -68,10 +68,17 @@ constexpr int WG = 256;
 
 // Units of WG nodes x kchunk planes x one set group, ordered set group
 // fastest, then node tile (one row of the node plane, about), then z chunk.
+//
+// Split planes (launches that carry the CPML of two x slabs at the domain
+// ends): tiles [0, tpa) cover the interior columns [a0, a1) of every row,
+// tiles [tpa, ntp) the row seams — row j - 1's columns [a1, nx] followed by
+// row j's columns [0, a0), adjacent in memory — so every wave holds only
+// interior elements or only x-slab (and end-column) elements.
 struct Plan {
   int nx, ny, nz;
   int kchunk, ngroups;
   int ntp, nkc, units;
+  int split, a0, a1, tpa, wb;
   const int* live;  // live spatial units (kc * ntp + tp), or null for all
 };
 
@@ -98,11 +105,33 @@ __device__ __forceinline__ bool unit_of(const Plan& p, int& grp, int& i, int& j,
   if (t < 0 || t >= p.ntp * p.nkc) return false;
   const int tp = t % p.ntp;
   const int kc = t / p.ntp;
-  const int q = tp * WG + (int)threadIdx.x;
   const int nx1 = p.nx + 1;
-  if (q >= nx1 * (p.ny + 1)) return false;
-  j = q / nx1;
-  i = q - j * nx1;
+  if (!p.split) {
+    const int q = tp * WG + (int)threadIdx.x;
+    if (q >= nx1 * (p.ny + 1)) return false;
+    j = q / nx1;
+    i = q - j * nx1;
+  } else if (tp < p.tpa) {  // interior columns
+    const int wa = p.a1 - p.a0;
+    const int q = tp * WG + (int)threadIdx.x;
+    j = q / wa;
+    if (j > p.ny) return false;
+    i = p.a0 + (q - j * wa);
+  } else {  // row seams: [a1, nx] of row jj - 1, then [0, a0) of row jj
+    const int q = (tp - p.tpa) * WG + (int)threadIdx.x;
+    const int jj = q / p.wb;
+    const int l = q - jj * p.wb;
+    const int wh = nx1 - p.a1;
+    if (l < wh) {
+      if (jj == 0 || jj > p.ny + 1) return false;
+      j = jj - 1;
+      i = p.a1 + l;
+    } else {
+      if (jj > p.ny) return false;
+      j = jj;
+      i = l - wh;
+    }
+  }
   k0 = kc * p.kchunk;
   k1 = min(k0 + p.kchunk, p.nz + 1);
   return true;
@@ -167,13 +196,14 @@ struct PFace {
   const unsigned char* air;
 };
 
-// The y and z faces in descriptor order, and per axis the (up to two) slab
-// intervals of the node index, [lo, lo + len).  The curl launch applies
-// these faces to the elements outside every x slab; the x-slab columns are
-// left entirely (all their faces, in order) to k_cpml_xcols_e/h.
+// Every face in descriptor order, and per axis the (up to two) slab
+// intervals of the node index, [lo, lo + len).  xin: the launch owns the
+// x-slab elements (split planes: they fill whole waves of their own); when
+// it does not, it leaves them, all their faces in order, to the x-column
+// pass (k_cpml_xcols_e/h) and applies the y and z faces elsewhere.
 template <typename T>
 struct PmlArgs {
-  int nf;
+  int nf, xin;
   int lo[3][2], len[3][2];
   PFace<T> f[MGH_CPML_MAX_FACES];
 };
@@ -186,7 +216,7 @@ __device__ __forceinline__ bool pml_hit(const PmlArgs<T>& pa, int i, int j, int 
                   (unsigned)(j - pa.lo[1][1]) < (unsigned)pa.len[1][1] ||
                   (unsigned)(k - pa.lo[2][0]) < (unsigned)pa.len[2][0] ||
                   (unsigned)(k - pa.lo[2][1]) < (unsigned)pa.len[2][1];
-  return yz && !xs;
+  return pa.xin ? (xs || yz) : (yz && !xs);
 }
 
 // Term `term` of face F on a component whose psi slab has x/y extents
@@ -917,8 +947,9 @@ inline int sets_per_thread(int nsets) {
   return (nsets % 3 == 0 && nsets % 2 == 1) ? 3 : kSetsPerThread;
 }
 
+// a0 < a1: split the planes at the interior columns [a0, a1) (see Plan)
 Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, const int* live = nullptr,
-               int nlive = 0) {
+               int nlive = 0, int a0 = 0, int a1 = 0) {
   Plan p;
   std::memset(&p, 0, sizeof(p));
   p.nx = nx;
@@ -927,6 +958,14 @@ Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, const int*
   p.kchunk = kchunk;
   p.ngroups = (nsets + ns - 1) / ns;
   p.ntp = (int)(((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG);
+  if (!live && a0 < a1) {
+    p.split = 1;
+    p.a0 = a0;
+    p.a1 = a1;
+    p.wb = (nx + 1 - a1) + a0;
+    p.tpa = (int)(((int64_t)(a1 - a0) * (ny + 1) + WG - 1) / WG);
+    p.ntp = p.tpa + (int)(((int64_t)p.wb * (ny + 2) + WG - 1) / WG);
+  }
   p.nkc = (nz + 1 + kchunk - 1) / kchunk;
   p.live = live;
   p.units = (live ? nlive : p.ntp * p.nkc) * p.ngroups;
@@ -954,8 +993,8 @@ int64_t slab_threads(int axis, int nx, int ny, int nz, int n) {
   return (int64_t)n * (ny + 1) * (nz + 1);
 }
 
-// the y and z faces of a CPML set for a fused launch (E or H side), and the
-// slab intervals of every face (x ones to exclude the x-slab columns)
+// every face of a CPML set for a fused launch (E or H side), descriptor
+// order, and every slab interval; xin is set by the caller
 template <typename T>
 PmlArgs<T> pml_args(const mgh::CpmlSet* pml, bool e_side, int nx, int ny, int nz) {
   PmlArgs<T> a;
@@ -967,7 +1006,6 @@ PmlArgs<T> pml_args(const mgh::CpmlSet* pml, bool e_side, int nx, int ny, int nz
     const int u = used[F.axis]++;  // at most two faces per axis (load_cpml_set)
     a.lo[F.axis][u] = F.k0;
     a.len[F.axis][u] = F.n;
-    if (F.axis == 0) continue;
     PFace<T>& P = a.f[a.nf++];
     P.axis = F.axis;
     P.k0 = F.k0;
@@ -980,13 +1018,32 @@ PmlArgs<T> pml_args(const mgh::CpmlSet* pml, bool e_side, int nx, int ny, int nz
     // term slab sizes: n times the component's extents along the other axes
     if (e_side) {
       if (F.axis == 2) { P.sz0 = n * nx * (ny + 1); P.sz1 = n * (nx + 1) * ny; }
-      else { P.sz0 = n * nx * (nz + 1); P.sz1 = n * (nx + 1) * nz; }
+      else if (F.axis == 1) { P.sz0 = n * nx * (nz + 1); P.sz1 = n * (nx + 1) * nz; }
+      else { P.sz0 = n * ny * (nz + 1); P.sz1 = n * (ny + 1) * nz; }
     } else {
       if (F.axis == 2) { P.sz0 = n * (nx + 1) * ny; P.sz1 = n * nx * (ny + 1); }
-      else { P.sz0 = n * (nx + 1) * nz; P.sz1 = n * nx * (nz + 1); }
+      else if (F.axis == 1) { P.sz0 = n * (nx + 1) * nz; P.sz1 = n * nx * (nz + 1); }
+      else { P.sz0 = n * (ny + 1) * nz; P.sz1 = n * ny * (nz + 1); }
     }
   }
   return a;
+}
+
+// Split planes for a CPML set: both x slabs at the domain ends ([0, a0) and
+// [a1, nx)) with interior columns between them.
+bool x_split(const mgh::CpmlSet* pml, int nx, int* a0, int* a1) {
+  if (!pml) return false;
+  int lo = -1, hi = -1;
+  for (int f = 0; f < pml->nfaces; ++f) {
+    const mgh::CpmlFace& F = pml->f[f];
+    if (F.axis != 0) continue;
+    if (F.k0 == 0) lo = f;
+    else if (F.k0 + F.n == nx) hi = f;
+  }
+  if (lo < 0 || hi < 0) return false;
+  *a0 = pml->f[lo].n;
+  *a1 = pml->f[hi].k0;
+  return *a0 < *a1;
 }
 
 // every face of a CPML set and its x faces' slabs, for the x-column pass
@@ -1028,9 +1085,12 @@ int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
           int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
   const int ns = sets_per_thread(nsets);
-  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
+  int a0 = 0, a1 = 0;
+  const bool split = !live && x_split(pml, nx, &a0, &a1);
+  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive, a0, a1);
   if (pl.units == 0) return MGH_OK;
-  const PmlArgs<T> pa = pml_args<T>(pml, true, nx, ny, nz);
+  PmlArgs<T> pa = pml_args<T>(pml, true, nx, ny, nz);
+  pa.xin = split;
   const bool fused = pml && pa.nf > 0;
   const dim3 grd(plan_blocks(pl));
 #define MGH_LAUNCH_E(NS_, PML_)                                                               \
@@ -1078,9 +1138,12 @@ int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
           int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
   const int ns = sets_per_thread(nsets);
-  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive);
+  int a0 = 0, a1 = 0;
+  const bool split = !live && x_split(pml, nx, &a0, &a1);
+  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive, a0, a1);
   if (pl.units == 0) return MGH_OK;
-  const PmlArgs<T> pa = pml_args<T>(pml, false, nx, ny, nz);
+  PmlArgs<T> pa = pml_args<T>(pml, false, nx, ny, nz);
+  pa.xin = split;
   const bool fused = pml && pa.nf > 0;
   const dim3 grd(plan_blocks(pl));
 #define MGH_LAUNCH_H(NS_, PML_)                                                              \
@@ -1128,6 +1191,8 @@ int run_cpml_h(const void* e, void* h, const void* bh, int nx, int ny, int nz, i
 template <typename T>
 int run_xcols_e(void* e, const void* h, const void* be, int nx, int ny, int nz, int nsets,
                 int64_t es, int64_t hs, const mgh::CpmlSet& pml, hipStream_t st) {
+  int a0, a1;
+  if (x_split(&pml, nx, &a0, &a1)) return MGH_OK;  // the curl launch took the x slabs
   int64_t nt = 0;
   const XCols<T> X = xcols_args<T>(pml, true, ny, nz, &nt);
   if (nt == 0) return MGH_OK;
@@ -1140,6 +1205,8 @@ int run_xcols_e(void* e, const void* h, const void* be, int nx, int ny, int nz, 
 template <typename T>
 int run_xcols_h(const void* e, void* h, const void* bh, int nx, int ny, int nz, int nsets,
                 int64_t es, int64_t hs, const mgh::CpmlSet& pml, hipStream_t st) {
+  int a0, a1;
+  if (x_split(&pml, nx, &a0, &a1)) return MGH_OK;  // the curl launch took the x slabs
   int64_t nt = 0;
   const XCols<T> X = xcols_args<T>(pml, false, ny, nz, &nt);
   if (nt == 0) return MGH_OK;
