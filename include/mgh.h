@@ -85,10 +85,12 @@ int mgh_update_h_tiles(int dtype, int nx, int ny, int nz, int nsets, const void*
 /* ---- (a3)+(a7) / (a4)+(a7): curl phases with CPML ------------------------
  * Same as mgh_update_e / mgh_update_h, followed by the CPML auxiliary
  * recursion of every listed face over its slab, in record order (same values
- * as mgh_update_* then mgh_cpml_*, bit for bit), as two launches: the curl
- * launch applies the y and z faces' terms to every element outside the x
- * slabs before its store (mgh_update_*_pml_core), then a launch over the
- * x-slab columns applies every face's terms there (mgh_cpml_xcols_*).
+ * as mgh_update_* then mgh_cpml_*, bit for bit): the curl launch applies the
+ * faces' terms to each slab element before its store (mgh_update_*_pml_core;
+ * its z planes split into interior-column and row-seam tiles so the x-slab
+ * elements fill waves of their own); when the x slabs are not both at the
+ * domain ends it leaves them to a launch over the x-slab columns
+ * (mgh_cpml_xcols_*, a no-op otherwise).
  * Replaces CPMLBoundary.update_E /
  * update_H (src/magnelio/boundaries/cpml.py:343-516), called right after the
  * curl update in e_phase_device / h_phase_device

@@ -42,17 +42,19 @@
 //
 // CPML, two forms with the same values.  With the curl (the
 // mgh_update_*_pml entry points): every d of every face is a difference of
-// neighbour words the curl already holds in registers, so an element in a y
-// or z slab applies those faces' terms, in descriptor order, to its updated
-// value before the single store; only psi, the layer coefficients and the
-// air mask add traffic.  One "node in some slab" test per element keeps the
-// interior on the plain path, and the face loop is not unrolled (its
-// per-face values are uniform scalar loads), which holds the register count
-// of the plain kernel.  Elements of the x slabs are left out of the curl
-// launch: they are a few lanes of most waves, and running their recursion
-// there stalls the whole wave (measured +25 us per phase on 256^3); a launch
-// over the x-slab columns (k_cpml_xcols_e/h) applies all their faces in
-// order after the curl.  Separate (mgh_cpml_e / mgh_cpml_h):
+// neighbour words the curl already holds in registers, so an element in a
+// slab applies its faces' terms, in descriptor order, to its updated value
+// before the single store; only psi, the layer coefficients and the air mask
+// add traffic.  One "node in some slab" test per element keeps the interior
+// on the plain path, and the face loop is not unrolled (its per-face values
+// are uniform scalar loads), which holds the register count of the plain
+// kernel.  The x-slab elements would be a few lanes of half the waves of a
+// row-tiled launch, and their recursion stalls the whole wave (measured +25
+// us per phase on 256^3), so such a launch splits its planes (Plan) and the
+// x-slab elements fill waves of their own; when the x slabs are not both at
+// the domain ends the launch leaves them to a pass over the x-slab columns
+// (k_cpml_xcols_e/h), which applies all their faces in order after the curl.
+// Separate (mgh_cpml_e / mgh_cpml_h):
 // one small launch per face over its slab, in descriptor order — the
 // structure of the reference (cpml.update_E / update_H per face after the
 // curl pass) — for listed launches and for dispersive meshes, whose ADE

@@ -66,10 +66,11 @@ def update_e(e: torch.Tensor, h: torch.Tensor, alpha_e: torch.Tensor, beta_e: to
     """E phase on K field sets: ``e`` is (K, n_E) or (n_E,), ``h`` likewise.
 
     ``cpml`` is a :class:`magnelio_amd.boundaries.DeviceCPML` (psi slabs for
-    the same K sets) or None: its y/z faces ride in the curl launch, the
-    x-slab columns follow in a launch of their own (``xcols=False`` leaves
-    that one to :func:`cpml_xcols_e`); ``live``: int32 device list of the
-    launch units to run (``tileskip.build_tile_plan``), None for all."""
+    the same K sets) or None: it rides in the curl launch, except the
+    x-slab columns when the x slabs are not both at the domain ends, which
+    follow in a launch of their own (``xcols=False`` leaves that one to
+    :func:`cpml_xcols_e`); ``live``: int32 device list of the launch units to
+    run (``tileskip.build_tile_plan``), None for all."""
     nx, ny, nz = dims
     ne, nh = n_e(nx, ny, nz), n_h(nx, ny, nz)
     e2 = e.view(-1, ne)

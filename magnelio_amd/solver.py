@@ -19,10 +19,11 @@ Step order (fit_td.py:950-1007), t = n dt:
   H phase    curl update, CPML faces                   mgh_update_h, mgh_cpml_h
 Without dispersion and listed (tile-skipping) launches, the CPML runs with
 the curl: the curl launch (mgh_update_e_pml_core / _h_pml_core) applies the
-y and z faces' terms, in face order, to every element outside the x slabs
-before its store, and a launch over the x-slab columns (mgh_cpml_xcols_e/h)
-applies every face's terms there — the same values as the separate slab
-passes.
+faces' terms, in face order, to each slab element before its store (its
+planes split so that the x-slab elements fill waves of their own); only when
+the x slabs are not both at the domain ends does a launch over the x-slab
+columns (mgh_cpml_xcols_e/h) apply theirs — the same values as the separate
+slab passes.
   sources    TF/SF H injection at t + 3dt/2
   monitors   DFT accumulation                          mgh_dft_*
   counter    n <- n + 1 on the device                  mgh_step_advance
