@@ -172,9 +172,11 @@ __device__ __forceinline__ void two_sum_acc(double* hi, float* lo, double v) {
 // per-wave loads.  A 512-step block takes the tile in two passes into the
 // same float32 accumulators; it halves the bin traffic per step.  The bins of
 // the tile are loaded before the MFMA chain so their HBM latency hides
-// behind the matrix work, and are read and written as complex pairs.
+// behind the matrix work, and are read and written as complex pairs.  Blocks
+// of up to 256 steps are held to 128 VGPRs (4 waves per SIMD, no spills; 3
+// without the bound); a 512-step block needs ~200 and runs at 2.
 template <int NB, bool COMP>
-__global__ __launch_bounds__(256) void k_dft_block(const float* __restrict__ X, int S, int S_pad,
+__global__ __launch_bounds__(256, (NB <= 64 ? 4 : 1)) void k_dft_block(const float* __restrict__ X, int S, int S_pad,
                                                    int B, int nb_valid,
                                                    const float* __restrict__ Pc,
                                                    const float* __restrict__ Ps, int F,
