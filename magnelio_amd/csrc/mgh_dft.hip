@@ -117,13 +117,27 @@ __global__ __launch_bounds__(256) void k_dft_sample_groups(
   const int64_t off = (int64_t)c0 * d[6] + (int64_t)c1 * d[7] + (int64_t)c2 * d[8];
   const T* f = d[4] ? h + (int64_t)set * h_set : e + (int64_t)set * e_set;
   const int nt = d[5];
-  float acc = 0.0f;
-  for (int t = 0; t < nt; ++t) {
-    const int ax = d[17 + t];
-    const int c = ax == 0 ? c0 : (ax == 1 ? c1 : (ax == 2 ? c2 : 0));
-    const float wt = wtab[d[25 + t] + c];
-    if (wt != 0.0f) acc += wt * (float)f[(int64_t)d[9 + t] + off];
+  // all tap weights, then all live taps' field words, then the sum in tap
+  // order: two memory round trips per sample instead of one per tap
+  float wv[DFT_NT], fv[DFT_NT];
+#pragma unroll
+  for (int t = 0; t < DFT_NT; ++t) {
+    wv[t] = 0.0f;
+    if (t < nt) {
+      const int ax = d[17 + t];
+      const int c = ax == 0 ? c0 : (ax == 1 ? c1 : (ax == 2 ? c2 : 0));
+      wv[t] = wtab[d[25 + t] + c];
+    }
   }
+#pragma unroll
+  for (int t = 0; t < DFT_NT; ++t) {
+    fv[t] = 0.0f;
+    if (wv[t] != 0.0f) fv[t] = (float)f[(int64_t)d[9 + t] + off];
+  }
+  float acc = 0.0f;
+#pragma unroll
+  for (int t = 0; t < DFT_NT; ++t)
+    if (wv[t] != 0.0f) acc += wv[t] * fv[t];
   const int s = d[0] + c0 + n0 * (c1 + n1 * c2);
   X[((int64_t)set * B + b) * S_pad + s] = acc;
 }
