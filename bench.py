@@ -556,7 +556,7 @@ def main_c3(args) -> int:
     field sets of one solver against the same six run one per solver."""
     import torch
 
-    from magnelio_amd.monitors import MonitorFieldFrequency
+    from magnelio_amd.monitors import MonitorFieldFrequency, blocks_for
     from magnelio_amd.sweep import ScatteringSweep
 
     t_setup = time.perf_counter()
@@ -564,6 +564,9 @@ def main_c3(args) -> int:
     nx, ny, nz = mesh.dims
     freqs = np.linspace(1e9, C3_FMAX, 2001 if args.freqs == 1001 else max(args.freqs, 1))
     block = 64 if args.dft_block == 256 else args.dft_block
+    # the warm-up captures the step graph (one bin-update period of the
+    # monitor, from the first aligned step after step 0) and replays it once
+    warmup = max(args.warmup, 2 * block * blocks_for(block)) if args.freqs > 0 else args.warmup
     kz = round(0.3 * nz)
 
     def monitors():
@@ -573,13 +576,13 @@ def main_c3(args) -> int:
                                       fields=("E",), block=block, compensated=not args.dft_plain,
                                       name="post_plane")]
 
-    total = args.warmup + args.steps
+    total = warmup + args.steps
     sweep = ScatteringSweep(mesh, ports, dt=dt, n_steps=total, f_axis=freqs, f_max=C3_FMAX,
                             monitors_factory=monitors)
     K = len(sweep.excited)
     setup_s = time.perf_counter() - t_setup
     sol = sweep.make_solver(sweep.excited)
-    t_batched = _timed_sweep_run(sol, args.warmup, args.steps)
+    t_batched = _timed_sweep_run(sol, warmup, args.steps)
     samples = sol.monitors[0].S if sol.monitors else 0
     bins_gb = sol.monitors[0].bytes() / 1e9 if sol.monitors else 0.0
     ok = bool(torch.isfinite(sol.e).all())
@@ -588,7 +591,7 @@ def main_c3(args) -> int:
     t_single = 0.0
     for ch in sweep.excited:
         s1 = sweep.make_solver([ch])
-        t_single += _timed_sweep_run(s1, args.warmup, args.steps)
+        t_single += _timed_sweep_run(s1, warmup, args.steps)
         ok = ok and bool(torch.isfinite(s1.e).all())
         del s1
         torch.cuda.empty_cache()
@@ -604,7 +607,7 @@ def main_c3(args) -> int:
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": t_batched / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",

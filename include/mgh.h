@@ -303,6 +303,19 @@ int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_val
                   double dt, double* blk, int s_h_start, double* bins, float* lo,
                   const long long* step_ctr, long long n_start, void* stream);
 
+/* mgh_dft_blocks: mgh_dft_block over nsub (1, 2 or 4) consecutive B-step
+ * blocks in one launch — X: nsets x nsub*B x S_pad, rows q*B .. q*B + B - 1
+ * are block q (first nb_valid rows valid), blk: nsub x F x 4 float64
+ * scratch; each block is its own contraction against the same Pc/Ps with its
+ * own exp(j w_f t0_q), the float64 contributions are summed and the bins read
+ * and written once (nsub = 1 is mgh_dft_block).  Requires Bp * nsub <= 256.
+ * Replaces the same DFTAccumulator.accumulate steps (monitors/_dft.py:37-52)
+ * as mgh_dft_block; the reference accumulates every step. */
+int mgh_dft_blocks(int nsets, const float* X, int S, int S_pad, int B, int nsub, int nb_valid,
+                   const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
+                   double dt, double* blk, int s_h_start, double* bins, float* lo,
+                   const long long* step_ctr, long long n_start, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

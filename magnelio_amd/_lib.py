@@ -143,6 +143,12 @@ SIGNATURES: dict[str, tuple] = {
          _c_int, _c_int, _c_double, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_ll,
          _c_void_p],
     ),
+    "mgh_dft_blocks": (
+        _c_int,
+        [_c_int, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p,
+         _c_void_p, _c_int, _c_int, _c_double, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p,
+         _c_ll, _c_void_p],
+    ),
 }
 
 

@@ -18,9 +18,10 @@
 // update of the float64 bins is fused into the MFMA epilogue, so the partial
 // products never reach HBM.  Each wave keeps its 16-sample x B-step operand
 // in registers and sweeps every frequency, so x is read once per block and
-// the bins are read and written once per block: the kernel is bound by the
+// the bins are read and written once per launch: the kernel is bound by the
 // bin traffic (16 B complex128 per (f, s), + 8 B of float32 compensation), which a
-// longer block divides.
+// longer block divides — or several consecutive blocks per launch, each its
+// own contraction and phase, summed before the single bin update.
 #include "mgh_common.h"
 
 namespace {
@@ -142,24 +143,27 @@ __global__ __launch_bounds__(256) void k_dft_sample_groups(
   X[((int64_t)set * B + b) * S_pad + s] = acc;
 }
 
-// Block phase factors exp(j w t0) for E and H stamps: blk[f*4 + 0..3] =
-// cos(w t0E), sin(w t0E), cos(w t0H), sin(w t0H).
+// Block phase factors exp(j w t0) for E and H stamps of each of the nsub
+// consecutive B-step blocks a contraction covers: blk[(q*F + f)*4 + 0..3] =
+// cos(w t0E), sin(w t0E), cos(w t0H), sin(w t0H) of block q.
 __global__ void k_dft_phase(const double* __restrict__ omega, int F, double dt,
                             const long long* __restrict__ ctr, long long n_start, int B,
-                            double* __restrict__ blk) {
+                            int nsub, double* __restrict__ blk) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
   if (f >= F) return;
   const long long n = *ctr;
-  const long long n0 = n - ((n - n_start) % B);
+  const long long n0 = n - ((n - n_start) % ((long long)B * nsub)) + (long long)q * B;
   const double te = (double)n0 * dt;
   const double th = te + 0.5 * dt;
   double s, c;
+  double* o = blk + ((int64_t)q * F + f) * 4;
   sincos(omega[f] * te, &s, &c);
-  blk[f * 4 + 0] = c;
-  blk[f * 4 + 1] = s;
+  o[0] = c;
+  o[1] = s;
   sincos(omega[f] * th, &s, &c);
-  blk[f * 4 + 2] = c;
-  blk[f * 4 + 3] = s;
+  o[2] = c;
+  o[3] = s;
 }
 
 // TwoSum of the float64 bin and the block's contribution; the rounding error
@@ -177,6 +181,11 @@ __device__ __forceinline__ void two_sum_acc(double* hi, float* lo, double v) {
 
 // NB: MFMA k-steps per block (the padded block length divided by 4).
 // COMP: TwoSum-compensated bins.
+// NSUB: consecutive blocks per launch (rows q*B .. q*B + B - 1 of X are block
+// q).  Each block is its own MFMA contraction against the same phasor tile
+// with its own phase factor; their float64 contributions are summed in
+// registers and the bins are read and written once for all of them, which
+// divides the bin traffic — the bound of short blocks — by NSUB.
 //
 // The four waves of a work-group take 64 consecutive samples and share each
 // frequency tile's phasor operands: the work-group copies the tile (cos and
@@ -186,21 +195,21 @@ __device__ __forceinline__ void two_sum_acc(double* hi, float* lo, double v) {
 // per-wave loads.  A 512-step block takes the tile in two passes into the
 // same float32 accumulators; it halves the bin traffic per step.  The bins of
 // the tile are loaded before the MFMA chain so their HBM latency hides
-// behind the matrix work, and are read and written as complex pairs.  Blocks
-// of up to 256 steps are held to 128 VGPRs (4 waves per SIMD, no spills; 3
-// without the bound); a 512-step block needs ~200 and runs at 2.
-template <int NB, bool COMP>
-__global__ __launch_bounds__(256, (NB <= 64 ? 4 : 1)) void k_dft_block(const float* __restrict__ X, int S, int S_pad,
-                                                   int B, int nb_valid,
-                                                   const float* __restrict__ Pc,
-                                                   const float* __restrict__ Ps, int F,
-                                                   const double* __restrict__ blk, int s_h_start,
-                                                   double* __restrict__ bins,
-                                                   float* __restrict__ lo) {
+// behind the matrix work, and are read and written as complex pairs.  Up to
+// 256 steps of samples per lane (NB x NSUB <= 64) the kernel is held to 128
+// VGPRs (4 waves per SIMD, no spills; 3 without the bound); a 512-step block
+// needs ~200 and runs at 2.
+template <int NB, bool COMP, int NSUB>
+__global__ __launch_bounds__(256, (NSUB > 1 ? 3 : (NB <= 64 ? 4 : 1))) void k_dft_block(
+    const float* __restrict__ X, int S, int S_pad, int B, int nb_valid,
+    const float* __restrict__ Pc, const float* __restrict__ Ps, int F,
+    const double* __restrict__ blk, int s_h_start, double* __restrict__ bins,
+    float* __restrict__ lo) {
   constexpr int NQ = NB / 4;                // float4 groups per lane
   constexpr int NQP = NQ < 16 ? NQ : 16;    // groups per LDS pass (256 steps)
   constexpr int NP = NQ / NQP;              // passes
   static_assert(NQ % NQP == 0, "block tile must be whole passes");
+  static_assert(NSUB == 1 || NP == 1, "several blocks per launch need a one-pass tile");
   __shared__ float4 sc[NQP * 64];
   __shared__ float4 ss[NQP * 64];
   const int lane = threadIdx.x & 63;
@@ -210,12 +219,20 @@ __global__ __launch_bounds__(256, (NB <= 64 ? 4 : 1)) void k_dft_block(const flo
   const bool active = s0 < S;  // idle waves still take part in the tile copies
   const int col = lane & 15;   // sample within the tile (B operand column / D column)
   const int krow = lane >> 4;  // 0..3: k within an MFMA step (A/B), row group of D
-  const float* Xs = X + (int64_t)set * B * S_pad;
-  float xb[NB];
+  const float* Xs = X + (int64_t)set * B * NSUB * S_pad;
+  float xb[NB * NSUB];
 #pragma unroll
-  for (int kk = 0; kk < NB; ++kk) {
-    const int b = 4 * kk + krow;
-    xb[kk] = (active && b < nb_valid) ? Xs[(int64_t)b * S_pad + s0 + col] : 0.0f;
+  for (int q = 0; q < NSUB; ++q) {
+#pragma unroll
+    for (int kk = 0; kk < NB; ++kk) {
+      const int b = 4 * kk + krow;
+      const int row = q * B + b;
+      // (one block: nb_valid <= B already bounds b)
+      const bool in = NSUB == 1 ? b < nb_valid : (b < B && row < nb_valid);
+      xb[q * NB + kk] = (active && in)
+                            ? Xs[(int64_t)row * S_pad + s0 + col]
+                            : 0.0f;
+    }
   }
   const int ph = (s0 >= s_h_start) ? 2 : 0;
   const int s = s0 + col;
@@ -237,53 +254,67 @@ __global__ __launch_bounds__(256, (NB <= 64 ? 4 : 1)) void k_dft_block(const flo
     // this tile's phasor operands, [q][lane] float4s, into LDS (pass by pass)
     const float4* gc = (const float4*)Pc + (int64_t)(f0 >> 4) * NQ * 64;
     const float4* gs = (const float4*)Ps + (int64_t)(f0 >> 4) * NQ * 64;
-    v4f cc = {0.f, 0.f, 0.f, 0.f};
-    v4f cs = {0.f, 0.f, 0.f, 0.f};
+    double2 dv[4];  // running sum of the earlier blocks' contributions
 #pragma unroll
-    for (int pass = 0; pass < NP; ++pass) {
-      __syncthreads();
-      for (int g = threadIdx.x; g < NQP * 64; g += 256) {
-        sc[g] = gc[pass * NQP * 64 + g];
-        ss[g] = gs[pass * NQP * 64 + g];
-      }
-      __syncthreads();
-      if (active) {
+    for (int q = 0; q < NSUB; ++q) {
+      v4f cc = {0.f, 0.f, 0.f, 0.f};
+      v4f cs = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = 0; q < NQP; ++q) {
-          const int qq = pass * NQP + q;
-          const float4 ac = sc[q * 64 + lane];
-          const float4 as = ss[q * 64 + lane];
-          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.x, xb[4 * qq + 0], cc, 0, 0, 0);
-          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, xb[4 * qq + 0], cs, 0, 0, 0);
-          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.y, xb[4 * qq + 1], cc, 0, 0, 0);
-          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, xb[4 * qq + 1], cs, 0, 0, 0);
-          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.z, xb[4 * qq + 2], cc, 0, 0, 0);
-          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, xb[4 * qq + 2], cs, 0, 0, 0);
-          cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.w, xb[4 * qq + 3], cc, 0, 0, 0);
-          cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, xb[4 * qq + 3], cs, 0, 0, 0);
+      for (int pass = 0; pass < NP; ++pass) {
+        if (q == 0) {  // one tile serves every block of the launch
+          __syncthreads();
+          for (int g = threadIdx.x; g < NQP * 64; g += 256) {
+            sc[g] = gc[pass * NQP * 64 + g];
+            ss[g] = gs[pass * NQP * 64 + g];
+          }
+          __syncthreads();
+        }
+        if (active) {
+          const float* xq = xb + q * NB;
+#pragma unroll
+          for (int u = 0; u < NQP; ++u) {
+            const int qq = pass * NQP + u;
+            const float4 ac = sc[u * 64 + lane];
+            const float4 as = ss[u * 64 + lane];
+            cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.x, xq[4 * qq + 0], cc, 0, 0, 0);
+            cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, xq[4 * qq + 0], cs, 0, 0, 0);
+            cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.y, xq[4 * qq + 1], cc, 0, 0, 0);
+            cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, xq[4 * qq + 1], cs, 0, 0, 0);
+            cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.z, xq[4 * qq + 2], cc, 0, 0, 0);
+            cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, xq[4 * qq + 2], cs, 0, 0, 0);
+            cc = __builtin_amdgcn_mfma_f32_16x16x4f32(ac.w, xq[4 * qq + 3], cc, 0, 0, 0);
+            cs = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, xq[4 * qq + 3], cs, 0, 0, 0);
+          }
         }
       }
-    }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int f = f0 + 4 * krow + r;
-      if (own && f < F) {
-        const double pre = (double)cc[r];
-        const double pim = (double)cs[r];
-        const double c = blk[f * 4 + ph];
-        const double sn = blk[f * 4 + ph + 1];
-        const double dre = c * pre - sn * pim;
-        const double dim = sn * pre + c * pim;
-        const int64_t o = (int64_t)f * S + s;
-        if (COMP) {
-          two_sum_acc(&bv[r].x, &lv[r].x, dre);
-          two_sum_acc(&bv[r].y, &lv[r].y, dim);
-          lset[o] = lv[r];
-        } else {
-          bv[r].x += dre;
-          bv[r].y += dim;
+      for (int r = 0; r < 4; ++r) {
+        const int f = f0 + 4 * krow + r;
+        if (own && f < F) {
+          const double pre = (double)cc[r];
+          const double pim = (double)cs[r];
+          const double c = blk[(q * F + f) * 4 + ph];
+          const double sn = blk[(q * F + f) * 4 + ph + 1];
+          const double dre = c * pre - sn * pim;
+          const double dim = sn * pre + c * pim;
+          if (q + 1 < NSUB) {  // an earlier block: into the running sum
+            dv[r].x = q == 0 ? dre : dv[r].x + dre;
+            dv[r].y = q == 0 ? dim : dv[r].y + dim;
+          } else {  // the last block: the launch's contribution into the bins
+            const double tre = q == 0 ? dre : dv[r].x + dre;
+            const double tim = q == 0 ? dim : dv[r].y + dim;
+            const int64_t o = (int64_t)f * S + s;
+            if (COMP) {
+              two_sum_acc(&bv[r].x, &lv[r].x, tre);
+              two_sum_acc(&bv[r].y, &lv[r].y, tim);
+              lset[o] = lv[r];
+            } else {
+              bv[r].x += tre;
+              bv[r].y += tim;
+            }
+            bset[o] = bv[r];
+          }
         }
-        bset[o] = bv[r];
       }
     }
   }
@@ -352,52 +383,71 @@ int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_s
   return MGH_OK;
 }
 
-// Contract the block (first nb_valid rows of X) and accumulate into the
-// float64 bins (nsets x F x S complex, interleaved re/im); lo (same shape or
-// NULL; float32) holds the compensation terms.  Pc/Ps: the float32 phasor tables
+// Contract nsub consecutive B-step blocks (X: nsets x nsub*B x S_pad, rows
+// q*B .. q*B + B - 1 are block q; the first nb_valid rows hold samples) and
+// accumulate into the float64 bins (nsets x F x S complex, interleaved
+// re/im) once; lo (same shape or NULL; float32) holds the compensation
+// terms.  nsub is 1, 2 or 4, with B rounded up to a multiple of 64 times nsub
+// at most 256.  Pc/Ps: the float32 phasor tables
 // P[f][b] = cos/sin(w_f b dt) dt (f < F_pad, b < Bp = B rounded up to a
 // multiple of 64, zero beyond B and F) stored in MFMA lane order, in float4
 // groups: element [t][q][l][c] (t < F_pad/16, q < Bp/16, l < 64, c < 4) =
 // P[16t + l%16][4(4q + c) + l/16] — a tile is a lane-contiguous block that
 // a work-group copies straight into LDS.  Samples with index >= s_h_start carry the H time stamp.
-// blk: F x 4 float64 scratch for the block phase, filled here from
-// *step_ctr (n = last step of the block, or one past it for a partial one).
-int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
-                  const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
-                  double dt, double* blk, int s_h_start, double* bins, float* lo,
-                  const long long* step_ctr, long long n_start, void* stream) {
+// blk: nsub x F x 4 float64 scratch for the block phases, filled here from
+// *step_ctr (n = last step of the last block, or one past it for a partial one).
+int mgh_dft_blocks(int nsets, const float* X, int S, int S_pad, int B, int nsub, int nb_valid,
+                   const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
+                   double dt, double* blk, int s_h_start, double* bins, float* lo,
+                   const long long* step_ctr, long long n_start, void* stream) {
   MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts");
   MGH_REQUIRE(F >= 1 && F_pad >= F && F_pad % 16 == 0, "bad frequency counts F=%d F_pad=%d", F,
               F_pad);
-  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX && nb_valid >= 0 && nb_valid <= B, "bad block length");
+  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "bad block length %d", B);
+  MGH_REQUIRE(nsub == 1 || nsub == 2 || nsub == 4, "blocks per launch must be 1, 2 or 4, got %d",
+              nsub);
+  MGH_REQUIRE(nsub == 1 || dft_bpad(B) * nsub <= 256,
+              "%d blocks of %d steps exceed 256 sample rows per lane", nsub, B);
+  MGH_REQUIRE(nb_valid >= 0 && nb_valid <= B * nsub, "bad valid row count %d", nb_valid);
   MGH_REQUIRE(X && Pc && Ps && omega && blk && bins && step_ctr, "null pointer");
   MGH_REQUIRE(s_h_start % 16 == 0, "s_h_start must be a multiple of 16");
   MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_dft_phase, dim3((F + 255) / 256), dim3(256), 0, st, omega, F, dt, step_ctr,
-                     n_start, B, blk);
+  hipLaunchKernelGGL(k_dft_phase, dim3((F + 255) / 256, nsub), dim3(256), 0, st, omega, F, dt,
+                     step_ctr, n_start, B, nsub, blk);
   MGH_CHECK_LAUNCH("k_dft_phase");
   const int waves = (S + 15) / 16;
   dim3 grd((waves + 3) / 4, nsets, 1);
   const int bp = dft_bpad(B);
-#define MGH_DFT_LAUNCH(NB)                                                                  \
-  do {                                                                                      \
-    if (lo)                                                                                 \
-      hipLaunchKernelGGL((k_dft_block<NB, true>), grd, dim3(256), 0, st, X, S, S_pad, B,    \
-                         nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, lo);     \
-    else                                                                                    \
-      hipLaunchKernelGGL((k_dft_block<NB, false>), grd, dim3(256), 0, st, X, S, S_pad, B,   \
-                         nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, lo);     \
+#define MGH_DFT_LAUNCH(NB, NSUB)                                                             \
+  do {                                                                                       \
+    if (lo)                                                                                  \
+      hipLaunchKernelGGL((k_dft_block<NB, true, NSUB>), grd, dim3(256), 0, st, X, S, S_pad, \
+                         B, nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, lo);   \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_dft_block<NB, false, NSUB>), grd, dim3(256), 0, st, X, S,       \
+                         S_pad, B, nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, \
+                         lo);                                                                \
   } while (0)
-  if (bp == 64) MGH_DFT_LAUNCH(16);
-  else if (bp == 128) MGH_DFT_LAUNCH(32);
-  else if (bp == 192) MGH_DFT_LAUNCH(48);
-  else if (bp == 256) MGH_DFT_LAUNCH(64);
-  else MGH_DFT_LAUNCH(128);
+  if (nsub == 4) MGH_DFT_LAUNCH(16, 4);
+  else if (nsub == 2 && bp == 64) MGH_DFT_LAUNCH(16, 2);
+  else if (nsub == 2) MGH_DFT_LAUNCH(32, 2);
+  else if (bp == 64) MGH_DFT_LAUNCH(16, 1);
+  else if (bp == 128) MGH_DFT_LAUNCH(32, 1);
+  else if (bp == 192) MGH_DFT_LAUNCH(48, 1);
+  else if (bp == 256) MGH_DFT_LAUNCH(64, 1);
+  else MGH_DFT_LAUNCH(128, 1);
 #undef MGH_DFT_LAUNCH
   MGH_CHECK_LAUNCH("k_dft_block");
-  (void)F_pad;
   return MGH_OK;
+}
+
+int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,
+                  const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
+                  double dt, double* blk, int s_h_start, double* bins, float* lo,
+                  const long long* step_ctr, long long n_start, void* stream) {
+  return mgh_dft_blocks(nsets, X, S, S_pad, B, 1, nb_valid, Pc, Ps, omega, F, F_pad, dt, blk,
+                        s_h_start, bins, lo, step_ctr, n_start, stream);
 }
 
 }  // extern "C"

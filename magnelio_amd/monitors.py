@@ -55,6 +55,18 @@ def natural(vals: np.ndarray, shape) -> np.ndarray:
     return arr.transpose(*range(len(lead)), *range(len(lead) + nd - 1, len(lead) - 1, -1))
 
 
+def blocks_for(block: int, requested: int | None = None) -> int:
+    """Consecutive blocks contracted per launch (the bins are read and
+    written once for all of them): ``requested`` (1, 2 or 4, checked), or by
+    default as many as fit 256 sample rows per lane."""
+    fits = [n for n in (4, 2, 1) if n == 1 or block_pad(block) * n <= 256]
+    if requested is None:
+        return fits[0]
+    if int(requested) not in fits:
+        raise ValueError(f"blocks_per_launch must be one of {sorted(fits)} for {block}-step blocks")
+    return int(requested)
+
+
 def block_pad(block: int) -> int:
     """Phasor-table row length of a block: whole 64-step register tiles up to
     256 steps, 512 beyond (the contraction kernel's variants)."""
@@ -258,13 +270,15 @@ class DFTMonitorBase:
     # (False: always the generic per-sample (index, weight) gather)
     structured = True
 
-    def __init__(self, freqs, name: str = "", block: int = BLOCK, compensated: bool = True):
+    def __init__(self, freqs, name: str = "", block: int = BLOCK, compensated: bool = True,
+                 blocks_per_launch: int | None = None):
         self.freqs = np.asarray(freqs, dtype=float).ravel()
         if self.freqs.size == 0:
             raise ValueError("freqs must be a non-empty 1D array")
         if not (1 <= block <= BMAX):
             raise ValueError(f"block must be 1..{BMAX}")
         self.block = int(block)
+        self.blocks_per_launch = blocks_for(self.block, blocks_per_launch)
         self.compensated = compensated
         self.name = name or f"dft_{id(self):x}"
         self._attached = False
@@ -359,8 +373,8 @@ class DFTMonitorBase:
         self.Pc = torch.from_numpy(lane_order(Pc)).to(dev, f32)
         self.Ps = torch.from_numpy(lane_order(Ps)).to(dev, f32)
         self.omega = torch.from_numpy(omega).to(dev, torch.float64)
-        self.blk = torch.zeros((self.F, 4), dtype=torch.float64, device=dev)
-        self.X = torch.zeros((K, self.block, self.S_pad), dtype=f32, device=dev)
+        self.blk = torch.zeros((self.blocks_per_launch * self.F, 4), dtype=torch.float64, device=dev)
+        self.X = torch.zeros((K, self.period, self.S_pad), dtype=f32, device=dev)
         self.bins = torch.zeros((K, self.F, self.S, 2), dtype=torch.float64, device=dev)
         # TwoSum compensation terms of the bins, float32 (csrc/mgh_dft.hip two_sum_acc)
         self.lo = torch.zeros(self.bins.shape, dtype=f32, device=dev) if self.compensated else None
@@ -368,6 +382,11 @@ class DFTMonitorBase:
         self.n_start = solver.n_completed
         self._pending = 0
         self._attached = True
+
+    @property
+    def period(self) -> int:
+        """Steps between bin updates (sample rows of the block buffer)."""
+        return self.block * self.blocks_per_launch
 
     def bytes(self) -> int:
         n = self.bins.numel() * (12 if self.compensated else 8)
@@ -380,21 +399,23 @@ class DFTMonitorBase:
             _lib.call("mgh_dft_sample_groups", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
                       solver.h.data_ptr(), solver.h.shape[1], self.grp_d.data_ptr(),
                       self.grp_d.shape[0], self.grp_blocks, self.wtab_d.data_ptr(), self.S,
-                      self.X.data_ptr(), self.S_pad, self.block, solver.step_ctr.data_ptr(),
+                      self.X.data_ptr(), self.S_pad, self.period, solver.step_ctr.data_ptr(),
                       self.n_start, sh)
         else:
             _lib.call("mgh_dft_sample", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
                       solver.h.data_ptr(), solver.h.shape[1], self.idx_d.data_ptr(),
-                      self.w_d.data_ptr(), self.S, self.X.data_ptr(), self.S_pad, self.block,
+                      self.w_d.data_ptr(), self.S, self.X.data_ptr(), self.S_pad, self.period,
                       solver.step_ctr.data_ptr(), self.n_start, sh)
-        if (n - self.n_start) % self.block == self.block - 1:
-            self._contract(solver, sh, self.block)
+        P = self.period
+        if (n - self.n_start) % P == P - 1:
+            self._contract(solver, sh, P)
             self._pending = 0
         else:
-            self._pending = (n - self.n_start) % self.block + 1
+            self._pending = (n - self.n_start) % P + 1
 
     def _contract(self, solver, sh: int, nb: int) -> None:
-        _lib.call("mgh_dft_block", self.nsets, self.X.data_ptr(), self.S, self.S_pad, self.block, nb,
+        _lib.call("mgh_dft_blocks", self.nsets, self.X.data_ptr(), self.S, self.S_pad, self.block,
+                  self.blocks_per_launch, nb,
                   self.Pc.data_ptr(), self.Ps.data_ptr(), self.omega.data_ptr(), self.F, self.F_pad,
                   float(self.dt), self.blk.data_ptr(), self.s_h_start, self.bins.data_ptr(),
                   self.lo.data_ptr() if self.lo is not None else None, solver.step_ctr.data_ptr(),
@@ -438,8 +459,8 @@ class MonitorFieldFrequency(DFTMonitorBase):
     """DFT of cell-centred fields in a box region (field_frequency.py)."""
 
     def __init__(self, freqs, corners=None, fields=("E", "H"), name: str = "", region=None,
-                 block: int = BLOCK, compensated: bool = True):
-        super().__init__(freqs, name, block, compensated)
+                 block: int = BLOCK, compensated: bool = True, blocks_per_launch: int | None = None):
+        super().__init__(freqs, name, block, compensated, blocks_per_launch)
         self.corners = corners
         self.region = region  # optional explicit (slice, slice, slice) of cells
         comps = []
@@ -482,8 +503,8 @@ class MonitorFarField(DFTMonitorBase):
     """
 
     def __init__(self, freqs, margin_cells: int = 3, name: str = "", block: int = BLOCK,
-                 compensated: bool = True):
-        super().__init__(freqs, name, block, compensated)
+                 compensated: bool = True, blocks_per_launch: int | None = None):
+        super().__init__(freqs, name, block, compensated, blocks_per_launch)
         if margin_cells < 1:
             raise ValueError("margin_cells must be at least 1")
         self.margin_cells = margin_cells

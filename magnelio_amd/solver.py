@@ -512,7 +512,8 @@ class FITTimeDomainSolver:
         ``graph_block``."""
         g = max(1, int(self.graph_block))
         for mon in self.monitors:
-            g = g * getattr(mon, "block", 1) // math.gcd(g, getattr(mon, "block", 1))
+            p = getattr(mon, "period", 1)
+            g = g * p // math.gcd(g, p)
         return g
 
     def run(self, n_steps: int | None = None) -> None:
@@ -597,7 +598,7 @@ class FITTimeDomainSolver:
         return done
 
     def _mon_aligned(self, n: int, G: int) -> bool:
-        return all((n - getattr(m, "n_start", 0)) % getattr(m, "block", 1) == 0 for m in self.monitors)
+        return all((n - getattr(m, "n_start", 0)) % getattr(m, "period", 1) == 0 for m in self.monitors)
 
     def _capture(self, n: int, G: int) -> None:
         """Capture G steps as one HIP graph; fall back to eager launches on failure.

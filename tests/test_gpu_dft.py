@@ -4,7 +4,8 @@
   (70, H samples from 48), two field sets, a full and a partial block, plain
   and compensated bins, blocks of 64-512 steps (one and two LDS passes) — a
   wrong MFMA operand or result lane layout gives O(1) errors, float32
-  products give ~1e-7;
+  products give ~1e-7; 2 or 4 consecutive blocks per launch (each with its
+  own phase, bins updated once), full and partial fills;
 * a region monitor inside the plain WR-90 run (C1, 300 steps: four full
   blocks and a partial one closed by ``finalize``) against the oracle's
   FieldDFT fed with float64 fields — FP64 and FP32 stepping;
@@ -79,6 +80,78 @@ def test_block_contraction_matches_numpy(B, gpu):
                 assert err < 1e-5
 
 
+@pytest.mark.parametrize("B,nsub", [(64, 2), (64, 4), (100, 2), (128, 2)])
+def test_blocks_per_launch_match_numpy(B, nsub, gpu):
+    """nsub consecutive blocks in one launch: each block's own contraction and
+    phase, the bins updated once (full, one-block-and-a-bit and short fills)."""
+    rng = np.random.default_rng(11)
+    K, S, s_h, F = 2, 70, 48, 37
+    S_pad, F_pad = 80, 48
+    dt = 1.3e-12
+    P_steps = B * nsub
+    omega = 2 * np.pi * rng.uniform(1e9, 30e9, F)
+    bp = block_pad(B)
+    b = np.arange(bp)
+    Pc = np.zeros((F_pad, bp), np.float32)
+    Ps = np.zeros((F_pad, bp), np.float32)
+    Pc[:F, :B] = (np.cos(np.outer(omega, b * dt)) * dt)[:, :B]
+    Ps[:F, :B] = (np.sin(np.outer(omega, b * dt)) * dt)[:, :B]
+    X = rng.standard_normal((K, P_steps, S_pad)).astype(np.float32)
+    n_start = 5
+    ctr_val = n_start + 3 * P_steps + 40
+    n0 = n_start + 3 * P_steps
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev)
+    Pcd, Psd = torch.from_numpy(lane_order(Pc)).to(dev), torch.from_numpy(lane_order(Ps)).to(dev)
+    om = torch.from_numpy(omega).to(dev)
+    blk = torch.zeros((nsub * F, 4), dtype=torch.float64, device=dev)
+    ctr = torch.tensor([ctr_val], dtype=torch.int64, device=dev)
+    P = (Pc[:F].astype(np.float64) + 1j * Ps[:F].astype(np.float64))
+    for nb in (P_steps, B + 17, 23):
+        for comp in (False, True):
+            print(f"dft blocks B={B} nsub={nsub} nb={nb} compensated={comp}", flush=True)
+            base = rng.standard_normal((K, F, S, 2)) * (8 * dt)
+            bins = torch.from_numpy(base.copy()).to(dev)
+            lo = torch.zeros(bins.shape, dtype=torch.float32, device=dev) if comp else None
+            stream = torch.cuda.current_stream(dev)
+            _lib.call("mgh_dft_blocks", K, Xd.data_ptr(), S, S_pad, B, nsub, nb, Pcd.data_ptr(),
+                      Psd.data_ptr(), om.data_ptr(), F, F_pad, dt, blk.data_ptr(), s_h,
+                      bins.data_ptr(), lo.data_ptr() if comp else None, ctr.data_ptr(), n_start,
+                      int(stream.cuda_stream))
+            torch.cuda.synchronize(dev)
+            got = bins.cpu().numpy()
+            if comp:
+                got = got + lo.cpu().numpy().astype(np.float64)
+            for k in range(K):
+                want = np.zeros((F, S), complex)
+                for q in range(nsub):
+                    rows = np.arange(q * B, min((q + 1) * B, nb))
+                    if rows.size == 0:
+                        continue
+                    t0 = (n0 + q * B) * dt + np.where(np.arange(S) >= s_h, 0.5 * dt, 0.0)
+                    rot = np.exp(1j * omega[:, None] * t0[None, :])
+                    want += (P[:, : rows.size] @ X[k, rows, :S].astype(np.float64)) * rot
+                g = (got[k, ..., 0] - base[k, ..., 0]) + 1j * (got[k, ..., 1] - base[k, ..., 1])
+                err = np.abs(g - want).max() / (math.sqrt(nb) * dt)
+                print(f"  set {k}: max err / (sqrt(nb) dt) = {err:.2e}", flush=True)
+                assert err < 1e-5
+
+
+def test_blocks_per_launch_checks(gpu):
+    dev = torch.device("cuda:0")
+    z = torch.zeros(64, dtype=torch.float64, device=dev)
+    stream = int(torch.cuda.current_stream(dev).cuda_stream)
+    # 4 blocks of 100 steps (128 padded) exceed the 256 rows a lane holds
+    with pytest.raises(ValueError, match="exceed 256"):
+        _lib.call("mgh_dft_blocks", 1, z.data_ptr(), 1, 16, 100, 4, 10, z.data_ptr(), z.data_ptr(),
+                  z.data_ptr(), 1, 16, 1e-12, z.data_ptr(), 0, z.data_ptr(), None, z.data_ptr(), 0,
+                  stream)
+    with pytest.raises(ValueError, match="must be 1, 2 or 4"):
+        _lib.call("mgh_dft_blocks", 1, z.data_ptr(), 1, 16, 64, 3, 10, z.data_ptr(), z.data_ptr(),
+                  z.data_ptr(), 1, 16, 1e-12, z.data_ptr(), 0, z.data_ptr(), None, z.data_ptr(), 0,
+                  stream)
+
+
 def _oracle_dft(steps):
     orc, _, dt, _ = cases.c1_oracle(np.float64)
     x, y, z = cases.c1_axes()
@@ -94,8 +167,8 @@ def oracle_bins():
     return _oracle_dft(N_STEPS)
 
 
-def _hip_bins(precision, use_graphs=False, block=64):
-    mon = MonitorFieldFrequency(FREQS, region=REGION, block=block)
+def _hip_bins(precision, use_graphs=False, block=64, per_launch=None):
+    mon = MonitorFieldFrequency(FREQS, region=REGION, block=block, blocks_per_launch=per_launch)
     sol, _, _ = cases.c1_solver(n_steps=N_STEPS, precision=precision, use_graphs=use_graphs,
                                 monitors=[mon])
     sol.run()
@@ -117,12 +190,14 @@ def _errors(got, want):
     return out
 
 
-@pytest.mark.parametrize("precision,gate,block", [("double", 1e-5, 64), ("single", 2e-5, 64),
-                                                  ("double", 1e-5, 256), ("single", 2e-5, 256),
-                                                  ("single", 2e-5, 512)])
-def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, gpu):
-    print(f"HIP C1 {precision} + region DFT (block {block}), {N_STEPS} steps", flush=True)
-    got = _hip_bins(precision, block=block)
+@pytest.mark.parametrize("precision,gate,block,per_launch", [
+    ("double", 1e-5, 64, None), ("single", 2e-5, 64, None), ("single", 2e-5, 64, 1),
+    ("single", 2e-5, 128, None), ("double", 1e-5, 256, None), ("single", 2e-5, 256, None),
+    ("single", 2e-5, 512, None)])
+def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, per_launch, gpu):
+    print(f"HIP C1 {precision} + region DFT (block {block}, {per_launch} per launch), "
+          f"{N_STEPS} steps", flush=True)
+    got = _hip_bins(precision, block=block, per_launch=per_launch)
     assert set(got) == set(oracle_bins)
     for c in oracle_bins:
         assert got[c].shape == oracle_bins[c].shape
@@ -133,9 +208,10 @@ def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, gpu):
 
 
 def test_region_monitor_graph_equals_eager(gpu):
+    # two 64-step blocks per launch: a 128-step graph (steps 128-255) inside the 300
     print("HIP C1 single + region DFT, graphs vs eager", flush=True)
-    eager = _hip_bins("single")
-    graph = _hip_bins("single", use_graphs=True)
+    eager = _hip_bins("single", per_launch=2)
+    graph = _hip_bins("single", use_graphs=True, per_launch=2)
     for c in eager:
         assert np.array_equal(eager[c], graph[c]), c
 

@@ -7,6 +7,7 @@ Huygens-face weighting of the far-field monitor (far_field.py:230-248).
 """
 
 import numpy as np
+import pytest
 
 from magnelio_amd import mesh as mm
 from magnelio_amd.boundaries import CPMLBoundary
@@ -192,3 +193,16 @@ def test_structured_launch_covers_each_sample_once():
         for d in desc:
             want[d[0]:d[0] + d[1]] = 1
         assert np.array_equal(hits, want)
+
+
+def test_blocks_per_launch_rule():
+    """Consecutive blocks per contraction: as many as fit 256 sample rows per
+    lane (blocks padded to 64 steps), or the requested count when it fits."""
+    from magnelio_amd.monitors import blocks_for
+
+    assert [blocks_for(b) for b in (1, 64, 65, 100, 128, 129, 256, 300, 512)] == [4, 4, 2, 2, 2, 1, 1,
+                                                                                  1, 1]
+    assert blocks_for(64, 1) == 1 and blocks_for(64, 2) == 2 and blocks_for(128, 2) == 2
+    for b, n in ((64, 3), (100, 4), (256, 2), (512, 4)):
+        with pytest.raises(ValueError):
+            blocks_for(b, n)
