@@ -133,7 +133,10 @@ __global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps,
   __shared__ double tot[PQ];
   const int q = threadIdx.x;
   if (q < PQ) {
+    // chunk order kept; the unrolled loop issues the partials' loads together
+    // (they were written on other XCDs, so each is a trip past this L2)
     double v = 0.0;
+#pragma unroll 16
     for (int c = 0; c < nch; ++c) v += P.work[((int64_t)s * nch + c) * PQ + q];
     tot[q] = v;
   }
@@ -251,6 +254,7 @@ __global__ __launch_bounds__(64) void k_port_record(mgh::PortSet ps,
   const int q = threadIdx.x;
   if (q < PQ) {
     double v = 0.0;
+#pragma unroll 16
     for (int c = 0; c < npc; ++c) v += part[((int64_t)s * nch + c) * PQ + q];
     tot[q] = v;
   }
