@@ -99,24 +99,30 @@ __global__ __launch_bounds__(PB) void k_port_project(const T* __restrict__ e, in
         if (m < M) acc[2 * m + part] += me * (P.prof_e[(int64_t)m * n_pts + p] * ev);
     }
   }
-  // DTBC history convolutions sum_{j=1..n} l_j hist[n-j], terms of this chunk
-  if (n >= 1 && (long long)c * MGH_PORT_CHUNK < n) {
+  // DTBC history convolutions sum_{j=1..n} l_j hist[n-j], terms of this chunk.
+  // The kernel slice is the same for every mode and the modes' flags are read
+  // up front, so the loads of all modes are predicated lanes of one straight
+  // stretch of code and issue together (a term outside j <= n, or of a mode
+  // without the DTBC, adds +0).
+  long long off;
+  if (n >= 1 && (long long)c * MGH_PORT_CHUNK < n && kernel_slice(P, n, &off)) {
+    bool dtbc[MGH_PORT_MAX_MODES];
+#pragma unroll
+    for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) dtbc[m] = m < M && P.params[8 * m] == 1.0;
 #pragma unroll
     for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) {
-      if (m >= M || P.params[8 * m] != 1.0) continue;
-      long long off;
-      if (!kernel_slice(P, n, &off)) continue;
       const double* l = P.kern + (int64_t)m * P.kern_stride + off;
       const double* wh = P.w_hist + ((int64_t)s * M + m) * P.cap;
       const double* sh = P.s_hist + ((int64_t)s * M + m) * P.cap;
 #pragma unroll
       for (int r = 0; r < PR; ++r) {
         const long long j = 1 + (long long)c * MGH_PORT_CHUNK + r * PB + threadIdx.x;
-        if (j <= n) {
-          const double lj = l[j];
-          acc[PH + 2 * m] += lj * wh[n - j];
-          acc[PH + 2 * m + 1] += lj * sh[n - j];
-        }
+        const bool ok = dtbc[m] && j <= n;
+        const double lj = ok ? l[j] : 0.0;
+        const double wv = ok ? wh[n - j] : 0.0;
+        const double sv = ok ? sh[n - j] : 0.0;
+        acc[PH + 2 * m] += lj * wv;
+        acc[PH + 2 * m + 1] += lj * sv;
       }
     }
   }
