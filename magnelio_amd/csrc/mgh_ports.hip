@@ -95,8 +95,12 @@ __global__ __launch_bounds__(PB) void k_port_project(const T* __restrict__ e, in
       const double me = P.me_int[p];
       const int part = p < P.n_u ? 0 : 1;
 #pragma unroll
-      for (int m = 0; m < MGH_PORT_MAX_MODES; ++m)
-        if (m < M) acc[2 * m + part] += me * (P.prof_e[(int64_t)m * n_pts + p] * ev);
+      for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) {
+        // modes past M read row 0 into partial words that are never stored,
+        // so every mode's load is straight-line code and they issue together
+        const int mm = m < M ? m : 0;
+        acc[2 * m + part] += me * (P.prof_e[(int64_t)mm * n_pts + p] * ev);
+      }
     }
   }
   // DTBC history convolutions sum_{j=1..n} l_j hist[n-j], terms of this chunk.
@@ -224,8 +228,11 @@ __global__ __launch_bounds__(PB) void k_port_write(T* __restrict__ e, int64_t e_
     if (p < n_pts) {
       double val = 0.0;
 #pragma unroll
-      for (int m = 0; m < MGH_PORT_MAX_MODES; ++m)
-        if (m < M) val += vc[m] * P.prof_e[(int64_t)m * n_pts + p];
+      for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) {
+        // modes past M: vc = 0, an exact zero term (loads issue together)
+        const int mm = m < M ? m : 0;
+        val += vc[m] * P.prof_e[(int64_t)mm * n_pts + p];
+      }
       const T tv = (T)val;
       es[P.idx_port[p]] = tv;
       if (rec) {
@@ -235,10 +242,10 @@ __global__ __launch_bounds__(PB) void k_port_write(T* __restrict__ e, int64_t e_
         const int part = p < P.n_u ? 0 : 1;
 #pragma unroll
         for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) {
-          if (m < M) {
-            acc[2 * m + part] += me * (P.prof_e[(int64_t)m * n_pts + p] * ev);
-            acc[PH + 2 * m + part] += mh * (P.prof_h[(int64_t)m * n_pts + p] * hv);
-          }
+          // modes past M: row 0 into partial words that are never stored
+          const int mm = m < M ? m : 0;
+          acc[2 * m + part] += me * (P.prof_e[(int64_t)mm * n_pts + p] * ev);
+          acc[PH + 2 * m + part] += mh * (P.prof_h[(int64_t)mm * n_pts + p] * hv);
         }
       }
     }
