@@ -151,12 +151,14 @@ __global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps,
     tot[q] = v;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double* st = P.state + ((int64_t)s * M) * 4;
-  double* vcs = P.work + vc_offset(nch, gridDim.y) + (int64_t)s * MGH_PORT_MAX_MODES;
-  const int exc = P.exc_mode ? P.exc_mode[s] : -1;
-  double env = 0.0;
-  for (int m = 0; m < M; ++m) {
+  // one lane per mode (the modes' recursions are independent); lane 0 then
+  // takes the envelope maximum in mode order
+  __shared__ double amp[MGH_PORT_MAX_MODES];
+  const int m = threadIdx.x;
+  if (m < M) {
+    double* st = P.state + ((int64_t)s * M) * 4;
+    double* vcs = P.work + vc_offset(nch, gridDim.y) + (int64_t)s * MGH_PORT_MAX_MODES;
+    const int exc = P.exc_mode ? P.exc_mode[s] : -1;
     const double* prm = P.params + 8 * m;
     double* sm = st + 4 * m;  // u, u_prev, V_port_prev, V_int_prev
     const double vint = tot[2 * m] + tot[2 * m + 1];
@@ -196,9 +198,12 @@ __global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps,
     vcs[m] = v;
     sm[2] = v;
     sm[3] = vint;
-    const double a = fabs(v) * prm[4];
-    env = a > env ? a : env;
+    amp[m] = fabs(v) * prm[4];
   }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double env = 0.0;
+  for (int q = 0; q < M; ++q) env = amp[q] > env ? amp[q] : env;
   if (P.absmax && env > P.absmax[s]) P.absmax[s] = env;
 }
 template <typename T>
@@ -272,14 +277,13 @@ __global__ __launch_bounds__(64) void k_port_record(mgh::PortSet ps,
     tot[q] = v;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int m = 0; m < M; ++m) {
-    const double scale = P.params[8 * m + 4];
-    const int64_t o = ((int64_t)s * M + m) * P.rec_cap + n;
-    // reference order: the u-edge part (h_u on the v edges) first
-    P.V_rec[o] = (tot[2 * m] + tot[2 * m + 1]) * scale;
-    P.I_rec[o] = (tot[PH + 2 * m + 1] + tot[PH + 2 * m]) * scale;
-  }
+  const int m = threadIdx.x;  // one lane per mode
+  if (m >= M) return;
+  const double scale = P.params[8 * m + 4];
+  const int64_t o = ((int64_t)s * M + m) * P.rec_cap + n;
+  // reference order: the u-edge part (h_u on the v edges) first
+  P.V_rec[o] = (tot[2 * m] + tot[2 * m + 1]) * scale;
+  P.I_rec[o] = (tot[PH + 2 * m + 1] + tot[PH + 2 * m]) * scale;
 }
 
 __global__ void k_step_advance(long long* ctr, long long by) {
