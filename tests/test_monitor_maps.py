@@ -206,3 +206,21 @@ def test_blocks_per_launch_rule():
     for b, n in ((64, 3), (100, 4), (256, 2), (512, 4)):
         with pytest.raises(ValueError):
             blocks_for(b, n)
+
+
+def test_graph_length_covers_bin_update_period():
+    """A captured graph spans whole bin-update periods of every monitor
+    (block x blocks per launch), so the contraction sits at a fixed place."""
+    from magnelio_amd.solver import FITTimeDomainSolver
+
+    g = mm.GridLines(np.linspace(0, 1e-2, 9), np.linspace(0, 1e-2, 9), np.linspace(0, 1e-2, 9))
+    mesh = mm.Mesh.from_grid(g)
+    dt = mm.courant_dt(g, "normal")
+    region = (slice(0, 2), slice(0, 2), slice(0, 2))
+    for per_launch, graph_block, period, steps in ((None, 1, 256, 256), (1, 1, 64, 64),
+                                                   (2, 1, 128, 128), (None, 100, 256, 6400)):
+        mon = MonitorFieldFrequency([1e9], region=region, block=64, blocks_per_launch=per_launch)
+        sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions={}, total_time_steps=10, dt=dt,
+                                  monitors=[mon], graph_block=graph_block)
+        assert mon.period == period
+        assert sol.graph_steps == steps
