@@ -5,10 +5,15 @@ single-large-job workload — 256^3 seeded non-uniform grid with 40 seeded
 dielectric bricks/spheres (eps_r ~ U[1, 10]), 8-layer CPML on all six faces,
 a TF/SF plane wave (+z, x-polarised Gaussian) and a 1001-frequency far-field
 (Huygens box) DFT monitor with compensated float64 bins, every part of the
-step inside the timed region — the HBM-roofline fraction of the dominant
-kernel measured live with HIP events, and the CPU baseline (the oracle's
-C/OpenMP restatement of the reference's Numba curl kernels) timed on the
-host's cores on a bounded sample of the same grid.
+step inside the timed region.  The timed steps run from a seeded mid-run
+state (step 1024 of the job: random fields, full DFT sample buffers), where
+each step carries its 1/256 share of the pipelined DFT contraction.  Beside
+it: the HBM-roofline fraction of the dominant kernel measured live with HIP
+events, the batched K = 6 sweep of the 192 x 96 x 288 dielectric-resonator
+guide against its one-excitation-per-launch path, and the CPU baseline (the
+oracle's C/OpenMP restatement of the reference's Numba curl kernels plus the
+NumPy CPML restatement) timed on the host's cores on a bounded sample of the
+same grid.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -348,13 +353,24 @@ def pmc_probe(args) -> int:
     return 0
 
 
-def cpu_baseline(mesh, coeffs, nsamp=(10, 20, 40)):
-    """Oracle C/OpenMP restatement of the Numba kernels on the same arrays."""
+def _host_threads() -> int:
+    """CPU threads this process may use: OMP_NUM_THREADS when set (the
+    host's share per GPU), else every CPU of its affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(mesh, coeffs, bcs, dt, nsamp=(10, 20, 40), cpml_steps=3):
+    """Oracle C/OpenMP restatement of the Numba kernels on the same arrays,
+    plus the NumPy restatement of the CPML slab updates (the reference's
+    CPML is NumPy array code on the CPU path)."""
     from oracle import build as ob
+    from oracle import cpml as oc
 
     lib = ob.load()
-    threads = len(os.sched_getaffinity(0))
-    threads = min(threads, 16)
+    threads = _host_threads()
     lib.fit_omp_set_threads(threads)
     nx, ny, nz = mesh.dims
     ae = coeffs.alpha_e.astype(np.float32)
@@ -374,7 +390,17 @@ def cpu_baseline(mesh, coeffs, nsamp=(10, 20, 40)):
     # t(N) = setup + N * cost (least squares over the sample points)
     cost = float(np.polyfit(np.array(nsamp, float), np.array(times), 1)[0])
     cost = max(cost, min(t / n for t, n in zip(times, nsamp)) * 0.5)
-    rate = mesh.grid.n_cells / cost / 1e9
+    g = mesh.grid
+    faces = [oc.CPMLFace(f, g.x, g.y, g.z, dt, np.float32, n_layers=bc.thickness_cells)
+             for f, bc in bcs.items()]
+    t0 = time.perf_counter()
+    for _ in range(cpml_steps):
+        for fc in faces:
+            fc.update_e(e, h, be)
+        for fc in faces:
+            fc.update_h(e, h, bh)
+    cpml_cost = (time.perf_counter() - t0) / cpml_steps
+    rate = mesh.grid.n_cells / (cost + cpml_cost) / 1e9
     cpu = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -389,10 +415,123 @@ def cpu_baseline(mesh, coeffs, nsamp=(10, 20, 40)):
         "unit": "Gcell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{nx}x{ny}x{nz} same grid and coefficients, E+H curl kernels only (Numba-order "
-                   "C/OpenMP; CPML, TF/SF and DFT not included), "
-                   f"{'/'.join(map(str, nsamp))} steps, fit t=a+N*c; {threads} threads of {cpu}"),
+        "sample": (f"{nx}x{ny}x{nz} same grid and coefficients: E+H curl kernels (Numba-order "
+                   f"C/OpenMP, {threads} threads, {'/'.join(map(str, nsamp))} steps, fit t=a+N*c: "
+                   f"{cost * 1e3:.1f} ms/step) + NumPy CPML on {len(faces)} faces ({cpml_steps} steps, "
+                   f"{cpml_cost * 1e3:.1f} ms/step, one thread as the reference's array code); "
+                   f"TF/SF and DFT not included; {cpu}"),
+        "curl_ms_per_step": cost * 1e3,
+        "cpml_ms_per_step": cpml_cost * 1e3,
     }
+
+
+def seed_mid_run(sol, n0: int, seed: int = 7) -> None:
+    """Put a set-up solver into a seeded state at step n0 of its run:
+    random fields (zero where beta_E = 0) and DFT sample buffers full of
+    random samples, the step counter at n0.  Steps from there do what the
+    steps of a long run do (a pipelined monitor contracts its previous
+    period's samples from the first of them on)."""
+    import torch
+
+    g = torch.Generator(device=sol.e.device).manual_seed(seed)
+    sol.e.normal_(generator=g).mul_(1e-3).mul_((sol.beta_e != 0).to(sol.e.dtype))
+    for mon in sol.monitors:
+        mon.X.normal_(generator=g).mul_(1e-3)
+    sol.step_ctr.fill_(n0)
+    sol._resume_step = n0
+    torch.cuda.synchronize()
+
+
+MID_RUN_STEP = 1024
+
+
+def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plain: bool = False,
+               single_path: bool = True) -> dict:
+    """The batched K = 6 sweep of the C3 guide (2 ports x 3 modes as six
+    field sets of one solver, graph replays, pipelined 2001-frequency
+    field-plane DFT) and the same six excitations run one per solver, each
+    timed over `steps` steps after `warmup` from the mid-run state."""
+    import torch
+
+    from magnelio_amd.monitors import MonitorFieldFrequency
+    from magnelio_amd.sweep import ScatteringSweep
+
+    mesh, ports, dt = c3_case()
+    nx, ny, nz = mesh.dims
+    f_axis = np.linspace(1e9, C3_FMAX, max(freqs, 1))
+    kz = round(0.3 * nz)
+
+    def monitors():
+        if freqs <= 0:
+            return []
+        return [MonitorFieldFrequency(f_axis, region=(slice(0, nx), slice(0, ny), slice(kz, kz + 1)),
+                                      fields=("E",), block=block, compensated=not plain,
+                                      name="post_plane")]
+
+    n0 = MID_RUN_STEP
+    G_MAX = 64  # the warm-up also replays one graph of at most this many steps
+    total = n0 + warmup + G_MAX + steps
+    # short graphs (a multiple of the DFT slice cadence), so that a short
+    # timed window replays graphs too
+    sweep = ScatteringSweep(mesh, ports, dt=dt, n_steps=total, f_axis=f_axis, f_max=C3_FMAX,
+                            monitors_factory=monitors, graph_block=8)
+    K = len(sweep.excited)
+
+    extra = {"steps": 0}
+
+    def timed(sol):
+        sol.setup()
+        seed_mid_run(sol, n0)
+        sol.run(warmup)
+        if sol.use_graphs:
+            # capture (records only) and one replay (its first launch uploads
+            # the graph) inside the warm-up, outside the timed region
+            G = sol.graph_steps
+            assert G <= G_MAX
+            if sol._graph is None:
+                sol._capture(sol.n_completed, G)
+            sol.run(G)
+            extra["steps"] = G
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol.run(steps)
+        torch.cuda.synchronize()
+        ok = bool(torch.isfinite(sol.e).all())
+        return time.perf_counter() - t0, ok
+
+    sol = sweep.make_solver(sweep.excited)
+    t_b, ok = timed(sol)
+    samples = sol.monitors[0].S if sol.monitors else 0
+    bins_gb = sol.monitors[0].bytes() / 1e9 if sol.monitors else 0.0
+    del sol
+    torch.cuda.empty_cache()
+    t_s = 0.0
+    if single_path:
+        for ch in sweep.excited:
+            s1 = sweep.make_solver([ch])
+            dt_s, ok1 = timed(s1)
+            t_s += dt_s
+            ok = ok and ok1
+            del s1
+            torch.cuda.empty_cache()
+    if not ok:
+        raise RuntimeError("non-finite field after the timed C3 steps")
+    cells = nx * ny * nz
+    rate_b = K * cells * steps / t_b / 1e9
+    roof = HBM_PEAK_GBS * 1e9 / (4.0 * (18 + 12 / K)) / 1e9
+    out = {"workload": (f"{nx}x{ny}x{nz} WR-90 guide, two seeded eps_r=30 posts, PEC iris with a "
+                        f"seeded window, modal DTBC ports on zmin/zmax x 3 modes = K={K} field sets, "
+                        "HIP graphs" + (f", pipelined field-plane DFT {f_axis.size} freqs "
+                                        f"({samples} samples/step per set, block {block})"
+                                        if freqs > 0 else "")),
+           "K": K, "value": rate_b, "unit": "Gcell-updates/s", "ms_per_step": t_b / steps * 1e3,
+           "steps": steps, "warmup": warmup + extra["steps"],
+           "roofline_gcell": roof, "frac_of_roofline": rate_b / roof, "dft_bins_gb": round(bins_gb, 2)}
+    if single_path:
+        rate_s = K * cells * steps / t_s / 1e9
+        out["single_excitation_path"] = {"value": rate_s, "ms_per_step_all_K": t_s / steps * 1e3,
+                                         "speedup_batched": rate_b / rate_s}
+    return out
 
 
 def main(argv=None) -> int:
@@ -405,8 +544,15 @@ def main(argv=None) -> int:
     ap.add_argument("--freqs", type=int, default=1001, help="DFT monitor frequencies (0: none)")
     ap.add_argument("--dft-block", type=int, default=256, help="steps per DFT contraction")
     ap.add_argument("--dft-plain", action="store_true", help="uncompensated float64 bins")
+    ap.add_argument("--dft-slice-steps", type=int, default=None,
+                    help="steps between pipelined contraction slices (default: chosen for occupancy)")
+    ap.add_argument("--dft-fsplit", type=int, default=None,
+                    help="frequency groups of a contraction slice (default: chosen for occupancy)")
+    ap.add_argument("--dft-batched", action="store_true",
+                    help="one contraction of the whole sample buffer per period instead of slices")
     ap.add_argument("--no-tfsf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="leave the batched K=6 sweep out of the line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="a short stencil-only run of the c2 grid plus a 1 GiB device copy, "
                          "for HBM traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
@@ -446,7 +592,8 @@ def main(argv=None) -> int:
     nx, ny, nz = mesh.dims
     dt = mm.courant_dt(mesh.grid, "normal")
     bcs = {f: CPMLBoundary(f, mesh.grid, thickness_cells=8) for f in mm.FACES}
-    total = args.warmup + args.steps
+    n0 = MID_RUN_STEP
+    total = n0 + args.warmup + args.steps
     sources, monitors = [], []
     if not args.no_tfsf:
         sources.append(PlaneWaveSource(direction=(0, 0, 1), polarization=(1, 0, 0),
@@ -456,43 +603,43 @@ def main(argv=None) -> int:
         # Huygens box 10 cells in from every face (8 CPML layers + 2)
         monitors.append(MonitorFarField(np.linspace(1e9, 20e9, args.freqs), margin_cells=2,
                                         block=args.dft_block, name="far_field",
-                                        compensated=not args.dft_plain))
+                                        compensated=not args.dft_plain,
+                                        pipelined=not args.dft_batched, fsplit=args.dft_fsplit,
+                                        slice_steps=args.dft_slice_steps))
     sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, total_time_steps=total, dt=dt,
                               precision="single", nsets=args.k, record=False, sources=sources,
                               monitors=monitors)
     sol.setup()
-    # a seeded initial field so the timed steps do real arithmetic from the start
-    g = torch.Generator(device=sol.e.device).manual_seed(7)
-    sol.e.normal_(generator=g).mul_(1e-3).mul_((sol.beta_e != 0).float())
-    torch.cuda.synchronize()
+    seed_mid_run(sol, n0)
     setup_s = time.perf_counter() - t_setup
 
-    stream = torch.cuda.current_stream()
     for i in range(args.warmup):
-        sol._step(i)
+        sol._step(n0 + i)
     torch.cuda.synchronize()
     sol.timing_events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                          for _ in range(args.steps)]
     sol.timed_events = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        sol._step(args.warmup + i)
+        sol._step(n0 + args.warmup + i)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     e_ms = float(np.mean([a.elapsed_time(b) for a, b in sol.timed_events]))
     if not torch.isfinite(sol.e).all():
         print("non-finite field after the timed steps", file=sys.stderr)
         return 1
-    del stream
 
     cells = nx * ny * nz
     K = args.k
     rate = K * cells * args.steps / wall / 1e9
-    # E phase: 6 coefficient words shared by the K sets + K x (read E, read H, write E),
-    # plus the psi words (read + written) of the CPML slabs the launch carries
+    # E phase, algorithmic bytes: 6 coefficient words shared by the K sets +
+    # K x (read E, read H, write E) per cell; the CPML psi words the launch
+    # reads and writes are counted apart
     fused = sol.cpml is not None and sol.tile_plan is None and sol._dispersion is None
-    bytes_e = 4.0 * cells * (6 + 9 * K) + (sol.cpml.slab_bytes("E", in_curl=True) if fused else 0)
+    bytes_e = 4.0 * cells * (6 + 9 * K)
+    psi_e = sol.cpml.slab_bytes("E", in_curl=True) if fused else 0.0
     achieved = bytes_e / (e_ms * 1e-3) / 1e9
+    mon = monitors[0] if monitors else None
     out = {
         "metric": "Yee Gcell-updates/s & % HBM roofline",
         "value": rate,
@@ -511,13 +658,22 @@ def main(argv=None) -> int:
                          "40 seeded dielectric bricks/spheres eps_r U[1,10], 8-layer CPML on six faces"
                          + ("" if args.no_tfsf else
                             ", TF/SF +z x-polarised Gaussian (f_max 20 GHz) on [0.15L, 0.85L]^3")
-                         + ("" if not monitors else
+                         + ("" if mon is None else
                             f", far-field Huygens-box DFT {args.freqs} freqs 1-20 GHz, 10 cells in, "
-                            f"{monitors[0].S} samples/step, block {args.dft_block}, "
+                            f"{mon.S} samples/step, pipelined contraction over {mon.period}-step "
+                            "periods (each step contracts 1/" f"{mon.period} of the previous period), "
                             + ("plain" if args.dft_plain else "compensated") + " FP64 bins")
-                         + "; all inside the timed steps"),
+                         + f"; timed steps from a seeded mid-run state at step {n0} "
+                         "(random fields, full sample buffers)"),
             "K": K,
-            "dft_bins_gb": round(monitors[0].bytes() / 1e9, 2) if monitors else 0.0,
+            "dft_bins_gb": round(mon.bytes() / 1e9, 2) if mon is not None else 0.0,
+            "dft_slice": ({"steps_between": mon.cadence, "frequency_groups": mon.fsplit,
+                           "pipelined": mon.pipelined} if mon is not None else None),
+            "dft_contraction_launches_timed": (sum(1 for i in range(args.steps)
+                                                   if (n0 + args.warmup + i - mon.n_start) % mon.cadence
+                                                   == mon.cadence - 1) if mon is not None else 0),
+            "dft_contractions_timed": (args.steps / mon.period if mon is not None and mon.pipelined
+                                       else None),
             "setup_s": round(setup_s, 2),
         },
         "roofline": {
@@ -527,109 +683,58 @@ def main(argv=None) -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": K_UPDATE_E_TRAFFIC.get((args.n, K, fused)),
+            "traffic_source": ("stored: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this launch "
+                               "(bench.py --pmc-probe), profiles/k_update_traffic_c2_256cube_k1.md"),
             "kernel": "k_update_e",
             "kernel_ms": e_ms,
             "alg_bytes_per_launch": bytes_e,
+            "cpml_psi_bytes_per_launch": psi_e,
         },
     }
+    coeffs = sol._coeffs  # float64 host alpha/beta (host-built: the mesh has a TF/SF source)
+    del sol, monitors, mon
+    torch.cuda.empty_cache()
+    if not args.no_c3:
+        out["c3_batched"] = c3_batched(args.warmup, args.steps)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mesh, sol._coeffs)
+        if coeffs is None:
+            from magnelio_amd import mesh as _m
+
+            coeffs = _m.update_coefficients(_m.build_m_eps(mesh), _m.build_m_mu(mesh),
+                                            _m.build_m_sigma(mesh), _m.build_m_sigma_m(mesh), dt,
+                                            _m.pec_flat(mesh.pec_mask_edges, mesh.dims))
+        out["cpu_baseline"] = cpu_baseline(mesh, coeffs, bcs, dt)
     print(json.dumps(out))
     return 0
-
-
-def _timed_sweep_run(sol, warmup: int, steps: int) -> float:
-    """Warm up (eager steps + graph capture), then time `steps` steps."""
-    import torch
-
-    sol.setup()
-    sol.run(warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sol.run(steps)
-    torch.cuda.synchronize()
-    return time.perf_counter() - t0
 
 
 def main_c3(args) -> int:
     """Batched multi-excitation sweep: K = 6 (port, mode) excitations as six
     field sets of one solver against the same six run one per solver."""
-    import torch
-
-    from magnelio_amd.monitors import MonitorFieldFrequency, blocks_for
-    from magnelio_amd.sweep import ScatteringSweep
-
     t_setup = time.perf_counter()
-    mesh, ports, dt = c3_case()
-    nx, ny, nz = mesh.dims
-    freqs = np.linspace(1e9, C3_FMAX, 2001 if args.freqs == 1001 else max(args.freqs, 1))
+    freqs = 2001 if args.freqs == 1001 else args.freqs
     block = 64 if args.dft_block == 256 else args.dft_block
-    # the warm-up captures the step graph (one bin-update period of the
-    # monitor, from the first aligned step after step 0) and replays it once
-    warmup = max(args.warmup, 2 * block * blocks_for(block)) if args.freqs > 0 else args.warmup
-    kz = round(0.3 * nz)
-
-    def monitors():
-        if args.freqs <= 0:
-            return []
-        return [MonitorFieldFrequency(freqs, region=(slice(0, nx), slice(0, ny), slice(kz, kz + 1)),
-                                      fields=("E",), block=block, compensated=not args.dft_plain,
-                                      name="post_plane")]
-
-    total = warmup + args.steps
-    sweep = ScatteringSweep(mesh, ports, dt=dt, n_steps=total, f_axis=freqs, f_max=C3_FMAX,
-                            monitors_factory=monitors)
-    K = len(sweep.excited)
-    setup_s = time.perf_counter() - t_setup
-    sol = sweep.make_solver(sweep.excited)
-    t_batched = _timed_sweep_run(sol, warmup, args.steps)
-    samples = sol.monitors[0].S if sol.monitors else 0
-    bins_gb = sol.monitors[0].bytes() / 1e9 if sol.monitors else 0.0
-    ok = bool(torch.isfinite(sol.e).all())
-    del sol
-    torch.cuda.empty_cache()
-    t_single = 0.0
-    for ch in sweep.excited:
-        s1 = sweep.make_solver([ch])
-        t_single += _timed_sweep_run(s1, warmup, args.steps)
-        ok = ok and bool(torch.isfinite(s1.e).all())
-        del s1
-        torch.cuda.empty_cache()
-    if not ok:
-        print("non-finite field after the timed steps", file=sys.stderr)
-        return 1
-    cells = nx * ny * nz
-    rate_b = K * cells * args.steps / t_batched / 1e9
-    rate_s = K * cells * args.steps / t_single / 1e9
+    r = c3_batched(args.warmup, args.steps, freqs=freqs, block=block, plain=args.dft_plain)
     out = {
         "metric": "batched sweep Gcell-updates/s (K excitations per launch) vs one excitation per launch",
-        "value": rate_b,
+        "value": r["value"],
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": warmup,
-        "ms_per_step": t_batched / args.steps * 1e3,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": (f"batched multi-excitation sweep {nx}x{ny}x{nz}: WR-90 guide, two seeded "
-                         "eps_r=30 posts, PEC iris with a seeded window, modal DTBC ports on zmin/zmax "
-                         f"x 3 modes = K={K} field sets, HIP graphs"
-                         + (f", field-plane DFT {freqs.size} freqs ({samples} samples/step per set, "
-                            f"block {block}, " + ("plain" if args.dft_plain else "compensated")
-                            + " FP64 bins)" if args.freqs > 0 else "")),
-            "K": K,
-            "dft_bins_gb": round(bins_gb, 2),
-            "setup_s": round(setup_s, 2),
-        },
-        "single_excitation_path": {
-            "value": rate_s,
-            "ms_per_step_all_K": t_single / args.steps * 1e3,
-            "speedup_batched": rate_b / rate_s,
-        },
+        "config": {"workload": "batched multi-excitation sweep " + r["workload"]
+                   + f"; timed steps from a seeded mid-run state at step {MID_RUN_STEP}",
+                   "K": r["K"], "dft_bins_gb": r["dft_bins_gb"],
+                   "wall_s": round(time.perf_counter() - t_setup, 1)},
+        "roofline_gcell": r["roofline_gcell"],
+        "frac_of_roofline": r["frac_of_roofline"],
+        "single_excitation_path": r["single_excitation_path"],
     }
     print(json.dumps(out))
     return 0

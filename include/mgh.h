@@ -265,13 +265,14 @@ int mgh_ade_update(int dtype, int nsets, void* e, long long e_set_stride, int n,
  * MonitorFarField (far_field.py:230-248); E stamped t = n dt, H t + dt/2.
  *
  * mgh_dft_sample: row ((n - n_start) mod B) of X (nsets x B x S_pad float32,
- * B <= 512)
- * receives, for each of S samples, sum_t w[s,t] * field[idx[s,t]] over 8
+ * B <= 512) receives — with x_buf_stride > 0 X holds two such buffers that
+ * many floats apart and the row goes to buffer ((n - n_start) div B) mod 2 — for each of S samples, sum_t w[s,t] * field[idx[s,t]] over 8
  * terms; idx >= 0 indexes E, idx < 0 indexes H at -(idx + 1) (device
  * layout); weight 0 marks an unused term. n is read from *step_ctr. */
 int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, const void* h,
                    long long h_set_stride, const int* idx, const float* w, int S, float* X,
-                   int S_pad, int B, const long long* step_ctr, long long n_start, void* stream);
+                   int S_pad, int B, long long x_buf_stride, const long long* step_ctr,
+                   long long n_start, void* stream);
 
 /* mgh_dft_sample_groups: the same row of X from structured groups — the
  * samples of one monitor component on one face or box, whose terms are affine
@@ -288,7 +289,8 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
 int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_stride,
                           const void* h, long long h_set_stride, const int* grp, int G,
                           int n_blocks, const float* wtab, int S, float* X, int S_pad, int B,
-                          const long long* step_ctr, long long n_start, void* stream);
+                          long long x_buf_stride, const long long* step_ctr, long long n_start,
+                          void* stream);
 
 /* mgh_dft_block: bins[set, f, s] += exp(j w_f t0) * sum_{b < nb_valid}
  * (Pc + j Ps)[f, b] * X[set, b, s], t0 = n0 dt for s < s_h_start and
@@ -315,6 +317,41 @@ int mgh_dft_blocks(int nsets, const float* X, int S, int S_pad, int B, int nsub,
                    const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
                    double dt, double* blk, int s_h_start, double* bins, float* lo,
                    const long long* step_ctr, long long n_start, void* stream);
+
+/* mgh_dft_blocks_range: mgh_dft_blocks over the sample waves [wave_lo,
+ * wave_hi) (16 samples each, W = ceil(S / 16) in all) with the first step
+ * of block 0 given as n0 instead of read from a counter, frequencies split
+ * over fsplit groups of work-groups.  Flushes what a pipelined monitor has
+ * not contracted when a run ends (the same accumulate steps). */
+int mgh_dft_blocks_range(int nsets, const float* X, int S, int S_pad, int B, int nsub,
+                         int nb_valid, int wave_lo, int wave_hi, long long n0, const float* Pc,
+                         const float* Ps, const double* omega, int F, int F_pad, double dt,
+                         double* blk, int s_h_start, double* bins, float* lo, int fsplit,
+                         void* stream);
+
+/* mgh_dft_phase_table: the block phases exp(j w t0) (E and H stamps) of
+ * n_periods consecutive periods of nsub B steps from step n0, nsub x F x 4
+ * float64 per period — the table mgh_dft_slice reads. */
+int mgh_dft_phase_table(const double* omega, int F, double dt, long long n0, int B, int nsub,
+                        int n_periods, double* blk, void* stream);
+
+/* mgh_dft_slice: one slice of a pipelined contraction.  X holds two sample
+ * buffers (x_buf_stride floats apart, each nsets x P x S_pad, P = nsub B);
+ * period p (P steps from n_start) is sampled into buffer p mod 2.  Called at
+ * the steps n (*step_ctr) with ((n - n_start) mod P + 1) mod cad = 0 (cad
+ * divides P), it contracts slice j = ((n - n_start) mod P) div cad of the
+ * P / cad slices of period p - 1 — waves [j W / (P/cad), (j + 1) W /
+ * (P/cad)), W = ceil(S / 16) — over all P rows of buffer (p - 1) mod 2 with
+ * that period's phases (phase_table from mgh_dft_phase_table from n_start,
+ * n_periods periods).  The P steps of a period contract every wave of the
+ * previous one once, so K steps carry K / P of a period's contraction
+ * work.  Nothing is contracted in period 0.  Replaces the same
+ * DFTAccumulator.accumulate steps (monitors/_dft.py:37-52). */
+int mgh_dft_slice(int nsets, const float* X, long long x_buf_stride, int S, int S_pad, int B,
+                  int nsub, const float* Pc, const float* Ps, int F, int F_pad,
+                  const double* phase_table, int n_periods, int s_h_start, double* bins, float* lo,
+                  const long long* step_ctr, long long n_start, int cad, int fsplit,
+                  void* stream);
 
 #ifdef __cplusplus
 }

@@ -130,12 +130,12 @@ SIGNATURES: dict[str, tuple] = {
     "mgh_dft_sample": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int,
-         _c_void_p, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p],
+         _c_void_p, _c_int, _c_int, _c_ll, _c_void_p, _c_ll, _c_void_p],
     ),
     "mgh_dft_sample_groups": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p, _c_int, _c_int, _c_void_p,
-         _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p],
+         _c_int, _c_void_p, _c_int, _c_int, _c_ll, _c_void_p, _c_ll, _c_void_p],
     ),
     "mgh_dft_block": (
         _c_int,
@@ -148,6 +148,22 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p,
          _c_void_p, _c_int, _c_int, _c_double, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p,
          _c_ll, _c_void_p],
+    ),
+    "mgh_dft_blocks_range": (
+        _c_int,
+        [_c_int, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_ll,
+         _c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_double, _c_void_p, _c_int, _c_void_p,
+         _c_void_p, _c_int, _c_void_p],
+    ),
+    "mgh_dft_phase_table": (
+        _c_int,
+        [_c_void_p, _c_int, _c_double, _c_ll, _c_int, _c_int, _c_int, _c_void_p, _c_void_p],
+    ),
+    "mgh_dft_slice": (
+        _c_int,
+        [_c_int, _c_void_p, _c_ll, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_int,
+         _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_ll, _c_int, _c_int,
+         _c_void_p],
     ),
 }
 

@@ -41,13 +41,15 @@ __global__ __launch_bounds__(256) void k_dft_sample(const T* __restrict__ e, int
                                                     const int* __restrict__ idx,
                                                     const float* __restrict__ w, int S,
                                                     float* __restrict__ X, int S_pad, int B,
+                                                    long long x_buf,
                                                     const long long* __restrict__ ctr,
                                                     long long n_start) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int set = blockIdx.y;
   if (s >= S) return;
-  const long long n = *ctr;
-  const int b = (int)((n - n_start) % B);
+  const long long rel = *ctr - n_start;
+  const int b = (int)(rel % B);
+  X += ((rel / B) & 1) * x_buf;  // the period's buffer (x_buf 0: one buffer)
   const T* es = e + (int64_t)set * e_set;
   const T* hs = h + (int64_t)set * h_set;
   float acc = 0.0f;
@@ -85,7 +87,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_dft_sample_groups(
     const T* __restrict__ e, int64_t e_set, const T* __restrict__ h, int64_t h_set,
     const int* __restrict__ grp, int G, const float* __restrict__ wtab, float* __restrict__ X,
-    int S_pad, int B, const long long* __restrict__ ctr, long long n_start) {
+    int S_pad, int B, long long x_buf, const long long* __restrict__ ctr, long long n_start) {
   const int blk = blockIdx.x;
   const int set = blockIdx.y;
   // group of this work-group: the last whose first block is <= blk (uniform)
@@ -113,8 +115,9 @@ __global__ __launch_bounds__(256) void k_dft_sample_groups(
     c1 = tc1 * 16 + (int)(threadIdx.x >> 4);
     if (c0 >= n0 || c1 >= n1 || c2 >= n2) return;
   }
-  const long long n = *ctr;
-  const int b = (int)((n - n_start) % B);
+  const long long rel = *ctr - n_start;
+  const int b = (int)(rel % B);
+  X += ((rel / B) & 1) * x_buf;  // the period's buffer (x_buf 0: one buffer)
   const int64_t off = (int64_t)c0 * d[6] + (int64_t)c1 * d[7] + (int64_t)c2 * d[8];
   const T* f = d[4] ? h + (int64_t)set * h_set : e + (int64_t)set * e_set;
   const int nt = d[5];
@@ -145,19 +148,29 @@ __global__ __launch_bounds__(256) void k_dft_sample_groups(
 
 // Block phase factors exp(j w t0) for E and H stamps of each of the nsub
 // consecutive B-step blocks a contraction covers: blk[(q*F + f)*4 + 0..3] =
-// cos(w t0E), sin(w t0E), cos(w t0H), sin(w t0H) of block q.
+// cos(w t0E), sin(w t0E), cos(w t0H), sin(w t0H) of block q.  The first step
+// of block 0 is n0_fixed + z P for work-group z (P = nsub B: a table of
+// consecutive periods) when n0_fixed >= 0, else the first step of the period
+// holding *ctr.
 __global__ void k_dft_phase(const double* __restrict__ omega, int F, double dt,
                             const long long* __restrict__ ctr, long long n_start, int B,
-                            int nsub, double* __restrict__ blk) {
+                            int nsub, long long n0_fixed, double* __restrict__ blk) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (f >= F) return;
-  const long long n = *ctr;
-  const long long n0 = n - ((n - n_start) % ((long long)B * nsub)) + (long long)q * B;
+  const long long P = (long long)B * nsub;
+  long long base;
+  if (n0_fixed >= 0) {
+    base = n0_fixed + (long long)blockIdx.z * P;
+  } else {
+    const long long n = *ctr;
+    base = n - ((n - n_start) % P);
+  }
+  const long long n0 = base + (long long)q * B;
   const double te = (double)n0 * dt;
   const double th = te + 0.5 * dt;
   double s, c;
-  double* o = blk + ((int64_t)q * F + f) * 4;
+  double* o = blk + (((int64_t)blockIdx.z * nsub + q) * F + f) * 4;
   sincos(omega[f] * te, &s, &c);
   o[0] = c;
   o[1] = s;
@@ -165,6 +178,30 @@ __global__ void k_dft_phase(const double* __restrict__ omega, int F, double dt,
   o[2] = c;
   o[3] = s;
 }
+
+// Which samples and frequencies a contraction launch covers.  Samples go in
+// waves of 16; work-group x of the launch takes waves wave_lo + 4x .. +3 and
+// work-group z the frequency tiles (16 frequencies each) z*tiles_per ..
+// (z+1)*tiles_per - 1.  With ctr set the launch is one slice of a
+// pipelined contraction: the period p holding *ctr (P = nsub*B steps from
+// n_start) is being sampled into buffer p mod 2 while row r = (*ctr -
+// n_start) mod P of it is written; the launch, made at the steps with
+// (r + 1) mod cad = 0, contracts slice j = r div cad of the P / cad slices
+// of the finished period p - 1 — waves [j W / (P/cad), (j + 1) W / (P/cad))
+// (W = all waves), all P rows, buffer (p - 1) mod 2 — with that period's
+// phases from the table blk (one nsub x F x 4 entry per period from
+// n_start).  Period 0 has nothing to contract.  The P steps of period p
+// contract every wave of period p - 1 once, before its buffer is
+// overwritten by period p + 1.
+struct DftRange {
+  const long long* ctr;  // pipelined slice: the step counter; NULL: fixed range
+  long long n_start;
+  long long x_buf;       // floats per sample buffer (pipelined)
+  int wave_lo, wave_hi;  // fixed range of sample waves
+  int tiles_per;         // frequency tiles per work-group z
+  int cad;               // pipelined: steps between slices (divides P)
+  int n_periods;         // pipelined: periods in the phase table
+};
 
 // TwoSum of the float64 bin and the block's contribution; the rounding error
 // (at most half an ulp of the bin) joins the float32 compensation word.
@@ -204,7 +241,7 @@ __global__ __launch_bounds__(256, (NSUB > 1 ? 3 : (NB <= 64 ? 4 : 1))) void k_df
     const float* __restrict__ X, int S, int S_pad, int B, int nb_valid,
     const float* __restrict__ Pc, const float* __restrict__ Ps, int F,
     const double* __restrict__ blk, int s_h_start, double* __restrict__ bins,
-    float* __restrict__ lo) {
+    float* __restrict__ lo, DftRange rg) {
   constexpr int NQ = NB / 4;                // float4 groups per lane
   constexpr int NQP = NQ < 16 ? NQ : 16;    // groups per LDS pass (256 steps)
   constexpr int NP = NQ / NQP;              // passes
@@ -212,11 +249,30 @@ __global__ __launch_bounds__(256, (NSUB > 1 ? 3 : (NB <= 64 ? 4 : 1))) void k_df
   static_assert(NSUB == 1 || NP == 1, "several blocks per launch need a one-pass tile");
   __shared__ float4 sc[NQP * 64];
   __shared__ float4 ss[NQP * 64];
+  int wlo = rg.wave_lo, whi = rg.wave_hi;
+  if (rg.ctr) {  // pipelined slice (uniform over the launch)
+    const long long P = (long long)B * NSUB;
+    const long long rel = *rg.ctr - rg.n_start;
+    const long long p = rel / P;
+    // no finished period yet; a period past the phase table (never for a
+    // run within its horizon) is left alone rather than read out of bounds
+    if (p <= 0 || p > rg.n_periods) return;
+    const long long ns = P / rg.cad;
+    const long long j = (rel % P) / rg.cad;
+    const long long W = (S + 15) / 16;
+    wlo = (int)(j * W / ns);
+    whi = (int)((j + 1) * W / ns);
+    X += ((p - 1) & 1) * rg.x_buf;
+    blk += (p - 1) * NSUB * F * 4;
+    nb_valid = (int)P;
+  }
+  // a work-group wholly past the range leaves before any barrier
+  if (wlo + (int)blockIdx.x * 4 >= whi) return;
   const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = wlo + (int)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int set = blockIdx.y;
   const int s0 = wave * 16;
-  const bool active = s0 < S;  // idle waves still take part in the tile copies
+  const bool active = wave < whi && s0 < S;  // idle waves still take part in the tile copies
   const int col = lane & 15;   // sample within the tile (B operand column / D column)
   const int krow = lane >> 4;  // 0..3: k within an MFMA step (A/B), row group of D
   const float* Xs = X + (int64_t)set * B * NSUB * S_pad;
@@ -239,7 +295,9 @@ __global__ __launch_bounds__(256, (NSUB > 1 ? 3 : (NB <= 64 ? 4 : 1))) void k_df
   const bool own = active && s < S;
   double2* bset = (double2*)(bins + (int64_t)set * F * S * 2);
   float2* lset = COMP ? (float2*)(lo + (int64_t)set * F * S * 2) : nullptr;
-  for (int f0 = 0; f0 < F; f0 += 16) {
+  const int f_begin = (int)blockIdx.z * rg.tiles_per * 16;
+  const int f_end = min(F, f_begin + rg.tiles_per * 16);
+  for (int f0 = f_begin; f0 < f_end; f0 += 16) {
     double2 bv[4];
     float2 lv[4];
 #pragma unroll
@@ -320,6 +378,51 @@ __global__ __launch_bounds__(256, (NSUB > 1 ? 3 : (NB <= 64 ? 4 : 1))) void k_df
   }
 }
 
+// Launch the contraction over a range (rg) with grid (gx, nsets, gz); the
+// phases are in blk.  Variant by padded block length and blocks per launch.
+static int launch_blocks(dim3 grd, hipStream_t st, const float* X, int S, int S_pad, int B,
+                         int nsub, int nb_valid, const float* Pc, const float* Ps, int F,
+                         const double* blk, int s_h_start, double* bins, float* lo,
+                         const DftRange& rg) {
+  const int bp = dft_bpad(B);
+#define MGH_DFT_LAUNCH(NB, NSUB)                                                              \
+  do {                                                                                        \
+    if (lo)                                                                                   \
+      hipLaunchKernelGGL((k_dft_block<NB, true, NSUB>), grd, dim3(256), 0, st, X, S, S_pad,  \
+                         B, nb_valid, Pc, Ps, F, blk, s_h_start, bins, lo, rg);               \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_dft_block<NB, false, NSUB>), grd, dim3(256), 0, st, X, S, S_pad, \
+                         B, nb_valid, Pc, Ps, F, blk, s_h_start, bins, lo, rg);               \
+  } while (0)
+  if (nsub == 4) MGH_DFT_LAUNCH(16, 4);
+  else if (nsub == 2 && bp == 64) MGH_DFT_LAUNCH(16, 2);
+  else if (nsub == 2) MGH_DFT_LAUNCH(32, 2);
+  else if (bp == 64) MGH_DFT_LAUNCH(16, 1);
+  else if (bp == 128) MGH_DFT_LAUNCH(32, 1);
+  else if (bp == 192) MGH_DFT_LAUNCH(48, 1);
+  else if (bp == 256) MGH_DFT_LAUNCH(64, 1);
+  else MGH_DFT_LAUNCH(128, 1);
+#undef MGH_DFT_LAUNCH
+  MGH_CHECK_LAUNCH("k_dft_block");
+  return MGH_OK;
+}
+
+// Checks shared by the contraction entry points.
+#define MGH_DFT_CHECK_CONTRACTION()                                                           \
+  do {                                                                                        \
+    MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts");               \
+    MGH_REQUIRE(F >= 1 && F_pad >= F && F_pad % 16 == 0, "bad frequency counts F=%d F_pad=%d", \
+                F, F_pad);                                                                    \
+    MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "bad block length %d", B);                           \
+    MGH_REQUIRE(nsub == 1 || nsub == 2 || nsub == 4,                                          \
+                "blocks per launch must be 1, 2 or 4, got %d", nsub);                         \
+    MGH_REQUIRE(nsub == 1 || dft_bpad(B) * nsub <= 256,                                       \
+                "%d blocks of %d steps exceed 256 sample rows per lane", nsub, B);            \
+    MGH_REQUIRE(X && Pc && Ps && bins, "null pointer");                                        \
+    MGH_REQUIRE(s_h_start % 16 == 0, "s_h_start must be a multiple of 16");                   \
+    MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");                                   \
+  } while (0)
+
 }  // namespace
 
 extern "C" {
@@ -330,20 +433,25 @@ extern "C" {
 // marks an unused term.
 int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, const void* h,
                    long long h_set_stride, const int* idx, const float* w, int S, float* X,
-                   int S_pad, int B, const long long* step_ctr, long long n_start, void* stream) {
+                   int S_pad, int B, long long x_buf_stride, const long long* step_ctr,
+                   long long n_start, void* stream) {
   MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8");
   MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts S=%d S_pad=%d", S, S_pad);
   MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..512, got %d", B);
   MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
   MGH_REQUIRE(e && h && idx && w && X && step_ctr, "null pointer");
+  MGH_REQUIRE(x_buf_stride == 0 || x_buf_stride >= (long long)nsets * B * S_pad,
+              "sample buffers overlap (stride %lld)", x_buf_stride);
   dim3 grd((S + 255) / 256, nsets, 1);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == 4)
     hipLaunchKernelGGL(k_dft_sample<float>, grd, dim3(256), 0, st, (const float*)e, e_set_stride,
-                       (const float*)h, h_set_stride, idx, w, S, X, S_pad, B, step_ctr, n_start);
+                       (const float*)h, h_set_stride, idx, w, S, X, S_pad, B, x_buf_stride, step_ctr,
+                       n_start);
   else
     hipLaunchKernelGGL(k_dft_sample<double>, grd, dim3(256), 0, st, (const double*)e, e_set_stride,
-                       (const double*)h, h_set_stride, idx, w, S, X, S_pad, B, step_ctr, n_start);
+                       (const double*)h, h_set_stride, idx, w, S, X, S_pad, B, x_buf_stride, step_ctr,
+                       n_start);
   MGH_CHECK_LAUNCH("k_dft_sample");
   return MGH_OK;
 }
@@ -361,7 +469,8 @@ int mgh_dft_sample(int dtype, int nsets, const void* e, long long e_set_stride, 
 int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_stride,
                           const void* h, long long h_set_stride, const int* grp, int G,
                           int n_blocks, const float* wtab, int S, float* X, int S_pad, int B,
-                          const long long* step_ctr, long long n_start, void* stream) {
+                          long long x_buf_stride, const long long* step_ctr, long long n_start,
+                          void* stream) {
   MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8");
   MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts S=%d S_pad=%d", S, S_pad);
   MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "block length must be 1..512, got %d", B);
@@ -369,16 +478,18 @@ int mgh_dft_sample_groups(int dtype, int nsets, const void* e, long long e_set_s
   MGH_REQUIRE(G >= 1 && G <= 4096, "bad group count %d", G);
   MGH_REQUIRE(n_blocks >= 1, "bad work-group count %d", n_blocks);
   MGH_REQUIRE(e && h && grp && wtab && X && step_ctr, "null pointer");
+  MGH_REQUIRE(x_buf_stride == 0 || x_buf_stride >= (long long)nsets * B * S_pad,
+              "sample buffers overlap (stride %lld)", x_buf_stride);
   dim3 grd((unsigned)n_blocks, nsets, 1);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == 4)
     hipLaunchKernelGGL(k_dft_sample_groups<float>, grd, dim3(256), 0, st, (const float*)e,
                        e_set_stride, (const float*)h, h_set_stride, grp, G, wtab, X, S_pad, B,
-                       step_ctr, n_start);
+                       x_buf_stride, step_ctr, n_start);
   else
     hipLaunchKernelGGL(k_dft_sample_groups<double>, grd, dim3(256), 0, st, (const double*)e,
                        e_set_stride, (const double*)h, h_set_stride, grp, G, wtab, X, S_pad, B,
-                       step_ctr, n_start);
+                       x_buf_stride, step_ctr, n_start);
   MGH_CHECK_LAUNCH("k_dft_sample_groups");
   return MGH_OK;
 }
@@ -400,46 +511,92 @@ int mgh_dft_blocks(int nsets, const float* X, int S, int S_pad, int B, int nsub,
                    const float* Pc, const float* Ps, const double* omega, int F, int F_pad,
                    double dt, double* blk, int s_h_start, double* bins, float* lo,
                    const long long* step_ctr, long long n_start, void* stream) {
-  MGH_REQUIRE(S >= 1 && S_pad >= S && S_pad % 16 == 0, "bad sample counts");
-  MGH_REQUIRE(F >= 1 && F_pad >= F && F_pad % 16 == 0, "bad frequency counts F=%d F_pad=%d", F,
-              F_pad);
-  MGH_REQUIRE(B >= 1 && B <= DFT_BMAX, "bad block length %d", B);
-  MGH_REQUIRE(nsub == 1 || nsub == 2 || nsub == 4, "blocks per launch must be 1, 2 or 4, got %d",
-              nsub);
-  MGH_REQUIRE(nsub == 1 || dft_bpad(B) * nsub <= 256,
-              "%d blocks of %d steps exceed 256 sample rows per lane", nsub, B);
+  MGH_DFT_CHECK_CONTRACTION();
+  MGH_REQUIRE(step_ctr && omega && blk, "null pointer");
   MGH_REQUIRE(nb_valid >= 0 && nb_valid <= B * nsub, "bad valid row count %d", nb_valid);
-  MGH_REQUIRE(X && Pc && Ps && omega && blk && bins && step_ctr, "null pointer");
-  MGH_REQUIRE(s_h_start % 16 == 0, "s_h_start must be a multiple of 16");
-  MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "bad nsets");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_dft_phase, dim3((F + 255) / 256, nsub), dim3(256), 0, st, omega, F, dt,
-                     step_ctr, n_start, B, nsub, blk);
+                     step_ctr, n_start, B, nsub, -1LL, blk);
   MGH_CHECK_LAUNCH("k_dft_phase");
   const int waves = (S + 15) / 16;
-  dim3 grd((waves + 3) / 4, nsets, 1);
-  const int bp = dft_bpad(B);
-#define MGH_DFT_LAUNCH(NB, NSUB)                                                             \
-  do {                                                                                       \
-    if (lo)                                                                                  \
-      hipLaunchKernelGGL((k_dft_block<NB, true, NSUB>), grd, dim3(256), 0, st, X, S, S_pad, \
-                         B, nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, lo);   \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_dft_block<NB, false, NSUB>), grd, dim3(256), 0, st, X, S,       \
-                         S_pad, B, nb_valid, Pc, Ps, F, (const double*)blk, s_h_start, bins, \
-                         lo);                                                                \
-  } while (0)
-  if (nsub == 4) MGH_DFT_LAUNCH(16, 4);
-  else if (nsub == 2 && bp == 64) MGH_DFT_LAUNCH(16, 2);
-  else if (nsub == 2) MGH_DFT_LAUNCH(32, 2);
-  else if (bp == 64) MGH_DFT_LAUNCH(16, 1);
-  else if (bp == 128) MGH_DFT_LAUNCH(32, 1);
-  else if (bp == 192) MGH_DFT_LAUNCH(48, 1);
-  else if (bp == 256) MGH_DFT_LAUNCH(64, 1);
-  else MGH_DFT_LAUNCH(128, 1);
-#undef MGH_DFT_LAUNCH
-  MGH_CHECK_LAUNCH("k_dft_block");
+  DftRange rg{nullptr, 0, 0, 0, waves, (F + 15) / 16, 1, 0};
+  return launch_blocks(dim3((waves + 3) / 4, nsets, 1), st, X, S, S_pad, B, nsub, nb_valid, Pc,
+                       Ps, F, blk, s_h_start, bins, lo, rg);
+}
+
+// mgh_dft_blocks over the sample waves [wave_lo, wave_hi) (16 samples each)
+// with the first step of block 0 given (n0), frequencies split over fsplit
+// groups of work-groups.
+int mgh_dft_blocks_range(int nsets, const float* X, int S, int S_pad, int B, int nsub,
+                         int nb_valid, int wave_lo, int wave_hi, long long n0, const float* Pc,
+                         const float* Ps, const double* omega, int F, int F_pad, double dt,
+                         double* blk, int s_h_start, double* bins, float* lo, int fsplit,
+                         void* stream) {
+  MGH_DFT_CHECK_CONTRACTION();
+  MGH_REQUIRE(omega && blk, "null pointer");
+  MGH_REQUIRE(nb_valid >= 0 && nb_valid <= B * nsub, "bad valid row count %d", nb_valid);
+  const int waves = (S + 15) / 16;
+  MGH_REQUIRE(wave_lo >= 0 && wave_lo <= wave_hi && wave_hi <= waves,
+              "bad wave range [%d, %d) of %d", wave_lo, wave_hi, waves);
+  MGH_REQUIRE(n0 >= 0, "bad first step %lld", n0);
+  MGH_REQUIRE(fsplit >= 1, "bad frequency split %d", fsplit);
+  if (wave_hi == wave_lo || nb_valid == 0) return MGH_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dft_phase, dim3((F + 255) / 256, nsub), dim3(256), 0, st, omega, F, dt,
+                     (const long long*)nullptr, 0LL, B, nsub, n0, blk);
+  MGH_CHECK_LAUNCH("k_dft_phase");
+  const int tiles = (F + 15) / 16;
+  const int per = (tiles + fsplit - 1) / fsplit;
+  DftRange rg{nullptr, 0, 0, wave_lo, wave_hi, per, 1, 0};
+  return launch_blocks(dim3((wave_hi - wave_lo + 3) / 4, nsets, (tiles + per - 1) / per), st, X,
+                       S, S_pad, B, nsub, nb_valid, Pc, Ps, F, blk, s_h_start, bins, lo, rg);
+}
+
+// Phase table of n_periods consecutive periods (nsub B steps each) from
+// step n0: blk[((z nsub + q) F + f) 4 + 0..3] for period z.
+int mgh_dft_phase_table(const double* omega, int F, double dt, long long n0, int B, int nsub,
+                        int n_periods, double* blk, void* stream) {
+  MGH_REQUIRE(omega && blk, "null pointer");
+  MGH_REQUIRE(F >= 1 && B >= 1 && B <= DFT_BMAX && nsub >= 1 && nsub <= 4, "bad table shape");
+  MGH_REQUIRE(n0 >= 0 && n_periods >= 1 && n_periods <= 65535, "bad periods n0=%lld n=%d", n0,
+              n_periods);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dft_phase, dim3((F + 255) / 256, nsub, n_periods), dim3(256), 0, st,
+                     omega, F, dt, (const long long*)nullptr, 0LL, B, nsub, n0, blk);
+  MGH_CHECK_LAUNCH("k_dft_phase");
   return MGH_OK;
+}
+
+// One pipelined slice (DftRange): X holds two sample buffers x_buf_stride
+// floats apart, each nsets x nsub*B x S_pad; the period holding *step_ctr is
+// sampled into one while this launch contracts slice ((n - n_start) mod P)
+// div cad of the P / cad slices of the other — the previous period — with
+// its phases from the table (mgh_dft_phase_table from n_start, n_periods
+// periods).  Made at the steps with ((n - n_start) mod P + 1) mod cad = 0.
+int mgh_dft_slice(int nsets, const float* X, long long x_buf_stride, int S, int S_pad, int B,
+                  int nsub, const float* Pc, const float* Ps, int F, int F_pad,
+                  const double* phase_table, int n_periods, int s_h_start, double* bins, float* lo,
+                  const long long* step_ctr, long long n_start, int cad, int fsplit,
+                  void* stream) {
+  MGH_DFT_CHECK_CONTRACTION();
+  MGH_REQUIRE(step_ctr && phase_table, "null pointer");
+  MGH_REQUIRE(x_buf_stride >= (long long)nsets * B * nsub * S_pad,
+              "sample buffers overlap (stride %lld)", x_buf_stride);
+  MGH_REQUIRE(fsplit >= 1, "bad frequency split %d", fsplit);
+  MGH_REQUIRE(n_periods >= 1, "bad phase table length %d", n_periods);
+  const long long P = (long long)B * nsub;
+  MGH_REQUIRE(cad >= 1 && P % cad == 0, "slice cadence %d does not divide the period %lld", cad,
+              P);
+  hipStream_t st = (hipStream_t)stream;
+  const long long waves = (S + 15) / 16;
+  const long long ns = P / cad;
+  const int slice = (int)((waves + ns - 1) / ns);  // the largest slice
+  if (slice == 0) return MGH_OK;
+  const int tiles = (F + 15) / 16;
+  const int per = (tiles + fsplit - 1) / fsplit;
+  DftRange rg{step_ctr, n_start, x_buf_stride, 0, 0, per, cad, n_periods};
+  return launch_blocks(dim3((slice + 3) / 4, nsets, (tiles + per - 1) / per), st, X, S, S_pad, B,
+                       nsub, (int)P, Pc, Ps, F, phase_table, s_h_start, bins, lo, rg);
 }
 
 int mgh_dft_block(int nsets, const float* X, int S, int S_pad, int B, int nb_valid,

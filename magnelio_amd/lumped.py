@@ -187,7 +187,7 @@ class DeviceLumped:
         dims, K = solver.dims, solver.nsets
         dev = solver.e.device
         dt = solver.dt
-        n_steps = int(solver.total_time_steps)
+        n_steps = solver.horizon
         seen = set()
         for op in self.ops:
             edges = set(op.flat.tolist())

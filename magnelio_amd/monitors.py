@@ -13,8 +13,13 @@ Reference counterparts (relative to the reference repository root):
   face linearly weighted between the two cell layers around the box plane)
 
 The per-step work is one sampling kernel (cell-centre samples of every
-monitored component into a 64-step block buffer) and, every block, one MFMA
-phasor contraction fused with the float64 bin update (csrc/mgh_dft.hip).
+monitored component into a block buffer) and an MFMA phasor contraction
+fused with the float64 bin update (csrc/mgh_dft.hip).  Pipelined (the
+default): the samples of period p (P = block x blocks_per_launch steps) go
+to one of two buffers while each step contracts 1/P of the samples of period
+p - 1 from the other, so every step carries the same share of the
+contraction; the run's end flushes what is left.  Otherwise one contraction
+of the whole buffer runs every P steps.
 E samples carry the time stamp t = n dt, H samples t + dt/2, where n is the
 index of the step whose H update has just finished — the reference's stamps
 (fit_td.py:996-1002), kept as they are.
@@ -271,8 +276,14 @@ class DFTMonitorBase:
     structured = True
 
     def __init__(self, freqs, name: str = "", block: int = BLOCK, compensated: bool = True,
-                 blocks_per_launch: int | None = None):
+                 blocks_per_launch: int | None = None, pipelined: bool = True,
+                 fsplit: int | None = None, slice_steps: int | None = None):
         self.freqs = np.asarray(freqs, dtype=float).ravel()
+        self.pipelined = bool(pipelined)
+        if fsplit is not None and int(fsplit) < 1:
+            raise ValueError("fsplit must be a positive int")
+        self.fsplit_req = fsplit
+        self.slice_steps_req = slice_steps
         if self.freqs.size == 0:
             raise ValueError("freqs must be a non-empty 1D array")
         if not (1 <= block <= BMAX):
@@ -374,14 +385,108 @@ class DFTMonitorBase:
         self.Ps = torch.from_numpy(lane_order(Ps)).to(dev, f32)
         self.omega = torch.from_numpy(omega).to(dev, torch.float64)
         self.blk = torch.zeros((self.blocks_per_launch * self.F, 4), dtype=torch.float64, device=dev)
-        self.X = torch.zeros((K, self.period, self.S_pad), dtype=f32, device=dev)
+        # pipelined: two sample buffers, period p in buffer p mod 2
+        nbuf = 2 if self.pipelined else 1
+        self.X = torch.zeros((nbuf, K, self.period, self.S_pad), dtype=f32, device=dev)
+        self.x_buf = K * self.period * self.S_pad if self.pipelined else 0
+        props = torch.cuda.get_device_properties(dev)
+        self.slice_steps, self.fsplit = self._slicing(K, int(props.multi_processor_count))
+        self.nsets = K
+        self.n_start = solver.n_completed
+        self._horizon = solver.horizon
+        self.phase_tab = None
+        if self.pipelined:
+            self._phase_table(torch.cuda.current_stream(dev))
         self.bins = torch.zeros((K, self.F, self.S, 2), dtype=torch.float64, device=dev)
         # TwoSum compensation terms of the bins, float32 (csrc/mgh_dft.hip two_sum_acc)
         self.lo = torch.zeros(self.bins.shape, dtype=f32, device=dev) if self.compensated else None
-        self.nsets = K
-        self.n_start = solver.n_completed
         self._pending = 0
         self._attached = True
+
+    # contraction slices: each wave keeps at least MIN_TILES frequency tiles
+    # of 16 and a launch should hold at least MIN_ROUNDS rounds of resident
+    # work-groups (so their load and matrix phases drift apart) and fill at
+    # least MIN_FILL of them.  Measured on the 256^3 far-field monitor
+    # (1001 frequencies, 256-step periods), ms per step: slices every 1 / 2 /
+    # 4 / 8 / 16 / 32 steps 0.597 / 0.584 / 0.577 / 0.574 / 0.572 / 0.569,
+    # one contraction per period 0.565.
+    MIN_TILES = 8
+    MIN_ROUNDS = 1.5
+    MIN_FILL = 0.9
+
+    def _occupancy(self) -> int:
+        """Resident waves per SIMD of the contraction variant (its
+        __launch_bounds__ in csrc/mgh_dft.hip)."""
+        if self.blocks_per_launch > 1:
+            return 3
+        return 4 if block_pad(self.block) <= 256 else 2
+
+    def _slicing(self, K: int, cus: int = 256) -> tuple[int, int]:
+        """(steps between contraction slices c, frequency groups fsplit).
+
+        A pipelined slice covers c / P of the previous period's sample waves
+        (4 per work-group) for all frequencies, split into fsplit groups of
+        work-groups.  Taken: the smallest power of two c dividing P for
+        which some split keeps MIN_TILES tiles per work-group and gives
+        MIN_ROUNDS rounds of resident work-groups (cus x waves per SIMD of
+        them), MIN_FILL of them full; of those splits the one with the
+        fewest groups (each group reads the samples once, and a wave's
+        sample operand serves more tiles).  Without one, the fullest rounds."""
+        tiles = self.F_pad // 16
+        if not self.pipelined:
+            return self.period, 1
+        P = self.period
+        waves = -(-self.S // 16)
+        slots = cus * self._occupancy()
+        cands = []
+        cs = [c for c in (1 << k for k in range(12)) if c <= P and P % c == 0]
+        if self.slice_steps_req is not None:
+            cs = [int(self.slice_steps_req)]
+            if cs[0] < 1 or P % cs[0]:
+                raise ValueError(f"slice_steps must divide the {P}-step period, got {cs[0]}")
+        splits = range(1, tiles + 1) if self.fsplit_req is None else [min(int(self.fsplit_req), tiles)]
+        for c in cs:
+            best = None
+            for fs in splits:
+                per = -(-tiles // fs)
+                z = -(-tiles // per)
+                if per < self.MIN_TILES and fs > 1 and self.fsplit_req is None:
+                    continue
+                wgs = -(-(-(-waves * c // P)) // 4) * K * z
+                if wgs < self.MIN_ROUNDS * slots and c < P:
+                    continue
+                fill = wgs / (-(-wgs // slots) * slots)
+                key = (True, -z, fill) if fill >= self.MIN_FILL else (False, fill, -z)
+                if best is None or key > best[0]:
+                    best = (key, z)
+            if best is None:
+                continue
+            cands.append((c, best[1], best[0]))
+            if best[0][0]:
+                return c, best[1]
+        c, z, _ = max(cands, key=lambda t: (t[2][1], -t[0]))
+        return c, z
+
+    def _phase_table(self, stream) -> None:
+        """Block phases of every period from n_start to past the horizon."""
+        P = self.period
+        n_per = max(1, -(-(self._horizon - self.n_start) // P) + 1)
+        self.n_periods = n_per
+        self.phase_tab = torch.empty((n_per, self.blocks_per_launch, self.F, 4), dtype=torch.float64,
+                                     device=self.omega.device)
+        _lib.call("mgh_dft_phase_table", self.omega.data_ptr(), self.F, float(self.dt), self.n_start,
+                  self.block, self.blocks_per_launch, n_per, self.phase_tab.data_ptr(),
+                  int(stream.cuda_stream))
+
+    @property
+    def cadence(self) -> int:
+        """Steps between contraction launches (a captured graph spans a
+        multiple of it and replays at starts congruent to its capture)."""
+        return self.slice_steps if self.pipelined else self.period
+
+    @property
+    def waves(self) -> int:
+        return -(-self.S // 16)
 
     @property
     def period(self) -> int:
@@ -399,34 +504,66 @@ class DFTMonitorBase:
             _lib.call("mgh_dft_sample_groups", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
                       solver.h.data_ptr(), solver.h.shape[1], self.grp_d.data_ptr(),
                       self.grp_d.shape[0], self.grp_blocks, self.wtab_d.data_ptr(), self.S,
-                      self.X.data_ptr(), self.S_pad, self.period, solver.step_ctr.data_ptr(),
-                      self.n_start, sh)
+                      self.X.data_ptr(), self.S_pad, self.period, self.x_buf,
+                      solver.step_ctr.data_ptr(), self.n_start, sh)
         else:
             _lib.call("mgh_dft_sample", code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
                       solver.h.data_ptr(), solver.h.shape[1], self.idx_d.data_ptr(),
                       self.w_d.data_ptr(), self.S, self.X.data_ptr(), self.S_pad, self.period,
-                      solver.step_ctr.data_ptr(), self.n_start, sh)
+                      self.x_buf, solver.step_ctr.data_ptr(), self.n_start, sh)
         P = self.period
-        if (n - self.n_start) % P == P - 1:
+        if self.pipelined:
+            c = self.slice_steps
+            if (n - self.n_start) % c == c - 1:
+                _lib.call("mgh_dft_slice", self.nsets, self.X.data_ptr(), self.x_buf, self.S,
+                          self.S_pad, self.block, self.blocks_per_launch, self.Pc.data_ptr(),
+                          self.Ps.data_ptr(), self.F, self.F_pad, self.phase_tab.data_ptr(),
+                          self.n_periods, self.s_h_start, self.bins.data_ptr(), self._lo_ptr(),
+                          solver.step_ctr.data_ptr(), self.n_start, c, self.fsplit, sh)
+        elif (n - self.n_start) % P == P - 1:
             self._contract(solver, sh, P)
-            self._pending = 0
-        else:
-            self._pending = (n - self.n_start) % P + 1
+
+    def _lo_ptr(self):
+        return self.lo.data_ptr() if self.lo is not None else None
 
     def _contract(self, solver, sh: int, nb: int) -> None:
         _lib.call("mgh_dft_blocks", self.nsets, self.X.data_ptr(), self.S, self.S_pad, self.block,
                   self.blocks_per_launch, nb,
                   self.Pc.data_ptr(), self.Ps.data_ptr(), self.omega.data_ptr(), self.F, self.F_pad,
                   float(self.dt), self.blk.data_ptr(), self.s_h_start, self.bins.data_ptr(),
-                  self.lo.data_ptr() if self.lo is not None else None, solver.step_ctr.data_ptr(),
-                  self.n_start, sh)
+                  self._lo_ptr(), solver.step_ctr.data_ptr(), self.n_start, sh)
+
+    def _contract_range(self, sh: int, buf: int, nb: int, w_lo: int, w_hi: int, n0: int) -> None:
+        _lib.call("mgh_dft_blocks_range", self.nsets, self.X[buf].data_ptr(), self.S, self.S_pad,
+                  self.block, self.blocks_per_launch, nb, w_lo, w_hi, n0, self.Pc.data_ptr(),
+                  self.Ps.data_ptr(), self.omega.data_ptr(), self.F, self.F_pad, float(self.dt),
+                  self.blk.data_ptr(), self.s_h_start, self.bins.data_ptr(), self._lo_ptr(),
+                  self.fsplit, sh)
 
     def finalize(self, solver) -> None:
-        """Contract a partially filled last block (the counter stands at n_done)."""
-        if self._pending:
-            stream = torch.cuda.current_stream(solver.e.device)
-            self._contract(solver, int(stream.cuda_stream), self._pending)
-            self._pending = 0
+        """Contract every recorded sample not yet in the bins; the sample
+        buffers are then cleared, so a run that goes on from here adds each
+        sample once (the counter stands at the number of completed steps)."""
+        stream = torch.cuda.current_stream(solver.e.device)
+        sh = int(stream.cuda_stream)
+        P = self.period
+        rel = solver.n_completed - 1 - self.n_start  # last recorded step
+        if rel < 0:
+            return
+        p, r = divmod(rel, P)
+        if self.pipelined:
+            W = self.waves
+            ns = P // self.slice_steps
+            done = (r + 1) // self.slice_steps  # slices of period p - 1 already contracted
+            if p >= 1 and done < ns:
+                self._contract_range(sh, (p - 1) % 2, P, done * W // ns, W,
+                                     self.n_start + (p - 1) * P)
+            self._contract_range(sh, p % 2, r + 1, 0, W, self.n_start + p * P)
+        elif r + 1 < P:  # a full period was contracted by its last step
+            self._contract(solver, sh, r + 1)
+        else:
+            return
+        self.X.zero_()
 
     def result(self, s: int = 0) -> dict:
         """{component: complex128 (F, *shape)} of field set s (host)."""
@@ -440,8 +577,7 @@ class DFTMonitorBase:
         return {k: (v[0] if len(v) == 1 else v) for k, v in out.items()}
 
     def state_dict(self) -> dict:
-        sd = {"X": self.X.cpu().numpy(), "bins": self.bins.cpu().numpy(), "pending": self._pending,
-              "n_start": self.n_start}
+        sd = {"X": self.X.cpu().numpy(), "bins": self.bins.cpu().numpy(), "n_start": self.n_start}
         if self.lo is not None:
             sd["lo"] = self.lo.cpu().numpy()
         return sd
@@ -451,16 +587,21 @@ class DFTMonitorBase:
         self.bins.copy_(torch.from_numpy(np.asarray(sd["bins"])))
         if self.lo is not None:
             self.lo.copy_(torch.from_numpy(np.asarray(sd["lo"])))
-        self._pending = int(sd["pending"])
-        self.n_start = int(sd["n_start"])
+        n_start = int(sd["n_start"])
+        if n_start != self.n_start:
+            self.n_start = n_start
+            if self.pipelined:
+                self._phase_table(torch.cuda.current_stream(self.X.device))
 
 
 class MonitorFieldFrequency(DFTMonitorBase):
     """DFT of cell-centred fields in a box region (field_frequency.py)."""
 
     def __init__(self, freqs, corners=None, fields=("E", "H"), name: str = "", region=None,
-                 block: int = BLOCK, compensated: bool = True, blocks_per_launch: int | None = None):
-        super().__init__(freqs, name, block, compensated, blocks_per_launch)
+                 block: int = BLOCK, compensated: bool = True, blocks_per_launch: int | None = None,
+                 pipelined: bool = True, fsplit: int | None = None, slice_steps: int | None = None):
+        super().__init__(freqs, name, block, compensated, blocks_per_launch, pipelined, fsplit,
+                         slice_steps)
         self.corners = corners
         self.region = region  # optional explicit (slice, slice, slice) of cells
         comps = []
@@ -503,8 +644,10 @@ class MonitorFarField(DFTMonitorBase):
     """
 
     def __init__(self, freqs, margin_cells: int = 3, name: str = "", block: int = BLOCK,
-                 compensated: bool = True, blocks_per_launch: int | None = None):
-        super().__init__(freqs, name, block, compensated, blocks_per_launch)
+                 compensated: bool = True, blocks_per_launch: int | None = None,
+                 pipelined: bool = True, fsplit: int | None = None, slice_steps: int | None = None):
+        super().__init__(freqs, name, block, compensated, blocks_per_launch, pipelined, fsplit,
+                         slice_steps)
         if margin_cells < 1:
             raise ValueError("margin_cells must be at least 1")
         self.margin_cells = margin_cells

@@ -66,6 +66,49 @@ def _face_bits(faces) -> int:
 
 PRECISIONS = {"single": (np.float32, torch.float32), "double": (np.float64, torch.float64)}
 
+# port-signal stall watchdog: arming level below the run peak and the
+# shortest fit window, in checks (fit_td.py:62-67)
+STALL_ARM_DB = 40.0
+STALL_MIN_WINDOW_CHECKS = 10
+
+
+class SignalStallDetector:
+    """Decides when the port-signal threshold cannot be reached before the
+    runtime cap (fit_td.py:76-132): once the envelope sits ``arm_db`` below
+    its peak, a least-squares line through the last ``window`` check
+    samples (dB against step) is extrapolated to the threshold; a flat or
+    rising envelope, or a crossing beyond ``cap_step``, ends the run now —
+    the outcome the cap would give, without the steps in between."""
+
+    def __init__(self, arm_db: float, window: int, cap_step: int) -> None:
+        self.arm_db = float(arm_db)
+        self.window = max(int(window), 2)
+        self.cap_step = int(cap_step)
+        self.slope_db_per_step: float | None = None
+        self._samples: list[tuple[int, float]] = []
+
+    def reset(self) -> None:
+        self._samples.clear()
+
+    def observe(self, n: int, sig_db: float, threshold_db: float) -> bool:
+        if sig_db > -self.arm_db:
+            self._samples.clear()
+            return False
+        self._samples.append((n, float(sig_db)))
+        if len(self._samples) > self.window:
+            del self._samples[0]
+        elif len(self._samples) < self.window:
+            return False
+        steps, vals = (np.array(c, dtype=np.float64) for c in zip(*self._samples))
+        slope = float(np.polyfit(steps, vals, 1)[0])
+        self.slope_db_per_step = slope
+        remaining = threshold_db - float(vals[-1])
+        if remaining >= 0.0:
+            return False
+        if slope >= 0.0:
+            return True
+        return n + remaining / slope > self.cap_step
+
 
 @dataclass
 class ChannelExcitation:
@@ -225,11 +268,13 @@ class FITTimeDomainSolver:
     boundary_conditions: dict = field(default_factory=dict)
     ports: list = field(default_factory=list)
     sources: list = field(default_factory=list)
-    total_time_steps: int = 1000
+    total_time_steps: int | None = 1000  # None: unbounded, ended by a stop criterion or the cap
     dt: float = 1e-12
     verbose: bool = False
     energy_stop_db: float | None = None
     port_signal_stop_db: float | None = None
+    port_signal_min_steps: int | None = None  # the port-signal criterion arms at this step
+    max_time_steps: int | None = None  # runtime cap of an unbounded run (absolute step)
     energy_check_interval: int | None = None
     monitors: list = field(default_factory=list)
     lumped: list = field(default_factory=list)
@@ -261,7 +306,14 @@ class FITTimeDomainSolver:
         self.timed_events: list = []
         self._resume_step = 0
         self._stop_reason = None
+        self._final_signal_db = None
+        self._stop_requested = False
+        self._aborted = False
         self._graph = None
+        self._graph_len = 0
+        self._graph_start = 0
+        self.energy_trace: list = []  # (step, energy per set) at every check
+        self.set_stops: list = [None] * self.nsets  # per set: (reason, step) once its criterion fired
 
     # ------------------------------------------------------------------
     def prepare_host(self) -> None:
@@ -419,7 +471,7 @@ class FITTimeDomainSolver:
             pec_cell = np.isin(mesh.material_id, pec_ids)
         self.cpml = (DeviceCPML(cpml_faces, dims, K, t_dt, dev, pec_e, pec_cell)
                      if cpml_faces else None)
-        n_steps = int(self.total_time_steps)
+        n_steps = self.horizon
         self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
         self.dports = (DevicePorts(self.ports, dims, K, n_steps, dev, self.excitations, dt, self.record)
                        if self.ports else None)
@@ -506,68 +558,174 @@ class FITTimeDomainSolver:
         return self._energy.cpu().numpy().copy()
 
     @property
+    def horizon(self) -> int:
+        """Steps the device buffers (port histories, recorders, source
+        tables) are sized for: ``total_time_steps``, or the runtime cap of an
+        unbounded run."""
+        if self.total_time_steps is not None:
+            return int(self.total_time_steps)
+        if self.max_time_steps is None:
+            raise ValueError("an unbounded run (total_time_steps=None) needs max_time_steps here: "
+                             "the port histories and recorders live on the device, sized for it")
+        return int(self.max_time_steps)
+
+    @property
     def graph_steps(self) -> int:
-        """Steps per captured graph: a multiple of every monitor block (the
-        block contraction sits at a fixed position inside it) and of
-        ``graph_block``."""
+        """Steps per captured graph: a multiple of ``graph_block`` and of
+        every monitor's launch cadence (a non-pipelined monitor contracts
+        once per period, a pipelined one every few steps)."""
         g = max(1, int(self.graph_block))
+        return g * self._cadence() // math.gcd(g, self._cadence())
+
+    def _cadence(self) -> int:
+        L = 1
         for mon in self.monitors:
-            p = getattr(mon, "period", 1)
-            g = g * p // math.gcd(g, p)
-        return g
+            c = int(getattr(mon, "cadence", 1))
+            L = L * c // math.gcd(L, c)
+        return L
+
+    def request_stop(self) -> None:
+        """Ask the marching loop to stop at the next step boundary
+        (fit_td.py:1252-1262): a plain flag, safe to set from a signal
+        handler; the run ends with ``stop_reason == "aborted"`` on a
+        consistent leapfrog state that ``state_dict`` can checkpoint."""
+        self._stop_requested = True
+
+    @property
+    def stop_reason(self) -> str | None:
+        """Why the last run ended: "steps", "energy", "port_signal",
+        "port_signal_stall", "runtime_cap" or "aborted"."""
+        return self._stop_reason
+
+    def _graph_ok(self, n: int, G: int) -> bool:
+        """A G-step graph may run from step n: G is a multiple of the
+        monitors' cadence and, once a graph exists, n is congruent to its
+        capture step (the host decides per step which contraction launches
+        it records, from the step index)."""
+        L = self._cadence()
+        if G % L:
+            return False
+        return self._graph is None or self._graph_len != G or (n - self._graph_start) % L == 0
+
+    def _replay(self, n: int, G: int) -> bool:
+        """Replay (capturing first if needed) a graph of G steps from step n."""
+        if self._graph is not None and self._graph_len != G:
+            self._graph = None
+        if self._graph is None:
+            self._capture(n, G)
+        if self._graph is None:
+            return False
+        self._graph.replay()
+        return True
 
     def run(self, n_steps: int | None = None) -> None:
+        """March to ``total_time_steps`` (or ``n_steps`` more), with the
+        reference's stop criteria (fit_td.py:682-1246): every
+        ``check_interval`` steps the stored energy and the port-signal
+        envelope of each field set are compared with that set's peak, and a
+        set whose criterion fires is done; the run stops once every set is.
+        An unbounded run (``total_time_steps=None``) ends at a criterion or
+        at ``max_time_steps`` ("runtime_cap"), and the port-signal stall
+        watchdog ends it early when the fitted envelope slope shows the
+        threshold unreachable before the cap ("port_signal_stall").  With
+        ``use_graphs`` the steps between two checks replay as one graph."""
         if not self._setup_done:
             self.setup()
-        total = int(self.total_time_steps) if n_steps is None else self._resume_step + int(n_steps)
-        if total > int(self.total_time_steps):
-            raise ValueError("cannot run past total_time_steps (recorder and DTBC histories are sized for it)")
-        interval = self.energy_check_interval or max(1, min(100, int(self.total_time_steps) // 20))
+        unbounded = self.total_time_steps is None
         checks = self.energy_stop_db is not None or self.port_signal_stop_db is not None
-        peak_e = getattr(self, "_peak_energy", 0.0)
-        peak_sig = getattr(self, "_peak_signal", 0.0)
+        if unbounded and not checks:
+            raise ValueError("total_time_steps=None (unbounded run) needs energy_stop_db or "
+                             "port_signal_stop_db to provide a stop criterion")
+        horizon = self.horizon
+        if unbounded and horizon <= self._resume_step:
+            raise ValueError(f"max_time_steps={horizon} does not advance past the current step "
+                             f"{self._resume_step}")
+        if self.port_signal_stop_db is not None and self.dports is None:
+            raise ValueError("port_signal_stop_db needs at least one modal port")
+        total = horizon if n_steps is None else self._resume_step + int(n_steps)
+        if total > horizon:
+            raise ValueError("cannot run past total_time_steps (recorder and DTBC histories are sized for it)")
+        interval = self.energy_check_interval or (100 if unbounded else max(1, min(100, horizon // 20)))
+        K = self.nsets
+        self._peak_e = np.broadcast_to(np.asarray(getattr(self, "_peak_e", 0.0), float), (K,)).copy()
+        self._peak_s = np.broadcast_to(np.asarray(getattr(self, "_peak_s", 0.0), float), (K,)).copy()
+        stall = None
+        if self.port_signal_stop_db is not None and unbounded:
+            window = max(STALL_MIN_WINDOW_CHECKS, (self.port_signal_min_steps or 0) // (2 * interval))
+            stall = [SignalStallDetector(STALL_ARM_DB, window, horizon) for _ in range(K)]
         self._stop_reason = None
-        G = self.graph_steps
+        self._stop_requested = False
+        self._aborted = False
+        G = interval if checks else self.graph_steps
         n = self._resume_step
         while n < total:
-            # graphs replay G steps that start on a block boundary; the stop
-            # checks need single steps, so a run with checks stays eager
-            if (self.use_graphs and not checks and n + G <= total and n >= 2 and
-                    self._mon_aligned(n, G)):
-                if self._graph is None:
-                    self._capture(n, G)
-                if self._graph is not None:
-                    self._graph.replay()
-                    n += G
-                    self._resume_step = n
-                    continue
-            self._step(n)
-            n += 1
+            if self._stop_requested:
+                break
+            # graphs replay whole check intervals: from just after a check
+            # (or any step, without checks) to the next one
+            if (self.use_graphs and n >= 2 and n + G <= total and self._graph_ok(n, G)
+                    and (not checks or (n - 1) % interval == 0) and self._replay(n, G)):
+                n += G
+            else:
+                self._step(n)
+                n += 1
             self._resume_step = n
             if checks and (n - 1) % interval == 0:
-                if self.energy_stop_db is not None:
-                    en = float(self.energy().max())
-                    if en > peak_e:
-                        peak_e = en
-                    elif peak_e > 0 and en < peak_e * 10 ** (-self.energy_stop_db / 10):
-                        self._stop_reason = "energy"
-                        break
-                if self.port_signal_stop_db is not None and self.dports is not None:
-                    env = max(float(a.max()) for a in self.dports.absmax)
-                    for a in self.dports.absmax:
-                        a.zero_()
-                    if env > peak_sig:
-                        peak_sig = env
-                    elif peak_sig > 0 and env < peak_sig * 10 ** (-self.port_signal_stop_db / 20):
-                        self._stop_reason = "port_signal"
-                        break
-        self._peak_energy, self._peak_signal = peak_e, peak_sig
-        if self._stop_reason is None and self._resume_step >= int(self.total_time_steps):
-            self._stop_reason = "steps"
+                reason = self._check(n - 1, stall)
+                if reason is not None:
+                    self._stop_reason = reason
+                    break
+        if self._stop_requested and self._stop_reason is None:
+            self._aborted = True
+            self._stop_reason = "aborted"
+        elif self._stop_reason is None and self._resume_step >= horizon:
+            self._stop_reason = "runtime_cap" if unbounded else "steps"
+            if unbounded:
+                import warnings  # noqa: PLC0415
+
+                warnings.warn(f"unbounded run hit the runtime cap of {horizon} steps before any stop "
+                              "criterion fired; the recorded signals are truncated at this step",
+                              RuntimeWarning, stacklevel=2)
         if self._stop_reason is not None:
             for mon in self.monitors:
                 mon.finalize(self)
         torch.cuda.synchronize(self._dev)
+
+    def _check(self, n: int, stall) -> str | None:
+        """One stop check after step n (fit_td.py:1014-1153), per field set;
+        returns the run's stop reason once every set is done."""
+        dbe, dbs = self.energy_stop_db, self.port_signal_stop_db
+        en = self.energy()
+        self.energy_trace.append((n, en))
+        for s in range(self.nsets):
+            if self.set_stops[s] is not None:
+                continue
+            if en[s] > self._peak_e[s]:
+                self._peak_e[s] = en[s]
+            elif dbe is not None and self._peak_e[s] > 0 and en[s] < self._peak_e[s] * 10 ** (-dbe / 10):
+                self.set_stops[s] = ("energy", n + 1)
+        if dbs is not None:
+            env = np.max(np.stack([a.cpu().numpy() for a in self.dports.absmax]), axis=0)
+            for a in self.dports.absmax:
+                a.zero_()
+            armed = self.port_signal_min_steps is None or n + 1 >= self.port_signal_min_steps
+            for s in range(self.nsets):
+                if self.set_stops[s] is not None:
+                    continue
+                if env[s] > self._peak_s[s]:
+                    self._peak_s[s] = env[s]
+                    if stall is not None:
+                        stall[s].reset()
+                elif self._peak_s[s] > 0:
+                    sig_db = 20.0 * math.log10(max(float(env[s]), 1e-300) / self._peak_s[s])
+                    self._final_signal_db = sig_db
+                    if armed and env[s] < self._peak_s[s] * 10.0 ** (-dbs / 20.0):
+                        self.set_stops[s] = ("port_signal", n + 1)
+                    elif armed and stall is not None and stall[s].observe(n, sig_db, -dbs):
+                        self.set_stops[s] = ("port_signal_stall", n + 1)
+        if all(st is not None for st in self.set_stops):
+            return max(self.set_stops, key=lambda st: st[1])[0]
+        return None
 
     def launch(self, n_steps: int) -> int:
         """Enqueue up to ``n_steps`` steps on the current stream and return
@@ -576,29 +734,22 @@ class FITTimeDomainSolver:
         A capture (first use of graphs) waits for the device once."""
         if not self._setup_done:
             self.setup()
-        total = min(int(self.total_time_steps), self._resume_step + int(n_steps))
+        total = min(self.horizon, self._resume_step + int(n_steps))
         G = self.graph_steps
         n = self._resume_step
         while n < total:
-            if self.use_graphs and n + G <= total and n >= 2 and self._mon_aligned(n, G):
-                if self._graph is None:
-                    self._capture(n, G)
-                if self._graph is not None:
-                    self._graph.replay()
-                    n += G
-                    continue
+            if self.use_graphs and n + G <= total and n >= 2 and self._graph_ok(n, G) and self._replay(n, G):
+                n += G
+                continue
             self._step(n)
             n += 1
         done = n - self._resume_step
         self._resume_step = n
-        if n >= int(self.total_time_steps):
+        if n >= self.horizon:
             self._stop_reason = "steps"
             for mon in self.monitors:
                 mon.finalize(self)
         return done
-
-    def _mon_aligned(self, n: int, G: int) -> bool:
-        return all((n - getattr(m, "n_start", 0)) % getattr(m, "period", 1) == 0 for m in self.monitors)
 
     def _capture(self, n: int, G: int) -> None:
         """Capture G steps as one HIP graph; fall back to eager launches on failure.
@@ -623,6 +774,8 @@ class FITTimeDomainSolver:
             cur.wait_stream(s)
             s.synchronize()
             self._graph = g
+            self._graph_len = G
+            self._graph_start = n
         except Exception as exc:  # pragma: no cover - depends on the runtime
             import warnings  # noqa: PLC0415
 
@@ -661,8 +814,10 @@ class FITTimeDomainSolver:
     def state_dict(self) -> dict:
         torch.cuda.synchronize(self._dev)
         sd = {"n_completed": self._resume_step,
-              "peak_energy": getattr(self, "_peak_energy", 0.0),
-              "peak_signal": getattr(self, "_peak_signal", 0.0),
+              "peak_energy": float(np.max(getattr(self, "_peak_e", 0.0))),
+              "peak_signal": float(np.max(getattr(self, "_peak_s", 0.0))),
+              "peak_energy_sets": np.array(getattr(self, "_peak_e", np.zeros(self.nsets)), float),
+              "peak_signal_sets": np.array(getattr(self, "_peak_s", np.zeros(self.nsets)), float),
               "e": layout.to_reference(self.e.cpu().numpy(), *self.dims, "E"),
               "h": layout.to_reference(self.h.cpu().numpy(), *self.dims, "H")}
         if self.cpml is not None:
@@ -721,7 +876,7 @@ class FITTimeDomainSolver:
             self.e.masked_fill_(self.tile_plan.zero_e.unsqueeze(0), 0)
         self.step_ctr.fill_(n)
         self._resume_step = n
-        self._peak_energy = float(sd.get("peak_energy", 0.0))
-        self._peak_signal = float(sd.get("peak_signal", 0.0))
+        self._peak_e = np.array(sd.get("peak_energy_sets", sd.get("peak_energy", 0.0)), float)
+        self._peak_s = np.array(sd.get("peak_signal_sets", sd.get("peak_signal", 0.0)), float)
         self._graph = None
         torch.cuda.synchronize(self._dev)

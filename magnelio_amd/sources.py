@@ -227,7 +227,7 @@ class PlaneWaveSource:
         co = solver._coeffs
         pe, ph = self.patches(grid, np.asarray(co.beta_e), np.asarray(co.beta_h), np_dt)
         dt = solver.dt
-        n_steps = int(solver.total_time_steps)
+        n_steps = solver.horizon
         n = np.arange(n_steps, dtype=np.float64) * dt
         t_e = (n + dt) - dt / 2                # inject_E(t + dt) at t_E - dt/2
         t_h = ((n + dt) + dt / 2) - dt / 2     # inject_H(t + dt + dt/2) at t_H - dt/2

@@ -75,3 +75,34 @@ class Oracle:
     def run(self, n_steps: int) -> None:
         for _ in range(n_steps):
             self.step()
+
+    def run_with_stops(self, n_steps: int, m_eps, m_mu, interval: int, energy_stop_db=None,
+                       port_signal_stop_db=None, port_signal_min_steps=None):
+        """March with the reference's stop checks (fit_td.py:1008-1153, the
+        bounded-run branch): after step n with n % interval == 0 the stored
+        energy 0.5 (sum M_eps e^2 + sum M_mu h^2) (float64 sums) and then the
+        port |V| envelope polled since the previous check are compared with
+        their running peaks.  Returns (reason, completed steps, energies)."""
+        peak_e = peak_s = 0.0
+        trace = []
+        while self.n < n_steps:
+            n = self.n
+            self.step()
+            if n % interval:
+                continue
+            e64, h64 = self.e.astype(np.float64), self.h.astype(np.float64)
+            en = 0.5 * (float(np.sum(m_eps * e64 * e64)) + float(np.sum(m_mu * h64 * h64)))
+            trace.append((n, en))
+            if en > peak_e:
+                peak_e = en
+            elif peak_e > 0.0 and energy_stop_db is not None and en < peak_e * 10 ** (-energy_stop_db / 10):
+                return "energy", n + 1, trace
+            if port_signal_stop_db is not None:
+                sig = max(p.poll_signal_absmax() for p in self.ports)
+                if sig > peak_s:
+                    peak_s = sig
+                elif peak_s > 0.0:
+                    armed = port_signal_min_steps is None or n + 1 >= port_signal_min_steps
+                    if armed and sig < peak_s * 10.0 ** (-port_signal_stop_db / 20.0):
+                        return "port_signal", n + 1, trace
+        return "steps", self.n, trace

@@ -248,6 +248,7 @@ class Port:
             self.tau[mi] = ndx / vp
         self.src_buffer = collections.deque(maxlen=0)
         self.V_port_prev = np.zeros(M)
+        self.absmax = 0.0  # |V| envelope since the last poll (operator.py:1530-1536)
         self.V_int_prev = np.zeros(M)
         self.exc = None
         self.wf = None
@@ -297,6 +298,14 @@ class Port:
         e[pl["e_v"]] = ev
         self.V_port_prev[:] = V
         self.V_int_prev[:] = V_int_new
+        env = float(np.max(np.abs(V) * self.record_scale))
+        if env > self.absmax:
+            self.absmax = env
+
+    def poll_signal_absmax(self) -> float:
+        """operator.py:1323-1334: the envelope since the last poll, reset."""
+        m, self.absmax = self.absmax, 0.0
+        return m
 
     def record(self, e, h):
         pl = self.pl

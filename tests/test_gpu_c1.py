@@ -3,11 +3,21 @@
 48 x 24 x 96 empty PEC guide, DTBC TE ports (3 modes) on z_min / z_max,
 port 1 mode 0 excited from the TE10 cut-off to 25 GHz.
 
-* FP32 HIP vs the float64 oracle after 4000 steps: field relative L2 and
-  S-parameter error (|dS| <= 1e-4, the reference's single-vs-double gate);
-* FP32 HIP vs the float32 oracle (same operation order): fields agree to
-  float32 round-off (the port reductions associate differently in float64);
-* FP64 HIP vs the float64 oracle: 1e-12;
+* FP32 HIP vs the float64 oracle after 300, 600 and 4000 steps: field error
+  at most 1e-5 of the run's peak field norm (the largest ||E|| / ||H|| of
+  the float64 oracle over every step up to the snapshot) and S-parameter
+  error |dS| <= 1e-4 (the reference's single-vs-double gate).  The peak norm
+  is the stated normalisation: by step 4000 the pulse has left the guide,
+  ||E|| has decayed 580x to the cut-off ringing tail and H is a static
+  remnant (99 % time-invariant), so an error of fixed absolute size —
+  float32 round-off collected while the pulse was inside — is 2.5e-4 of the
+  tail's own norm in E.  The float32 oracle in the reference's operation
+  order carries the same error (measured on the CPU: 2.5e-4 relative, 4.0e-7
+  of the peak at step 4000); the HIP path must add nothing beyond it;
+* FP32 HIP vs the float32 oracle (same operation order) for 4000 steps:
+  fields agree to float32 round-off (the port reductions associate
+  differently in float64);
+* FP64 HIP vs the float64 oracle: 1e-10;
 * K = 6 excitations in one batched launch == six single runs, bit for bit;
 * HIP-graph replay == eager launches, bit for bit;
 * checkpoint / resume continues bit-exactly;
@@ -47,33 +57,70 @@ def _oracle_sparams(ports, dt, f):
 
 
 SNAPS = (300, 600, 4000)
+FIELD_GATE = 1e-5  # of the peak field norm
+
+
+def _run_oracle(dtype):
+    """The oracle to every snapshot, with the peak field norms seen so far."""
+    orc, ports, dt, _ = cases.c1_oracle(dtype)
+    snaps, peak_e, peak_h = {}, 0.0, 0.0
+    for n in SNAPS:
+        while orc.n < n:
+            orc.step()
+            peak_e = max(peak_e, float(np.linalg.norm(orc.e.astype(np.float64))))
+            peak_h = max(peak_h, float(np.linalg.norm(orc.h.astype(np.float64))))
+        snaps[n] = (orc.e.astype(np.float64), orc.h.astype(np.float64), peak_e, peak_h)
+    orc.snaps = snaps
+    return orc, ports, dt
 
 
 @pytest.fixture(scope="module")
 def c1_oracle64():
     print("oracle C1 float64, 4000 steps (CPU)", flush=True)
-    orc, ports, dt, _ = cases.c1_oracle(np.float64)
-    snaps = {}
-    for n in SNAPS:
-        orc.run(n - orc.n)
-        snaps[n] = (orc.e.copy(), orc.h.copy())
-    orc.snaps = snaps
-    return orc, ports, dt
+    return _run_oracle(np.float64)
 
 
-def test_c1_fp32_vs_fp64_oracle(c1_oracle64, gpu):
-    orc, oports, dt = c1_oracle64
+@pytest.fixture(scope="module")
+def c1_oracle32():
+    print("oracle C1 float32, 4000 steps (CPU)", flush=True)
+    return _run_oracle(np.float32)
+
+
+@pytest.fixture(scope="module")
+def c1_hip32():
+    """FP32 HIP run of C1 with field snapshots (shared by the FP32 tests)."""
     print("HIP C1 fp32 4000 steps", flush=True)
-    sol, ports, dt_h = cases.c1_solver(n_steps=4000)
-    assert dt_h == dt
-    errs = {}
+    sol, ports, dt = cases.c1_solver(n_steps=4000)
+    snaps = {}
     for n in SNAPS:
         sol.run(n - sol.n_completed)
         e, h = sol.fields(0)
-        oe, oh = orc.snaps[n]
-        errs[n] = (_rel_l2(e.astype(np.float64), oe), _rel_l2(h.astype(np.float64), oh))
-        print(f"C1 fp32 vs fp64 oracle at step {n}: rel L2 e {errs[n][0]:.3e} h {errs[n][1]:.3e}",
-              flush=True)
+        snaps[n] = (e.astype(np.float64), h.astype(np.float64))
+    return sol, ports, dt, snaps
+
+
+def test_c1_fp32_vs_fp64_oracle(c1_oracle64, c1_oracle32, c1_hip32, gpu):
+    orc, oports, dt = c1_oracle64
+    o32 = c1_oracle32[0]
+    sol, ports, dt_h, snaps = c1_hip32
+    assert dt_h == dt
+    for n in SNAPS:
+        e, h = snaps[n]
+        oe, oh, pe, ph = orc.snaps[n]
+        ee, eh = np.linalg.norm(e - oe), np.linalg.norm(h - oh)
+        # the float32 oracle (reference operation order) against the same float64 run
+        re, rh = (np.linalg.norm(o32.snaps[n][0] - oe), np.linalg.norm(o32.snaps[n][1] - oh))
+        print(f"C1 step {n}: HIP fp32 vs fp64 oracle  e {ee / pe:.3e} h {eh / ph:.3e} of peak "
+              f"({ee / np.linalg.norm(oe):.3e} / {eh / np.linalg.norm(oh):.3e} of the snapshot); "
+              f"oracle fp32 e {re / pe:.3e} h {rh / ph:.3e} of peak", flush=True)
+        assert ee <= FIELD_GATE * pe and eh <= FIELD_GATE * ph, n
+        # nothing beyond the float32 error of the reference's own order
+        assert ee <= 1.5 * re + 1e-9 * pe and eh <= 1.5 * rh + 1e-9 * ph, n
+    # while the pulse is inside the guide the snapshot norm is the peak's order:
+    # the gate holds relative to the snapshot itself too
+    for n in (300, 600):
+        e, h = snaps[n]
+        assert _rel_l2(e, orc.snaps[n][0]) <= 1e-5 and _rel_l2(h, orc.snaps[n][1]) <= 1e-5
     f_c = oports[0].modes[0]["wc"] / (2 * math.pi)
     f = np.linspace(1.1 * f_c, 24e9, 201)
     S_h = _hip_sparams(sol, ports, dt, f)
@@ -81,12 +128,21 @@ def test_c1_fp32_vs_fp64_oracle(c1_oracle64, gpu):
     dmax = max(float(np.nanmax(np.abs(S_h[k] - S_o[k]))) for k in S_o)
     print(f"C1 fp32 vs fp64 oracle: max |dS| {dmax:.3e}", flush=True)
     assert dmax <= 1e-4
-    # while the pulse is in the guide: the 1e-5 field gate
-    for n in (300, 600):
-        assert errs[n][0] <= 1e-5 and errs[n][1] <= 1e-5
-    # step 4000 holds only the decayed cut-off ringing tail (~1e-3 of the
-    # peak field): float32 round-off relative to that small norm
-    assert errs[4000][0] <= 1e-3 and errs[4000][1] <= 1e-3
+
+
+def test_c1_fp32_vs_fp32_oracle_same_order(c1_oracle32, c1_hip32, gpu):
+    o32 = c1_oracle32[0]
+    snaps = c1_hip32[3]
+    for n in SNAPS:
+        e, h = snaps[n]
+        oe, oh, pe, ph = o32.snaps[n]
+        ee, eh = np.linalg.norm(e - oe), np.linalg.norm(h - oh)
+        print(f"C1 fp32 vs fp32 oracle step {n}: e {ee / pe:.3e} h {eh / ph:.3e} of peak "
+              f"({ee / np.linalg.norm(oe):.3e} / {eh / np.linalg.norm(oh):.3e} of the snapshot)",
+              flush=True)
+        assert ee <= 1e-6 * pe and eh <= 1e-6 * ph, n
+        if n <= 600:
+            assert ee <= 1e-6 * np.linalg.norm(oe), n
 
 
 def test_c1_fp64_vs_oracle(c1_oracle64, gpu):
@@ -94,7 +150,7 @@ def test_c1_fp64_vs_oracle(c1_oracle64, gpu):
     sol, ports, _ = cases.c1_solver(n_steps=4000, precision="double")
     sol.run()
     e, h = sol.fields(0)
-    err_e, err_h = _rel_l2(e, orc.e), _rel_l2(h, orc.h)
+    err_e, err_h = _rel_l2(e, orc.snaps[4000][0]), _rel_l2(h, orc.snaps[4000][1])
     print(f"C1 fp64 vs fp64 oracle: rel L2 e {err_e:.3e} h {err_h:.3e}", flush=True)
     assert err_e <= 1e-10 and err_h <= 1e-10
     sig = sol.recorded(0)
@@ -103,17 +159,6 @@ def test_c1_fp64_vs_oracle(c1_oracle64, gpu):
             V, I = sig[(p.name, m)]
             ref_v = np.array(p.V[m])
             assert np.max(np.abs(V - ref_v)) <= 1e-9 * np.max(np.abs(oports[0].V[0]))
-
-
-def test_c1_fp32_vs_fp32_oracle_same_order(gpu):
-    orc, oports, dt, _ = cases.c1_oracle(np.float32)
-    orc.run(600)
-    sol, ports, _ = cases.c1_solver(n_steps=600)
-    sol.run()
-    e, h = sol.fields(0)
-    err_e = _rel_l2(e.astype(np.float64), orc.e.astype(np.float64))
-    print(f"C1 fp32 vs fp32 oracle 600 steps: rel L2 e {err_e:.3e}", flush=True)
-    assert err_e <= 1e-6
 
 
 def test_batched_k6_equals_single_runs(gpu):

@@ -9,7 +9,13 @@
 * a region monitor inside the plain WR-90 run (C1, 300 steps: four full
   blocks and a partial one closed by ``finalize``) against the oracle's
   FieldDFT fed with float64 fields — FP64 and FP32 stepping;
-* the same run replayed through HIP graphs == eager launches, bit for bit.
+* the same run replayed through HIP graphs == eager launches, bit for bit;
+* the pipelined contraction (mgh_dft_slice): the slices of one period of
+  steps, driven by the device counter, add the previous period's block
+  contraction to the bins (numpy), including slices with no waves and split
+  frequency groups; the explicit-range flush (mgh_dft_blocks_range) too;
+* a run flushed by ``finalize`` in mid-period and then continued gives the
+  bins of an uninterrupted run (each sample contracted once).
 """
 
 import math
@@ -137,6 +143,94 @@ def test_blocks_per_launch_match_numpy(B, nsub, gpu):
                 assert err < 1e-5
 
 
+@pytest.mark.parametrize("S,nsub,fsplit,cad", [(70, 1, 1, 1), (70, 4, 2, 4), (3000, 2, 3, 8),
+                                               (5, 1, 1, 64), (3000, 1, 2, 2)])
+def test_pipelined_slices_match_numpy(S, nsub, fsplit, cad, gpu):
+    """One period of pipelined slices (a slice every `cad` steps, indices
+    from the device counter, phases from the period table) == the
+    contraction of the previous period's buffer; then a flush of part of
+    the other buffer through mgh_dft_blocks_range."""
+    rng = np.random.default_rng(5)
+    K, F, B = 2, 37, 64
+    s_h = (S // 2) // 16 * 16
+    S_pad, F_pad = -(-S // 16) * 16, 48
+    P_steps = B * nsub
+    dt = 1.1e-12
+    omega = 2 * np.pi * rng.uniform(1e9, 30e9, F)
+    b = np.arange(B)
+    Pc = np.zeros((F_pad, B), np.float32)
+    Ps = np.zeros((F_pad, B), np.float32)
+    Pc[:F] = np.cos(np.outer(omega, b * dt)) * dt
+    Ps[:F] = np.sin(np.outer(omega, b * dt)) * dt
+    X = rng.standard_normal((2, K, P_steps, S_pad)).astype(np.float32)
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev)
+    Pcd, Psd = torch.from_numpy(lane_order(Pc)).to(dev), torch.from_numpy(lane_order(Ps)).to(dev)
+    om = torch.from_numpy(omega).to(dev)
+    blk = torch.zeros((nsub * F, 4), dtype=torch.float64, device=dev)
+    base = rng.standard_normal((K, F, S, 2)) * (8 * dt)
+    bins = torch.from_numpy(base.copy()).to(dev)
+    lo = torch.zeros(bins.shape, dtype=torch.float32, device=dev)
+    n_start = 3
+    P = Pc[:F].astype(np.float64) + 1j * Ps[:F].astype(np.float64)
+    stream = int(torch.cuda.current_stream(dev).cuda_stream)
+    xbuf = K * P_steps * S_pad
+
+    def want_of(buf, n0, rows, waves=None):
+        w = np.zeros((K, F, S), complex)
+        cols = np.arange(S) if waves is None else np.arange(waves[0] * 16, min(waves[1] * 16, S))
+        for k in range(K):
+            for q in range(nsub):
+                rr = np.arange(q * B, min((q + 1) * B, rows))
+                if rr.size == 0:
+                    continue
+                t0 = (n0 + q * B) * dt + np.where(cols >= s_h, 0.5 * dt, 0.0)
+                rot = np.exp(1j * omega[:, None] * t0[None, :])
+                w[k][:, cols] += (P[:, : rr.size] @ X[buf, k, rr][:, cols].astype(np.float64)) * rot
+        return w
+
+    def got_minus(ref):
+        g = bins.cpu().numpy() + lo.cpu().numpy().astype(np.float64)
+        return (g[..., 0] - ref[..., 0]) + 1j * (g[..., 1] - ref[..., 1])
+
+    n_per = 8
+    tab = torch.zeros((n_per, nsub, F, 4), dtype=torch.float64, device=dev)
+    _lib.call("mgh_dft_phase_table", om.data_ptr(), F, dt, n_start, B, nsub, n_per, tab.data_ptr(),
+              stream)
+
+    def slice_at(n):
+        ctr = torch.tensor([n], dtype=torch.int64, device=dev)
+        _lib.call("mgh_dft_slice", K, Xd.data_ptr(), xbuf, S, S_pad, B, nsub, Pcd.data_ptr(),
+                  Psd.data_ptr(), F, F_pad, tab.data_ptr(), n_per, s_h, bins.data_ptr(),
+                  lo.data_ptr(), ctr.data_ptr(), n_start, cad, fsplit, stream)
+
+    # period 2 runs: its steps contract period 1, held in buffer 1
+    for r in range(P_steps):
+        if (r + 1) % cad == 0:
+            slice_at(n_start + 2 * P_steps + r)
+    torch.cuda.synchronize(dev)
+    err = np.abs(got_minus(base) - want_of(1, n_start + P_steps, P_steps)).max() / (
+        math.sqrt(P_steps) * dt)
+    print(f"pipelined S={S} nsub={nsub} fsplit={fsplit}: max err {err:.2e}", flush=True)
+    assert err < 1e-5
+    # period 0 contracts nothing
+    before = bins.cpu().numpy().copy()
+    slice_at(n_start + cad - 1)
+    torch.cuda.synchronize(dev)
+    assert np.array_equal(bins.cpu().numpy(), before)
+    # explicit flush of 23 rows of buffer 0 over a wave range
+    W = -(-S // 16)
+    wl, wh = W // 3, W
+    n0 = n_start + 7 * P_steps
+    _lib.call("mgh_dft_blocks_range", K, Xd[0].data_ptr(), S, S_pad, B, nsub, 23, wl, wh, n0,
+              Pcd.data_ptr(), Psd.data_ptr(), om.data_ptr(), F, F_pad, dt, blk.data_ptr(), s_h,
+              bins.data_ptr(), lo.data_ptr(), fsplit, stream)
+    torch.cuda.synchronize(dev)
+    err = np.abs(got_minus(before) - want_of(0, n0, 23, (wl, wh))).max() / (math.sqrt(23) * dt)
+    print(f"range flush: max err {err:.2e}", flush=True)
+    assert err < 1e-5
+
+
 def test_blocks_per_launch_checks(gpu):
     dev = torch.device("cuda:0")
     z = torch.zeros(64, dtype=torch.float64, device=dev)
@@ -167,8 +261,9 @@ def oracle_bins():
     return _oracle_dft(N_STEPS)
 
 
-def _hip_bins(precision, use_graphs=False, block=64, per_launch=None):
-    mon = MonitorFieldFrequency(FREQS, region=REGION, block=block, blocks_per_launch=per_launch)
+def _hip_bins(precision, use_graphs=False, block=64, per_launch=None, pipelined=True):
+    mon = MonitorFieldFrequency(FREQS, region=REGION, block=block, blocks_per_launch=per_launch,
+                                pipelined=pipelined)
     sol, _, _ = cases.c1_solver(n_steps=N_STEPS, precision=precision, use_graphs=use_graphs,
                                 monitors=[mon])
     sol.run()
@@ -190,14 +285,16 @@ def _errors(got, want):
     return out
 
 
-@pytest.mark.parametrize("precision,gate,block,per_launch", [
-    ("double", 1e-5, 64, None), ("single", 2e-5, 64, None), ("single", 2e-5, 64, 1),
-    ("single", 2e-5, 128, None), ("double", 1e-5, 256, None), ("single", 2e-5, 256, None),
-    ("single", 2e-5, 512, None)])
-def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, per_launch, gpu):
-    print(f"HIP C1 {precision} + region DFT (block {block}, {per_launch} per launch), "
-          f"{N_STEPS} steps", flush=True)
-    got = _hip_bins(precision, block=block, per_launch=per_launch)
+@pytest.mark.parametrize("precision,gate,block,per_launch,pipelined", [
+    ("double", 1e-5, 64, None, True), ("single", 2e-5, 64, None, True),
+    ("single", 2e-5, 64, 1, True), ("single", 2e-5, 128, None, True),
+    ("double", 1e-5, 256, None, True), ("single", 2e-5, 256, None, True),
+    ("single", 2e-5, 512, None, True), ("double", 1e-5, 64, None, False),
+    ("single", 2e-5, 64, 1, False), ("single", 2e-5, 512, None, False)])
+def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, per_launch, pipelined, gpu):
+    print(f"HIP C1 {precision} + region DFT (block {block}, {per_launch} per launch, "
+          f"pipelined {pipelined}), {N_STEPS} steps", flush=True)
+    got = _hip_bins(precision, block=block, per_launch=per_launch, pipelined=pipelined)
     assert set(got) == set(oracle_bins)
     for c in oracle_bins:
         assert got[c].shape == oracle_bins[c].shape
@@ -207,13 +304,31 @@ def test_region_monitor_vs_oracle(oracle_bins, precision, gate, block, per_launc
     assert max(errs.values()) < gate
 
 
-def test_region_monitor_graph_equals_eager(gpu):
-    # two 64-step blocks per launch: a 128-step graph (steps 128-255) inside the 300
-    print("HIP C1 single + region DFT, graphs vs eager", flush=True)
-    eager = _hip_bins("single", per_launch=2)
-    graph = _hip_bins("single", use_graphs=True, per_launch=2)
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_region_monitor_graph_equals_eager(pipelined, gpu):
+    # two 64-step blocks per launch: non-pipelined, a 128-step graph (steps
+    # 128-255) inside the 300; pipelined, one-step graphs from step 2 on
+    print(f"HIP C1 single + region DFT, graphs vs eager (pipelined {pipelined})", flush=True)
+    eager = _hip_bins("single", per_launch=2, pipelined=pipelined)
+    graph = _hip_bins("single", use_graphs=True, per_launch=2, pipelined=pipelined)
     for c in eager:
         assert np.array_equal(eager[c], graph[c]), c
+
+
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_finalize_mid_period_then_continue(oracle_bins, pipelined, gpu):
+    """finalize() at step 150 (mid-period: 256-step periods), then the run
+    goes on to 300: every sample is contracted once — the bins match the
+    oracle as closely as an uninterrupted run."""
+    print(f"HIP C1 double, finalize at 150 then continue (pipelined {pipelined})", flush=True)
+    mon = MonitorFieldFrequency(FREQS, region=REGION, block=64, pipelined=pipelined)
+    sol, _, _ = cases.c1_solver(n_steps=N_STEPS, precision="double", monitors=[mon])
+    sol.run(150)
+    mon.finalize(sol)
+    sol.run()
+    errs = _errors(mon.result(0), oracle_bins)
+    print("  " + " ".join(f"{c} {v:.2e}" for c, v in errs.items()), flush=True)
+    assert max(errs.values()) < 1e-5
 
 
 def _farfield_bins(structured, precision, steps=150):

@@ -209,8 +209,9 @@ def test_blocks_per_launch_rule():
 
 
 def test_graph_length_covers_bin_update_period():
-    """A captured graph spans whole bin-update periods of every monitor
-    (block x blocks per launch), so the contraction sits at a fixed place."""
+    """A captured graph spans whole launch cadences of every monitor — the
+    period (block x blocks per launch) of a non-pipelined one — and replays
+    only from steps congruent to its capture step."""
     from magnelio_amd.solver import FITTimeDomainSolver
 
     g = mm.GridLines(np.linspace(0, 1e-2, 9), np.linspace(0, 1e-2, 9), np.linspace(0, 1e-2, 9))
@@ -219,8 +220,119 @@ def test_graph_length_covers_bin_update_period():
     region = (slice(0, 2), slice(0, 2), slice(0, 2))
     for per_launch, graph_block, period, steps in ((None, 1, 256, 256), (1, 1, 64, 64),
                                                    (2, 1, 128, 128), (None, 100, 256, 6400)):
-        mon = MonitorFieldFrequency([1e9], region=region, block=64, blocks_per_launch=per_launch)
+        mon = MonitorFieldFrequency([1e9], region=region, block=64, blocks_per_launch=per_launch,
+                                    pipelined=False)
         sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions={}, total_time_steps=10, dt=dt,
                                   monitors=[mon], graph_block=graph_block)
-        assert mon.period == period
+        assert mon.period == period == mon.cadence
         assert sol.graph_steps == steps
+        assert sol._graph_ok(5, steps) and not sol._graph_ok(5, steps + 1)
+        sol._graph, sol._graph_len, sol._graph_start = object(), steps, 5  # a captured graph
+        assert sol._graph_ok(5 + period, steps) and not sol._graph_ok(6 + period, steps)
+
+
+@pytest.mark.parametrize("S,F,K,block,want", [(1336704, 1001, 1, 256, (4, 6)),
+                                              (55296, 2001, 6, 64, (8, 9)),
+                                              (900, 5, 1, 64, (256, 1)),
+                                              (1336704, 7, 2, 64, (32, 1))])
+def test_pipelined_slicing_choice(S, F, K, block, want):
+    """Slice cadence and frequency split of a pipelined monitor: >= 8
+    frequency tiles per work-group, the shortest power-of-two cadence whose
+    launch holds >= 1.5 rounds of resident work-groups (256 CUs x waves per
+    SIMD of the contraction variant) and fills >= 90 % of them; a small
+    monitor contracts once per 256-step period."""
+    mon = MonitorFieldFrequency(np.linspace(1e9, 2e9, F), region=(slice(0, 1),) * 3, block=block)
+    mon.S, mon.F_pad = S, -(-F // 16) * 16
+    c, fs = mon._slicing(K)
+    assert (c, fs) == want
+    tiles = mon.F_pad // 16
+    per = -(-tiles // fs)
+    assert fs == 1 or per >= mon.MIN_TILES
+    if c < mon.period:
+        slots = 256 * mon._occupancy()
+        wgs = -(-(-(-(-(-S // 16)) * c // mon.period)) // 4) * K * (-(-tiles // per))
+        assert wgs / (-(-wgs // slots) * slots) >= mon.MIN_FILL
+        assert wgs >= mon.MIN_ROUNDS * slots
+
+
+def _slices(W, P):
+    """Wave range of each pipelined slice r (csrc/mgh_dft.hip DftRange)."""
+    return [(r * W // P, (r + 1) * W // P) for r in range(P)]
+
+
+@pytest.mark.parametrize("W,P", [(1, 256), (7, 64), (83544, 64), (3456, 32), (255, 256),
+                                 (257, 256), (100, 128)])
+def test_pipelined_slices_cover_every_wave_once(W, P):
+    """The P slices of a period (here P: slices per period) cover waves
+    [0, W) exactly once, each within the launch's grid (ceil(ceil(W / P) /
+    4) work-groups of 4 waves)."""
+    hits = np.zeros(W, dtype=np.int64)
+    gx = -(-(-(-W // P)) // 4)
+    for lo, hi in _slices(W, P):
+        assert 0 <= hi - lo <= 4 * gx
+        hits[lo:hi] += 1
+    assert np.all(hits == 1)
+
+
+@pytest.mark.parametrize("steps", [1, 5, 63, 64, 65, 255, 256, 257, 300, 511, 512, 700])
+@pytest.mark.parametrize("cad", [1, 4, 64])
+def test_pipelined_flush_accounts_every_row_once(steps, cad):
+    """Replay of the pipelined schedule on the host: a slice every `cad`
+    steps over the previous period plus the flush at the run's end
+    (monitors.finalize) contract every (row, wave) of every period once."""
+    W, P = 37, 64
+    ns = P // cad
+    count = {}
+
+    def add(period, rows, lo, hi):
+        for r in range(rows):
+            for w in range(lo, hi):
+                count[(period, r, w)] = count.get((period, r, w), 0) + 1
+
+    for n in range(steps):
+        p, r = divmod(n, P)
+        if p >= 1 and (r + 1) % cad == 0:
+            lo, hi = _slices(W, ns)[r // cad]
+            add(p - 1, P, lo, hi)
+    p, r = divmod(steps - 1, P)
+    done = (r + 1) // cad
+    if p >= 1 and done < ns:
+        add(p - 1, P, done * W // ns, W)
+    add(p, r + 1, 0, W)
+    want = {(n // P, n % P, w) for n in range(steps) for w in range(W)}
+    assert set(count) == want
+    assert all(v == 1 for v in count.values())
+
+
+def test_stall_detector_on_synthetic_envelopes():
+    """The port-signal stall watchdog (fit_td.py:76-132): an exponential
+    ring-down that crosses the threshold before the cap is never stopped, a
+    plateau or a rising envelope is, once armed and the window is full, and
+    an algebraic decay whose extrapolated crossing lies past the cap is."""
+    from magnelio_amd.solver import SignalStallDetector
+
+    def first_stop(env_db, cap, window=10, thr=-60.0):
+        det = SignalStallDetector(40.0, window, cap)
+        for k, db in enumerate(env_db):
+            if db <= thr:
+                return None  # the criterion itself fires first
+            if det.observe(100 * k, db, thr):
+                return 100 * k
+        return None
+
+    steps = np.arange(200)
+    assert first_stop(-0.5 * steps, cap=20000) is None  # 0.5 dB per check: crosses at k = 120
+    flat = np.concatenate([-0.5 * steps[:90], np.full(110, -45.0)])
+    # armed at k = 81 (-40.5 dB), window full at k = 90; the fitted slope
+    # flattens as the plateau fills the window and the projected crossing
+    # passes the cap before the window is wholly flat (k = 99)
+    stop = first_stop(flat, cap=20000)
+    assert stop is not None and 100 * 90 < stop <= 100 * 99
+    algebraic = -40.0 - 20 * np.log10(1 + steps / 50.0)
+    stop = first_stop(algebraic, cap=20000)
+    assert stop is not None and stop < 100 * 200
+    # the same decay with a cap beyond its projected crossing is left alone
+    assert first_stop(algebraic[:40], cap=10 ** 9) is None
+    # below the arming level nothing is collected
+    det = SignalStallDetector(40.0, 10, 100)
+    assert not any(det.observe(k, -30.0, -60.0) for k in range(50))

@@ -83,7 +83,7 @@ def test_tables_match_oracle_injection(direction, polarization, dtype, reach_end
     solver = SimpleNamespace(
         mesh=SimpleNamespace(grid=grid), dims=dims, _np_dtype=dtype,
         e=torch.zeros((1, yee.n_e(*dims))), _coeffs=SimpleNamespace(beta_e=be, beta_h=bh),
-        dt=dt, total_time_steps=n_steps, nsets=1)
+        dt=dt, total_time_steps=n_steps, horizon=n_steps, nsets=1)
     src.attach(solver)
     # fields of the size of the corrections, so every addition shows
     e = (rng.standard_normal(yee.n_e(*dims)) * 1e-8).astype(dtype)
