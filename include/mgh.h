@@ -200,6 +200,18 @@ int mgh_weighted_sq_sum(int dtype, long long n, int nsets, const void* x, long l
                         const void* w, double* out, int accumulate, double scale, double* work,
                         long long work_len, void* stream);
 
+/* mgh_weighted_sq_sum_noalloc: mgh_weighted_sq_sum without a workspace
+ * argument and without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a static device array of the library (512 KiB per
+ * GPU, held until the process ends), so nsets must be <= 64 (error
+ * otherwise).  All calls on one GPU share it: calls that overlap on one GPU
+ * race. */
+int mgh_weighted_sq_sum_noalloc(int dtype, long long n, int nsets, const void* x,
+                                long long set_stride, const void* w, double* out, int accumulate,
+                                double scale, void* stream);
+
 /* ---- (a12) lumped RLC elements and discrete ports -----------------------
  * Replaces LumpedElementOperator.update_e (src/magnelio/ports/_lumped/
  * operator.py:134-166) with SeriesRLC / ParallelRLC companions

@@ -93,6 +93,11 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_ll, _c_int, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int, _c_double,
          _c_void_p, _c_ll, _c_void_p],
     ),
+    "mgh_weighted_sq_sum_noalloc": (
+        _c_int,
+        [_c_int, _c_ll, _c_int, _c_void_p, _c_ll, _c_void_p, _c_void_p, _c_int, _c_double,
+         _c_void_p],
+    ),
     "mgh_lumped_step": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_int, _c_void_p, _c_void_p],

@@ -9,6 +9,8 @@
 // deterministic: fixed per-block partial sums, then one ordered final pass.
 #include "mgh_common.h"
 
+#include <atomic>
+
 namespace {
 
 constexpr int RB = 256;           // threads per block
@@ -53,6 +55,61 @@ __global__ __launch_bounds__(RB) void k_wsq_final(const double* __restrict__ par
   if (threadIdx.x == 0) out[s] = accumulate ? out[s] + scale * red[0] : scale * red[0];
 }
 
+// Launch the two reduction passes with the partial sums in work.
+int wsq_launch(int dtype, long long n, int nsets, const void* x, long long set_stride,
+               const void* w, double* out, int accumulate, double scale, double* work,
+               hipStream_t st) {
+  int nblk = (int)((n + RB - 1) / RB);
+  if (nblk > RMAX_BLOCKS) nblk = RMAX_BLOCKS;
+  if (nblk < 1) nblk = 1;
+  dim3 g1(nblk, nsets, 1);
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_wsq_partial<float>, g1, dim3(RB), 0, st, (const float*)x, set_stride,
+                       (const float*)w, n, work);
+  else
+    hipLaunchKernelGGL(k_wsq_partial<double>, g1, dim3(RB), 0, st, (const double*)x, set_stride,
+                       (const double*)w, n, work);
+  MGH_CHECK_LAUNCH("k_wsq_partial");
+  hipLaunchKernelGGL(k_wsq_final, dim3(nsets), dim3(RB), 0, st, (const double*)work, nblk, out,
+                     accumulate, scale);
+  MGH_CHECK_LAUNCH("k_wsq_final");
+  return MGH_OK;
+}
+
+constexpr int NOALLOC_MAX_SETS = 64;
+
+}  // namespace
+
+// Scratch of the allocation-free reduction: the per-block partial sums of
+// up to NOALLOC_MAX_SETS field sets (512 KiB).  Every GPU the library runs
+// on holds its own copy until the process ends.  In host code the name is
+// only a stand-in: kernels get the device address that hipGetSymbolAddress
+// returns for the current device, resolved once per device.
+static __device__ __align__(256) double scratch[RMAX_BLOCKS * NOALLOC_MAX_SETS];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::atomic<double*> g_scratch[MAX_DEVICES];
+
+int scratch_on_current_device(double** out) {
+  int dev = 0;
+  MGH_CHECK_HIP(hipGetDevice(&dev));
+  MGH_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "device %d beyond the %d this library tracks", dev,
+              MAX_DEVICES);
+  double* p = g_scratch[dev].load(std::memory_order_acquire);
+  if (p == nullptr) {
+    void* q = nullptr;
+    MGH_CHECK_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    MGH_REQUIRE(q != nullptr, "no device address for the reduction scratch on device %d", dev);
+    p = (double*)q;
+    // two threads resolving the same device store the same address
+    g_scratch[dev].store(p, std::memory_order_release);
+  }
+  *out = p;
+  return MGH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -73,22 +130,29 @@ int mgh_weighted_sq_sum(int dtype, long long n, int nsets, const void* x, long l
   MGH_REQUIRE(x && w && out && work, "null device pointer");
   MGH_REQUIRE(work_len >= (long long)RMAX_BLOCKS * nsets, "workspace too small (%lld < %lld)",
               work_len, (long long)RMAX_BLOCKS * nsets);
-  hipStream_t st = (hipStream_t)stream;
-  int nblk = (int)((n + RB - 1) / RB);
-  if (nblk > RMAX_BLOCKS) nblk = RMAX_BLOCKS;
-  if (nblk < 1) nblk = 1;
-  dim3 g1(nblk, nsets, 1);
-  if (dtype == 4)
-    hipLaunchKernelGGL(k_wsq_partial<float>, g1, dim3(RB), 0, st, (const float*)x, set_stride,
-                       (const float*)w, n, work);
-  else
-    hipLaunchKernelGGL(k_wsq_partial<double>, g1, dim3(RB), 0, st, (const double*)x, set_stride,
-                       (const double*)w, n, work);
-  MGH_CHECK_LAUNCH("k_wsq_partial");
-  hipLaunchKernelGGL(k_wsq_final, dim3(nsets), dim3(RB), 0, st, (const double*)work, nblk, out,
-                     accumulate, scale);
-  MGH_CHECK_LAUNCH("k_wsq_final");
-  return MGH_OK;
+  return wsq_launch(dtype, n, nsets, x, set_stride, w, out, accumulate, scale, work,
+                    (hipStream_t)stream);
+}
+
+// mgh_weighted_sq_sum with its partial sums in the library's static scratch
+// (at most NOALLOC_MAX_SETS sets): no workspace argument, no allocation.
+// Every call on one GPU shares that GPU's scratch; calls that overlap on one
+// GPU (different streams or threads) race.  Each partial is written before
+// it is read, so nothing needs clearing between calls.
+int mgh_weighted_sq_sum_noalloc(int dtype, long long n, int nsets, const void* x,
+                                long long set_stride, const void* w, double* out, int accumulate,
+                                double scale, void* stream) {
+  MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8, got %d", dtype);
+  MGH_REQUIRE(n >= 0 && nsets >= 1, "bad sizes n=%lld nsets=%d", n, nsets);
+  MGH_REQUIRE(nsets <= NOALLOC_MAX_SETS,
+              "%d field sets exceed the static scratch of the allocation-free reduction (%d)",
+              nsets, NOALLOC_MAX_SETS);
+  MGH_REQUIRE(x && w && out, "null device pointer");
+  double* work = nullptr;
+  const int rc = scratch_on_current_device(&work);
+  if (rc != MGH_OK) return rc;
+  return wsq_launch(dtype, n, nsets, x, set_stride, w, out, accumulate, scale, work,
+                    (hipStream_t)stream);
 }
 
 }  // extern "C"
