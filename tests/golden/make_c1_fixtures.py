@@ -1,0 +1,148 @@
+"""Write golden vectors of whole FIT-TD runs from the reference's own solver.
+
+Run in the build container only (the reference never travels to the GPU
+machine; only the .npz files this writes do):
+
+    python tests/golden/make_c1_fixtures.py /root/reference/src
+
+The reference's NumPy tier (``FITTimeDomainSolver``, src/magnelio/solver/
+fit_td.py:682-1246, with ``PortOperatorModal`` ports, ``PortSignalRecorder``
+and ``compute_s_parameters``, post/modal_sparameters.py:335) runs two reduced
+plain cases for 300 steps:
+
+* ``c1_guide``:  the C1 WR-90 guide on a 12 x 8 x 20 uniform grid (1 mm in
+  z), air, PEC x/y walls, 3-mode numerical ports on z_min / z_max;
+* ``c1_loaded``: the same guide on a seeded graded cross-section (adjacent
+  spacing ratio <= 1.3) with a lossy dielectric block (eps_r 4, sigma
+  0.05 S/m) and a PEC post in its middle — graded dual masses, material
+  averaging, the PEC edge OR rule and the port-plane flattening all enter;
+
+port 1 mode 0 excited with the modulated Gaussian from the TE10 cut-off to
+25 GHz, dt = courant_dt(grid, "normal"), float64 (and float32 for the
+guide).  Stored: the ports' numerical mode profiles and cut-offs, alpha/beta
+(E, H) of the solver, e/h after steps 1, 10 and 300 (reference flat layout),
+the recorded V/I of every channel and the S-parameters of the excited column
+on 41 frequencies.
+"""
+
+from __future__ import annotations
+
+import math
+import sys
+import warnings
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+A, B = 22.86e-3, 10.16e-3
+F_MAX = 25e9
+SNAPS = (1, 10, 300)
+
+
+def graded_axis(n, length, seed):
+    rng = np.random.default_rng(seed)
+    w = np.exp(np.concatenate([[0.0], np.cumsum(rng.uniform(np.log(1 / 1.3), np.log(1.3), n - 1))]))
+    w = w / w.sum() * length
+    return np.concatenate([[0.0], np.cumsum(w)])
+
+
+def cases():
+    z = np.linspace(0.0, 20e-3, 21)
+    return {
+        "c1_guide": dict(axes=(np.linspace(0.0, A, 13), np.linspace(0.0, B, 9), z), loaded=False,
+                         precisions=("double", "single")),
+        "c1_loaded": dict(axes=(graded_axis(12, A, 21), graded_axis(8, B, 22), z), loaded=True,
+                          precisions=("double",)),
+    }
+
+
+# the loaded case: dielectric block and PEC post (x0, y0, z0, x1, y1, z1)
+BLOCK = (0.25 * A, 0.0, 8e-3, 0.75 * A, 0.6 * B, 12e-3)
+POST = (0.45 * A, 0.0, 9e-3, 0.55 * A, B, 11e-3)
+BLOCK_EPS, BLOCK_SIGMA = 4.0, 0.05
+
+
+def main(ref_src: str) -> None:
+    sys.path.insert(0, ref_src)
+    from magnelio.materials.material import Material
+    from magnelio.mesh.grid import GridLines
+    from magnelio.mesh.mesher import Mesh
+    from magnelio._operators.material_matrices import build_M_eps, build_M_mu
+    from magnelio.ports._modal import BoxFace, PortSpecNumerical, build_modal_port
+    from magnelio.ports.recorder import PortSignalRecorder
+    from magnelio.post.modal_sparameters import compute_s_parameters
+    from magnelio.signals.signal_1d import Signal1D
+    from magnelio.signals.waveforms import modulated_gaussian
+    from magnelio.solver.fit_td import FITTimeDomainSolver
+    from magnelio.solver.stability import courant_dt
+
+    for name, cfg in cases().items():
+        x, y, z = cfg["axes"]
+        grid = GridLines(x=x, y=y, z=z)
+        regions = []
+        if cfg["loaded"]:
+            regions = [(Material(name="block", epsilon=(BLOCK_EPS,) * 3, sigma=(BLOCK_SIGMA,) * 3), BLOCK),
+                       (Material.pec(), POST)]
+        mesh = Mesh.from_grid(grid, regions=regions).with_boundary_conditions(
+            {"xmin": "PEC", "xmax": "PEC", "ymin": "PEC", "ymax": "PEC", "zmin": "PMC", "zmax": "PMC"})
+        dt = courant_dt(grid, "normal")
+        out = {"x": x, "y": y, "z": z, "dt": np.float64(dt), "snaps": np.array(SNAPS)}
+        for prec in cfg["precisions"]:
+            m_eps, m_mu = build_M_eps(mesh), build_M_mu(mesh)
+            ops = [build_modal_port(PortSpecNumerical(name=lbl, plane=face, n_modes=3), mesh, m_eps,
+                                    m_mu, dt=dt, f_calc=F_MAX)
+                   for lbl, face in (("port1", BoxFace.Z_MIN), ("port2", BoxFace.Z_MAX))]
+            f_c = ops[0].discrete_modes[0].mode.omega_c / (2 * math.pi)
+            if prec == "double":  # the ports' numerical modes (the oracle takes them as given)
+                for op in ops:
+                    for m, dm in enumerate(op.discrete_modes):
+                        out[f"{op.name}_e_u{m}"] = np.asarray(dm.e_u_profile, float)
+                        out[f"{op.name}_e_v{m}"] = np.asarray(dm.e_v_profile, float)
+                        out[f"{op.name}_omega_c{m}"] = np.float64(dm.mode.omega_c)
+
+            def wf(t, lo=f_c):
+                return modulated_gaussian(t, F_MAX, lo)
+
+            ops[0].set_excitation(0, wf)
+            rec = PortSignalRecorder(dt=dt, ports=ops)
+            sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions={}, ports=ops, recorder=rec,
+                                      total_time_steps=SNAPS[0], dt=dt, verbose=False,
+                                      backend="numpy", precision=prec,
+                                      energy_stop_db=None, port_signal_stop_db=None)
+            p = f"{prec}_"
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                for n in SNAPS:
+                    sol.total_time_steps = n
+                    sol.run()
+                    out[f"{p}e{n}"] = np.asarray(sol._fields.e_flat, dtype=np.float64).copy()
+                    out[f"{p}h{n}"] = np.asarray(sol._fields.h_flat, dtype=np.float64).copy()
+            out[p + "alpha_e"] = np.asarray(sol._alpha_E, np.float64)
+            out[p + "beta_e"] = np.asarray(sol._beta_E, np.float64)
+            out[p + "alpha_h"] = np.asarray(sol._alpha_H, np.float64)
+            out[p + "beta_h"] = np.asarray(sol._beta_H, np.float64)
+            sigs = rec.finalize(n_steps_actual=SNAPS[-1])
+            for (lbl, m), (V, I) in sigs.items():
+                out[f"{p}V_{lbl}_{m}"] = np.asarray(V.values, float)
+                out[f"{p}I_{lbl}_{m}"] = np.asarray(I.values, float)
+            f_axis = np.linspace(1.1 * f_c, 24e9, 41)
+            t_axis = np.arange(SNAPS[-1]) * dt
+            ref = Signal1D(t=t_axis, values=np.array([wf(float(t)) for t in t_axis]), dt=dt,
+                           label="excitation")
+            s = compute_s_parameters(
+                recorder_signals=sigs,
+                port_modes={op.name: [dm.mode for dm in op.discrete_modes] for op in ops},
+                excited=("port1", 0), reference_signal=ref, f_axis=f_axis,
+                port_normal_dx={op.name: op.plane.normal_dx for op in ops},
+                port_line_params={(op.name, m): lp for op in ops
+                                  for m, lp in op.dtbc_line_params.items()})
+            out["f_axis"] = f_axis
+            for (lbl, m), v in s.items():
+                out[f"{p}S_{lbl}_{m}"] = np.asarray(v, complex)
+        np.savez_compressed(HERE / f"{name}.npz", **out)
+        print(f"wrote {name}.npz ({len(out)} arrays)")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src")

@@ -1,0 +1,88 @@
+"""Write the golden vectors of the modal-port calibration from the reference.
+
+Run in the build container only (the reference never travels to the GPU
+machine; only the .npz files this writes do):
+
+    python tests/golden/make_port_fixtures.py /root/reference/src
+
+Pins what no closed form fixes: the numerical V/I calibration
+(``record_scale``, ``source_scale``, ``state_scale``;
+src/magnelio/ports/_modal/operator.py:778-941), the DTBC chain parameters
+(r, q, z0; ``_chain_params`` / ``_chain_z0``, operator.py:671-776) and the
+M_eps-normalised numerical mode profiles (``build_modal_port``,
+ports/_modal/factory.py:1367) on
+
+* ``wr90``:   a WR-90 guide on a uniform 12 x 6 cross-section, 20 cells long;
+* ``graded``: the same guide on a seeded non-uniform cross-section (adjacent
+  spacing ratio <= 1.3), uniform along z;
+
+both with PEC x/y walls and 3-mode numerical ports (lowest cut-offs) on
+z_min and z_max, dt = courant_dt(grid, "normal"), f_calc = 25 GHz.
+"""
+
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+A, B, L = 22.86e-3, 10.16e-3, 20e-3
+
+
+def graded_axis(n, length, seed):
+    rng = np.random.default_rng(seed)
+    w = np.exp(np.concatenate([[0.0], np.cumsum(rng.uniform(np.log(1 / 1.3), np.log(1.3), n - 1))]))
+    w = w / w.sum() * length
+    return np.concatenate([[0.0], np.cumsum(w)])
+
+
+def cases():
+    return {
+        "wr90": (np.linspace(0.0, A, 13), np.linspace(0.0, B, 7), np.linspace(0.0, L, 21)),
+        "graded": (graded_axis(12, A, 11), graded_axis(6, B, 12), np.linspace(0.0, L, 21)),
+    }
+
+
+def main(ref_src: str) -> None:
+    sys.path.insert(0, ref_src)
+    from magnelio._operators.material_matrices import build_M_eps, build_M_mu
+    from magnelio.mesh.grid import GridLines
+    from magnelio.mesh.mesher import Mesh
+    from magnelio.ports._modal import BoxFace, PortSpecNumerical, build_modal_port
+    from magnelio.solver.stability import courant_dt
+
+    for name, (x, y, z) in cases().items():
+        grid = GridLines(x=x, y=y, z=z)
+        mesh = Mesh.from_grid(grid).with_boundary_conditions(
+            {"xmin": "PEC", "xmax": "PEC", "ymin": "PEC", "ymax": "PEC", "zmin": "PMC", "zmax": "PMC"})
+        dt = courant_dt(grid, "normal")
+        m_eps, m_mu = build_M_eps(mesh), build_M_mu(mesh)
+        out = {"x": x, "y": y, "z": z, "dt": np.float64(dt), "f_calc": np.float64(25e9)}
+        for label, face in (("port1", BoxFace.Z_MIN), ("port2", BoxFace.Z_MAX)):
+            op = build_modal_port(PortSpecNumerical(name=label, plane=face, n_modes=3), mesh, m_eps,
+                                  m_mu, dt=dt, f_calc=25e9)
+            M = len(op.discrete_modes)
+            lp = op.dtbc_line_params
+            p = f"{label}_"
+            out[p + "record_scale"] = np.asarray(op.record_scale, float)
+            out[p + "source_scale"] = np.asarray(op._source_scale, float)
+            out[p + "state_scale"] = np.asarray(op.state_scale, float)
+            out[p + "omega_c"] = np.array([dm.mode.omega_c for dm in op.discrete_modes])
+            out[p + "dtbc"] = np.array([[lp[m][0], lp[m][1], np.nan if lp[m][2] is None else lp[m][2]]
+                                        if m in lp else [np.nan] * 3 for m in range(M)])
+            out[p + "kinds"] = np.array(op.termination_kinds)
+            for m, dm in enumerate(op.discrete_modes):
+                for comp in ("e_u", "e_v", "h_u", "h_v"):
+                    out[f"{p}{comp}{m}"] = np.asarray(getattr(dm, comp + "_profile"), float)
+            pl = op.plane
+            out[p + "e_u_indices"] = np.asarray(pl.e_u_indices, np.int64)
+            out[p + "e_v_indices"] = np.asarray(pl.e_v_indices, np.int64)
+        np.savez_compressed(HERE / f"ports_{name}.npz", **out)
+        print(f"wrote ports_{name}.npz: " + ", ".join(f"{k}" for k in out if k.startswith("port1_")
+                                                       and not k[-1].isdigit()))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src")

@@ -1,0 +1,103 @@
+"""Whole-run golden vectors of the reference's own solver (CPU side).
+
+``tests/golden/c1_{guide,loaded}.npz`` hold 300 steps of the reference's
+NumPy tier on reduced C1 cases (tests/golden/make_c1_fixtures.py): the
+uniform WR-90 guide, and a graded cross-section with a lossy dielectric
+block and a PEC post.  They pin the oracle's whole composition — graded dual
+masses, material averaging, the PEC edge OR rule, port-plane flattening,
+the curl order, the modal DTBC port step and calibration, the recorder and
+the S-parameter post — and the product's host coefficient build.
+
+* product alpha/beta (host build) == the reference's, bit for bit;
+* oracle alpha/beta (oracle.mesh) == the reference's, bit for bit;
+* oracle float64 fields after 1, 10, 300 steps (1e-12; measured <= 2.3e-13,
+  the port dot products associate differently), every recorded V/I and the
+  S-parameters on 41 frequencies (1e-12; measured 2e-15) against the
+  reference's;
+* oracle float32 (the reference's order) equal to the reference's float32
+  run of the guide bit for bit, every step snapshot.
+"""
+
+import numpy as np
+import pytest
+
+import cases
+from magnelio_amd import mesh as pm
+from oracle import sparams as osp
+
+CASES = ("c1_guide", "c1_loaded")
+
+
+def _product_coefficients(name):
+    mesh = cases.golden_c1_mesh(name)
+    dims = mesh.dims
+    f = cases.golden_c1(name)
+    dt = pm.courant_dt(mesh.grid, "normal")
+    assert dt == float(f["dt"])
+    me, mu = pm.build_m_eps(mesh), pm.build_m_mu(mesh)
+    pec = mesh.pec_mask_edges
+    for face in ("zmin", "zmax"):
+        me = pm.flatten_port_mass(me, dims, face)
+        mu = pm.flatten_port_mu(mu, dims, face)
+        pec = pm.flatten_port_pec(pec, dims, face)
+    return pm.update_coefficients(me, mu, pm.build_m_sigma(mesh), pm.build_m_sigma_m(mesh), dt,
+                                  pm.pec_flat(pec, dims))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_product_coefficients_equal_reference(name):
+    f = cases.golden_c1(name)
+    co = _product_coefficients(name)
+    for k in ("alpha_e", "beta_e", "alpha_h", "beta_h"):
+        assert np.array_equal(getattr(co, k), f["double_" + k]), k
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_coefficients_equal_reference(name):
+    f = cases.golden_c1(name)
+    orc, _, _, _ = cases.golden_c1_oracle(name)
+    for k, v in (("alpha_e", orc.ae), ("beta_e", orc.be), ("alpha_h", orc.ah), ("beta_h", orc.bh)):
+        assert np.array_equal(v, f["double_" + k]), k
+
+
+def _sparams(ports, dt, f_axis):
+    sig = {(p.name, m): (np.array(p.V[m]), np.array(p.I[m])) for p in ports for m in range(p.n_modes)}
+    lines = {(p.name, m): p.line[m] for p in ports for m in range(p.n_modes)}
+    return osp.s_params(sig, lines, ("port1", 0), f_axis, dt)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_run_matches_reference(name):
+    f = cases.golden_c1(name)
+    orc, ports, dt, _ = cases.golden_c1_oracle(name)
+    for n in f["snaps"]:
+        orc.run(int(n) - orc.n)
+        for comp, got in (("e", orc.e), ("h", orc.h)):
+            want = f[f"double_{comp}{n}"]
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            assert err <= 1e-12, (n, comp, err)
+    # channels that carry no mode stay at round-off: gate every channel
+    # against the largest V (I) of the run
+    vmax = max(np.abs(f[k]).max() for k in f if k.startswith("double_V_"))
+    imax = max(np.abs(f[k]).max() for k in f if k.startswith("double_I_"))
+    for p in ports:
+        for m in range(3):
+            np.testing.assert_allclose(p.V[m], f[f"double_V_{p.name}_{m}"], rtol=0, atol=1e-10 * vmax)
+            np.testing.assert_allclose(p.I[m], f[f"double_I_{p.name}_{m}"], rtol=0, atol=1e-10 * imax)
+    S = _sparams(ports, dt, f["f_axis"])
+    for (lbl, m), s in S.items():
+        want = f[f"double_S_{lbl}_{m}"]
+        ok = np.isfinite(want)
+        assert np.array_equal(ok, np.isfinite(s))
+        assert np.max(np.abs(s[ok] - want[ok])) <= 1e-12, (lbl, m)
+
+
+def test_oracle_float32_matches_reference_float32():
+    f = cases.golden_c1("c1_guide")
+    orc, _, _, _ = cases.golden_c1_oracle("c1_guide", np.float32)
+    for n in f["snaps"]:
+        orc.run(int(n) - orc.n)
+        for comp, got in (("e", orc.e), ("h", orc.h)):
+            want = f[f"single_{comp}{n}"]
+            # same operation order in float32: bit for bit
+            assert np.array_equal(got.astype(np.float64), want), (n, comp)

@@ -1,0 +1,63 @@
+"""The HIP engine against the reference's own runs (golden vectors).
+
+tests/golden/c1_{guide,loaded}.npz: 300 steps of the reference's NumPy tier
+on the uniform WR-90 guide and on a graded cross-section with a lossy
+dielectric block and a PEC post (tests/golden/make_c1_fixtures.py).
+
+* HIP float64: fields after 1, 10 and 300 steps within 1e-10, V/I within
+  1e-9 of the run's largest, S-parameters within 1e-9;
+* HIP float32 on the guide: fields within 1e-6 of the reference's float32
+  run (the port reductions associate differently in float64), S within
+  1e-4 of the reference's float64 S (the single-vs-double gate).
+"""
+
+import numpy as np
+import pytest
+
+import cases
+from magnelio_amd.signals import s_parameters
+
+pytestmark = pytest.mark.gpu
+
+
+def _hip_sparams(sol, ports, dt, f_axis):
+    sig = sol.recorded(0)
+    modes = {p.name: p.modes for p in ports}
+    lines = {(p.name, m): lp for p in ports for m, lp in p.dtbc_line_params.items()}
+    ndx = {p.name: p.plane.normal_dx for p in ports}
+    return s_parameters(sig, modes, ("port1", 0), f_axis, dt, port_normal_dx=ndx, port_line_params=lines)
+
+
+@pytest.mark.parametrize("name,precision", [("c1_guide", "double"), ("c1_loaded", "double"),
+                                            ("c1_guide", "single")])
+def test_hip_run_matches_reference(name, precision, gpu):
+    print(f"HIP golden {name} {precision}", flush=True)
+    f = cases.golden_c1(name)
+    sol, ports, dt = cases.golden_c1_solver(name, precision)
+    assert dt == float(f["dt"])
+    p = precision + "_"
+    gate = 1e-10 if precision == "double" else 1e-6
+    for n in f["snaps"]:
+        sol.run(int(n) - sol.n_completed)
+        e, h = sol.fields(0)
+        for comp, got in (("e", e), ("h", h)):
+            want = f[f"{p}{comp}{n}"]
+            err = float(np.linalg.norm(got.astype(np.float64) - want) / np.linalg.norm(want))
+            print(f"  step {n} {comp}: rel {err:.2e}", flush=True)
+            assert err <= gate, (n, comp)
+    if precision == "double":
+        sig = sol.recorded(0)
+        vmax = max(np.abs(f[k]).max() for k in f if k.startswith("double_V_"))
+        imax = max(np.abs(f[k]).max() for k in f if k.startswith("double_I_"))
+        for (lbl, m), (V, I) in sig.items():
+            assert np.max(np.abs(V - f[f"double_V_{lbl}_{m}"])) <= 1e-9 * vmax
+            assert np.max(np.abs(I - f[f"double_I_{lbl}_{m}"])) <= 1e-9 * imax
+    S = _hip_sparams(sol, ports, dt, f["f_axis"])
+    sgate = 1e-9 if precision == "double" else 1e-4
+    dmax = 0.0
+    for (lbl, m), s in S.items():
+        want = f[f"double_S_{lbl}_{m}"]
+        ok = np.isfinite(want) & np.isfinite(s)
+        dmax = max(dmax, float(np.max(np.abs(s[ok] - want[ok]))))
+    print(f"  max |dS| vs the reference's float64 S: {dmax:.2e}", flush=True)
+    assert dmax <= sgate
