@@ -193,3 +193,46 @@ def golden_c1_solver(name, precision="double", n_steps=300):
                               total_time_steps=n_steps, dt=dt, precision=precision, nsets=1,
                               excitations=[ChannelExcitation("port1", 0, wf)])
     return sol, ports, dt
+
+
+# ---------------------------------------------------------------------------
+# CPML box with TF/SF and DFT monitors (tests/golden/box_tfsf_dft.npz,
+# written by tests/golden/make_box_fixtures.py)
+
+def golden_box():
+    import os  # noqa: PLC0415
+    import sys  # noqa: PLC0415
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    if here not in sys.path:
+        sys.path.insert(0, here)
+    import make_box_fixtures as mk  # noqa: PLC0415
+
+    return dict(np.load(os.path.join(here, "box_tfsf_dft.npz"))), mk
+
+
+def golden_box_oracle(dtype=np.float64):
+    """Oracle of the golden box: (Oracle, region monitor, far-field monitor)."""
+    from oracle import cpml as oc  # noqa: PLC0415
+    from oracle import monitors as omon  # noqa: PLC0415
+    from oracle import tfsf as ot  # noqa: PLC0415
+
+    f, mk = golden_box()
+    x, y, z = f["x"], f["y"], f["z"]
+    dims = (x.size - 1, y.size - 1, z.size - 1)
+    mid = om.fill_aabb(x, y, z, [(i + 1, box) for i, (_, box) in enumerate(mk.BRICKS)])
+    eps = [[1.0] + [e for e, _ in mk.BRICKS]] * 3
+    me = om.mass_e(x, y, z, mid, eps, om.EPS0)
+    mmu = om.mass_h(x, y, z, mid, [[1.0] * (len(mk.BRICKS) + 1)] * 3, om.MU0)
+    dt = float(f["dt"])
+    ae, be = om.alpha_beta(me, np.zeros_like(me), dt, np.zeros(me.size, bool))
+    ah, bh = om.alpha_beta(mmu, np.zeros_like(mmu), dt)
+    faces = [oc.CPMLFace(fc, x, y, z, dt, dtype, n_layers=mk.LAYERS) for fc in mk.FACES]
+    src = ot.PlaneWave(x, y, z, be, bh, dt, direction=(0, 0, 1), polarization=(1, 0, 0),
+                       box=ot.snap_box(x, y, z, mk.TFSF), f_max=20e9, dtype=dtype)
+    rs = f["region_slices"]
+    region = omon.FieldDFT(mk.FREQS, x, y, z, tuple(slice(int(a), int(b)) for a, b in rs))
+    far = omon.FarFieldDFT(mk.FREQS, x, y, z, mk.FACES, {fc: mk.LAYERS + 2 for fc in mk.FACES})
+    orc = ofd.Oracle(dims, ae, be, ah, bh, dt, cpml=faces, sources=[src], monitors=[region, far],
+                     dtype=dtype)
+    return orc, region, far
