@@ -73,6 +73,25 @@ C3_SIZE = (22.86e-3, 10.16e-3, 60e-3)
 C3_FMAX = 25e9
 
 
+def c3_geometry(dims=(192, 96, 288), seed=3, size=C3_SIZE):
+    """Seeded layout of the batched workload: two eps_r = 30 posts (axis
+    along y) at z = 0.3 L and 0.7 L, [(xc, zc, r)], and the PEC iris layer
+    (z in [zi, zi + dz)) with its rectangular window (wx, wy, ww, wh)."""
+    rng = np.random.default_rng(seed)
+    W, H, L = size
+    posts = []
+    for zc in (0.3 * L, 0.7 * L):
+        xc = W / 2 + rng.uniform(-0.15, 0.15) * W
+        r = rng.uniform(0.08, 0.12) * W
+        posts.append((float(xc), float(zc), float(r)))
+    ww, wh = rng.uniform(0.3, 0.6) * W, rng.uniform(0.4, 0.8) * H
+    wx = rng.uniform(0.0, W - ww)
+    wy = rng.uniform(0.0, H - wh)
+    dz = L / dims[2]
+    zi = round(0.5 * dims[2]) * dz
+    return posts, (float(zi), float(dz)), (float(wx), float(wy), float(ww), float(wh))
+
+
 def c3_case(dims=(192, 96, 288), seed=3, size=C3_SIZE, n_modes=3, f_max=C3_FMAX):
     """Batched multi-excitation workload: a WR-90 guide with two seeded eps_r = 30
     dielectric posts (axis along y) and a PEC iris wall at mid-length with a
@@ -81,21 +100,13 @@ def c3_case(dims=(192, 96, 288), seed=3, size=C3_SIZE, n_modes=3, f_max=C3_FMAX)
     from magnelio_amd import mesh as mm
     from magnelio_amd import ports as pp
 
-    rng = np.random.default_rng(seed)
     W, H, L = size
     axes = [np.linspace(0.0, s, n + 1) for s, n in zip(size, dims)]
     grid = mm.GridLines(*axes)
-    dz = L / dims[2]
+    posts, (zi, dz), (wx, wy, ww, wh) = c3_geometry(dims, seed, size)
     diel = mm.Material.dielectric(30.0, name="resonator")
-    shapes = []
-    for zc in (0.3 * L, 0.7 * L):
-        xc = W / 2 + rng.uniform(-0.15, 0.15) * W
-        r = rng.uniform(0.08, 0.12) * W
-        shapes.append((diel, lambda X, Y, Z, xc=xc, zc=zc, r=r: (X - xc) ** 2 + (Z - zc) ** 2 < r * r))
-    ww, wh = rng.uniform(0.3, 0.6) * W, rng.uniform(0.4, 0.8) * H
-    wx = rng.uniform(0.0, W - ww)
-    wy = rng.uniform(0.0, H - wh)
-    zi = round(0.5 * dims[2]) * dz
+    shapes = [(diel, lambda X, Y, Z, xc=xc, zc=zc, r=r: (X - xc) ** 2 + (Z - zc) ** 2 < r * r)
+              for xc, zc, r in posts]
     regions = [(mm.Material.pec(), (0.0, 0.0, zi, W, H, zi + dz)),
                (mm.Material.air(), (wx, wy, zi, wx + ww, wy + wh, zi + dz))]
     # the shapes fill after the boxes; keep the posts clear of the iris layer
@@ -214,29 +225,43 @@ def lorentz_pole(d_eps, f0, gamma):
     return complex(-0.5 * gamma, wd), complex(0.0, -d_eps * w0 * w0 / (2.0 * wd))
 
 
-def c4_case(n=320, seed=4, f_max=20e9, n_modes=2):
-    """Precision study: a 32 x 20 x 48 mm PEC guide on an n^3 grid with a
-    seeded slab (z in [0.4, 0.6] L) of two-pole Lorentz material (eps_inf
-    U[2, 4], pole frequencies U[5, 20] GHz, seeded residues, damping f0/10)
-    with bulk conductivity 0.05 S/m, modal DTBC ports with n_modes modes on
-    zmin and zmax.  Every open end carries a port, so the DTBC is the
-    absorber and no CPML is placed."""
-    from magnelio_amd import mesh as mm
-    from magnelio_amd import ports as pp
-
+def c4_layout(seed=4):
+    """Seeded precision-study material: (eps_inf, sigma, poles) of the slab,
+    poles as pole-residue pairs of two Lorentz terms (d_eps U[0.5, 2],
+    f0 U[5, 20] GHz, damping f0 / 10)."""
     rng = np.random.default_rng(seed)
-    W, H, L = C4_SIZE
-    grid = mm.GridLines(*(np.linspace(0.0, s, n + 1) for s in C4_SIZE))
     poles = tuple(lorentz_pole(float(rng.uniform(0.5, 2.0)), f0, 2 * math.pi * f0 / 10)
                   for f0 in rng.uniform(5e9, 20e9, 2))
     eps_inf = float(rng.uniform(2.0, 4.0))
-    slab = mm.Material(name="lorentz", epsilon=(eps_inf,) * 3, sigma=(0.05,) * 3, poles=poles)
-    mesh = mm.Mesh.from_grid(grid, regions=[(slab, (0.0, 0.0, 0.4 * L, W, H, 0.6 * L))],
-                             pec_faces=("xmin", "xmax", "ymin", "ymax"))
+    return eps_inf, 0.05, poles
+
+
+C4_LAYERS = 8
+
+
+def c4_case(n=320, seed=4, f_max=20e9, n_modes=2):
+    """Precision study: a 32 x 20 x 48 mm guide with PEC x/y walls on an
+    n^3 grid, a seeded slab (z in [0.4, 0.6] L) of two-pole Lorentz
+    material (eps_inf U[2, 4], pole frequencies U[5, 20] GHz, seeded
+    residues, damping f0/10) with bulk conductivity 0.05 S/m, a modal DTBC
+    port with n_modes modes on the open end z_min and an 8-layer CPML
+    absorbing the open end z_max.  Returns (mesh, ports, dt, boundary
+    conditions, info)."""
+    from magnelio_amd import mesh as mm
+    from magnelio_amd import ports as pp
+    from magnelio_amd.boundaries import CPMLBoundary, PECBoundary
+
+    W, H, L = C4_SIZE
+    grid = mm.GridLines(*(np.linspace(0.0, s, n + 1) for s in C4_SIZE))
+    eps_inf, sigma, poles = c4_layout(seed)
+    slab = mm.Material(name="lorentz", epsilon=(eps_inf,) * 3, sigma=(sigma,) * 3, poles=poles)
+    walls = ("xmin", "xmax", "ymin", "ymax")
+    mesh = mm.Mesh.from_grid(grid, regions=[(slab, (0.0, 0.0, 0.4 * L, W, H, 0.6 * L))], pec_faces=walls)
     dt = mm.courant_dt(grid, "normal")
-    ports = [pp.build_port_local(name, face, mesh, dt=dt, f_calc=f_max, n_modes=n_modes)
-             for name, face in (("port1", "zmin"), ("port2", "zmax"))]
-    return mesh, ports, dt, {"eps_inf": eps_inf, "poles_hz": [p[0].imag / (2 * math.pi) for p in poles]}
+    ports = [pp.build_port_local("port1", "zmin", mesh, dt=dt, f_calc=f_max, n_modes=n_modes)]
+    bcs = {f: PECBoundary(f) for f in walls}
+    bcs["zmax"] = CPMLBoundary("zmax", grid, thickness_cells=C4_LAYERS)
+    return mesh, ports, dt, bcs, {"eps_inf": eps_inf, "poles_hz": [p[0].imag / (2 * math.pi) for p in poles]}
 
 
 def main_c4(args) -> int:
@@ -248,7 +273,7 @@ def main_c4(args) -> int:
     from magnelio_amd.sweep import ScatteringSweep
 
     t_setup = time.perf_counter()
-    mesh, ports, dt, info = c4_case(args.n if args.n != 256 else 320)
+    mesh, ports, dt, bcs, info = c4_case(args.n if args.n != 256 else 320)
     nx, ny, nz = mesh.dims
     steps = args.steps if args.steps != 512 else 3000
     f_axis = np.linspace(6e9, 19e9, 131)
@@ -263,7 +288,8 @@ def main_c4(args) -> int:
     out_runs = {}
     for prec in ("single", "double"):
         sw = ScatteringSweep(mesh, ports, dt=dt, n_steps=steps, f_axis=f_axis, f_max=20e9,
-                             excited=excited, precision=prec, monitors_factory=monitors)
+                             excited=excited, precision=prec, monitors_factory=monitors,
+                             boundary_conditions=bcs)
         res = sw.run()
         sol = sw.solvers[0]
         out_runs[prec] = {"S": res.s_params.matrix, "seconds": res.seconds,
@@ -298,9 +324,10 @@ def main_c4(args) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"{nx}x{ny}x{nz} PEC guide 32x20x48 mm, seeded two-pole Lorentz slab with "
-                         "sigma 0.05 S/m (eps_inf %.2f, poles %.1f/%.1f GHz), DTBC ports on zmin/zmax "
-                         "x 2 modes, 2 excitations, %d steps, field-plane DFT 64 freqs compensated"
+            "workload": (f"{nx}x{ny}x{nz} guide 32x20x48 mm with PEC x/y walls, seeded two-pole Lorentz "
+                         "slab with sigma 0.05 S/m (eps_inf %.2f, poles %.1f/%.1f GHz), DTBC port on the "
+                         "open end zmin x 2 modes, 8-layer CPML on the open end zmax, 2 excitations, %d "
+                         "steps, field-plane DFT 64 freqs compensated"
                          % (info["eps_inf"], info["poles_hz"][0] / 1e9, info["poles_hz"][1] / 1e9, steps)),
             "setup_s": round(time.perf_counter() - t_setup - a["seconds"] - b["seconds"], 1),
         },

@@ -236,3 +236,153 @@ def golden_box_oracle(dtype=np.float64):
     orc = ofd.Oracle(dims, ae, be, ah, bh, dt, cpml=faces, sources=[src], monitors=[region, far],
                      dtype=dtype)
     return orc, region, far
+
+
+# ---------------------------------------------------------------------------
+# C3 (batched multi-excitation) at reduced size: WR-90 guide, two eps_r = 30
+# posts, a PEC iris with a seeded window, 3-mode ports on z_min / z_max
+
+C3_SMALL = (48, 24, 72)
+C3_SIZE = (22.86e-3, 10.16e-3, 60e-3)
+
+
+def c3_oracle(excite, dims=C3_SMALL, dtype=np.float64):
+    """Oracle of the reduced C3 case with one excited (port, mode): cells
+    filled from the seeded layout (bench.c3_geometry) with the cell-centre
+    tests (iris box, window box, then the posts), closed-form TE modes on
+    the uniform feed.  Returns (Oracle, ports, dt, (me, mm))."""
+    from bench import c3_geometry  # noqa: PLC0415
+
+    W, H, L = C3_SIZE
+    x, y, z = (np.linspace(0.0, s, n + 1) for s, n in zip(C3_SIZE, dims))
+    posts, (zi, dz), (wx, wy, ww, wh) = c3_geometry(dims, 3, C3_SIZE)
+    AIR, PEC, DIEL = 0, 1, 2
+    mid = om.fill_aabb(x, y, z, [(PEC, (0.0, 0.0, zi, W, H, zi + dz)),
+                                 (AIR, (wx, wy, zi, wx + ww, wy + wh, zi + dz))])
+    xc_, zc_ = 0.5 * (x[:-1] + x[1:]), 0.5 * (z[:-1] + z[1:])
+    for px, pz, r in posts:
+        inside = ((xc_[:, None] - px) ** 2 + (zc_[None, :] - pz) ** 2) < r * r
+        mid[np.broadcast_to(inside[:, None, :], mid.shape)] = DIEL
+    me = om.mass_e(x, y, z, mid, [[1.0, 1.0, 30.0]] * 3, om.EPS0)
+    mmu = om.mass_h(x, y, z, mid, [[1.0, 1.0, 1.0]] * 3, om.MU0)
+    nx, ny, nz = dims
+    pec = om.bbox_walls(om.pec_edge_mask(mid == PEC), nx, ny, nz, ["xmin", "xmax", "ymin", "ymax"])
+    for f in ("zmin", "zmax"):
+        me = om.flatten_port(me, nx, ny, nz, f, "E")
+        mmu = om.flatten_port(mmu, nx, ny, nz, f, "H")
+        pec = om.flatten_port(pec, nx, ny, nz, f, "E")
+    dt = om.courant_dt(x, y, z, 0.95)
+    ae, be = om.alpha_beta(me, np.zeros_like(me), dt, pec)
+    ah, bh = om.alpha_beta(mmu, np.zeros_like(mmu), dt)
+    ports = []
+    for name, face in (("port1", "zmin"), ("port2", "zmax")):
+        pl = op.plane(face, x, y, z)
+        modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES]
+        ports.append(op.Port(name, pl, modes, me, mmu, dt, 2 * math.pi * 25e9))
+    p = ports[0] if excite[0] == "port1" else ports[1]
+    f_lo = p.modes[excite[1]]["wc"] / (2 * math.pi)
+    p.set_excitation(excite[1], lambda t, lo=f_lo: modulated_gaussian(t, 25e9, lo))
+    orc = ofd.Oracle(dims, ae, be, ah, bh, dt, ports=ports, dtype=dtype)
+    return orc, ports, dt, (me, mmu)
+
+
+# ---------------------------------------------------------------------------
+# C5 serving jobs: the oracle of one job from its seeded geometry
+
+def c5_job_oracle(job, excite, dtype=np.float64):
+    """Oracle of a serving job (bench.c5_jobs) with one excited (port,
+    mode): the WR-90 guide of the job's grid, its bricks (z clamped to the
+    middle 60 %) and spheres (cell-centre test) filled in the job's order,
+    PEC x/y walls (and z_max with one port), closed-form TE ports.
+    Returns (Oracle, ports, dt)."""
+    from bench import C5_SIZE  # noqa: PLC0415
+
+    dims, n_ports, objs = job.build.tasks["mesh"][1].args
+    n_modes = job.build.tasks["port1"][0][3]
+    x, y, z = (np.linspace(0.0, s, n + 1) for s, n in zip(C5_SIZE, dims))
+    zlo, zhi = 0.2 * C5_SIZE[2], 0.8 * C5_SIZE[2]
+    boxes, spheres, eps = [], [], [1.0]
+    for e, sphere, ctr, half in objs:
+        eps.append(e)
+        if sphere:
+            spheres.append((len(eps) - 1, ctr, float(half.min())))
+        else:
+            lo, hi = ctr - half, ctr + half
+            boxes.append((len(eps) - 1, (lo[0], lo[1], max(lo[2], zlo), hi[0], hi[1], min(hi[2], zhi))))
+    mid = om.fill_aabb(x, y, z, boxes)
+    cx, cy, cz = np.meshgrid(*(0.5 * (a[:-1] + a[1:]) for a in (x, y, z)), indexing="ij")
+    for m, c, r in spheres:
+        inside = ((cx - c[0]) ** 2 + (cy - c[1]) ** 2 + (cz - c[2]) ** 2 < r * r) & (cz > zlo) & (cz < zhi)
+        mid[inside] = m
+    me = om.mass_e(x, y, z, mid, [eps] * 3, om.EPS0)
+    mmu = om.mass_h(x, y, z, mid, [[1.0] * len(eps)] * 3, om.MU0)
+    nx, ny, nz = dims
+    walls = ["xmin", "xmax", "ymin", "ymax"] + (["zmax"] if n_ports == 1 else [])
+    pec = om.bbox_walls(np.zeros(me.size, bool), nx, ny, nz, walls)
+    faces = ("zmin", "zmax")[:n_ports]
+    for f in faces:
+        me = om.flatten_port(me, nx, ny, nz, f, "E")
+        mmu = om.flatten_port(mmu, nx, ny, nz, f, "H")
+        pec = om.flatten_port(pec, nx, ny, nz, f, "E")
+    dt = om.courant_dt(x, y, z, 0.95)
+    ae, be = om.alpha_beta(me, np.zeros_like(me), dt, pec)
+    ah, bh = om.alpha_beta(mmu, np.zeros_like(mmu), dt)
+    ports = []
+    for name, face in zip(("port1", "port2"), faces):
+        pl = op.plane(face, x, y, z)
+        modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES[:n_modes]]
+        ports.append(op.Port(name, pl, modes, me, mmu, dt, 2 * math.pi * job.f_max))
+    p = next(q for q in ports if q.name == excite[0])
+    f_lo = p.modes[excite[1]]["wc"] / (2 * math.pi)
+    p.set_excitation(excite[1], lambda t, lo=f_lo, hi=job.f_max: modulated_gaussian(t, hi, lo))
+    return ofd.Oracle(dims, ae, be, ah, bh, dt, ports=ports, dtype=dtype), ports, dt
+
+
+# ---------------------------------------------------------------------------
+# C4 (precision study) at reduced size: Lorentz slab with sigma, a DTBC port
+# on z_min, CPML on z_max
+
+def lowest_te_modes(x, y, n_modes, eps_r=1.0):
+    """(m, n) of the n_modes lowest discrete TE cut-offs of a uniform PEC
+    rectangle (closed form k_c^2 = sum (2/d sin(m pi / 2N))^2)."""
+    nx, ny = x.size - 1, y.size - 1
+    dx, dy = float(x[1] - x[0]), float(y[1] - y[0])
+    cands = [(m, n) for m in range(0, 4) for n in range(0, 4) if (m, n) != (0, 0)]
+    kc = {mn: (2 / dx * math.sin(mn[0] * math.pi / (2 * nx))) ** 2
+          + (2 / dy * math.sin(mn[1] * math.pi / (2 * ny))) ** 2 for mn in cands}
+    return sorted(cands, key=lambda mn: kc[mn])[:n_modes]
+
+
+def c4_oracle(n, excite, dtype=np.float64, n_modes=2, seed=4, f_max=20e9):
+    """Oracle of the reduced C4 case (bench.c4_case on an n^3 grid) with one
+    excited mode of port 1.  Returns (Oracle, ports, dt, (me, mm))."""
+    from bench import C4_LAYERS, C4_SIZE, c4_layout  # noqa: PLC0415
+    from oracle import cpml as oc  # noqa: PLC0415
+    from oracle import dispersion as od  # noqa: PLC0415
+
+    W, H, L = C4_SIZE
+    x, y, z = (np.linspace(0.0, s, n + 1) for s in C4_SIZE)
+    dims = (n, n, n)
+    eps_inf, sigma, poles = c4_layout(seed)
+    mid = om.fill_aabb(x, y, z, [(1, (0.0, 0.0, 0.4 * L, W, H, 0.6 * L))])
+    me = om.mass_e(x, y, z, mid, [[1.0, eps_inf]] * 3, om.EPS0)
+    ms = om.mass_e(x, y, z, mid, [[0.0, sigma]] * 3, 1.0)
+    mmu = om.mass_h(x, y, z, mid, [[1.0, 1.0]] * 3, om.MU0)
+    dt = om.courant_dt(x, y, z, 0.95)
+    walls = om.bbox_walls(np.zeros(me.size, bool), *dims, ["xmin", "xmax", "ymin", "ymax"])
+    disp = od.Dispersion(x, y, z, mid, {1: list(poles)}, dt, pec_e=walls)
+    me = om.flatten_port(me, *dims, "zmin", "E")
+    mmu = om.flatten_port(mmu, *dims, "zmin", "H")
+    pec = om.flatten_port(walls, *dims, "zmin", "E")
+    ae, be = om.alpha_beta(me, ms, dt, pec, w=disp.w_half)
+    ah, bh = om.alpha_beta(mmu, np.zeros_like(mmu), dt)
+    disp.bind(be, dtype)
+    pl = op.plane("zmin", x, y, z)
+    modes = [op.rect_te_mode(pl, x, y, z, m, k, me) for m, k in lowest_te_modes(x, y, n_modes)]
+    port = op.Port("port1", pl, modes, me, mmu, dt, 2 * math.pi * f_max)
+    f_lo = port.modes[excite[1]]["wc"] / (2 * math.pi)
+    port.set_excitation(excite[1], lambda t, lo=f_lo: modulated_gaussian(t, f_max, lo))
+    face = oc.CPMLFace("zmax", x, y, z, dt, dtype, pec_e_flat=pec, pec_cell=np.zeros(dims, bool),
+                       n_layers=C4_LAYERS)
+    orc = ofd.Oracle(dims, ae, be, ah, bh, dt, cpml=[face], ports=[port], dtype=dtype, dispersion=disp)
+    return orc, [port], dt, (me, mmu)
