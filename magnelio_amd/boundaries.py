@@ -273,6 +273,34 @@ class DeviceCPML:
                     out[f"{f.face}_psi_{name}{t}"] = np.ascontiguousarray(np.moveaxis(a, (1, 2, 3), (3, 2, 1)))
         return out
 
+    def reference_state(self, s: int) -> dict:
+        """{face: {"_psi_Ey": ..., ...}} of field set s: the reference's
+        CPMLBoundary.state_dict (cpml.py:174-187, :197-204) — each term's psi
+        slab in C order with the normal extent cut to the layers."""
+        out = {}
+        for f, pe, ph in zip(self.faces, self.psi_e, self.psi_h):
+            e_sh, h_sh = f.psi_shapes(self.dims)
+            d = {}
+            for side, lst, shs, comps in (("E", pe, e_sh, E_TERM_COMPS[f.axis]),
+                                          ("H", ph, h_sh, H_TERM_COMPS[f.axis])):
+                for ten, sh, c in zip(lst, shs, comps):
+                    a = ten[s].cpu().numpy().reshape(sh[2], sh[1], sh[0])
+                    d[f"_psi_{side}{'xyz'[c]}"] = np.ascontiguousarray(np.moveaxis(a, (0, 1, 2), (2, 1, 0)))
+            out[f.face] = d
+        return out
+
+    def load_reference_state(self, s: int, faces: dict) -> None:
+        """Field set s from the reference's per-face psi dicts."""
+        import torch  # noqa: PLC0415
+
+        for f, pe, ph in zip(self.faces, self.psi_e, self.psi_h):
+            d = faces[f.face]
+            for side, lst, comps in (("E", pe, E_TERM_COMPS[f.axis]), ("H", ph, H_TERM_COMPS[f.axis])):
+                for ten, c in zip(lst, comps):
+                    a = np.asarray(d[f"_psi_{side}{'xyz'[c]}"])
+                    dev = np.ascontiguousarray(np.moveaxis(a, (0, 1, 2), (2, 1, 0)).ravel())
+                    ten[s].copy_(torch.from_numpy(dev).to(ten.device, ten.dtype))
+
     def load_state_dict(self, sd: dict) -> None:
         import torch  # noqa: PLC0415
 

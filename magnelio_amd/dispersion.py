@@ -157,3 +157,33 @@ class DeviceDispersion:
     def load_state_dict(self, sd: dict) -> None:
         self.J.copy_(torch.from_numpy(np.asarray(sd["J"])))
         self.fprev.copy_(torch.from_numpy(np.asarray(sd["f_prev"])))
+
+    def reference_state(self, s: int) -> dict:
+        """{str(mat_id): {"J0": ..., ...}} of field set s, the reference's
+        DispersionOperator.state_dict (_dispersion.py:571-576): per block
+        the pole currents over its edges, real for a real pole, complex for
+        a stored-once conjugate pair (f_prev is rewritten before every use)."""
+        J = self.J[s].cpu().numpy()
+        out, off = {}, 0
+        for mid, idx, _, poles in self.blocks:
+            n = idx.size
+            d = {}
+            for i, p in enumerate(poles):
+                re = J[i, off:off + n, 0]
+                d[f"J{i}"] = (re + 1j * J[i, off:off + n, 1]) if p.complex else re.copy()
+            out[str(mid)] = d
+            off += n
+        return out
+
+    def load_reference_state(self, s: int, state: dict) -> None:
+        J = self.J[s].cpu().numpy()
+        off = 0
+        for mid, idx, _, poles in self.blocks:
+            n = idx.size
+            d = state[str(mid)]
+            for i, p in enumerate(poles):
+                a = np.asarray(d[f"J{i}"])
+                J[i, off:off + n, 0] = a.real
+                J[i, off:off + n, 1] = a.imag if p.complex else 0.0
+            off += n
+        self.J[s].copy_(torch.from_numpy(J))

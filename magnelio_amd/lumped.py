@@ -277,6 +277,25 @@ class DeviceLumped:
                 sd[f"{op.name}_I"] = I.cpu().numpy()
         return sd
 
+    def set_state(self, s: int, n: int) -> dict:
+        """Companion state and V/I records of field set s (first n steps)."""
+        sd = {}
+        for op, st, V, I in zip(self.ops, self.state, self.V_rec, self.I_rec):
+            d = {"state": st[s].cpu().numpy()}
+            if V is not None:
+                d["V"] = V[s, :n].cpu().numpy()
+                d["I"] = I[s, :n].cpu().numpy()
+            sd[op.name] = d
+        return sd
+
+    def load_set_state(self, s: int, sd: dict, n: int) -> None:
+        for op, st, V, I in zip(self.ops, self.state, self.V_rec, self.I_rec):
+            d = sd[op.name]
+            st[s].copy_(torch.from_numpy(np.asarray(d["state"])))
+            if V is not None and n:
+                V[s, :n].copy_(torch.from_numpy(np.asarray(d["V"])))
+                I[s, :n].copy_(torch.from_numpy(np.asarray(d["I"])))
+
     def load_state_dict(self, sd: dict) -> None:
         for op, st, V, I in zip(self.ops, self.state, self.V_rec, self.I_rec):
             st.copy_(torch.from_numpy(np.asarray(sd[f"{op.name}_state"])))

@@ -60,6 +60,23 @@ def natural(vals: np.ndarray, shape) -> np.ndarray:
     return arr.transpose(*range(len(lead)), *range(len(lead) + nd - 1, len(lead) - 1, -1))
 
 
+def unnatural(arr: np.ndarray) -> np.ndarray:
+    """Inverse of :func:`natural`: (F, *shape) -> (F, n) samples, first
+    grid axis fastest."""
+    nd = arr.ndim - 1
+    return np.ascontiguousarray(np.moveaxis(arr, tuple(range(1, nd + 1)), tuple(range(nd, 0, -1)))
+                                ).reshape(arr.shape[0], -1)
+
+
+def _corners_array(corners) -> np.ndarray:
+    """(2, 3) corners, None as -inf / +inf, axes sorted (the stored form)."""
+    if corners is None:
+        return np.array([[-np.inf] * 3, [np.inf] * 3])
+    a = np.array([[(-np.inf if v is None else float(v)) for v in corners[0]],
+                  [(np.inf if v is None else float(v)) for v in corners[1]]])
+    return np.sort(a, axis=0)
+
+
 def blocks_for(block: int, requested: int | None = None) -> int:
     """Consecutive blocks contracted per launch (the bins are read and
     written once for all of them): ``requested`` (1, 2 or 4, checked), or by
@@ -394,6 +411,9 @@ class DFTMonitorBase:
         self.nsets = K
         self.n_start = solver.n_completed
         self._horizon = solver.horizon
+        import weakref  # noqa: PLC0415
+
+        self._solver = weakref.ref(solver)
         self.phase_tab = None
         if self.pipelined:
             self._phase_table(torch.cuda.current_stream(dev))
@@ -587,11 +607,64 @@ class DFTMonitorBase:
         self.bins.copy_(torch.from_numpy(np.asarray(sd["bins"])))
         if self.lo is not None:
             self.lo.copy_(torch.from_numpy(np.asarray(sd["lo"])))
-        n_start = int(sd["n_start"])
+        self._set_n_start(int(sd["n_start"]))
+
+    def _set_n_start(self, n_start: int) -> None:
         if n_start != self.n_start:
             self.n_start = n_start
             if self.pipelined:
                 self._phase_table(torch.cuda.current_stream(self.X.device))
+
+    # per field set -------------------------------------------------------
+    def set_state(self, s: int) -> dict:
+        """Field set s's sample buffers and bins (exact resume)."""
+        d = {"X": self.X[:, s].cpu().numpy(), "bins": self.bins[s].cpu().numpy(), "n_start": self.n_start}
+        if self.lo is not None:
+            d["lo"] = self.lo[s].cpu().numpy()
+        return d
+
+    def load_set_state(self, s: int, d: dict) -> None:
+        self.X[:, s].copy_(torch.from_numpy(np.asarray(d["X"])))
+        self.bins[s].copy_(torch.from_numpy(np.asarray(d["bins"])))
+        if self.lo is not None:
+            self.lo[s].copy_(torch.from_numpy(np.asarray(d["lo"])))
+        self._set_n_start(int(d["n_start"]))
+
+    def complete_bins(self, s: int = 0) -> np.ndarray:
+        """(F, S) complex128 bins of field set s with every recorded sample
+        in them: pending samples are contracted into a copy (the monitor's
+        own state is left as it was)."""
+        solver = self._solver()
+        rel = solver.n_completed - self.n_start if solver is not None else 0
+        if rel > 0 and solver is not None:
+            keep = (self.X.clone(), self.bins.clone(), self.lo.clone() if self.lo is not None else None)
+            self.finalize(solver)
+            b = self.bins[s].cpu().numpy()
+            lo = self.lo[s].cpu().numpy() if self.lo is not None else None
+            self.X.copy_(keep[0])
+            self.bins.copy_(keep[1])
+            if self.lo is not None:
+                self.lo.copy_(keep[2])
+        else:
+            b = self.bins[s].cpu().numpy()
+            lo = self.lo[s].cpu().numpy() if self.lo is not None else None
+        if lo is not None:
+            b = b + lo.astype(np.float64)
+        return b[..., 0] + 1j * b[..., 1]
+
+    def _load_bins(self, s: int, z: np.ndarray) -> None:
+        """Field set s's bins from (F, S) complex values; no samples pending."""
+        b = np.zeros((self.F, self.S, 2))
+        b[..., 0], b[..., 1] = z.real, z.imag
+        self.bins[s].copy_(torch.from_numpy(b))
+        if self.lo is not None:
+            self.lo[s].zero_()
+        self.X[:, s].zero_()
+
+    def _component_bins(self, s: int) -> list:
+        """[(comp, shape, (F, *shape) complex)] in sample-group order."""
+        z = self.complete_bins(s)
+        return [(comp, shape, natural(z[:, start:start + n], shape)) for comp, shape, start, n in self._layout]
 
 
 class MonitorFieldFrequency(DFTMonitorBase):
@@ -624,6 +697,8 @@ class MonitorFieldFrequency(DFTMonitorBase):
                     sl.append(snap_range(cc[a], lo, hi))
             sl = tuple(sl)
         self.slices = sl
+        cc = grid.centres()
+        self._centres = tuple(np.asarray(cc[a][sl[a]], float) for a in range(3))
         sb = _SampleBuilder(grid)
         I, J, K = np.meshgrid(np.arange(sl[0].start, sl[0].stop), np.arange(sl[1].start, sl[1].stop),
                               np.arange(sl[2].start, sl[2].stop), indexing="ij")
@@ -632,6 +707,24 @@ class MonitorFieldFrequency(DFTMonitorBase):
         # per-step gathers of neighbouring samples coalesce
         I, J, K = (a.transpose(2, 1, 0) for a in (I, J, K))
         self._groups = [(c, shape, sb.centred(c, I, J, K)) for c in self.components]
+
+
+    def result_dump(self, s: int = 0) -> dict:
+        """The reference's dump of this monitor (field_frequency.py:304-331):
+        region cell centres, frequencies and the raw complex bins per
+        component of field set s, every recorded sample included."""
+        bins = {comp: b for comp, _, b in self._component_bins(s)}
+        return {"components": list(self.components), "fields": sorted({c[0] for c in self.components}),
+                "interval": None, "symmetry": [], "freqs": np.asarray(self.freqs, float),
+                "corners": _corners_array(self.corners), "grid_x": self._centres[0],
+                "grid_y": self._centres[1], "grid_z": self._centres[2], "bins": bins}
+
+    def load_result_dump(self, dump: dict, s: int = 0) -> None:
+        """Field set s's bins from the reference's dump (resume)."""
+        z = np.zeros((self.F, self.S), complex)
+        for comp, shape, start, n in self._layout:
+            z[:, start:start + n] = unnatural(np.asarray(dump["bins"][comp]))
+        self._load_bins(s, z)
 
 
 class MonitorFarField(DFTMonitorBase):
@@ -675,6 +768,17 @@ class MonitorFarField(DFTMonitorBase):
         centres = grid.centres()
         self._groups = []
         self.faces = []
+        self._geometry = {}
+        # walls handled by image theory in the far-field post (far_field.py:150-162)
+        self._image_planes = []
+        for face, bc in boundary_conditions.items():
+            kind = getattr(bc, "kind", type(bc).__name__.replace("Boundary", ""))
+            if str(kind).upper() in ("PEC", "PMC"):
+                axis = "xyz".index(face[0])
+                at_low = face.endswith("min")
+                self._image_planes.append({"axis": axis, "kind": str(kind).upper(), "at_low": at_low,
+                                           "position": float(nodes[axis][0 if at_low else -1]),
+                                           "physical_halfspace": True})
         for face in open_faces:
             axis = "xyz".index(face[0])
             nn = lo_n[axis] if face.endswith("min") else hi_n[axis]
@@ -687,6 +791,12 @@ class MonitorFarField(DFTMonitorBase):
             shape = A.shape
             A, Bm = A.T, Bm.T  # samples numbered along the lower axis fastest
             self.faces.append((face, axis, nn, wgt, shape))
+            widths = (grid.dx, grid.dy, grid.dz)
+            self._geometry[face] = dict(axis=axis, sign=-1.0 if face.endswith("min") else 1.0,
+                                        plane=float(nodes[axis][nn]),
+                                        c1=centres[t1][r1].copy(), c2=centres[t2][r2].copy(),
+                                        w1=np.asarray(widths[t1])[r1].copy(),
+                                        w2=np.asarray(widths[t2])[r2].copy())
             for comp in TANGENTIALS[axis]:
                 terms = []
                 for layer, lw in ((nn - 1, 1.0 - wgt), (nn, wgt)):
@@ -696,3 +806,46 @@ class MonitorFarField(DFTMonitorBase):
                     for ii, ww in sb.centred(comp, *ijk):
                         terms.append((ii, ww * lw))
                 self._groups.append((comp, shape, terms))
+
+    def result_dump(self, s: int = 0) -> dict:
+        """The reference's dump of this monitor (far_field.py:403-452): per
+        Huygens face its geometry and the raw complex bins of field set s,
+        every recorded sample included — the input of the far-field post."""
+        per_face: dict = {}
+        k = {}
+        comps = self._component_bins(s)
+        by_comp: dict = {}
+        for comp, _, b in comps:
+            by_comp.setdefault(comp, []).append(b)
+        for face, axis, _, _, _ in self.faces:
+            for comp in TANGENTIALS[axis]:
+                i = k.get(comp, 0)
+                k[comp] = i + 1
+                per_face.setdefault(face, {})[comp] = by_comp[comp][i]
+        faces = []
+        for face, axis, _, _, _ in self.faces:
+            g = self._geometry[face]
+            faces.append({"name": face, "axis": int(g["axis"]), "sign": g["sign"], "plane": g["plane"],
+                          "c1": g["c1"], "c2": g["c2"], "w1": g["w1"], "w2": g["w2"],
+                          "bins": per_face[face]})
+        return {"name": self.name, "freqs": np.asarray(self.freqs, float),
+                "margin_cells": int(self.margin_cells), "faces": faces,
+                "image_planes": [dict(p) for p in self._image_planes]}
+
+    def load_result_dump(self, dump: dict, s: int = 0) -> None:
+        """Field set s's bins from the reference's dump (resume)."""
+        by_face = {str(f["name"]): f["bins"] for f in (dump["faces"].values()
+                                                      if isinstance(dump["faces"], dict)
+                                                      else dump["faces"])}
+        z = np.zeros((self.F, self.S), complex)
+        k = {}
+        order = []
+        for face, axis, _, _, _ in self.faces:
+            for comp in TANGENTIALS[axis]:
+                order.append((face, comp))
+        # groups were laid out E components first, then H, each in face order
+        e_first = [fc for fc in order if fc[1].startswith("E")] + [fc for fc in order if fc[1].startswith("H")]
+        for (face, comp), (c2, shape, start, n) in zip(e_first, self._layout):
+            assert comp == c2
+            z[:, start:start + n] = unnatural(np.asarray(by_face[face][comp]))
+        self._load_bins(s, z)

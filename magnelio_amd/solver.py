@@ -289,6 +289,10 @@ class FITTimeDomainSolver:
     coefficients: str = "auto"  # alpha/beta built on the "device", the "host", or "auto"
     tile_skip: str = "auto"  # dead-unit skipping: "auto" (when it skips >= 2 %), "on", "off"
     device: str = "cuda:0"
+    # (m_eps, m_mu, m_sigma, m_sigma_m) flat float64 vectors in the reference
+    # layout, unflattened — e.g. the reference's own build_M_* of a conformal
+    # mesh; None: built here from the material ids
+    mass_matrices: tuple | None = None
 
     def __post_init__(self):
         if self.precision not in PRECISIONS:
@@ -331,9 +335,18 @@ class FITTimeDomainSolver:
         if self._device_coefficients():
             self._prepare_device_coefficients()
             return
-        m_eps = _mesh.build_m_eps(mesh)
-        m_mu = _mesh.build_m_mu(mesh)
-        m_sig = _mesh.build_m_sigma(mesh)
+        if self.mass_matrices is not None:
+            ne, nh = kernels.n_e(*dims), kernels.n_h(*dims)
+            m_eps, m_mu, m_sig, m_sig_m = (np.asarray(a, dtype=np.float64).copy()
+                                           for a in self.mass_matrices)
+            if m_eps.size != ne or m_sig.size != ne or m_mu.size != nh or m_sig_m.size != nh:
+                raise ValueError("mass_matrices must be (m_eps, m_mu, m_sigma, m_sigma_m) of lengths "
+                                 f"({ne}, {nh}, {ne}, {nh})")
+        else:
+            m_eps = _mesh.build_m_eps(mesh)
+            m_mu = _mesh.build_m_mu(mesh)
+            m_sig = _mesh.build_m_sigma(mesh)
+            m_sig_m = _mesh.build_m_sigma_m(mesh)
         pec = mesh.pec_mask_edges
         pec_raw = _mesh.pec_flat(pec, dims)
         port_faces = []
@@ -355,7 +368,7 @@ class FITTimeDomainSolver:
         self._dispersion = DeviceDispersion.from_mesh(mesh, dt, pec_raw)
         if self._dispersion is not None:
             w_e = self._dispersion.w_half
-        co = _mesh.update_coefficients(m_eps, m_mu, m_sig, _mesh.build_m_sigma_m(mesh), dt, pec_e, w_e, w_h)
+        co = _mesh.update_coefficients(m_eps, m_mu, m_sig, m_sig_m, dt, pec_e, w_e, w_h)
         self._coeffs = co
         self._m_eps_host, self._m_mu_host = m_eps, m_mu
 
@@ -368,14 +381,17 @@ class FITTimeDomainSolver:
 
     def _device_coefficients(self) -> bool:
         """Whether alpha/beta and the masses are built on the device: not for
-        ADE materials or TF/SF sources (their host set-up reads the float64
-        coefficients) nor for port faces on more than one axis."""
+        given mass matrices, ADE materials or TF/SF sources (their host
+        set-up reads the float64 coefficients) nor for port faces on more
+        than one axis."""
         if self.coefficients not in ("auto", "host", "device"):
             raise ValueError(f"coefficients must be 'auto', 'host' or 'device', got {self.coefficients!r}")
         if self.coefficients == "host":
             return False
         why = None
-        if any(getattr(m, "poles", ()) for m in self.mesh.material_library.values()):
+        if self.mass_matrices is not None:
+            why = "given mass matrices"
+        elif any(getattr(m, "poles", ()) for m in self.mesh.material_library.values()):
             why = "dispersive materials"
         elif self.sources:
             why = "sources"
@@ -811,72 +827,162 @@ class FITTimeDomainSolver:
             out.update(self.dlumped.recorded(s, n))
         return out
 
-    def state_dict(self) -> dict:
+    def state_dict(self, set_index: int | None = None) -> dict:
+        """Checkpoint in the reference's schema (fit_td.py:1264-1311).
+
+        One field set: ``n_completed``, ``peak_energy``, ``peak_signal``,
+        ``e`` / ``h`` (reference flat layout), ``boundaries`` {face: CPML
+        psi as cpml.py:197-204 names them}, ``ports`` {name: V_port_prev,
+        V_interior_prev, src_buffer, src_maxlen, dtbc {mode: u, u_prev, n,
+        w_hist, s_hist}} (operator.py:993-1017, dtbc.py:365-381),
+        ``monitors`` {name: result_dump} and ``dispersion`` {mat_id: J
+        per pole} (_dispersion.py:571-576); what only this engine keeps —
+        recorded V/I, sample buffers not yet contracted, lumped companion
+        states — under ``"hip"``, which the reference does not read.  With
+        K > 1 sets and no ``set_index``: {"nsets": K, "sets": [per set]}."""
         torch.cuda.synchronize(self._dev)
-        sd = {"n_completed": self._resume_step,
-              "peak_energy": float(np.max(getattr(self, "_peak_e", 0.0))),
-              "peak_signal": float(np.max(getattr(self, "_peak_s", 0.0))),
-              "peak_energy_sets": np.array(getattr(self, "_peak_e", np.zeros(self.nsets)), float),
-              "peak_signal_sets": np.array(getattr(self, "_peak_s", np.zeros(self.nsets)), float),
-              "e": layout.to_reference(self.e.cpu().numpy(), *self.dims, "E"),
-              "h": layout.to_reference(self.h.cpu().numpy(), *self.dims, "H")}
-        if self.cpml is not None:
-            sd["cpml"] = self.cpml.state_dict()
+        K = self.nsets
+        if set_index is None and K > 1:
+            return {"nsets": K, "sets": [self.state_dict(s) for s in range(K)]}
+        s = 0 if set_index is None else int(set_index)
+        if not 0 <= s < K:
+            raise ValueError(f"set_index {s} out of range [0, {K})")
+        n = self._resume_step
+        dims = self.dims
+        peak_e = np.broadcast_to(np.asarray(getattr(self, "_peak_e", 0.0), float), (K,))
+        peak_s = np.broadcast_to(np.asarray(getattr(self, "_peak_s", 0.0), float), (K,))
+        sd = {"n_completed": int(n), "peak_energy": float(peak_e[s]), "peak_signal": float(peak_s[s]),
+              "e": layout.to_reference(self.e[s].cpu().numpy(), *dims, "E"),
+              "h": layout.to_reference(self.h[s].cpu().numpy(), *dims, "H"),
+              "boundaries": self.cpml.reference_state(s) if self.cpml is not None else {},
+              "ports": {}, "monitors": {}}
+        hip = {"ports": {}, "monitors": {}}
         if self.dports is not None:
-            n = self._resume_step
-            sd["ports"] = {}
-            for P, st, wh, shh, V, I in zip(self.ports, self.dports.state, self.dports.w_hist,
-                                           self.dports.s_hist, self.dports.V_rec, self.dports.I_rec):
-                stn = st.cpu().numpy()
-                sd["ports"][P.name] = {
-                    "u": stn[:, :, 0].copy(), "u_prev": stn[:, :, 1].copy(),
-                    "V_port_prev": stn[:, :, 2].copy(), "V_interior_prev": stn[:, :, 3].copy(),
-                    "w_hist": wh[:, :, :n].cpu().numpy().copy(), "s_hist": shh[:, :, :n].cpu().numpy().copy(),
-                    "V_rec": V[:, :, :n].cpu().numpy().copy(), "I_rec": I[:, :, :n].cpu().numpy().copy(),
-                }
+            for pi, P in enumerate(self.ports):
+                st = self.dports.state[pi][s].cpu().numpy()
+                wh, shh = self.dports.w_hist[pi], self.dports.s_hist[pi]
+                d = {"V_port_prev": st[:, 2].copy(), "V_interior_prev": st[:, 3].copy(), "dtbc": {}}
+                exc = self.excitations[s]
+                excited = exc is not None and exc.port == P.name
+                buf, maxlen = [], 0
+                if excited:
+                    m = exc.mode
+                    tau = float(P.tau[m])
+                    maxlen = max(int(math.ceil((tau + self.dt) / self.dt)) + 3, 5)
+                    if P.dtbc_r[m] is None:  # the Mur TF/SF ring buffer: scaled source samples
+                        scale = float(P.source_scale[m])
+                        buf = [scale * exc.waveform((j + 1) * self.dt) for j in range(max(0, n - maxlen), n)]
+                d["src_buffer"] = np.array(buf, dtype=float)
+                d["src_maxlen"] = int(maxlen)
+                for m in range(P.n_modes):
+                    if P.dtbc_r[m] is None:
+                        continue
+                    hist = wh.shape[2] >= n and n > 0
+                    d["dtbc"][str(m)] = {"u": float(st[m, 0]), "u_prev": float(st[m, 1]), "n": int(n),
+                                         "w_hist": wh[s, m, :n].cpu().numpy().copy() if hist else np.zeros(n),
+                                         "s_hist": shh[s, m, :n].cpu().numpy().copy() if hist else np.zeros(n)}
+                sd["ports"][P.name] = d
+                hip["ports"][P.name] = {"V_rec": self.dports.V_rec[pi][s, :, :n].cpu().numpy().copy(),
+                                        "I_rec": self.dports.I_rec[pi][s, :, :n].cpu().numpy().copy(),
+                                        "absmax": float(self.dports.absmax[pi][s].item())}
         for i, mon in enumerate(self.monitors):
-            if hasattr(mon, "state_dict"):
-                sd[f"monitor{i}"] = mon.state_dict()
+            name = getattr(mon, "name", "") or f"monitor{i}"
+            if hasattr(mon, "result_dump"):
+                sd["monitors"][name] = mon.result_dump(s)
+            if hasattr(mon, "set_state"):
+                hip["monitors"][name] = mon.set_state(s)
         if self.dlumped is not None:
-            sd["lumped"] = self.dlumped.state_dict()
+            hip["lumped"] = self.dlumped.set_state(s, n)
         if self._dispersion is not None:
-            sd["dispersion"] = self._dispersion.state_dict()
+            sd["dispersion"] = self._dispersion.reference_state(s)
+        if self.set_stops[s] is not None:
+            hip["set_stop"] = {"reason": self.set_stops[s][0], "step": int(self.set_stops[s][1])}
+        sd["hip"] = hip
         return sd
 
-    def load_state_dict(self, sd: dict) -> None:
+    def load_state_dict(self, sd: dict, set_index: int | None = None) -> None:
+        """Restore a checkpoint of :meth:`state_dict` — or one the reference's
+        solver wrote (fit_td.py:1313-1353): a reference checkpoint has no
+        ``"hip"`` key, so recorded V/I before it start at zero and DFT
+        monitors take the bins of its ``monitors`` dumps, when present.  The
+        solver must be built as the one that wrote it; call before run()."""
         if not self._setup_done:
             self.setup()
-        dims = self.dims
-        n = int(sd["n_completed"])
-        self.e.copy_(torch.from_numpy(layout.to_device(np.asarray(sd["e"]), *dims, "E")).to(self._dev, self._t_dtype))
-        self.h.copy_(torch.from_numpy(layout.to_device(np.asarray(sd["h"]), *dims, "H")).to(self._dev, self._t_dtype))
-        if self.cpml is not None:
-            self.cpml.load_state_dict(sd["cpml"])
-        if self.dports is not None:
-            for P, st, wh, shh, V, I in zip(self.ports, self.dports.state, self.dports.w_hist,
-                                           self.dports.s_hist, self.dports.V_rec, self.dports.I_rec):
-                p = sd["ports"][P.name]
-                stn = np.stack([p["u"], p["u_prev"], p["V_port_prev"], p["V_interior_prev"]], axis=-1)
-                st.copy_(torch.from_numpy(stn))
-                if n and wh.shape[2] >= n:
-                    wh[:, :, :n].copy_(torch.from_numpy(p["w_hist"]))
-                    shh[:, :, :n].copy_(torch.from_numpy(p["s_hist"]))
-                if n:
-                    V[:, :, :n].copy_(torch.from_numpy(p["V_rec"]))
-                    I[:, :, :n].copy_(torch.from_numpy(p["I_rec"]))
-        for i, mon in enumerate(self.monitors):
-            if hasattr(mon, "load_state_dict"):
-                mon.load_state_dict(sd[f"monitor{i}"])
-        if self.dlumped is not None:
-            self.dlumped.load_state_dict(sd["lumped"])
-        if self._dispersion is not None:
-            self._dispersion.load_state_dict(sd["dispersion"])
+        if "sets" in sd:
+            if len(sd["sets"]) != self.nsets:
+                raise ValueError(f"checkpoint holds {len(sd['sets'])} field sets, the solver {self.nsets}")
+            for s, sub in enumerate(sd["sets"]):
+                self._load_set(s, sub)
+            n = int(sd["sets"][0]["n_completed"])
+        else:
+            if set_index is None and self.nsets > 1:
+                raise ValueError("a one-set checkpoint needs set_index on a solver with several sets")
+            self._load_set(0 if set_index is None else int(set_index), sd)
+            n = int(sd["n_completed"])
         if self.tile_plan is not None:
             # dead E edges: zero, as the dense launch leaves them at its next E update
             self.e.masked_fill_(self.tile_plan.zero_e.unsqueeze(0), 0)
         self.step_ctr.fill_(n)
         self._resume_step = n
-        self._peak_e = np.array(sd.get("peak_energy_sets", sd.get("peak_energy", 0.0)), float)
-        self._peak_s = np.array(sd.get("peak_signal_sets", sd.get("peak_signal", 0.0)), float)
         self._graph = None
         torch.cuda.synchronize(self._dev)
+
+    def _load_set(self, s: int, sd: dict) -> None:
+        dims = self.dims
+        n = int(sd["n_completed"])
+        if n > self.horizon:
+            raise ValueError(f"checkpoint at step {n} lies past this solver's {self.horizon} steps")
+        K = self.nsets
+        self._peak_e = np.broadcast_to(np.asarray(getattr(self, "_peak_e", 0.0), float), (K,)).copy()
+        self._peak_s = np.broadcast_to(np.asarray(getattr(self, "_peak_s", 0.0), float), (K,)).copy()
+        self._peak_e[s] = float(sd.get("peak_energy", 0.0))
+        self._peak_s[s] = float(sd.get("peak_signal", 0.0))
+        self.e[s].copy_(torch.from_numpy(layout.to_device(np.asarray(sd["e"]), *dims, "E")).to(self._dev, self._t_dtype))
+        self.h[s].copy_(torch.from_numpy(layout.to_device(np.asarray(sd["h"]), *dims, "H")).to(self._dev, self._t_dtype))
+        if self.cpml is not None:
+            self.cpml.load_reference_state(s, sd["boundaries"])
+        hip = sd.get("hip", {})
+        if self.dports is not None:
+            for pi, P in enumerate(self.ports):
+                p = sd["ports"][P.name]
+                comp = p.get("complement")
+                if comp is not None and any(np.any(np.asarray(v) != 0) for v in comp.values()):
+                    raise ValueError(f"port {P.name!r}: the checkpoint's complement-absorber state "
+                                     "is not served by this backend")
+                st = self.dports.state[pi]
+                stn = st[s].cpu().numpy()
+                stn[:, 2] = np.asarray(p["V_port_prev"], float)
+                stn[:, 3] = np.asarray(p["V_interior_prev"], float)
+                stn[:, 0:2] = 0.0
+                wh, shh = self.dports.w_hist[pi], self.dports.s_hist[pi]
+                for m_str, t in p.get("dtbc", {}).items():
+                    m = int(m_str)
+                    if P.dtbc_r[m] is None:
+                        raise ValueError(f"checkpoint carries DTBC state for mode {m} but port "
+                                         f"{P.name!r} terminates it on Mur")
+                    if int(t["n"]) != n:
+                        raise ValueError(f"port {P.name!r} mode {m}: DTBC at step {t['n']}, fields at {n}")
+                    stn[m, 0], stn[m, 1] = float(t["u"]), float(t["u_prev"])
+                    if n:
+                        wh[s, m, :n].copy_(torch.from_numpy(np.asarray(t["w_hist"], float)))
+                        shh[s, m, :n].copy_(torch.from_numpy(np.asarray(t["s_hist"], float)))
+                st[s].copy_(torch.from_numpy(stn))
+                hp = hip.get("ports", {}).get(P.name)
+                V, I = self.dports.V_rec[pi], self.dports.I_rec[pi]
+                if hp is not None and n and V.shape[2] >= n:
+                    V[s, :, :n].copy_(torch.from_numpy(np.asarray(hp["V_rec"])))
+                    I[s, :, :n].copy_(torch.from_numpy(np.asarray(hp["I_rec"])))
+                if hp is not None:
+                    self.dports.absmax[pi][s] = float(hp.get("absmax", 0.0))
+        for i, mon in enumerate(self.monitors):
+            name = getattr(mon, "name", "") or f"monitor{i}"
+            if name in hip.get("monitors", {}) and hasattr(mon, "load_set_state"):
+                mon.load_set_state(s, hip["monitors"][name])
+            elif name in sd.get("monitors", {}) and hasattr(mon, "load_result_dump"):
+                mon.load_result_dump(sd["monitors"][name], s)
+        if self.dlumped is not None and "lumped" in hip:
+            self.dlumped.load_set_state(s, hip["lumped"], n)
+        if self._dispersion is not None and "dispersion" in sd:
+            self._dispersion.load_reference_state(s, sd["dispersion"])
+        stop = hip.get("set_stop")
+        self.set_stops[s] = (stop["reason"], int(stop["step"])) if stop else None

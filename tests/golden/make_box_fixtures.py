@@ -14,7 +14,9 @@ defaults m 3, R 1e-8, kappa_max 7, alpha_max 0.02), a ``PlaneWaveSource``
 region and a ``MonitorFarField`` Huygens box 2 cells inside the absorber
 (monitors/field_frequency.py:249-294, far_field.py:123-248), 5 frequencies,
 150 steps, dt = courant_dt(grid, "normal"), float64 and float32.  Stored:
-e/h after 150 steps, every accumulator's bins, the CPML psi state.
+e/h after 150 steps, every accumulator's bins, the CPML psi state; after
+step 75 the solver's state_dict() and both monitors' result_dump()
+(flattened "a/b" keys).
 """
 
 from __future__ import annotations
@@ -26,9 +28,12 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+from _flat import flatten  # noqa: E402
 DIMS = (18, 16, 20)
 SIZE = 24e-3
 STEPS = 150
+CHECKPOINT = 75  # the reference's state_dict() (psi, fields) is stored after this step
 FREQS = np.array([4e9, 7e9, 10e9, 13e9, 16e9])
 LAYERS = 4
 FACES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
@@ -79,13 +84,18 @@ def main(ref_src: str) -> None:
         region = MonitorFieldFrequency(freqs=FREQS, corners=REGION, fields=["E", "H"], name="region")
         far = MonitorFarField(freqs=FREQS, name="far", margin_cells=2)
         sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, sources=[src],
-                                  monitors=[region, far], total_time_steps=STEPS, dt=dt,
+                                  monitors=[region, far], total_time_steps=CHECKPOINT, dt=dt,
                                   verbose=False, backend="numpy", precision=prec,
                                   energy_stop_db=None, port_signal_stop_db=None)
+        p = prec + "_"
         with warnings.catch_warnings():
             warnings.simplefilter("ignore")
             sol.run()
-        p = prec + "_"
+            out.update(flatten(sol.state_dict(), f"{p}state"))
+            for mon in (region, far):
+                out.update(flatten(mon.result_dump(), f"{p}dump{CHECKPOINT}_{mon.name}"))
+            sol.total_time_steps = STEPS
+            sol.run()
         out[p + "e"] = np.asarray(sol._fields.e_flat, np.float64).copy()
         out[p + "h"] = np.asarray(sol._fields.h_flat, np.float64).copy()
         for comp in region._components:

@@ -20,7 +20,8 @@ plain cases for 300 steps:
 port 1 mode 0 excited with the modulated Gaussian from the TE10 cut-off to
 25 GHz, dt = courant_dt(grid, "normal"), float64 (and float32 for the
 guide).  Stored: the ports' numerical mode profiles and cut-offs, alpha/beta
-(E, H) of the solver, e/h after steps 1, 10 and 300 (reference flat layout),
+(E, H) of the solver, e/h after steps 1, 10, 150 and 300 (reference flat
+layout), the solver's state_dict() after step 150 (flattened "a/b" keys),
 the recorded V/I of every channel and the S-parameters of the excited column
 on 41 frequencies.
 """
@@ -35,9 +36,12 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+from _flat import flatten  # noqa: E402
 A, B = 22.86e-3, 10.16e-3
 F_MAX = 25e9
-SNAPS = (1, 10, 300)
+SNAPS = (1, 10, 150, 300)
+CHECKPOINT = 150  # the reference's state_dict() is stored after this step
 
 
 def graded_axis(n, length, seed):
@@ -118,6 +122,8 @@ def main(ref_src: str) -> None:
                     sol.run()
                     out[f"{p}e{n}"] = np.asarray(sol._fields.e_flat, dtype=np.float64).copy()
                     out[f"{p}h{n}"] = np.asarray(sol._fields.h_flat, dtype=np.float64).copy()
+                    if n == CHECKPOINT:
+                        out.update(flatten(sol.state_dict(), f"{p}state"))
             out[p + "alpha_e"] = np.asarray(sol._alpha_E, np.float64)
             out[p + "beta_e"] = np.asarray(sol._beta_E, np.float64)
             out[p + "alpha_h"] = np.asarray(sol._alpha_H, np.float64)

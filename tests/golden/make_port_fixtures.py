@@ -17,7 +17,11 @@ ports/_modal/factory.py:1367) on
   spacing ratio <= 1.3), uniform along z;
 
 both with PEC x/y walls and 3-mode numerical ports (lowest cut-offs) on
-z_min and z_max, dt = courant_dt(grid, "normal"), f_calc = 25 GHz.
+z_min and z_max, dt = courant_dt(grid, "normal"), f_calc = 25 GHz.  Also
+stored, under ``op_``: every attribute of the built ``PortOperatorModal``
+that a backend reads to run the port (plane maps and geometry, calibrated
+mode profiles, mass slices, Mur and DTBC chain parameters) — the adapter
+tests build reference-shaped stand-ins from them.
 """
 
 from __future__ import annotations
@@ -79,6 +83,26 @@ def main(ref_src: str) -> None:
             pl = op.plane
             out[p + "e_u_indices"] = np.asarray(pl.e_u_indices, np.int64)
             out[p + "e_v_indices"] = np.asarray(pl.e_v_indices, np.int64)
+            o = p + "op_"
+            out[o + "face"] = np.array(pl.face.value)
+            for k in ("coordinate", "normal_dx"):
+                out[o + "plane_" + k] = np.float64(getattr(pl, k))
+            for k in ("e_u_indices", "h_v_indices", "u_edge_uv", "u_edge_lengths", "e_v_indices",
+                      "h_u_indices", "v_edge_uv", "v_edge_lengths", "e_u_indices_interior",
+                      "e_v_indices_interior", "u_node_window", "v_node_window"):
+                out[o + "plane_" + k] = np.asarray(getattr(pl, k))
+            for k in ("_me_u_port", "_me_v_port", "_me_u_int", "_me_v_int", "_mh_u", "_mh_v",
+                      "_mur_r", "_tau_m"):
+                out[o + k] = np.asarray(getattr(op, k), float)
+            for k in ("_dtbc_r", "_dtbc_q", "_dtbc_z0", "_dtbc_pair_spread"):
+                out[o + k] = np.array([np.nan if v is None else v for v in getattr(op, k)], float)
+            out[o + "_dt"] = np.float64(op._dt)
+            out[o + "_chain_slab_defect"] = np.float64(op._chain_slab_defect or 0.0)
+            for m, dm in enumerate(op.discrete_modes):
+                out[f"{o}mode{m}_name"] = np.array(dm.mode.name)
+                out[f"{o}mode{m}_type"] = np.array(dm.mode.mode_type.value)
+                out[f"{o}mode{m}_omega_c"] = np.float64(dm.mode.omega_c)
+                out[f"{o}mode{m}_epsilon_r"] = np.float64(dm.mode.epsilon_r)
         np.savez_compressed(HERE / f"ports_{name}.npz", **out)
         print(f"wrote ports_{name}.npz: " + ", ".join(f"{k}" for k in out if k.startswith("port1_")
                                                        and not k[-1].isdigit()))

@@ -1,0 +1,152 @@
+"""Checkpoints that interchange with the reference's (fit_td.py:1264-1353).
+
+* WR-90 guide (tests/golden/c1_guide.npz): after 150 float64 steps the HIP
+  solver's state_dict() holds the reference's keys with the reference's
+  values — fields, every port's V_port_prev / V_interior_prev / src_buffer
+  / src_maxlen and per-mode DTBC u, u_prev, n, w_hist, s_hist; a fresh HIP
+  solver loaded with the REFERENCE's step-150 checkpoint runs on to the
+  reference's step-300 fields and V/I;
+* CPML box (tests/golden/box_tfsf_dft.npz): after 75 steps the CPML psi
+  under the reference's names (_psi_Ey ...) and both monitors'
+  result_dump() (region bins + geometry; far-field faces, geometry, bins)
+  match the reference's; the reference's step-75 checkpoint and dumps load
+  into a fresh solver that runs on to the reference's step-150 fields and
+  bins.
+"""
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import cases
+
+sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+from _flat import unflatten  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(got, want, rtol, what, scale=None):
+    got, want = np.asarray(got, float), np.asarray(want, float)
+    assert got.shape == want.shape, (what, got.shape, want.shape)
+    ref = scale if scale is not None else max(float(np.abs(want).max()), 1e-300)
+    err = float(np.abs(got - want).max()) / ref if want.size else 0.0
+    assert err <= rtol, (what, err)
+    return err
+
+
+def test_c1_checkpoint_interchange(gpu):
+    f = cases.golden_c1("c1_guide")
+    ref = unflatten(f, "double_state")
+    n0 = int(ref["n_completed"])
+    print(f"HIP C1 guide float64 to step {n0}, state_dict vs the reference's", flush=True)
+    sol, ports, dt = cases.golden_c1_solver("c1_guide", "double")
+    sol.run(n0)
+    sd = sol.state_dict()
+    assert sd["n_completed"] == n0
+    _close(sd["e"], ref["e"], 1e-10, "e")
+    _close(sd["h"], ref["h"], 1e-10, "h")
+    assert set(sd["ports"]) == set(ref["ports"])
+    for name, rp in ref["ports"].items():
+        hp = sd["ports"][name]
+        vs = max(float(np.abs(rp["V_port_prev"]).max()), 1e-300)
+        _close(hp["V_port_prev"], rp["V_port_prev"], 1e-9, (name, "V_port_prev"), vs)
+        _close(hp["V_interior_prev"], rp["V_interior_prev"], 1e-9, (name, "V_interior_prev"), vs)
+        assert hp["src_maxlen"] == int(rp["src_maxlen"]), name
+        assert np.asarray(hp["src_buffer"]).size == np.asarray(rp["src_buffer"]).size
+        assert set(hp["dtbc"]) == set(rp["dtbc"]), name
+        for m, rt in rp["dtbc"].items():
+            ht = hp["dtbc"][m]
+            assert ht["n"] == int(rt["n"]) == n0
+            # w = u - u_inc cancels while the incident wave sits on the
+            # boundary: gate the histories on the scale of the channel's
+            # amplitudes (incident s, boundary u, port voltage), not on w
+            sc = max(float(np.abs(rt["s_hist"]).max()), abs(float(rt["u"])), abs(float(rt["u_prev"])),
+                     float(np.abs(rt["w_hist"]).max()), vs)
+            _close([ht["u"], ht["u_prev"]], [rt["u"], rt["u_prev"]], 1e-9, (name, m, "u"), sc)
+            _close(ht["w_hist"], rt["w_hist"], 1e-9, (name, m, "w_hist"), sc)
+            _close(ht["s_hist"], rt["s_hist"], 1e-12, (name, m, "s_hist"), sc)
+    print("HIP C1 guide: fresh solver from the reference's checkpoint, on to step 300", flush=True)
+    fresh, _, _ = cases.golden_c1_solver("c1_guide", "double")
+    fresh.load_state_dict(ref)
+    fresh.run()
+    e, h = fresh.fields(0)
+    err = _close(e, f["double_e300"], 1e-9, "e300", np.linalg.norm(f["double_e300"]) / np.sqrt(e.size))
+    print(f"  step 300 e: max err / rms {err:.2e}", flush=True)
+    assert np.linalg.norm(e - f["double_e300"]) <= 1e-10 * np.linalg.norm(f["double_e300"])
+    assert np.linalg.norm(h - f["double_h300"]) <= 1e-10 * np.linalg.norm(f["double_h300"])
+    vmax = max(np.abs(f[k]).max() for k in f if k.startswith("double_V_"))
+    for (lbl, m), (V, I) in fresh.recorded(0).items():
+        assert np.max(np.abs(V[n0:] - f[f"double_V_{lbl}_{m}"][n0:])) <= 1e-9 * vmax
+
+
+def _box_solver(precision="double"):
+    from magnelio_amd import mesh as mm
+    from magnelio_amd.boundaries import CPMLBoundary
+    from magnelio_amd.monitors import MonitorFarField, MonitorFieldFrequency
+    from magnelio_amd.solver import FITTimeDomainSolver
+    from magnelio_amd.sources import PlaneWaveSource
+
+    f, mk = cases.golden_box()
+    g = mm.GridLines(f["x"], f["y"], f["z"])
+    mesh = mm.Mesh.from_grid(g, regions=[(mm.Material.dielectric(e), box) for e, box in mk.BRICKS],
+                             pec_faces=())
+    bcs = {fc: CPMLBoundary(fc, g, thickness_cells=mk.LAYERS) for fc in mk.FACES}
+    src = PlaneWaveSource(direction=(0, 0, 1), polarization=(1, 0, 0), corners=mk.TFSF, f_max=20e9)
+    mons = [MonitorFieldFrequency(mk.FREQS, corners=mk.REGION, fields=("E", "H"), block=64, name="region"),
+            MonitorFarField(mk.FREQS, margin_cells=2, block=64, name="far")]
+    sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, sources=[src], monitors=mons,
+                              total_time_steps=mk.STEPS, dt=mm.courant_dt(g, "normal"),
+                              precision=precision, record=False)
+    return f, mk, sol
+
+
+def _dump_close(got, want, what, gate=1e-6):
+    peak = max(float(np.abs(np.asarray(b)).max()) for b in want.values())
+    for c, b in want.items():
+        g, w = np.asarray(got[c]), np.asarray(b)
+        _close(np.stack([g.real, g.imag]), np.stack([w.real, w.imag]), gate, (what, c), peak)
+
+
+def test_box_checkpoint_interchange(gpu):
+    f, mk, sol = _box_solver()
+    ref = unflatten(f, "double_state")
+    n0 = int(ref["n_completed"])
+    print(f"HIP box float64 to step {n0}, state_dict vs the reference's", flush=True)
+    sol.run(n0)
+    sd = sol.state_dict()
+    _close(sd["e"], ref["e"], 1e-10, "e")
+    assert set(sd["boundaries"]) == set(ref["boundaries"])
+    for face, rb in ref["boundaries"].items():
+        assert set(sd["boundaries"][face]) == set(rb), face
+        for k, v in rb.items():
+            _close(sd["boundaries"][face][k], v, 1e-9, (face, k), max(float(np.abs(v).max()), 1e-30))
+    rr = unflatten(f, f"double_dump{n0}_region")
+    hr = sd["monitors"]["region"]
+    for k in ("grid_x", "grid_y", "grid_z", "freqs", "corners"):
+        assert np.allclose(hr[k], rr[k], rtol=1e-14, atol=0), k
+    assert list(hr["components"]) == [str(c) for c in rr["components"]]
+    _dump_close(hr["bins"], rr["bins"], "region")
+    rf = unflatten(f, f"double_dump{n0}_far")
+    hf = sd["monitors"]["far"]
+    rfaces = {str(v["name"]): v for v in rf["faces"].values()}
+    assert [fc["name"] for fc in hf["faces"]] == [str(rf["faces"][str(i)]["name"]) for i in range(len(rfaces))]
+    for fc in hf["faces"]:
+        r = rfaces[fc["name"]]
+        for k in ("axis", "sign", "plane", "c1", "c2", "w1", "w2"):
+            assert np.allclose(fc[k], r[k], rtol=1e-14, atol=0), (fc["name"], k)
+        _dump_close(fc["bins"], r["bins"], ("far", fc["name"]))
+    print("HIP box: fresh solver from the reference's checkpoint and dumps, on to step 150", flush=True)
+    f, mk, fresh = _box_solver()
+    ck = dict(ref)
+    ck["monitors"] = {"region": rr, "far": rf}
+    fresh.load_state_dict(ck)
+    fresh.run()
+    e, h = fresh.fields(0)
+    assert np.linalg.norm(e - f["double_e"]) <= 1e-10 * np.linalg.norm(f["double_e"])
+    assert np.linalg.norm(h - f["double_h"]) <= 1e-10 * np.linalg.norm(f["double_h"])
+    got = fresh.monitors[0].result_dump(0)["bins"]
+    _dump_close(got, {c: f[f"double_region_{c}"] for c in got}, "region final")
+    print("  step 150 fields and bins match the reference's uninterrupted run", flush=True)
