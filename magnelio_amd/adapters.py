@@ -29,6 +29,8 @@ imported), so the reference's meshes, ports and results stay what they are.
                           (conformal cells, thin wires, donors), which the
                           solver then uses instead of its own cell-category
                           build
+* ``recorder_signals``  the solver's recorded V/I as PortSignalRecorder.
+                          finalize returns them (ports/recorder.py:286-340)
 """
 
 from __future__ import annotations
@@ -219,3 +221,24 @@ def solver_from_reference(*, mesh, boundary_conditions=None, ports=(), sources=(
         max_time_steps=max_time_steps, energy_check_interval=energy_check_interval,
         precision=precision or "single", nsets=len(exc), excitations=exc,
         mass_matrices=mass_matrices, **solver_kw)
+
+
+def recorder_signals(solver, set_index: int = 0, signal_type=None) -> dict:
+    """Field set ``set_index``'s recorded V/I in the shape the reference's
+    ``PortSignalRecorder.finalize`` returns (ports/recorder.py:286-340):
+    ``{(port, mode): (V, I)}`` over ``t = arange(N) * dt``, with the
+    per-mode record scale already applied.  ``signal_type`` — the
+    reference's ``Signal1D`` — wraps each series as
+    ``signal_type(t=, values=, dt=, label=)`` with the recorder's labels
+    ``"{port}_mode{m}_V"`` / ``"_I"``; without it the pairs are plain
+    float64 arrays."""
+    rec = solver.recorded(set_index)
+    if signal_type is None:
+        return rec
+    dt = float(solver.dt)
+    out = {}
+    for (name, m), (V, I) in rec.items():
+        t = np.arange(V.size) * dt
+        out[(name, m)] = (signal_type(t=t, values=V, dt=dt, label=f"{name}_mode{m}_V"),
+                          signal_type(t=t, values=I, dt=dt, label=f"{name}_mode{m}_I"))
+    return out

@@ -178,3 +178,28 @@ def test_resolve_backend_never_falls_back(monkeypatch):
         with pytest.raises(backend.HipDeviceUnavailable):
             backend.resolve_backend("auto")
         backend._PROBED.clear()
+
+
+def test_recorder_signals_shape_of_finalize():
+    """recorder_signals gives PortSignalRecorder.finalize's mapping: one
+    (V, I) pair per (port, mode) over t = arange(N) dt, labelled
+    "{port}_mode{m}_V" / "_I" (ports/recorder.py:319-340)."""
+    from dataclasses import dataclass
+
+    @dataclass(frozen=True)
+    class Sig:
+        t: np.ndarray
+        values: np.ndarray
+        dt: float
+        label: str = ""
+
+    V = np.arange(5.0)
+    rec = {("port1", 0): (V, -V), ("port2", 1): (2 * V, V + 1)}
+    sol = NS(dt=1.5e-12, recorded=lambda s: rec if s == 0 else {})
+    assert adapters.recorder_signals(sol, 0) is rec
+    out = adapters.recorder_signals(sol, 0, Sig)
+    assert set(out) == set(rec)
+    v, i = out[("port2", 1)]
+    assert v.label == "port2_mode1_V" and i.label == "port2_mode1_I"
+    assert np.array_equal(v.values, 2 * V) and np.array_equal(i.values, V + 1)
+    assert np.allclose(v.t, np.arange(5) * 1.5e-12) and v.dt == 1.5e-12
