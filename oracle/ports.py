@@ -25,9 +25,10 @@ eigenpairs of an independently assembled curl-curl pencil to 1e-12, are
 M_eps-orthonormal and divergence-free; the calibration makes the unit
 travelling wave satisfy V/I = Z_modal exactly; the discrete TE10 chain
 parameters satisfy r = c dt / dz and q = w_c dt.  ``record_scale`` /
-``source_scale`` / ``state_scale`` have no closed form: parity unpinned
-beyond their defining identities (checked in the tests) — the absolute
-V/I unit cancels in every S-parameter between identical ports.
+``source_scale`` / ``state_scale`` have no closed form; they are pinned by
+the reference's own values on the uniform WR-90 guide
+(tests/test_golden_ports.py, tests/golden/ports_wr90.npz), and the whole
+port step by the reference's 300-step runs (tests/test_golden_c1.py).
 """
 
 from __future__ import annotations
