@@ -248,9 +248,6 @@ __device__ __forceinline__ void pml_term(const PFace<T>& F, int term, int n0, in
 
 // The CPML terms of one component, faces in descriptor order.  AXA/AXB: the
 // component's two normal axes; TA/TB: its term index on those faces; SA/SB:
-// the signs; dA/dB: the one-sided differences along them.
-// The CPML terms of one component, faces in descriptor order.  AXA/AXB: the
-// component's two normal axes; TA/TB: its term index on those faces; SA/SB:
 // the signs; dA/dB: the one-sided differences along them.  The loop is kept
 // rolled, its face records read as uniform scalar loads: unrolled, the
 // compiler holds every face's record in scalar registers, spills them and
@@ -318,13 +315,25 @@ __global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __r
       if (s0 + q >= nsets) break;
       const T* hs = h + (int64_t)(s0 + q) * h_set;
       T* es = e + (int64_t)(s0 + q) * e_set;
-      // H words around the node (zero where the face does not exist)
-      const T hy = (inx && knz) ? hs[oHy + (int64_t)k * pHy + rN] : T(0);
-      const T hx = (iny && knz) ? hs[(int64_t)k * pHx + rN1] : T(0);
-      const T hz = (inx && iny) ? hs[oHz + (int64_t)k * pHz + rN] : T(0);
+      // Every word of this set at this plane is loaded before its first store.
+      // The three E components share one base pointer, and the loads sit in
+      // the branches of components that may not exist, so a load placed after
+      // an earlier component's store would stay there: each component would
+      // cost the wave a memory round trip of its own.
+      T hy = T(0), hx = T(0), hz = T(0), hz_jm = T(0), hz_im = T(0), hy_im = T(0), hx_jm = T(0);
+      T ex = T(0), ey = T(0), ez = T(0);
+      if (inx && knz) hy = hs[oHy + (int64_t)k * pHy + rN];
+      if (iny && knz) hx = hs[(int64_t)k * pHx + rN1];
+      if (inx && iny) hz = hs[oHz + (int64_t)k * pHz + rN];
+      if (inx && j > 0) hz_jm = hs[oHz + (int64_t)k * pHz + rN - nx];
+      if (iny && i > 0) hz_im = hs[oHz + (int64_t)k * pHz + rN - 1];
+      if (knz && i > 0) hy_im = hs[oHy + (int64_t)k * pHy + rN - 1];
+      if (knz && j > 0) hx_jm = hs[(int64_t)k * pHx + rN1 - (nx + 1)];
+      if (inx) ex = es[iEx];
+      if (iny) ey = es[iEy];
+      if (knz) ez = es[iEz];
 
       if (inx) {  // Ex(i,j,k): dHz/dy - dHy/dz
-        const T hz_jm = (j > 0) ? hs[oHz + (int64_t)k * pHz + rN - nx] : T(0);
         T v;
         if constexpr (CURL64) {
           double c = 0.0;
@@ -332,23 +341,22 @@ __global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __r
           if (j > 0) c -= (double)hz_jm;
           if (knz) c -= (double)hy;
           if (k > 0) c += (double)hy_prev[q];
-          v = (T)((double)(a_x * es[iEx]) + (double)b_x * c);
+          v = (T)((double)(a_x * ex) + (double)b_x * c);
         } else {
           T c = T(0);
           if (iny) c += hz;
           if (j > 0) c -= hz_jm;
           if (knz) c -= hy;
           if (k > 0) c += hy_prev[q];
-          v = (a_x * es[iEx]) + (b_x * c);
+          v = (a_x * ex) + (b_x * c);
         }
         if constexpr (PML) {  // Ex: z faces d = Hy(k) - Hy(k-1) (+), y faces Hz(j) - Hz(j-1) (-)
-          if (hit) pml_comp<T, 2, 0, +1, 1, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q, hy - hy_prev[q],
-                                                   hz - hz_jm, b_x, &v);
+          if (hit) pml_comp<T, 2, 0, +1, 1, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q,
+                                                   hy - hy_prev[q], hz - hz_jm, b_x, &v);
         }
         es[iEx] = v;
       }
       if (iny) {  // Ey(i,j,k): dHx/dz - dHz/dx
-        const T hz_im = (i > 0) ? hs[oHz + (int64_t)k * pHz + rN - 1] : T(0);
         T v;
         if constexpr (CURL64) {
           double c = 0.0;
@@ -356,24 +364,22 @@ __global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __r
           if (k > 0) c -= (double)hx_prev[q];
           if (inx) c -= (double)hz;
           if (i > 0) c += (double)hz_im;
-          v = (T)((double)(a_y * es[iEy]) + (double)b_y * c);
+          v = (T)((double)(a_y * ey) + (double)b_y * c);
         } else {
           T c = T(0);
           if (knz) c += hx;
           if (k > 0) c -= hx_prev[q];
           if (inx) c -= hz;
           if (i > 0) c += hz_im;
-          v = (a_y * es[iEy]) + (b_y * c);
+          v = (a_y * ey) + (b_y * c);
         }
         if constexpr (PML) {  // Ey: z faces d = Hx(k) - Hx(k-1) (-), x faces Hz(i) - Hz(i-1) (+)
-          if (hit) pml_comp<T, 2, 1, -1, 0, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q, hx - hx_prev[q],
-                                                   hz - hz_im, b_y, &v);
+          if (hit) pml_comp<T, 2, 1, -1, 0, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q,
+                                                   hx - hx_prev[q], hz - hz_im, b_y, &v);
         }
         es[iEy] = v;
       }
       if (knz) {  // Ez(i,j,k): dHy/dx - dHx/dy
-        const T hy_im = (i > 0) ? hs[oHy + (int64_t)k * pHy + rN - 1] : T(0);
-        const T hx_jm = (j > 0) ? hs[(int64_t)k * pHx + rN1 - (nx + 1)] : T(0);
         T v;
         if constexpr (CURL64) {
           double c = 0.0;
@@ -381,18 +387,18 @@ __global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __r
           if (i > 0) c -= (double)hy_im;
           if (iny) c -= (double)hx;
           if (j > 0) c += (double)hx_jm;
-          v = (T)((double)(a_z * es[iEz]) + (double)b_z * c);
+          v = (T)((double)(a_z * ez) + (double)b_z * c);
         } else {
           T c = T(0);
           if (inx) c += hy;
           if (i > 0) c -= hy_im;
           if (iny) c -= hx;
           if (j > 0) c += hx_jm;
-          v = (a_z * es[iEz]) + (b_z * c);
+          v = (a_z * ez) + (b_z * c);
         }
         if constexpr (PML) {  // Ez: y faces d = Hx(j) - Hx(j-1) (+), x faces Hy(i) - Hy(i-1) (-)
-          if (hit) pml_comp<T, 1, 1, +1, 0, 1, -1>(pa, nx + 1, ny + 1, i, j, k, s0 + q, hx - hx_jm,
-                                                   hy - hy_im, b_z, &v);
+          if (hit) pml_comp<T, 1, 1, +1, 0, 1, -1>(pa, nx + 1, ny + 1, i, j, k, s0 + q,
+                                                   hx - hx_jm, hy - hy_im, b_z, &v);
         }
         es[iEz] = v;
       }
@@ -454,38 +460,50 @@ __global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __r
       if (s0 + q >= nsets) break;
       const T* es = e + (int64_t)(s0 + q) * e_set;
       T* hs = h + (int64_t)(s0 + q) * h_set;
-      // E at plane k+1 (exists for k+1 <= nz)
-      const T ey_kp = (iny && knz) ? es[oEy + (int64_t)(k + 1) * pEy + rN1] : T(0);
-      const T ex_kp = (inx && knz) ? es[(int64_t)(k + 1) * pEx + rN] : T(0);
-      const T ez = knz ? es[oEz + (int64_t)k * pEz + rN1] : T(0);
+      // Every word of this set at this plane is loaded before its first store
+      // (see k_update_e).  E at plane k+1 exists for k+1 <= nz.
+      T ey_kp = T(0), ex_kp = T(0), ez = T(0), ez_jp = T(0), ez_ip = T(0), ey_ip = T(0),
+        ex_jp = T(0), hxv = T(0), hyv = T(0), hzv = T(0);
+      if (iny && knz) ey_kp = es[oEy + (int64_t)(k + 1) * pEy + rN1];
+      if (inx && knz) ex_kp = es[(int64_t)(k + 1) * pEx + rN];
+      if (knz) ez = es[oEz + (int64_t)k * pEz + rN1];
+      if (hasHx) {
+        ez_jp = es[oEz + (int64_t)k * pEz + rN1 + (nx + 1)];
+        hxv = hs[iHx];
+      }
+      if (hasHy) {
+        ez_ip = es[oEz + (int64_t)k * pEz + rN1 + 1];
+        hyv = hs[iHy];
+      }
+      if (hasHz) {
+        ey_ip = es[oEy + (int64_t)k * pEy + rN1 + 1];
+        ex_jp = es[(int64_t)k * pEx + rN + nx];
+        hzv = hs[iHz];
+      }
       if (hasHx) {  // Hx(i,j,k): dEz/dy - dEy/dz
-        const T ez_jp = es[oEz + (int64_t)k * pEz + rN1 + (nx + 1)];
         const T c = ((ez_jp - ez) - ey_kp) + ey_k[q];
-        T v = (a_x * hs[iHx]) - (b_x * c);
+        T v = (a_x * hxv) - (b_x * c);
         if constexpr (PML) {  // Hx: z faces d = Ey(k+1) - Ey(k) (-), y faces Ez(j+1) - Ez(j) (+)
-          if (hit) pml_comp<T, 2, 0, -1, 1, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q, ey_kp - ey_k[q],
-                                                   ez_jp - ez, b_x, &v);
+          if (hit) pml_comp<T, 2, 0, -1, 1, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q,
+                                                   ey_kp - ey_k[q], ez_jp - ez, b_x, &v);
         }
         hs[iHx] = v;
       }
       if (hasHy) {  // Hy(i,j,k): dEx/dz - dEz/dx
-        const T ez_ip = es[oEz + (int64_t)k * pEz + rN1 + 1];
         const T c = ((ex_kp - ex_k[q]) - ez_ip) + ez;
-        T v = (a_y * hs[iHy]) - (b_y * c);
+        T v = (a_y * hyv) - (b_y * c);
         if constexpr (PML) {  // Hy: z faces d = Ex(k+1) - Ex(k) (+), x faces Ez(i+1) - Ez(i) (-)
-          if (hit) pml_comp<T, 2, 1, +1, 0, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q, ex_kp - ex_k[q],
-                                                   ez_ip - ez, b_y, &v);
+          if (hit) pml_comp<T, 2, 1, +1, 0, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q,
+                                                   ex_kp - ex_k[q], ez_ip - ez, b_y, &v);
         }
         hs[iHy] = v;
       }
       if (hasHz) {  // Hz(i,j,k): dEy/dx - dEx/dy
-        const T ey_ip = es[oEy + (int64_t)k * pEy + rN1 + 1];
-        const T ex_jp = es[(int64_t)k * pEx + rN + nx];
         const T c = ((ey_ip - ey_k[q]) - ex_jp) + ex_k[q];
-        T v = (a_z * hs[iHz]) - (b_z * c);
+        T v = (a_z * hzv) - (b_z * c);
         if constexpr (PML) {  // Hz: y faces d = Ex(j+1) - Ex(j) (-), x faces Ey(i+1) - Ey(i) (+)
-          if (hit) pml_comp<T, 1, 1, -1, 0, 1, +1>(pa, nx, ny, i, j, k, s0 + q, ex_jp - ex_k[q],
-                                                   ey_ip - ey_k[q], b_z, &v);
+          if (hit) pml_comp<T, 1, 1, -1, 0, 1, +1>(pa, nx, ny, i, j, k, s0 + q,
+                                                   ex_jp - ex_k[q], ey_ip - ey_k[q], b_z, &v);
         }
         hs[iHz] = v;
       }
