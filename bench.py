@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_update_e launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (bench.py --pmc-probe), FETCH scaled x2.0 and WRITE x1.0 from a 1 GiB device copy:
 # {(n, K, CPML fused into the launch): bytes}
-K_UPDATE_E_TRAFFIC = {(256, 1, True): 1.331e9}
+K_UPDATE_E_TRAFFIC = {(256, 1, True): 1.308e9}
 
 
 def c2_mesh(n: int = 256, seed: int = 2024, n_objects: int = 40, length: float = 64e-3):
