@@ -254,6 +254,11 @@ int mgh_tfsf_apply(int dtype, int nsets, void* field, long long set_stride, int 
  * (fit_td.py:382-403).  n dispersive E edges idx (int64, device layout,
  * disjoint), g = EPS0 A_dual/L and beta_E at those edges (field dtype), the
  * edge's block (material) blk (int32).  fprev: nsets x n.
+ * The same two calls serve the mu(w) mirror (the H side of the same
+ * operator, _dispersion.py:45-76; fit_td.py:835-836, :906-907): e is then
+ * the H vector, idx its dispersive faces, g = MU0 A_primal/L_dual and beta
+ * beta_H, the stash at the top of the step and the update right after the
+ * H curl (before the CPML H corrections).
  *   save:   fprev = e[idx]
  *   update: j = sum_q hw_q Re((1 + k_q) J_q) (first term seeds);
  *           e[idx] -= beta j;  gd = g (e[idx] - fprev);

@@ -63,18 +63,19 @@ def _triple(v) -> tuple:
 
 
 def material_from_reference(mat) -> _mesh.Material:
-    """Material (src/magnelio/materials/material.py:18-101); a dispersive one
+    """Material (src/magnelio/materials/material.py:18-145); a dispersive one
     carries its DispersionModel's pole-residue pairs (materials/dispersion.py:
-    40-103).  A permeability model (``dispersion_mu``) is not served."""
-    if getattr(mat, "dispersion_mu", None) is not None:
-        raise NotImplementedError(f"material {mat.name!r}: mu(omega) dispersion is not served by "
-                                  "this backend")
-    disp = getattr(mat, "dispersion", None)
-    poles = tuple((complex(a), complex(r)) for a, r in disp.poles) if disp is not None else ()
+    40-103) — ``dispersion`` for eps(w), ``dispersion_mu`` for mu(w) (whose
+    high-frequency limit is the material's ``mu``, material.py:106-117)."""
+    def poles_of(model):
+        return tuple((complex(a), complex(r)) for a, r in model.poles) if model is not None else ()
+
     return _mesh.Material(name=str(mat.name), epsilon=_triple(mat.epsilon), mu=_triple(mat.mu),
                           sigma=_triple(getattr(mat, "sigma", 0.0)),
                           sigma_m=_triple(getattr(mat, "sigma_m", 0.0)),
-                          is_pec=bool(getattr(mat, "is_pec", False)), poles=poles)
+                          is_pec=bool(getattr(mat, "is_pec", False)),
+                          poles=poles_of(getattr(mat, "dispersion", None)),
+                          poles_mu=poles_of(getattr(mat, "dispersion_mu", None)))
 
 
 def _pec_faces(bc) -> tuple:

@@ -1,22 +1,29 @@
-"""Pole-residue (ADE) dielectrics on the device, E side.
+"""Pole-residue (ADE) materials on the device: eps(w) on the E edges and
+mu(w) on the H faces.
 
-Reference: ``DispersionOperator`` (src/magnelio/solver/_dispersion.py:160-533)
+Reference: ``DispersionOperator`` (src/magnelio/solver/_dispersion.py:160-569)
 and its coefficient folding in ``FITTimeDomainSolver.setup``
-(src/magnelio/solver/fit_td.py:382-403).
+(src/magnelio/solver/fit_td.py:382-403 for E, :430-461 for H).
 
 A dispersive material is a :class:`magnelio_amd.mesh.Material` with
 ``epsilon`` = eps_inf and ``poles`` = ((a_p, r_p), ...) in rad/s, a real
 ``a_p`` giving a real pole and a complex one a conjugate pair stored once:
-eps(w) = eps_inf + sum_p r_p / (j w - a_p) (+ conjugate partners).
+eps(w) = eps_inf + sum_p r_p / (j w - a_p) (+ conjugate partners); and/or
+with ``mu`` = mu_inf and ``poles_mu`` for mu(w).
 
 Host side (FP64, once): one block per dispersive material (sorted by id)
-over the E edges its cells own (the clamped one-sided lookup of the mass
-matrices, PEC edges excluded), g = EPS0 A_dual / L_primal, the trapezoidal
-pole coefficients k = (1 + a dt/2)/(1 - a dt/2), c = r/(1 - a dt/2), and
+over the elements its cells own — the E edges by the clamped one-sided
+lookup of M_sigma, PEC edges excluded, g = EPS0 A_dual / L_primal; the H
+faces by the lookup of M_sigma_m, faces with M_mu = 0 excluded,
+g = MU0 A_primal / L_dual — the trapezoidal pole coefficients
+k = (1 + a dt/2)/(1 - a dt/2), c = r/(1 - a dt/2), and
 W = g sum_p weight_p Re(c_p) whose half-step 0.5 dt W the solver folds into
-alpha_E / beta_E.  Device side (csrc/mgh_ade.hip): the field stash at the
-top of the step and the fused history subtraction + pole advance right
-after the curl kernel, states in the field precision.
+alpha / beta of that side.  The H side is the same operator: the curl term
+enters only as the right-hand side the kernel multiplies by beta, whatever
+its sign, so the completion f -= beta j_hist and the pole advance are
+identical (_dispersion.py:51-76).  Device side (csrc/mgh_ade.hip): the field
+stash at the top of the step and the fused history subtraction + pole
+advance right after that side's curl kernel, states in the field precision.
 """
 
 from __future__ import annotations
@@ -25,7 +32,7 @@ import numpy as np
 import torch
 
 from . import _lib, layout
-from .mesh import EPS0, _avg_d, _clamp_idx
+from .mesh import EPS0, MU0, _avg_d, _clamp_idx
 
 TAB = 8
 
@@ -37,6 +44,21 @@ def _geom_e(grid) -> np.ndarray:
     gy = (ax[:, None, None] * az[None, None, :] / dy[None, :, None]).ravel()
     gz = (ax[:, None, None] * ay[None, :, None] / dz[None, None, :]).ravel()
     return np.concatenate([gx, gy, gz])
+
+
+def _geom_h(grid) -> np.ndarray:
+    ax, ay, az = _avg_d(grid.dx), _avg_d(grid.dy), _avg_d(grid.dz)
+    dx, dy, dz = grid.dx, grid.dy, grid.dz
+    gx = (dy[None, :, None] * dz[None, None, :] / ax[:, None, None]).ravel()
+    gy = (dx[:, None, None] * dz[None, None, :] / ay[None, :, None]).ravel()
+    gz = (dx[:, None, None] * dy[None, :, None] / az[None, None, :]).ravel()
+    return np.concatenate([gx, gy, gz])
+
+
+def _face_owner(mat_id: np.ndarray) -> np.ndarray:
+    nx, ny, nz = mat_id.shape
+    il, jl, kl = _clamp_idx(nx), _clamp_idx(ny), _clamp_idx(nz)
+    return np.concatenate([mat_id[il].ravel(), mat_id[:, jl].ravel(), mat_id[:, :, kl].ravel()])
 
 
 def _edge_owner(mat_id: np.ndarray) -> np.ndarray:
@@ -76,11 +98,12 @@ class _Pole:
 
 
 class DeviceDispersion:
-    """E-side ADE state of every dispersive material (one launch per hook)."""
+    """ADE state of every dispersive material of one side (one launch per hook)."""
 
-    def __init__(self, blocks, n_e: int, dt: float):
-        self.blocks = blocks  # [(mat_id, edge idx (reference layout), g float64, [poles])]
-        W = np.zeros(n_e)
+    def __init__(self, blocks, n_f: int, dt: float, side: str = "E"):
+        self.side = side
+        self.blocks = blocks  # [(mat_id, element idx (reference layout), g float64, [poles])]
+        W = np.zeros(n_f)
         for _, idx, g, poles in blocks:
             W[idx] += g * sum(p.w_coeff() for p in poles)
         self.W = W
@@ -88,24 +111,32 @@ class DeviceDispersion:
         self._bound = False
 
     @classmethod
-    def from_mesh(cls, mesh, dt: float, pec_e_flat: np.ndarray):
-        """None when no material is dispersive (the plain path stays untouched)."""
+    def from_mesh(cls, mesh, dt: float, excluded: np.ndarray, side: str = "E"):
+        """None when no material of ``side`` is dispersive (the plain path
+        stays untouched).  ``excluded``: elements that carry no state — the
+        unflattened PEC edge mask (E, _dispersion.py:296-306) or M_mu <= 0
+        (H, the frozen faces, fit_td.py:436-441)."""
+        if side not in ("E", "H"):
+            raise ValueError(f"side must be 'E' or 'H', got {side!r}")
+        attr = "poles" if side == "E" else "poles_mu"
         disp = [(mid, mat) for mid, mat in sorted(mesh.material_library.items())
-                if getattr(mat, "poles", ()) and not mat.is_pec]
+                if getattr(mat, attr, ()) and not mat.is_pec]
         if not disp:
             return None
-        own = _edge_owner(mesh.material_id)
-        geom = _geom_e(mesh.grid)
-        excluded = np.asarray(pec_e_flat, dtype=bool)
+        if side == "E":
+            own, const, geom = _edge_owner(mesh.material_id), EPS0, _geom_e(mesh.grid)
+        else:
+            own, const, geom = _face_owner(mesh.material_id), MU0, _geom_h(mesh.grid)
+        excluded = np.asarray(excluded, dtype=bool)
         blocks = []
         for mid, mat in disp:
             idx = np.nonzero((own == mid) & ~excluded)[0]
             if idx.size == 0:
                 continue
-            blocks.append((mid, idx, EPS0 * geom[idx], [_Pole(a, r, dt) for a, r in mat.poles]))
+            blocks.append((mid, idx, const * geom[idx], [_Pole(a, r, dt) for a, r in getattr(mat, attr)]))
         if not blocks:
             return None
-        return cls(blocks, own.size, dt)
+        return cls(blocks, own.size, dt, side)
 
     # -- device ---------------------------------------------------------
     def bind(self, solver) -> None:
@@ -113,7 +144,7 @@ class DeviceDispersion:
         dims, K = solver.dims, solver.nsets
         np_dt = solver._np_dtype
         dev = solver.e.device
-        beta = np.asarray(solver._coeffs.beta_e)
+        beta = np.asarray(solver._coeffs.beta_e if self.side == "E" else solver._coeffs.beta_h)
         idx = np.concatenate([b[1] for b in self.blocks])
         self.n = int(idx.size)
         self.pmax = max(len(b[3]) for b in self.blocks)
@@ -129,7 +160,7 @@ class DeviceDispersion:
         def up(a, dtype):
             return torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype)
 
-        self.idx = up(layout.index_to_device(idx, *dims, "E"), torch.int64)
+        self.idx = up(layout.index_to_device(idx, *dims, self.side), torch.int64)
         self.g = up(g, t_dt)
         self.beta = up(beta[idx].astype(np_dt), t_dt)
         self.blk = up(blk, torch.int32)
@@ -141,12 +172,17 @@ class DeviceDispersion:
         self.nsets = K
         self._bound = True
 
+    def _field(self, solver):
+        return solver.e if self.side == "E" else solver.h
+
     def save(self, solver, stream) -> None:
-        _lib.call("mgh_ade_save", self.code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
+        f = self._field(solver)
+        _lib.call("mgh_ade_save", self.code, self.nsets, f.data_ptr(), f.shape[1],
                   self.n, self.idx.data_ptr(), self.fprev.data_ptr(), int(stream.cuda_stream))
 
     def update(self, solver, stream) -> None:
-        _lib.call("mgh_ade_update", self.code, self.nsets, solver.e.data_ptr(), solver.e.shape[1],
+        f = self._field(solver)
+        _lib.call("mgh_ade_update", self.code, self.nsets, f.data_ptr(), f.shape[1],
                   self.n, self.idx.data_ptr(), self.g.data_ptr(), self.beta.data_ptr(),
                   self.fprev.data_ptr(), self.blk.data_ptr(), self.npoles.data_ptr(),
                   self.tab.data_ptr(), self.pmax, self.J.data_ptr(), int(stream.cuda_stream))

@@ -50,6 +50,9 @@ class Material:
     # pole-residue ADE model: list of (pole a_p, residue c_p) complex pairs in
     # rad/s (conjugate partners implied), eps(w) = eps_inf + sum c/(jw - a) + cc
     poles: tuple = ()
+    # the same for the permeability, mu(w) = mu_inf + sum c/(jw - a) + cc, with
+    # ``mu`` = mu_inf
+    poles_mu: tuple = ()
 
     @classmethod
     def air(cls) -> "Material":

@@ -360,14 +360,19 @@ class FITTimeDomainSolver:
         pec_e = _mesh.pec_flat(pec, dims)
         self._pec_e = pec_e
         w_e = w_h = None
-        self._dispersion = None
+        self._dispersion = self._dispersion_mu = None
         for src in self.sources:
             if hasattr(src, "prepare"):
                 src.prepare(self)
         # ADE: blocks over the unflattened PEC mask (_dispersion.py:296-306)
-        self._dispersion = DeviceDispersion.from_mesh(mesh, dt, pec_raw)
+        self._dispersion = DeviceDispersion.from_mesh(mesh, dt, pec_raw, "E")
         if self._dispersion is not None:
             w_e = self._dispersion.w_half
+        # mu(w): faces with M_mu = 0 (after the port flattening) stay frozen
+        # (fit_td.py:436-441)
+        self._dispersion_mu = DeviceDispersion.from_mesh(mesh, dt, m_mu <= 0, "H")
+        if self._dispersion_mu is not None:
+            w_h = self._dispersion_mu.w_half
         co = _mesh.update_coefficients(m_eps, m_mu, m_sig, m_sig_m, dt, pec_e, w_e, w_h)
         self._coeffs = co
         self._m_eps_host, self._m_mu_host = m_eps, m_mu
@@ -391,7 +396,8 @@ class FITTimeDomainSolver:
         why = None
         if self.mass_matrices is not None:
             why = "given mass matrices"
-        elif any(getattr(m, "poles", ()) for m in self.mesh.material_library.values()):
+        elif any(getattr(m, "poles", ()) or getattr(m, "poles_mu", ())
+                 for m in self.mesh.material_library.values()):
             why = "dispersive materials"
         elif self.sources:
             why = "sources"
@@ -418,7 +424,7 @@ class FITTimeDomainSolver:
         g = mesh.grid
         geo = np.concatenate([g.dx, g.dy, g.dz, _mesh._avg_d(g.dx), _mesh._avg_d(g.dy), _mesh._avg_d(g.dz)])
         self._port_faces = {P.face for P in self.ports}
-        self._dispersion = None
+        self._dispersion = self._dispersion_mu = None
         self._coeffs = None
         self._m_eps_host = self._m_mu_host = None
         if any(isinstance(bc, CPMLBoundary) for bc in self.boundary_conditions.values()):
@@ -469,8 +475,9 @@ class FITTimeDomainSolver:
         K = self.nsets
         self.e = torch.zeros((K, ne), dtype=t_dt, device=dev)
         self.h = torch.zeros((K, nh), dtype=t_dt, device=dev)
-        if self._dispersion is not None:
-            self._dispersion.bind(self)
+        for disp in (self._dispersion, self._dispersion_mu):
+            if disp is not None:
+                disp.bind(self)
 
         cpml_faces = []
         for face, bc in self.boundary_conditions.items():
@@ -527,12 +534,15 @@ class FITTimeDomainSolver:
         ev = self.timing_events.pop() if self.timing_events else None
         if self._dispersion is not None:
             self._dispersion.save(self, stream)
+        if self._dispersion_mu is not None:
+            self._dispersion_mu.save(self, stream)
         if ev is not None:
             ev[0].record(stream)
         tp = self.tile_plan
-        # CPML rides in the curl launches unless a listed launch or the
+        # CPML rides in the curl launches unless a listed launch or a
         # dispersion completion (which sits between curl and CPML) forbids it
-        fused = self.cpml if (self.cpml is not None and tp is None and self._dispersion is None) else None
+        fused = self.cpml if (self.cpml is not None and tp is None and self._dispersion is None
+                              and self._dispersion_mu is None) else None
         kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
                          kchunk=self.kchunk, live=tp.live_e if tp is not None else None, cpml=fused,
                          xcols=False, stream=stream)
@@ -559,6 +569,10 @@ class FITTimeDomainSolver:
                          stream=stream)
         if fused is not None:
             kernels.cpml_xcols_h(self.e, self.h, self.beta_h, dims, fused, stream=stream)
+        # reference H phase order: curl, mu(w) completion, CPML faces
+        # (fit_td.py:891-921)
+        if self._dispersion_mu is not None:
+            self._dispersion_mu.update(self, stream)
         if self.cpml is not None and fused is None:
             kernels.cpml_h(self.e, self.h, self.beta_h, dims, self.cpml, stream=stream)
         for src in self.sources:
@@ -895,6 +909,8 @@ class FITTimeDomainSolver:
             hip["lumped"] = self.dlumped.set_state(s, n)
         if self._dispersion is not None:
             sd["dispersion"] = self._dispersion.reference_state(s)
+        if self._dispersion_mu is not None:
+            sd["dispersion_mu"] = self._dispersion_mu.reference_state(s)
         if self.set_stops[s] is not None:
             hip["set_stop"] = {"reason": self.set_stops[s][0], "step": int(self.set_stops[s][1])}
         sd["hip"] = hip
@@ -984,5 +1000,7 @@ class FITTimeDomainSolver:
             self.dlumped.load_set_state(s, hip["lumped"], n)
         if self._dispersion is not None and "dispersion" in sd:
             self._dispersion.load_reference_state(s, sd["dispersion"])
+        if self._dispersion_mu is not None and "dispersion_mu" in sd:
+            self._dispersion_mu.load_reference_state(s, sd["dispersion_mu"])
         stop = hip.get("set_stop")
         self.set_stops[s] = (stop["reason"], int(stop["step"])) if stop else None

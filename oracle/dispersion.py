@@ -1,5 +1,5 @@
-"""Oracle: trapezoidal ADE for pole-residue dielectrics, E side (CPU).
-Test infrastructure only.
+"""Oracle: trapezoidal ADE for pole-residue materials, eps(w) on the E edges
+and mu(w) on the H faces (CPU).  Test infrastructure only.
 
 Restates src/magnelio/solver/_dispersion.py (paths relative to the
 reference repository root), the NumPy branch that the reference's array
@@ -13,6 +13,15 @@ backends run:
 * blocks             :258-381   one block per dispersive material (sorted by
                                 id): edges owned (clamped one-sided lookup)
                                 by that material and not PEC; g = EPS0 A_dual/L
+* H side (mu(w))     :45-76, :308-318   the same operator over the H faces:
+                                owner = the clamped lookup of build_M_sigma_m,
+                                g = MU0 A_primal / L_dual, frozen faces
+                                (M_mu <= 0) excluded, W folded into
+                                alpha_H / beta_H (fit_td.py:430-461); the
+                                field stash at the top of the step next to
+                                the E one, the completion right after the H
+                                curl, before the CPML (fit_td.py:826-836,
+                                :891-921)
 * precision          :383-414   states, g, f_prev, beta in the field dtype;
                                 k, c Python scalars (weak: they take the
                                 array's precision)
@@ -27,16 +36,17 @@ backends run:
 
 Pinned by tests/test_oracle_dispersion.py: a Drude DC pole equals the
 sigma = eps0 r conductor (the reference's test_solver_dispersion.py:129-162
-gate), each pole recursion realises the bilinear transform of r s/(s - a)
-on a sampled sinusoid, and a stored-once conjugate pair equals the explicit
-sum of its two complex poles.
+gate) and, on the H side, a mu-Drude DC pole equals the sigma* = mu0 r
+magnetic conductor; each pole recursion realises the bilinear transform of
+r s/(s - a) on a sampled sinusoid, and a stored-once conjugate pair equals
+the explicit sum of its two complex poles.
 """
 
 from __future__ import annotations
 
 import numpy as np
 
-from .mesh import EPS0, dual_widths
+from .mesh import EPS0, MU0, dual_widths
 
 
 def geom_e(x, y, z):
@@ -47,6 +57,25 @@ def geom_e(x, y, z):
     gy = (ax[:, None, None] * az[None, None, :] / dy[None, :, None]).ravel()
     gz = (ax[:, None, None] * ay[None, :, None] / dz[None, None, :]).ravel()
     return np.concatenate([gx, gy, gz])
+
+
+def geom_h(x, y, z):
+    """A_primal / L_dual for every H face (material_matrices.py:1250-1269)."""
+    dx, dy, dz = np.diff(x), np.diff(y), np.diff(z)
+    ax, ay, az = dual_widths(dx), dual_widths(dy), dual_widths(dz)
+    gx = (dy[None, :, None] * dz[None, None, :] / ax[:, None, None]).ravel()
+    gy = (dx[:, None, None] * dz[None, None, :] / ay[None, :, None]).ravel()
+    gz = (dx[:, None, None] * dy[None, :, None] / az[None, None, :]).ravel()
+    return np.concatenate([gx, gy, gz])
+
+
+def face_owner(mat_id):
+    """Owning cell material of every H face (_dispersion.py:309-316)."""
+    nx, ny, nz = mat_id.shape
+    ic = np.clip(np.arange(nx + 1), 0, max(nx - 1, 0))
+    jc = np.clip(np.arange(ny + 1), 0, max(ny - 1, 0))
+    kc = np.clip(np.arange(nz + 1), 0, max(nz - 1, 0))
+    return np.concatenate([mat_id[ic].ravel(), mat_id[:, jc].ravel(), mat_id[:, :, kc].ravel()])
 
 
 def edge_owner(mat_id):
@@ -80,18 +109,24 @@ class Block:
 
 
 class Dispersion:
-    """E-side ADE over the dispersive materials ({mat_id: [(a, r), ...]})."""
+    """ADE over the dispersive materials ({mat_id: [(a, r), ...]}) of one side:
+    "E" (excluded: the PEC edge mask) or "H" (excluded: the frozen faces,
+    M_mu <= 0)."""
 
-    def __init__(self, x, y, z, mat_id, poles_by_mid, dt, pec_e=None, pair_weight=None):
-        own = edge_owner(mat_id)
-        excluded = np.zeros(own.size, bool) if pec_e is None else np.asarray(pec_e, bool)
-        geom = geom_e(x, y, z)
+    def __init__(self, x, y, z, mat_id, poles_by_mid, dt, pec_e=None, pair_weight=None,
+                 side="E", excluded=None):
+        if side == "E":
+            own, geom, const = edge_owner(mat_id), geom_e(x, y, z), EPS0
+            excluded = pec_e if excluded is None else excluded
+        else:
+            own, geom, const = face_owner(mat_id), geom_h(x, y, z), MU0
+        excluded = np.zeros(own.size, bool) if excluded is None else np.asarray(excluded, bool)
         self.blocks = []
         for mid in sorted(poles_by_mid):
             idx = np.nonzero((own == mid) & ~excluded)[0]
             if idx.size == 0 or not poles_by_mid[mid]:
                 continue
-            g = EPS0 * geom[idx]
+            g = const * geom[idx]
             poles = [Pole(a, r, dt, idx.size, pair_weight) for a, r in poles_by_mid[mid]]
             self.blocks.append(Block(idx, g, poles))
         W = np.zeros(own.size)

@@ -3,13 +3,14 @@
 Follows src/magnelio/solver/fit_td.py:950-1007 for the parts this build
 covers, with t = n dt:
 
-  E phase  curl update (stencil or Numba order), CPML E corrections in
-           boundary order                                    :826-889
+  stash    ADE f_prev of the eps(w) edges and mu(w) faces   :826-836
+  E phase  curl update (stencil or Numba order), eps(w) completion,
+           CPML E corrections in boundary order              :837-889
   sources  TF/SF E injection at t + dt                       :965-967
   ports    update_e(fields, t + dt) in port order            :975-976
            (modal ports, then lumped ports / elements)
   record   V/I of every port (e at t^{n+1}, h at t^{n+1/2})  :981-982
-  H phase  curl update, CPML H corrections                   :891-927
+  H phase  curl update, mu(w) completion, CPML H corrections :891-927
   sources  TF/SF H injection at t + 3dt/2                    :991-993
   monitors record(fields, n, t, dt)                          :996-1002
 """
@@ -24,7 +25,7 @@ from . import yee
 class Oracle:
     def __init__(self, dims, alpha_e, beta_e, alpha_h, beta_h, dt, *, cpml=(), ports=(),
                  sources=(), monitors=(), lumped=(), dtype=np.float64, order="stencil",
-                 dispersion=None):
+                 dispersion=None, dispersion_mu=None):
         self.dims = dims
         self.dt = dt
         self.dtype = dtype
@@ -39,6 +40,7 @@ class Oracle:
         self.lumped = list(lumped)
         self.order = order
         self.dispersion = dispersion
+        self.dispersion_mu = dispersion_mu
         self.n = 0
 
     def step(self) -> None:
@@ -47,6 +49,8 @@ class Oracle:
         dims = self.dims
         if self.dispersion is not None:
             self.dispersion.save_field(self.e)
+        if self.dispersion_mu is not None:
+            self.dispersion_mu.save_field(self.h)
         yee.update_e(self.e, self.h, self.ae, self.be, *dims, order=self.order)
         if self.dispersion is not None:
             self.dispersion.update_field(self.e)
@@ -64,6 +68,8 @@ class Oracle:
             if getattr(el, "recorded", False):
                 el.record(self.e, self.h)
         yee.update_h(self.e, self.h, self.ah, self.bh, *dims)
+        if self.dispersion_mu is not None:
+            self.dispersion_mu.update_field(self.h)
         for f in self.cpml:
             f.update_h(self.e, self.h, self.bh)
         for s in self.sources:

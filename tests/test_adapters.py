@@ -117,9 +117,10 @@ def test_mesh_from_reference():
     assert np.array_equal(mm.build_m_eps(got), mm.build_m_eps(own))
     assert np.array_equal(mm.build_m_sigma(got), mm.build_m_sigma(own))
     assert got.material_library[1].poles == ((complex(-1e9, 2e10), complex(0, -3e10)),)
-    stand.material_library[1].dispersion_mu = NS(poles=())
-    with pytest.raises(NotImplementedError, match="mu"):
-        adapters.mesh_from_reference(stand)
+    assert got.material_library[1].poles_mu == ()
+    stand.material_library[1].dispersion_mu = NS(poles=((-2e9, 5e9),))
+    got = adapters.mesh_from_reference(stand)
+    assert got.material_library[1].poles_mu == ((complex(-2e9), complex(5e9)),)
 
 
 def test_boundaries_sources_monitors():
