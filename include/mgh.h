@@ -275,6 +275,29 @@ int mgh_ade_update(int dtype, int nsets, void* e, long long e_set_stride, int n,
                    const int* blk, const int* npoles, const void* tab, int pmax, void* J,
                    void* stream);
 
+/* ---- SIBC walls: lossy walls in the time-domain H update ------------------
+ * Replaces SIBCOperator.update_field (src/magnelio/solver/_sibc.py:213-249),
+ * called right after the H curl and the mu(w) completion, before the CPML H
+ * corrections (fit_td.py:909-916); the stash at the top of the step
+ * (save_field, :194-199, fit_td.py:839-840) is mgh_ade_save over the entry
+ * indices, and W = g r_inst is folded into M_sigma_m on the host
+ * (fit_td.py:419-429).  Entries: the wall blocks' faces concatenated in
+ * block order (n_entries; device-layout H indices entry_idx int64, block id
+ * blk int32; g = A/l_dual^2, beta_H and hprev (nsets x n_entries) in the
+ * field dtype).  Distinct faces face_idx (n_faces, int64) with their
+ * entries face_entries[face_ptr[f] .. face_ptr[f + 1]) in block order
+ * (int32), so a face booked by two walls takes both corrections in the
+ * reference's order:
+ *   h[f] = h[f] + beta_e (g_e sum_p hw_p u_p,e)     for each entry e of f in order
+ *   h_mid = 0.5 (h[idx_e] + hprev_e);  u_p,e = u_p,e k_p;  u_p,e += q_p h_mid
+ * tab: n_blocks x bmax x 4 field-dtype values (hw = (1 + k)/2, k, q, unused),
+ * nbr branches per block (int32); u: nsets x bmax x n_entries. */
+int mgh_sibc_update(int dtype, int nsets, void* h, long long h_set_stride, int n_faces,
+                    const long long* face_idx, const int* face_ptr, const int* face_entries,
+                    int n_entries, const long long* entry_idx, const int* blk, const void* g,
+                    const void* beta, const void* hprev, const int* nbr, const void* tab, int bmax,
+                    void* u, void* stream);
+
 /* ---- (a16) DFT monitors ----------------------------------------------------
  * Replace DFTAccumulator.accumulate (src/magnelio/monitors/_dft.py:37-52) fed
  * with cell-centred samples (src/magnelio/monitors/base.py:51-173) from the

@@ -127,6 +127,12 @@ SIGNATURES: dict[str, tuple] = {
         [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
          _c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p],
     ),
+    "mgh_sibc_update": (
+        _c_int,
+        [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_int,
+         _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int,
+         _c_void_p, _c_void_p],
+    ),
     "mgh_tfsf_apply": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,

@@ -29,6 +29,8 @@ imported), so the reference's meshes, ports and results stay what they are.
                           (conformal cells, thin wires, donors), which the
                           solver then uses instead of its own cell-category
                           build
+* SIBC walls            the reference's SIBCSpec passes to the solver as
+                          it is (``sibc=``)
 * ``recorder_signals``  the solver's recorded V/I as PortSignalRecorder.
                           finalize returns them (ports/recorder.py:286-340)
 """
@@ -192,11 +194,13 @@ def solver_from_reference(*, mesh, boundary_conditions=None, ports=(), sources=(
                           total_time_steps=None, dt, energy_stop_db=None, port_signal_stop_db=None,
                           port_signal_min_steps=None, max_time_steps=None, energy_check_interval=None,
                           precision=None, excitations=None, f_calc=None, mass_matrices=None,
-                          monitor_options=None, **solver_kw):
+                          monitor_options=None, sibc=None, **solver_kw):
     """This backend's FITTimeDomainSolver from the reference solver's fields.
 
     ``excitations``: per field set a (port name, mode, waveform) — by default
-    the excitation set on the reference ports (one set)."""
+    the excitation set on the reference ports (one set).  ``sibc``: the
+    reference's SIBCSpec (_sibc.py:65-89), taken as it is — its surfaces'
+    comp / flat_idx / weight / inv_l_dual and its fits' c0 / branches."""
     from .solver import ChannelExcitation, FITTimeDomainSolver  # noqa: PLC0415
 
     pm = mesh_from_reference(mesh)
@@ -221,7 +225,7 @@ def solver_from_reference(*, mesh, boundary_conditions=None, ports=(), sources=(
         port_signal_stop_db=port_signal_stop_db, port_signal_min_steps=port_signal_min_steps,
         max_time_steps=max_time_steps, energy_check_interval=energy_check_interval,
         precision=precision or "single", nsets=len(exc), excitations=exc,
-        mass_matrices=mass_matrices, **solver_kw)
+        mass_matrices=mass_matrices, sibc=sibc, **solver_kw)
 
 
 def recorder_signals(solver, set_index: int = 0, signal_type=None) -> dict:

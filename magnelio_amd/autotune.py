@@ -108,7 +108,7 @@ def kchunk_for(solver, candidates=CANDIDATES, path: Path | None = None) -> int:
     timed now and stored."""
     path = path or cache_path()
     cpml = solver.cpml if (solver.cpml is not None and solver._dispersion is None
-                          and solver._dispersion_mu is None) else None
+                          and solver._dispersion_mu is None and solver._sibc is None) else None
     faces = [f.face for f in solver.cpml.faces] if solver.cpml is not None else []
     key = cache_key(solver.e.device, solver.dims, solver.nsets, solver.e.dtype, faces)
     data = _load(path)

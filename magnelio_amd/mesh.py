@@ -63,6 +63,15 @@ class Material:
         return cls(name="PEC", is_pec=True)
 
     @classmethod
+    def lossy_metal(cls, name: str, sigma: float, mu: float = 1.0) -> "Material":
+        """A good conductor: PEC in the field solve, its finite sigma and mu
+        used only by wall-loss models (surface impedance walls,
+        src/magnelio/materials/material.py:148-185)."""
+        if not 0.0 < sigma < float("inf"):
+            raise ValueError(f"lossy_metal needs a finite sigma > 0, got {sigma!r}")
+        return cls(name=name, mu=(mu,) * 3, sigma=(sigma,) * 3, is_pec=True)
+
+    @classmethod
     def dielectric(cls, eps_r: float, sigma: float = 0.0, name: str = "dielectric") -> "Material":
         return cls(name=name, epsilon=(eps_r,) * 3, sigma=(sigma,) * 3)
 

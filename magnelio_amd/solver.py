@@ -47,6 +47,7 @@ import torch
 from . import _lib, autotune, kernels, layout
 from . import mesh as _mesh
 from .dispersion import DeviceDispersion
+from .sibc import DeviceSIBC
 from .lumped import DeviceLumped
 from .tileskip import build_tile_plan
 from .boundaries import CPMLBoundary, DeviceCPML, PECBoundary, PMCBoundary
@@ -293,6 +294,9 @@ class FITTimeDomainSolver:
     # layout, unflattened — e.g. the reference's own build_M_* of a conformal
     # mesh; None: built here from the material ids
     mass_matrices: tuple | None = None
+    # surface-impedance walls: a spec with ``surfaces`` and ``fits`` (the
+    # reference's SIBCSpec or magnelio_amd.sibc.SIBCSpec), None for plain PEC
+    sibc: object | None = None
 
     def __post_init__(self):
         if self.precision not in PRECISIONS:
@@ -360,7 +364,7 @@ class FITTimeDomainSolver:
         pec_e = _mesh.pec_flat(pec, dims)
         self._pec_e = pec_e
         w_e = w_h = None
-        self._dispersion = self._dispersion_mu = None
+        self._dispersion = self._dispersion_mu = self._sibc = None
         for src in self.sources:
             if hasattr(src, "prepare"):
                 src.prepare(self)
@@ -368,6 +372,12 @@ class FITTimeDomainSolver:
         self._dispersion = DeviceDispersion.from_mesh(mesh, dt, pec_raw, "E")
         if self._dispersion is not None:
             w_e = self._dispersion.w_half
+        # SIBC walls: the instantaneous part multiplies h_mid like sigma*, so
+        # W folds into M_sigma_m (fit_td.py:419-429); frozen faces excluded
+        if self.sibc is not None:
+            self._sibc = DeviceSIBC.from_spec(self.sibc, dims, dt, frozen=m_mu <= 0)
+            if self._sibc is not None:
+                m_sig_m = m_sig_m + self._sibc.W
         # mu(w): faces with M_mu = 0 (after the port flattening) stay frozen
         # (fit_td.py:436-441)
         self._dispersion_mu = DeviceDispersion.from_mesh(mesh, dt, m_mu <= 0, "H")
@@ -396,6 +406,8 @@ class FITTimeDomainSolver:
         why = None
         if self.mass_matrices is not None:
             why = "given mass matrices"
+        elif self.sibc is not None:
+            why = "surface-impedance walls"
         elif any(getattr(m, "poles", ()) or getattr(m, "poles_mu", ())
                  for m in self.mesh.material_library.values()):
             why = "dispersive materials"
@@ -424,7 +436,7 @@ class FITTimeDomainSolver:
         g = mesh.grid
         geo = np.concatenate([g.dx, g.dy, g.dz, _mesh._avg_d(g.dx), _mesh._avg_d(g.dy), _mesh._avg_d(g.dz)])
         self._port_faces = {P.face for P in self.ports}
-        self._dispersion = self._dispersion_mu = None
+        self._dispersion = self._dispersion_mu = self._sibc = None
         self._coeffs = None
         self._m_eps_host = self._m_mu_host = None
         if any(isinstance(bc, CPMLBoundary) for bc in self.boundary_conditions.values()):
@@ -475,7 +487,7 @@ class FITTimeDomainSolver:
         K = self.nsets
         self.e = torch.zeros((K, ne), dtype=t_dt, device=dev)
         self.h = torch.zeros((K, nh), dtype=t_dt, device=dev)
-        for disp in (self._dispersion, self._dispersion_mu):
+        for disp in (self._dispersion, self._dispersion_mu, self._sibc):
             if disp is not None:
                 disp.bind(self)
 
@@ -536,13 +548,15 @@ class FITTimeDomainSolver:
             self._dispersion.save(self, stream)
         if self._dispersion_mu is not None:
             self._dispersion_mu.save(self, stream)
+        if self._sibc is not None:
+            self._sibc.save(self, stream)
         if ev is not None:
             ev[0].record(stream)
         tp = self.tile_plan
         # CPML rides in the curl launches unless a listed launch or a
         # dispersion completion (which sits between curl and CPML) forbids it
         fused = self.cpml if (self.cpml is not None and tp is None and self._dispersion is None
-                              and self._dispersion_mu is None) else None
+                              and self._dispersion_mu is None and self._sibc is None) else None
         kernels.update_e(self.e, self.h, self.alpha_e, self.beta_e, dims, curl64=self.curl64,
                          kchunk=self.kchunk, live=tp.live_e if tp is not None else None, cpml=fused,
                          xcols=False, stream=stream)
@@ -569,10 +583,12 @@ class FITTimeDomainSolver:
                          stream=stream)
         if fused is not None:
             kernels.cpml_xcols_h(self.e, self.h, self.beta_h, dims, fused, stream=stream)
-        # reference H phase order: curl, mu(w) completion, CPML faces
-        # (fit_td.py:891-921)
+        # reference H phase order: curl, mu(w) completion, SIBC completion,
+        # CPML faces (fit_td.py:891-921)
         if self._dispersion_mu is not None:
             self._dispersion_mu.update(self, stream)
+        if self._sibc is not None:
+            self._sibc.update(self, stream)
         if self.cpml is not None and fused is None:
             kernels.cpml_h(self.e, self.h, self.beta_h, dims, self.cpml, stream=stream)
         for src in self.sources:
@@ -911,6 +927,8 @@ class FITTimeDomainSolver:
             sd["dispersion"] = self._dispersion.reference_state(s)
         if self._dispersion_mu is not None:
             sd["dispersion_mu"] = self._dispersion_mu.reference_state(s)
+        if self._sibc is not None:
+            sd["sibc"] = self._sibc.reference_state(s)
         if self.set_stops[s] is not None:
             hip["set_stop"] = {"reason": self.set_stops[s][0], "step": int(self.set_stops[s][1])}
         sd["hip"] = hip
@@ -1002,5 +1020,7 @@ class FITTimeDomainSolver:
             self._dispersion.load_reference_state(s, sd["dispersion"])
         if self._dispersion_mu is not None and "dispersion_mu" in sd:
             self._dispersion_mu.load_reference_state(s, sd["dispersion_mu"])
+        if self._sibc is not None and "sibc" in sd:
+            self._sibc.load_reference_state(s, sd["sibc"])
         stop = hip.get("set_stop")
         self.set_stops[s] = (stop["reason"], int(stop["step"])) if stop else None

@@ -3,14 +3,16 @@
 Follows src/magnelio/solver/fit_td.py:950-1007 for the parts this build
 covers, with t = n dt:
 
-  stash    ADE f_prev of the eps(w) edges and mu(w) faces   :826-836
+  stash    ADE f_prev of the eps(w) edges and mu(w) faces,
+           SIBC h_prev of the wall faces                     :826-841
   E phase  curl update (stencil or Numba order), eps(w) completion,
            CPML E corrections in boundary order              :837-889
   sources  TF/SF E injection at t + dt                       :965-967
   ports    update_e(fields, t + dt) in port order            :975-976
            (modal ports, then lumped ports / elements)
   record   V/I of every port (e at t^{n+1}, h at t^{n+1/2})  :981-982
-  H phase  curl update, mu(w) completion, CPML H corrections :891-927
+  H phase  curl update, mu(w) completion, SIBC completion,
+           CPML H corrections                                :891-927
   sources  TF/SF H injection at t + 3dt/2                    :991-993
   monitors record(fields, n, t, dt)                          :996-1002
 """
@@ -25,7 +27,7 @@ from . import yee
 class Oracle:
     def __init__(self, dims, alpha_e, beta_e, alpha_h, beta_h, dt, *, cpml=(), ports=(),
                  sources=(), monitors=(), lumped=(), dtype=np.float64, order="stencil",
-                 dispersion=None, dispersion_mu=None):
+                 dispersion=None, dispersion_mu=None, sibc=None):
         self.dims = dims
         self.dt = dt
         self.dtype = dtype
@@ -41,6 +43,7 @@ class Oracle:
         self.order = order
         self.dispersion = dispersion
         self.dispersion_mu = dispersion_mu
+        self.sibc = sibc
         self.n = 0
 
     def step(self) -> None:
@@ -51,6 +54,8 @@ class Oracle:
             self.dispersion.save_field(self.e)
         if self.dispersion_mu is not None:
             self.dispersion_mu.save_field(self.h)
+        if self.sibc is not None:
+            self.sibc.save_field(self.h)
         yee.update_e(self.e, self.h, self.ae, self.be, *dims, order=self.order)
         if self.dispersion is not None:
             self.dispersion.update_field(self.e)
@@ -70,6 +75,8 @@ class Oracle:
         yee.update_h(self.e, self.h, self.ah, self.bh, *dims)
         if self.dispersion_mu is not None:
             self.dispersion_mu.update_field(self.h)
+        if self.sibc is not None:
+            self.sibc.update_field(self.h)
         for f in self.cpml:
             f.update_h(self.e, self.h, self.bh)
         for s in self.sources:

@@ -239,6 +239,56 @@ def golden_box_oracle(dtype=np.float64):
 
 
 # ---------------------------------------------------------------------------
+# SIBC cavity (tests/golden/sibc_cavity.npz): a graded PEC cavity with a
+# lossy-metal brick and a plain PEC brick, walls and fits of the reference
+
+
+def golden_sibc():
+    """(fixture, its surfaces as oracle.sibc.Surface, {tag: oracle.sibc.Fit})."""
+    from oracle import sibc as osb  # noqa: PLC0415
+
+    import os  # noqa: PLC0415
+
+    f = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sibc_cavity.npz")))
+    surfs, fits = [], {}
+    for i in range(int(f["n_surfaces"])):
+        t = str(f[f"surf{i}_tag"])
+        tag = int(t) if t.isdigit() else t
+        surfs.append(osb.Surface(tag, f[f"surf{i}_comp"], f[f"surf{i}_flat_idx"], f[f"surf{i}_weight"],
+                                 f[f"surf{i}_inv_l_dual"]))
+        fits[tag] = osb.Fit(float(f[f"surf{i}_c0"]), [tuple(r) for r in f[f"surf{i}_branches"]])
+    return f, surfs, fits
+
+
+def golden_sibc_oracle(dtype=np.float64, surfaces=None, fits=None):
+    """Oracle of the golden SIBC cavity seeded as the reference's run:
+    (Oracle, SIBC, (m_eps, m_mu))."""
+    from oracle import sibc as osb  # noqa: PLC0415
+
+    f, surfs, fts = golden_sibc()
+    surfaces = surfs if surfaces is None else surfaces
+    fits = fts if fits is None else fits
+    x, y, z = f["x"], f["y"], f["z"]
+    dims = (x.size - 1, y.size - 1, z.size - 1)
+    mid = f["material_id"]
+    nmat = int(mid.max()) + 1
+    me = om.mass_e(x, y, z, mid, [[1.0] * nmat] * 3, om.EPS0)
+    mmu = om.mass_h(x, y, z, mid, [[1.0] * nmat] * 3, om.MU0)
+    dt = float(f["dt"])
+    pec = f["pec_edges"]
+    sib = osb.SIBC.build(surfaces, fits, dims, dt, frozen=mmu <= 0)
+    ae, be = om.alpha_beta(me, np.zeros_like(me), dt, pec)
+    ah, bh = om.alpha_beta(mmu, sib.W if sib is not None else np.zeros_like(mmu), dt)
+    if sib is not None:
+        sib.bind(bh, dtype)
+    orc = ofd.Oracle(dims, ae, be, ah, bh, dt, dtype=dtype, sibc=sib)
+    e = np.random.default_rng(11).standard_normal(me.size)
+    e[pec] = 0.0
+    orc.e[:] = e.astype(dtype)
+    return orc, sib, (me, mmu)
+
+
+# ---------------------------------------------------------------------------
 # C3 (batched multi-excitation) at reduced size: WR-90 guide, two eps_r = 30
 # posts, a PEC iris with a seeded window, 3-mode ports on z_min / z_max
 

@@ -161,6 +161,11 @@ def test_solver_from_reference_fields():
     assert sol.nsets == 2 and sol.precision == "double" and sol.horizon == 500
     assert sol.energy_stop_db == 60.0 and sol.port_signal_min_steps == 7 and sol.energy_check_interval == 20
     assert isinstance(sol.boundary_conditions["zmin"], CPMLBoundary)
+    spec = NS(surfaces=(NS(tag="xmin", comp=np.array([2], np.uint8), flat_idx=np.array([0]),
+                           weight=np.array([1e-6]), inv_l_dual=np.array([1e3])),),
+              fits={"xmin": NS(c0=0.01, branches=((1e10, 0.02),))})
+    sol = adapters.solver_from_reference(mesh=stand, total_time_steps=10, dt=1e-13, sibc=spec)
+    assert sol.sibc is spec
 
 
 def test_resolve_backend_never_falls_back(monkeypatch):
