@@ -149,16 +149,21 @@ int mgh_cpml_h(int dtype, int nx, int ny, int nz, int nsets, const void* e,
  * DTBCTermination.advance (src/magnelio/ports/_modal/dtbc.py:417-463) and
  * PortSignalRecorder.record (src/magnelio/ports/recorder.py:216-250), called
  * after the E phase in FITTimeDomainSolver.run (fit_td.py:975-982).
- * One work-group per (port, field set); float64 projections, history
- * convolutions and recursion; the step index n is read from *step_ctr.
- * port_desc: n_ports records of 29 int64 words (see csrc/mgh_ports.h):
+ * Float64 projections, history convolutions and recursion in four
+ * launches over (plane chunk, port, field set); the step index n is read
+ * from *step_ctr.  With the complement absorber (operator.py:1395-1427,
+ * :1499-1522, live while a channel of the port runs on Mur) the write adds,
+ * per plane edge, live (c_int_prev + r (c - c_port_prev)) of the interior
+ * remainder c = e_int - sum_m V_int,m p_m.
+ * port_desc: n_ports records of 32 int64 words (see csrc/mgh_ports.h):
  *   0 n_u, 1 n_v, 2 M, 3-5 idx_port/idx_int/idx_h (int64 device-layout
  *   indices), 6-8 me_port/me_int/mh, 9-10 prof_e/prof_h (M x n),
  *   11 params (M x 8: kind, r^2, q^2, mur_r, record_scale, 2-2r^2-q^2),
  *   12-14 kern/kern_stride/k_init (packed DTBC kernels), 15 state
  *   (nsets x M x 4), 16-18 w_hist/s_hist/cap, 19 exc_mode (int32 nsets),
  *   20-21 src_tab/tab_len, 22 mur_tab, 23-25 V_rec/I_rec/rec_cap,
- *   26 absmax; 27-28 spare. */
+ *   26 absmax, 27-28 work/work_len (scratch), 29-31 comp_r/comp_live (n)
+ *   and comp_state (nsets x 2 x n) of the complement absorber, or 0. */
 int mgh_port_step(int dtype, int nsets, void* e, long long e_set_stride, const void* h,
                   long long h_set_stride, const long long* port_desc, int n_ports,
                   const long long* step_ctr, void* stream);

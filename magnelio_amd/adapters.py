@@ -136,8 +136,10 @@ def port_from_operator(op, f_calc: float | None = None) -> PortModel:
                             e_u=np.asarray(dm.e_u_profile, float), e_v=np.asarray(dm.e_v_profile, float),
                             h_u=np.asarray(dm.h_u_profile, float), h_v=np.asarray(dm.h_v_profile, float),
                             mode_type=kind))
-    if getattr(op, "_comp_r_u", None) is not None:
-        raise NotImplementedError(f"port {op.name!r}: the complement absorber is not served")
+    comp_r = comp_live = None
+    if getattr(op, "_comp_r_u", None) is not None:  # the complement absorber (operator.py:596-619)
+        comp_r = np.concatenate([np.asarray(op._comp_r_u, float), np.asarray(op._comp_r_v, float)])
+        comp_live = np.concatenate([np.asarray(op._comp_live_u, float), np.asarray(op._comp_live_v, float)])
     return PortModel(name=str(op.name), plane=plane, modes=modes, dt=float(op._dt),
                      omega_calc=2 * math.pi * f_calc if f_calc else float("nan"),
                      me_u_port=np.asarray(op._me_u_port, float), me_v_port=np.asarray(op._me_v_port, float),
@@ -150,7 +152,8 @@ def port_from_operator(op, f_calc: float | None = None) -> PortModel:
                      dtbc_z0=[_opt(v) for v in op._dtbc_z0],
                      pair_spread=[_opt(v) for v in op._dtbc_pair_spread],
                      mur_r=np.asarray(op._mur_r, float).copy(), tau=np.asarray(op._tau_m, float).copy(),
-                     slab_defect=float(getattr(op, "_chain_slab_defect", 0.0) or 0.0))
+                     slab_defect=float(getattr(op, "_chain_slab_defect", 0.0) or 0.0),
+                     comp_r=comp_r, comp_live=comp_live)
 
 
 def cpml_from_reference(bc, grid) -> CPMLBoundary:

@@ -21,11 +21,15 @@
 //   26 absmax                            float64 (nsets) |V| envelope or 0
 //   27 work, 28 work_len                 float64 scratch of the split step, at least
 //                                         mgh::port_work_len(n, M, cap, nsets) words
+//   29 comp_r, 30 comp_live              float64 (n) complement absorber: per-edge Mur
+//                                         coefficient and 0/1 live mask, or 0 (off)
+//   31 comp_state                        float64 (nsets, 2, n): interior remainder and
+//                                         written remainder of the previous step
 #pragma once
 
 #include "mgh_common.h"
 
-#define MGH_PORT_WORDS 29
+#define MGH_PORT_WORDS 32
 #define MGH_PORT_MAX_PORTS 8
 #define MGH_PORT_MAX_MODES 8
 // points of a port plane (and DTBC history terms) per work-group chunk
@@ -62,6 +66,9 @@ struct PortDev {
   double* absmax;
   double* work;
   long long work_len;
+  const double* comp_r;
+  const double* comp_live;
+  double* comp_state;
 };
 
 // chunks of the projection pass: the plane and the history, whichever is longer
@@ -118,6 +125,12 @@ inline int load_port(const long long* w, PortDev* P, int nsets) {
   P->absmax = (double*)w[26];
   P->work = (double*)w[27];
   P->work_len = w[28];
+  P->comp_r = (const double*)w[29];
+  P->comp_live = (const double*)w[30];
+  P->comp_state = (double*)w[31];
+  MGH_REQUIRE((P->comp_r == nullptr) == (P->comp_live == nullptr) &&
+                  (P->comp_r == nullptr) == (P->comp_state == nullptr),
+              "port: complement absorber needs r, live and state together");
   MGH_REQUIRE(P->idx_port && P->idx_int && P->idx_h && P->me_port && P->me_int && P->mh &&
                   P->prof_e && P->prof_h && P->params && P->state,
               "port: null array pointer");

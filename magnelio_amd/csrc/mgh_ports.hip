@@ -224,6 +224,14 @@ __global__ __launch_bounds__(PB) void k_port_write(T* __restrict__ e, int64_t e_
   for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) vc[m] = m < M ? vcs[m] : 0.0;
   T* es = e + (int64_t)s * e_set;
   const T* hs = h + (int64_t)s * h_set;
+  // complement absorber: the interior projections of this step (V_int_prev
+  // after the advance) remove the port's channels from the interior plane
+  const bool comp = P.comp_r != nullptr;
+  double vi[MGH_PORT_MAX_MODES];
+#pragma unroll
+  for (int m = 0; m < MGH_PORT_MAX_MODES; ++m)
+    vi[m] = (comp && m < M) ? P.state[((int64_t)s * M + m) * 4 + 3] : 0.0;
+  double* cst = comp ? P.comp_state + (int64_t)s * 2 * n_pts : nullptr;
   double acc[PQ];
 #pragma unroll
   for (int q = 0; q < PQ; ++q) acc[q] = 0.0;
@@ -237,6 +245,16 @@ __global__ __launch_bounds__(PB) void k_port_write(T* __restrict__ e, int64_t e_
         // modes past M: vc = 0, an exact zero term (loads issue together)
         const int mm = m < M ? m : 0;
         val += vc[m] * P.prof_e[(int64_t)mm * n_pts + p];
+      }
+      if (comp) {
+        // remainder at the interior plane, channels removed in mode order,
+        // advanced to the plane by the edge's Mur-1 and added to the write
+        double rem = (double)es[P.idx_int[p]];
+        for (int m = 0; m < M; ++m) rem -= vi[m] * P.prof_e[(int64_t)m * n_pts + p];
+        const double cp = P.comp_live[p] * (cst[p] + P.comp_r[p] * (rem - cst[n_pts + p]));
+        val += cp;
+        cst[p] = rem;
+        cst[n_pts + p] = cp;
       }
       const T tv = (T)val;
       es[P.idx_port[p]] = tv;

@@ -400,6 +400,10 @@ class PortModel:
     slab_defect: float
     excitation: tuple | None = None  # (mode index, waveform callable)
     meta: dict = field(default_factory=dict)
+    # complement absorber (factory.py:687-735): per plane edge [u | v] the
+    # Mur coefficient r and the 0/1 live mask, or None
+    comp_r: np.ndarray | None = None
+    comp_live: np.ndarray | None = None
 
     @property
     def n_modes(self) -> int:
@@ -412,6 +416,12 @@ class PortModel:
     @property
     def termination_kinds(self) -> list[str]:
         return ["dtbc" if r is not None else "mur" for r in self.dtbc_r]
+
+    @property
+    def complement_active(self) -> bool:
+        """The absorber is configured and some channel runs on Mur
+        (operator.py:1337-1346); fully certified ports keep the exact path."""
+        return self.comp_r is not None and any(r is None for r in self.dtbc_r)
 
     @property
     def dtbc_line_params(self) -> dict:
@@ -590,11 +600,33 @@ def build_port(name: str, face: str, mesh, m_eps: np.ndarray, m_mu: np.ndarray, 
         qs.append(q_m if use else None)
         z0s.append(_chain_z0(dm, r_m, me_u, me_v, mh_u, mh_v, dt) if use and q_m > 0.0 else None)
         spreads.append(spread)
+    comp_r, comp_live = complement_absorber(plane, mesh, m_eps, dt)
     return PortModel(name=name, plane=plane, modes=modes, dt=dt, omega_calc=omega_calc,
                      me_u_port=me_u, me_v_port=me_v, me_u_int=me_ui, me_v_int=me_vi, mh_u=mh_u,
                      mh_v=mh_v, record_scale=rec, source_scale=src, state_scale=st, dtbc_r=rs,
                      dtbc_q=qs, dtbc_z0=z0s, pair_spread=spreads, mur_r=mur_r, tau=tau,
-                     slab_defect=defect)
+                     slab_defect=defect, comp_r=comp_r, comp_live=comp_live)
+
+
+def complement_absorber(plane: PortPlane, mesh, m_eps_flat: np.ndarray, dt: float):
+    """Per plane edge [u | v]: the Mur coefficient of the complement absorber
+    r = (c dt - dx_n)/(c dt + dx_n) with the local phase velocity
+    c = C0 / sqrt(M_eps / M_eps_vacuum) of the port-flattened masses (0 where
+    M_eps <= 0), and the live mask (neither PEC in the mesh's own edge mask
+    nor frozen) (factory.py:687-735)."""
+    from .dispersion import _geom_e  # noqa: PLC0415
+
+    dims = mesh.dims
+    m_vac = _mesh.flatten_port_mass(_mesh.EPS0 * _geom_e(mesh.grid), dims, plane.face)
+    idx = np.concatenate([plane.e_u_indices, plane.e_v_indices])
+    m_e = np.asarray(m_eps_flat[idx], float)
+    dead = m_e <= 0.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c_loc = C0 / np.sqrt(m_e / np.asarray(m_vac[idx], float))
+        r = (c_loc * dt - plane.normal_dx) / (c_loc * dt + plane.normal_dx)
+    r = np.where(dead, 0.0, r)
+    pec = _mesh.pec_flat(mesh.pec_mask_edges, dims)[idx] | dead
+    return r, (~pec).astype(float)
 
 
 def build_port_local(name: str, face: str, mesh, *, dt: float, f_calc: float, n_modes: int = 1,

@@ -123,7 +123,7 @@ class ChannelExcitation:
 class DevicePorts:
     """Device arrays + descriptors of the modal ports for K field sets."""
 
-    WORDS = 29
+    WORDS = 32
 
     def __init__(self, ports: list[PortModel], dims, nsets: int, n_steps: int, device,
                  excitations: list[ChannelExcitation | None], dt: float, record: bool = True):
@@ -134,6 +134,7 @@ class DevicePorts:
         self._keep = []
         self.desc = np.zeros(self.WORDS * len(ports), dtype=np.int64)
         self.V_rec, self.I_rec, self.state, self.w_hist, self.s_hist, self.absmax = [], [], [], [], [], []
+        self.comp = []  # per port: (nsets, 2, n_u + n_v) complement states or None
         f64 = torch.float64
         for pi, P in enumerate(ports):
             pl = P.plane
@@ -219,6 +220,15 @@ class DevicePorts:
             work_len = nsets * (2 * nch * PORT_PQ + PORT_MAX_MODES)
             work = dev(np.zeros(work_len))
             w[27], w[28] = work.data_ptr(), work_len
+            comp = None
+            if P.complement_active:
+                # per-edge Mur of the port-unrepresented remainder
+                # (operator.py:1395-1427, :1499-1522): r, live, and per set
+                # [interior remainder | written remainder] of the last step
+                t_cr, t_cl = dev(P.comp_r), dev(P.comp_live)
+                comp = dev(np.zeros((nsets, 2, n_pts)))
+                w[29], w[30], w[31] = t_cr.data_ptr(), t_cl.data_ptr(), comp.data_ptr()
+            self.comp.append(comp)
             self.V_rec.append(V)
             self.I_rec.append(I)
             self.state.append(state)
@@ -911,6 +921,12 @@ class FITTimeDomainSolver:
                     d["dtbc"][str(m)] = {"u": float(st[m, 0]), "u_prev": float(st[m, 1]), "n": int(n),
                                          "w_hist": wh[s, m, :n].cpu().numpy().copy() if hist else np.zeros(n),
                                          "s_hist": shh[s, m, :n].cpu().numpy().copy() if hist else np.zeros(n)}
+                if P.comp_r is not None:  # the absorber's states (operator.py:1010-1016)
+                    n_u = P.plane.e_u_indices.size
+                    cs = (self.dports.comp[pi][s].cpu().numpy() if self.dports.comp[pi] is not None
+                          else np.zeros((2, P.comp_r.size)))
+                    d["complement"] = {"int_u": cs[0, :n_u].copy(), "int_v": cs[0, n_u:].copy(),
+                                       "port_u": cs[1, :n_u].copy(), "port_v": cs[1, n_u:].copy()}
                 sd["ports"][P.name] = d
                 hip["ports"][P.name] = {"V_rec": self.dports.V_rec[pi][s, :, :n].cpu().numpy().copy(),
                                         "I_rec": self.dports.I_rec[pi][s, :, :n].cpu().numpy().copy(),
@@ -980,9 +996,15 @@ class FITTimeDomainSolver:
             for pi, P in enumerate(self.ports):
                 p = sd["ports"][P.name]
                 comp = p.get("complement")
-                if comp is not None and any(np.any(np.asarray(v) != 0) for v in comp.values()):
-                    raise ValueError(f"port {P.name!r}: the checkpoint's complement-absorber state "
-                                     "is not served by this backend")
+                if comp is not None:
+                    parts = [np.asarray(comp[k], float) for k in ("int_u", "int_v", "port_u", "port_v")]
+                    if self.dports.comp[pi] is not None:
+                        cs = np.stack([np.concatenate(parts[:2]), np.concatenate(parts[2:])])
+                        self.dports.comp[pi][s].copy_(torch.from_numpy(cs))
+                    elif any(np.any(a != 0) for a in parts):
+                        raise ValueError(f"port {P.name!r}: the checkpoint carries complement-absorber "
+                                         "state but the port's absorber is not live (every channel on "
+                                         "the DTBC, or no absorber configured)")
                 st = self.dports.state[pi]
                 stn = st[s].cpu().numpy()
                 stn[:, 2] = np.asarray(p["V_port_prev"], float)

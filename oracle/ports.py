@@ -19,6 +19,15 @@ Restates (paths relative to the reference repository root):
                     TF/SF with the ring-buffer look-back :1539-1562, DTBC)
 * projections       ports/_modal/operator.py:1277-1321
 * recorder          src/magnelio/ports/recorder.py:216-250
+* complement        ports/_modal/factory.py:687-735 (per-edge Mur coefficient
+                    r = (c dt - dx_n)/(c dt + dx_n), c = C0/sqrt(M_eps/M_eps_vac)
+                    on the plane edges of the flattened masses, 0 where
+                    M_eps <= 0; live = not PEC (the mesh's own mask) and not
+                    frozen), operator.py:1395-1427, :1499-1522 (live while a
+                    channel of the port runs on Mur: the interior-plane
+                    remainder comp = e_int - sum_m V_int,m p_m, advanced by
+                    live (comp_int_prev + r (comp - comp_port_prev)) and added
+                    to the modal write)
 
 Parity pins (tests/test_oracle_ports.py): the closed-form modes are
 eigenpairs of an independently assembled curl-curl pencil to 1e-12, are
@@ -166,10 +175,18 @@ def z_te(omega, omega_c, eps_r=1.0):
 class Port:
     """Oracle modal port: calibrated TE channels, Mur/DTBC termination."""
 
-    def __init__(self, name, pl, modes, m_eps, m_mu, dt, omega_calc, eps_r=1.0, termination="auto"):
+    def __init__(self, name, pl, modes, m_eps, m_mu, dt, omega_calc, eps_r=1.0, termination="auto",
+                 complement=None):
         """modes: list of (e_u, e_v, omega_c) from :func:`rect_te_mode`;
-        termination "auto" (exact DTBC) or "mur" (modal Mur-1 everywhere)."""
+        termination "auto" (exact DTBC) or "mur" (modal Mur-1 everywhere);
+        complement: (r_u, r_v, live_u, live_v) of :func:`complement_arrays`
+        or None."""
         self.name, self.pl, self.dt = name, pl, dt
+        self.comp = None if complement is None else tuple(np.asarray(a, float) for a in complement)
+        self.comp_int_u = np.zeros(pl["e_u"].size)
+        self.comp_int_v = np.zeros(pl["e_v"].size)
+        self.comp_port_u = np.zeros(pl["e_u"].size)
+        self.comp_port_v = np.zeros(pl["e_v"].size)
         self.me_u, self.me_v = m_eps[pl["e_u"]], m_eps[pl["e_v"]]
         self.me_ui, self.me_vi = m_eps[pl["e_u_int"]], m_eps[pl["e_v_int"]]
         self.mh_u, self.mh_v = m_mu[pl["h_u"]], m_mu[pl["h_v"]]
@@ -272,9 +289,20 @@ class Port:
         return np.array([float(np.dot(mu, md["e_u"] * eu)) + float(np.dot(mv, md["e_v"] * ev))
                          for md in self.modes])
 
+    @property
+    def complement_active(self):
+        return self.comp is not None and any(term is None for term in self.dtbc)
+
     def update_e(self, e, t):
         pl = self.pl
         V_int_new = self._V(e, pl["e_u_int"], pl["e_v_int"], self.me_ui, self.me_vi)
+        comp_on = self.complement_active
+        if comp_on:
+            cu = np.asarray(e[pl["e_u_int"]], float).copy()
+            cv = np.asarray(e[pl["e_v_int"]], float).copy()
+            for m, md in enumerate(self.modes):
+                cu -= V_int_new[m] * md["e_u"]
+                cv -= V_int_new[m] * md["e_v"]
         V = self.V_int_prev + self.mur_r * (V_int_new - self.V_port_prev)
         if self.exc is not None and self.dtbc[self.exc] is None:
             m = self.exc
@@ -295,6 +323,14 @@ class Port:
         for m, md in enumerate(self.modes):
             eu += V[m] * md["e_u"]
             ev += V[m] * md["e_v"]
+        if comp_on:
+            r_u, r_v, live_u, live_v = self.comp
+            pu = live_u * (self.comp_int_u + r_u * (cu - self.comp_port_u))
+            pv = live_v * (self.comp_int_v + r_v * (cv - self.comp_port_v))
+            eu += pu
+            ev += pv
+            self.comp_int_u, self.comp_int_v = cu, cv
+            self.comp_port_u, self.comp_port_v = pu, pv
         e[pl["e_u"]] = eu
         e[pl["e_v"]] = ev
         self.V_port_prev[:] = V
@@ -317,6 +353,24 @@ class Port:
         for m in range(self.n_modes):
             self.V[m].append(float(Vp[m]))
             self.I[m].append(float(Ip[m]))
+
+
+def complement_arrays(pl, m_eps, m_eps_vac, pec_e, dt):
+    """(r_u, r_v, live_u, live_v) of the complement absorber on the plane's
+    edges: ``m_eps`` / ``m_eps_vac`` flattened at the port face, ``pec_e``
+    the mesh's own PEC edge mask (factory.py:687-735)."""
+    out_r, out_dead = [], []
+    for key in ("e_u", "e_v"):
+        idx = pl[key]
+        me = np.asarray(m_eps[idx], float)
+        dead = me <= 0.0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            c = C0 / np.sqrt(me / np.asarray(m_eps_vac[idx], float))
+            r = (c * dt - pl["normal_dx"]) / (c * dt + pl["normal_dx"])
+        out_r.append(np.where(dead, 0.0, r))
+        out_dead.append(dead)
+    live = [(~(np.asarray(pec_e, bool)[pl[k]] | d)).astype(float) for k, d in zip(("e_u", "e_v"), out_dead)]
+    return out_r[0], out_r[1], live[0], live[1]
 
 
 def interp_delayed(buffer: collections.deque, delay: float, dt: float) -> float:

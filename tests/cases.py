@@ -35,6 +35,16 @@ def modulated_gaussian(t: float, f_max: float, f_min: float) -> float:
     return math.exp(-x * x) * math.cos(2.0 * math.pi * fc * (t - t0))
 
 
+def oracle_complement(pl, x, y, z, mid, me_flat, pec_mesh, dt):
+    """The complement-absorber arrays of an oracle port, as the reference's
+    factory configures them on every port (factory.py:687-735): the plane's
+    face flattened in a vacuum M_eps, the mesh's own PEC mask."""
+    dims = (x.size - 1, y.size - 1, z.size - 1)
+    nmat = int(np.max(mid)) + 1 if np.size(mid) else 1
+    mvac = om.flatten_port(om.mass_e(x, y, z, mid, [[1.0] * nmat] * 3, om.EPS0), *dims, pl["face"], "E")
+    return op.complement_arrays(pl, me_flat, mvac, pec_mesh, dt)
+
+
 def c1_oracle(dtype=np.float64, dims=C1_DIMS, size=C1_SIZE, excite=("port1", 0), order="stencil",
               termination="auto", band=None):
     """Oracle side: coefficients, two ports, excitation; returns (Oracle, ports, dt)."""
@@ -45,6 +55,7 @@ def c1_oracle(dtype=np.float64, dims=C1_DIMS, size=C1_SIZE, excite=("port1", 0),
     mm = om.mass_h(x, y, z, mid, [[1.0]] * 3, om.MU0)
     dt = om.courant_dt(x, y, z, 0.95)
     pec = om.bbox_walls(np.zeros(me.size, bool), nx, ny, nz, ["xmin", "xmax", "ymin", "ymax"])
+    pec_mesh = pec
     for f in ("zmin", "zmax"):
         me = om.flatten_port(me, nx, ny, nz, f, "E")
         mm = om.flatten_port(mm, nx, ny, nz, f, "H")
@@ -56,7 +67,8 @@ def c1_oracle(dtype=np.float64, dims=C1_DIMS, size=C1_SIZE, excite=("port1", 0),
     for name, face in (("port1", "zmin"), ("port2", "zmax")):
         pl = op.plane(face, x, y, z)
         modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES]
-        ports.append(op.Port(name, pl, modes, me, mm, dt, omega_calc, termination=termination))
+        ports.append(op.Port(name, pl, modes, me, mm, dt, omega_calc, termination=termination,
+                             complement=oracle_complement(pl, x, y, z, mid, me, pec_mesh, dt)))
     if excite is not None:
         p = ports[0] if excite[0] == "port1" else ports[1]
         f_lo, f_hi = band if band is not None else (p.modes[excite[1]]["wc"] / (2 * math.pi), C1_FMAX)
@@ -126,24 +138,45 @@ def golden_c1(name):
     return dict(np.load(path))
 
 
+def _c1_fixture_module():
+    import os  # noqa: PLC0415
+    import sys  # noqa: PLC0415
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    if here not in sys.path:
+        sys.path.insert(0, here)
+    import make_c1_fixtures as mk  # noqa: PLC0415
+
+    return mk
+
+
 def golden_c1_oracle(name, dtype=np.float64):
     """Oracle of a golden case: masses, PEC and alpha/beta from oracle.mesh,
     ports from the reference's numerical mode profiles (the oracle's Port
-    calibrates them itself).  Returns (Oracle, ports, dt, (me, mu))."""
+    calibrates them itself), each with its complement absorber; c1_mur's
+    port 1 terminates on Mur.  Returns (Oracle, ports, dt, (me, mu))."""
     f = golden_c1(name)
     x, y, z = f["x"], f["y"], f["z"]
     dims = (x.size - 1, y.size - 1, z.size - 1)
-    loaded = name == "c1_loaded"
-    boxes = [(1, GOLDEN_BLOCK), (2, GOLDEN_POST)] if loaded else []
+    boxes = []
+    eps_block = GOLDEN_EPS
+    if name == "c1_loaded":
+        boxes = [(1, GOLDEN_BLOCK), (2, GOLDEN_POST)]
+    elif name == "c1_mur":
+        mk = _c1_fixture_module()
+        boxes, eps_block = [(1, mk.MUR_BLOCK)], mk.MUR_EPS
     mid = om.fill_aabb(x, y, z, boxes)
-    eps = [[1.0, GOLDEN_EPS, 1.0]] * 3
-    sig = [[0.0, GOLDEN_SIGMA, 0.0]] * 3
+    eps = [[1.0, eps_block, 1.0]] * 3
+    sig = [[0.0, GOLDEN_SIGMA if name == "c1_loaded" else 0.0, 0.0]] * 3
     me = om.mass_e(x, y, z, mid, eps, om.EPS0)
+    mvac = om.mass_e(x, y, z, mid, [[1.0, 1.0, 1.0]] * 3, om.EPS0)
     mm = om.mass_h(x, y, z, mid, [[1.0, 1.0, 1.0]] * 3, om.MU0)
     ms = om.mass_e(x, y, z, mid, sig, 1.0)
-    pec = om.bbox_walls(om.pec_edge_mask(mid == 2), *dims, ["xmin", "xmax", "ymin", "ymax"])
+    pec_mesh = om.bbox_walls(om.pec_edge_mask(mid == 2), *dims, ["xmin", "xmax", "ymin", "ymax"])
+    pec = pec_mesh
     for face in ("zmin", "zmax"):
         me = om.flatten_port(me, *dims, face, "E")
+        mvac = om.flatten_port(mvac, *dims, face, "E")
         mm = om.flatten_port(mm, *dims, face, "H")
         pec = om.flatten_port(pec, *dims, face, "E")
     dt = float(f["dt"])
@@ -154,7 +187,10 @@ def golden_c1_oracle(name, dtype=np.float64):
         pl = op.plane(face, x, y, z)
         modes = [(f[f"{label}_e_u{m}"], f[f"{label}_e_v{m}"], float(f[f"{label}_omega_c{m}"]))
                  for m in range(3)]
-        ports.append(op.Port(label, pl, modes, me, mm, dt, 2 * math.pi * C1_FMAX))
+        term = "mur" if (name == "c1_mur" and label == "port1") else "auto"
+        comp = op.complement_arrays(pl, me, mvac, pec_mesh, dt)
+        ports.append(op.Port(label, pl, modes, me, mm, dt, 2 * math.pi * C1_FMAX, termination=term,
+                             complement=comp))
     f_lo = ports[0].modes[0]["wc"] / (2 * math.pi)
     ports[0].set_excitation(0, lambda t: modulated_gaussian(t, C1_FMAX, f_lo))
     orc = ofd.Oracle(dims, ae, be, ah, bh, dt, ports=ports, dtype=dtype)
@@ -171,6 +207,9 @@ def golden_c1_mesh(name):
     if name == "c1_loaded":
         regions = [(pm.Material(name="block", epsilon=(GOLDEN_EPS,) * 3, sigma=(GOLDEN_SIGMA,) * 3),
                     GOLDEN_BLOCK), (pm.Material.pec(), GOLDEN_POST)]
+    elif name == "c1_mur":
+        mk = _c1_fixture_module()
+        regions = [(pm.Material(name="feed_block", epsilon=(mk.MUR_EPS,) * 3), mk.MUR_BLOCK)]
     return pm.Mesh.from_grid(g, regions=regions, pec_faces=("xmin", "xmax", "ymin", "ymax"))
 
 
@@ -317,6 +356,7 @@ def c3_oracle(excite, dims=C3_SMALL, dtype=np.float64):
     mmu = om.mass_h(x, y, z, mid, [[1.0, 1.0, 1.0]] * 3, om.MU0)
     nx, ny, nz = dims
     pec = om.bbox_walls(om.pec_edge_mask(mid == PEC), nx, ny, nz, ["xmin", "xmax", "ymin", "ymax"])
+    pec_mesh = pec
     for f in ("zmin", "zmax"):
         me = om.flatten_port(me, nx, ny, nz, f, "E")
         mmu = om.flatten_port(mmu, nx, ny, nz, f, "H")
@@ -328,7 +368,8 @@ def c3_oracle(excite, dims=C3_SMALL, dtype=np.float64):
     for name, face in (("port1", "zmin"), ("port2", "zmax")):
         pl = op.plane(face, x, y, z)
         modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES]
-        ports.append(op.Port(name, pl, modes, me, mmu, dt, 2 * math.pi * 25e9))
+        ports.append(op.Port(name, pl, modes, me, mmu, dt, 2 * math.pi * 25e9,
+                             complement=oracle_complement(pl, x, y, z, mid, me, pec_mesh, dt)))
     p = ports[0] if excite[0] == "port1" else ports[1]
     f_lo = p.modes[excite[1]]["wc"] / (2 * math.pi)
     p.set_excitation(excite[1], lambda t, lo=f_lo: modulated_gaussian(t, 25e9, lo))
@@ -369,6 +410,7 @@ def c5_job_oracle(job, excite, dtype=np.float64):
     nx, ny, nz = dims
     walls = ["xmin", "xmax", "ymin", "ymax"] + (["zmax"] if n_ports == 1 else [])
     pec = om.bbox_walls(np.zeros(me.size, bool), nx, ny, nz, walls)
+    pec_mesh = pec
     faces = ("zmin", "zmax")[:n_ports]
     for f in faces:
         me = om.flatten_port(me, nx, ny, nz, f, "E")
@@ -381,7 +423,8 @@ def c5_job_oracle(job, excite, dtype=np.float64):
     for name, face in zip(("port1", "port2"), faces):
         pl = op.plane(face, x, y, z)
         modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES[:n_modes]]
-        ports.append(op.Port(name, pl, modes, me, mmu, dt, 2 * math.pi * job.f_max))
+        ports.append(op.Port(name, pl, modes, me, mmu, dt, 2 * math.pi * job.f_max,
+                             complement=oracle_complement(pl, x, y, z, mid, me, pec_mesh, dt)))
     p = next(q for q in ports if q.name == excite[0])
     f_lo = p.modes[excite[1]]["wc"] / (2 * math.pi)
     p.set_excitation(excite[1], lambda t, lo=f_lo, hi=job.f_max: modulated_gaussian(t, hi, lo))
@@ -429,7 +472,8 @@ def c4_oracle(n, excite, dtype=np.float64, n_modes=2, seed=4, f_max=20e9):
     disp.bind(be, dtype)
     pl = op.plane("zmin", x, y, z)
     modes = [op.rect_te_mode(pl, x, y, z, m, k, me) for m, k in lowest_te_modes(x, y, n_modes)]
-    port = op.Port("port1", pl, modes, me, mmu, dt, 2 * math.pi * f_max)
+    port = op.Port("port1", pl, modes, me, mmu, dt, 2 * math.pi * f_max,
+                   complement=oracle_complement(pl, x, y, z, mid, me, walls, dt))
     f_lo = port.modes[excite[1]]["wc"] / (2 * math.pi)
     port.set_excitation(excite[1], lambda t, lo=f_lo: modulated_gaussian(t, f_max, lo))
     face = oc.CPMLFace("zmax", x, y, z, dt, dtype, pec_e_flat=pec, pec_cell=np.zeros(dims, bool),

@@ -3,7 +3,7 @@
 Run in the build container only (the reference never travels to the GPU
 machine; only the .npz files this writes do):
 
-    python tests/golden/make_c1_fixtures.py /root/reference/src
+    python tests/golden/make_c1_fixtures.py /root/reference/src [case ...]
 
 The reference's NumPy tier (``FITTimeDomainSolver``, src/magnelio/solver/
 fit_td.py:682-1246, with ``PortOperatorModal`` ports, ``PortSignalRecorder``
@@ -16,6 +16,10 @@ plain cases for 300 steps:
   spacing ratio <= 1.3) with a lossy dielectric block (eps_r 4, sigma
   0.05 S/m) and a PEC post in its middle — graded dual masses, material
   averaging, the PEC edge OR rule and the port-plane flattening all enter;
+* ``c1_mur``:    the guide with a dielectric block (eps_r 2.5) in z cells 2-4
+  next to port 1: its feed slabs fail the DTBC certificate, its channels run
+  on modal Mur and the complement absorber (a per-edge Mur on the
+  port-unrepresented remainder of the plane) is live; port 2 keeps the DTBC;
 
 port 1 mode 0 excited with the modulated Gaussian from the TE10 cut-off to
 25 GHz, dt = courant_dt(grid, "normal"), float64 (and float32 for the
@@ -58,6 +62,12 @@ def cases():
                          precisions=("double", "single")),
         "c1_loaded": dict(axes=(graded_axis(12, A, 21), graded_axis(8, B, 22), z), loaded=True,
                           precisions=("double",)),
+        # a dielectric block from the third cell at z_min: port 1's feed
+        # slabs differ, its DTBC slab certificate fails and its channels
+        # terminate on Mur with the complement absorber live (operator.py:
+        # 596-619, :1337-1346); port 2 keeps the DTBC
+        "c1_mur": dict(axes=(np.linspace(0.0, A, 13), np.linspace(0.0, B, 9), z), loaded="mur",
+                       precisions=("double", "single")),
     }
 
 
@@ -65,9 +75,12 @@ def cases():
 BLOCK = (0.25 * A, 0.0, 8e-3, 0.75 * A, 0.6 * B, 12e-3)
 POST = (0.45 * A, 0.0, 9e-3, 0.55 * A, B, 11e-3)
 BLOCK_EPS, BLOCK_SIGMA = 4.0, 0.05
+# the Mur case: a block in z cells 2-4 next to port 1
+MUR_BLOCK = (0.3 * A, 0.0, 2e-3, 0.7 * A, 0.5 * B, 5e-3)
+MUR_EPS = 2.5
 
 
-def main(ref_src: str) -> None:
+def main(ref_src: str, only=None) -> None:
     sys.path.insert(0, ref_src)
     from magnelio.materials.material import Material
     from magnelio.mesh.grid import GridLines
@@ -82,10 +95,14 @@ def main(ref_src: str) -> None:
     from magnelio.solver.stability import courant_dt
 
     for name, cfg in cases().items():
+        if only and name not in only:
+            continue
         x, y, z = cfg["axes"]
         grid = GridLines(x=x, y=y, z=z)
         regions = []
-        if cfg["loaded"]:
+        if cfg["loaded"] == "mur":
+            regions = [(Material(name="feed_block", epsilon=(MUR_EPS,) * 3), MUR_BLOCK)]
+        elif cfg["loaded"]:
             regions = [(Material(name="block", epsilon=(BLOCK_EPS,) * 3, sigma=(BLOCK_SIGMA,) * 3), BLOCK),
                        (Material.pec(), POST)]
         mesh = Mesh.from_grid(grid, regions=regions).with_boundary_conditions(
@@ -104,6 +121,11 @@ def main(ref_src: str) -> None:
                         out[f"{op.name}_e_u{m}"] = np.asarray(dm.e_u_profile, float)
                         out[f"{op.name}_e_v{m}"] = np.asarray(dm.e_v_profile, float)
                         out[f"{op.name}_omega_c{m}"] = np.float64(dm.mode.omega_c)
+                    out[f"{op.name}_mur"] = np.array([t is None for t in op._dtbc])
+                    out[f"{op.name}_mur_r"] = np.asarray(op._mur_r, float)
+                    if op._comp_r_u is not None:
+                        for k in ("r_u", "r_v", "live_u", "live_v"):
+                            out[f"{op.name}_comp_{k}"] = np.asarray(getattr(op, f"_comp_{k}"), float)
 
             def wf(t, lo=f_c):
                 return modulated_gaussian(t, F_MAX, lo)
@@ -151,4 +173,5 @@ def main(ref_src: str) -> None:
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src")
+    # python make_c1_fixtures.py [reference src] [case ...]
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src", sys.argv[2:] or None)

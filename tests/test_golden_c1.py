@@ -25,7 +25,7 @@ import cases
 from magnelio_amd import mesh as pm
 from oracle import sparams as osp
 
-CASES = ("c1_guide", "c1_loaded")
+CASES = ("c1_guide", "c1_loaded", "c1_mur")
 
 
 def _product_coefficients(name):
@@ -62,8 +62,10 @@ def test_oracle_coefficients_equal_reference(name):
 
 def _sparams(ports, dt, f_axis):
     sig = {(p.name, m): (np.array(p.V[m]), np.array(p.I[m])) for p in ports for m in range(p.n_modes)}
-    lines = {(p.name, m): p.line[m] for p in ports for m in range(p.n_modes)}
-    return osp.s_params(sig, lines, ("port1", 0), f_axis, dt)
+    lines = {(p.name, m): p.line[m] for p in ports for m in range(p.n_modes) if m in p.line}
+    modes = {(p.name, m): (md["wc"], md["eps_r"]) for p in ports for m, md in enumerate(p.modes)}
+    ndx = {p.name: p.pl["normal_dx"] for p in ports}
+    return osp.s_params(sig, lines, ("port1", 0), f_axis, dt, modes=modes, normal_dx=ndx)
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -92,9 +94,57 @@ def test_oracle_run_matches_reference(name):
         assert np.max(np.abs(s[ok] - want[ok])) <= 1e-12, (lbl, m)
 
 
-def test_oracle_float32_matches_reference_float32():
-    f = cases.golden_c1("c1_guide")
-    orc, _, _, _ = cases.golden_c1_oracle("c1_guide", np.float32)
+def test_complement_absorber_arrays_equal_reference():
+    """The oracle's per-edge Mur coefficients and live masks of every port
+    (factory.py:687-735) equal the reference's; c1_mur's port 1 is the one
+    on Mur, with its absorber live."""
+    f = cases.golden_c1("c1_mur")
+    assert f["port1_mur"].all() and not f["port2_mur"].any()
+    _, ports, _, _ = cases.golden_c1_oracle("c1_mur")
+    for p in ports:
+        for k, got in zip(("r_u", "r_v", "live_u", "live_v"), p.comp):
+            np.testing.assert_allclose(got, f[f"{p.name}_comp_{k}"], rtol=1e-14, atol=0)
+    assert ports[0].complement_active and not ports[1].complement_active
+
+
+@pytest.mark.parametrize("name", ["c1_guide", "c1_mur"])
+def test_product_ports_carry_reference_complement(name):
+    """The product's port build: the reference's terminations (c1_mur's
+    port 1 on Mur, everything else on the DTBC) and complement arrays."""
+    f = cases.golden_c1(name)
+    mesh = cases.golden_c1_mesh(name)
+    dt = pm.courant_dt(mesh.grid, "normal")
+    from magnelio_amd import ports as pp  # noqa: PLC0415
+
+    me, mu = pm.build_m_eps(mesh), pm.build_m_mu(mesh)
+    for label, face in (("port1", "zmin"), ("port2", "zmax")):
+        P = pp.build_port(label, face, mesh, me, mu, dt=dt, f_calc=cases.C1_FMAX, n_modes=3)
+        if f"{label}_mur" in f:
+            assert [k == "mur" for k in P.termination_kinds] == list(f[f"{label}_mur"])
+            np.testing.assert_allclose(P.mur_r, f[f"{label}_mur_r"], rtol=1e-12)
+            n_u = P.plane.e_u_indices.size
+            np.testing.assert_allclose(P.comp_r[:n_u], f[f"{label}_comp_r_u"], rtol=1e-14, atol=0)
+            np.testing.assert_allclose(P.comp_r[n_u:], f[f"{label}_comp_r_v"], rtol=1e-14, atol=0)
+            assert np.array_equal(P.comp_live, np.concatenate([f[f"{label}_comp_live_u"],
+                                                               f[f"{label}_comp_live_v"]]))
+        assert P.complement_active == (name == "c1_mur" and label == "port1")
+
+
+def test_oracle_checkpoint_complement_state_equals_reference():
+    f = cases.golden_c1("c1_mur")
+    orc, ports, _, _ = cases.golden_c1_oracle("c1_mur")
+    orc.run(150)
+    p = ports[0]
+    for k, got in (("int_u", p.comp_int_u), ("int_v", p.comp_int_v), ("port_u", p.comp_port_u),
+                   ("port_v", p.comp_port_v)):
+        want = f[f"double_state/ports/port1/complement/{k}"]
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.abs(want).max(), k
+
+
+@pytest.mark.parametrize("name", ["c1_guide", "c1_mur"])
+def test_oracle_float32_matches_reference_float32(name):
+    f = cases.golden_c1(name)
+    orc, _, _, _ = cases.golden_c1_oracle(name, np.float32)
     for n in f["snaps"]:
         orc.run(int(n) - orc.n)
         for comp, got in (("e", orc.e), ("h", orc.h)):

@@ -8,7 +8,9 @@ dielectric block and a PEC post (tests/golden/make_c1_fixtures.py).
   1e-9 of the run's largest, S-parameters within 1e-9;
 * HIP float32 on the guide: fields within 1e-6 of the reference's float32
   run (the port reductions associate differently in float64), S within
-  1e-4 of the reference's float64 S (the single-vs-double gate).
+  1e-4 of the reference's float64 S (the single-vs-double gate);
+* c1_mur (tests/golden/c1_mur.npz): port 1 on modal Mur with the complement
+  absorber live, port 2 on the DTBC — the same gates in both precisions.
 """
 
 import numpy as np
@@ -29,14 +31,20 @@ def _hip_sparams(sol, ports, dt, f_axis):
 
 
 @pytest.mark.parametrize("name,precision", [("c1_guide", "double"), ("c1_loaded", "double"),
-                                            ("c1_guide", "single")])
+                                            ("c1_guide", "single"), ("c1_mur", "double"),
+                                            ("c1_mur", "single")])
 def test_hip_run_matches_reference(name, precision, gpu):
     print(f"HIP golden {name} {precision}", flush=True)
     f = cases.golden_c1(name)
     sol, ports, dt = cases.golden_c1_solver(name, precision)
     assert dt == float(f["dt"])
     p = precision + "_"
-    gate = 1e-10 if precision == "double" else 1e-6
+    # float32: the port's float64 projections are fixed-order device sums,
+    # the reference's are NumPy dots; a last-bit difference can flip the
+    # float32 rounding of a written plane edge.  With the complement absorber
+    # (c1_mur) the written remainder carries such a flip on through the Mur
+    # state, so those runs agree to the single-vs-double bound, not to 1e-6.
+    gate = 1e-10 if precision == "double" else (1e-5 if name == "c1_mur" else 1e-6)
     for n in f["snaps"]:
         sol.run(int(n) - sol.n_completed)
         e, h = sol.fields(0)
