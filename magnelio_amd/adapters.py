@@ -44,7 +44,7 @@ import numpy as np
 from . import mesh as _mesh
 from .boundaries import CPMLBoundary, PECBoundary, PMCBoundary
 from .monitors import MonitorFarField, MonitorFieldFrequency
-from .ports import PortModel, PortPlane, TEMode
+from .ports import DiscreteMode, PortModel, PortPlane
 from .sources import PlaneWaveSource
 
 _FACES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
@@ -130,9 +130,9 @@ def port_from_operator(op, f_calc: float | None = None) -> PortModel:
     for dm in op.discrete_modes:
         md = dm.mode
         kind = str(getattr(md.mode_type, "value", md.mode_type))
-        if kind != "TE":
+        if kind not in ("TE", "TM"):
             raise NotImplementedError(f"port {op.name!r}: {kind} modes are not served by this backend")
-        modes.append(TEMode(name=str(md.name), omega_c=float(md.omega_c), epsilon_r=float(md.epsilon_r),
+        modes.append(DiscreteMode(name=str(md.name), omega_c=float(md.omega_c), epsilon_r=float(md.epsilon_r),
                             e_u=np.asarray(dm.e_u_profile, float), e_v=np.asarray(dm.e_v_profile, float),
                             h_u=np.asarray(dm.h_u_profile, float), h_v=np.asarray(dm.h_v_profile, float),
                             mode_type=kind))

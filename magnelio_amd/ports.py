@@ -12,26 +12,32 @@ Reference counterparts (relative to the reference repository root):
 * 2-D curl-curl restriction        ports/_modal/curl_curl_2d.py:52-124, :416-471
 * numerical TE modes (eigsh)       ports/_modal/numerical_2d.py:454-583, sign
                                    convention :758-802, Z at f_calc :828-853
+* numerical TM modes               normal-E node problem ports/_modal/
+                                   curl_curl_2d.py:127-218 (normal edges
+                                   :236-292, plane gradient :295-347), eigsh
+                                   numerical_2d.py:585-753; TE + TM merge by
+                                   cut-off ports/_modal/factory.py:1627-1660
 * travelling-wave H profiles       ports/_modal/tem_laplace.py:102-167
 * V/I calibration, physical scales ports/_modal/operator.py:778-941 (numerical
                                    path; staircase planes: flux factor 1)
 * DTBC chain certificate, z0       ports/_modal/operator.py:671-776
 * slab-defect certificate          ports/_modal/factory.py:1189-1244
 * Mur coefficient / delay          ports/_modal/operator.py:652-661
+* complement absorber arrays       ports/_modal/factory.py:687-735
 * feed validation                  ports/_modal/factory.py:875-950
 * build order                      ports/_modal/factory.py:1367-1877
 
-Scope: whole-face ports on hollow (single-dielectric) feeds whose requested
-modes are TE.  A feed whose lowest modes include a TM mode is rejected
-(the TM node operators are not part of this build).  Degenerate-mode gauge
-fixing and the complement absorber are not built: every certified channel
-runs the exact DTBC, for which the reference keeps neither.
+Scope: whole-face ports on hollow (single-dielectric) feeds; the channels
+are the lowest ``n_modes`` discrete cut-offs over the TE and TM families
+together (TEM / quasi-TEM multi-conductor feeds are not built).
+Degenerate-mode gauge fixing is not built: a degenerate pair keeps the
+eigen-solver's basis.
 """
 
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 import scipy.sparse as sp
@@ -52,6 +58,7 @@ FACES = {
 DTBC_PAIR_SPREAD_TOL = 1e-8
 DTBC_WEIGHT_FLOOR = 1e-12
 DTBC_SLAB_DEFECT_TOL = 1e-8
+DEGENERATE_CUTOFF_RTOL = 1e-3
 
 
 def _e_shape(c, nx, ny, nz):
@@ -220,11 +227,12 @@ def _arpack_v0(n: int) -> np.ndarray:
 
 def z_te(omega: float, omega_c: float, eps_r: float) -> complex:
     """TE wave impedance j w mu0 / gamma (src/magnelio/ports/_modal/mode.py:168-213)."""
-    k2 = omega**2 * MU0 * EPS0 * eps_r
-    kc2 = omega_c**2 * MU0 * EPS0 * eps_r
-    diff = kc2 - k2
-    gamma = 1j * math.sqrt(-diff) if diff <= 0.0 else complex(math.sqrt(diff))
-    return 1j * omega * MU0 / gamma
+    return 1j * omega * MU0 / gamma_mode(omega, omega_c, eps_r)
+
+
+def z_tm(omega: float, omega_c: float, eps_r: float) -> complex:
+    """TM wave impedance gamma / (j w eps0 eps_r) (mode.py:206-211)."""
+    return gamma_mode(omega, omega_c, eps_r) / (1j * omega * EPS0 * eps_r)
 
 
 def gamma_mode(omega: float, omega_c: float, eps_r: float) -> complex:
@@ -235,8 +243,8 @@ def gamma_mode(omega: float, omega_c: float, eps_r: float) -> complex:
 
 
 @dataclass
-class TEMode:
-    """One discrete TE channel on a port plane."""
+class DiscreteMode:
+    """One discrete hollow-guide channel (TE or TM) on a port plane."""
 
     name: str
     omega_c: float
@@ -248,6 +256,8 @@ class TEMode:
     mode_type: str = "TE"
 
     def z_modal(self, omega: float) -> complex:
+        if self.mode_type == "TM":
+            return z_tm(omega, self.omega_c, self.epsilon_r)
         return z_te(omega, self.omega_c, self.epsilon_r)
 
     def gamma(self, omega: float) -> complex:
@@ -277,7 +287,7 @@ def _resolve_sign(plane: PortPlane, v: np.ndarray) -> np.ndarray:
 
 
 def solve_te_modes(plane: PortPlane, grid, m_eps, m_mu, pec_e_flat, n_modes: int,
-                   f_calc: float, eps_r: float = 1.0) -> list[TEMode]:
+                   f_calc: float, eps_r: float = 1.0) -> list[DiscreteMode]:
     """Lowest TE modes of K e = w_c^2 M e on the plane (free, non-PEC edges)."""
     prim = np.concatenate([plane.e_u_indices, plane.e_v_indices])
     hn = normal_h_indices(plane.face, grid)
@@ -313,12 +323,6 @@ def solve_te_modes(plane: PortPlane, grid, m_eps, m_mu, pec_e_flat, n_modes: int
     order = np.argsort(vals)[:n_modes]
     if order.size < n_modes:
         raise RuntimeError(f"only {order.size} TE modes found on port {plane.face}")
-    # TM guard: on a rectangular PEC-walled plane the lowest TM cut-off is
-    # TM11; a requested set reaching it would need the TM family.
-    tm11 = (math.pi * c_eff) ** 2 * (1 / L_u**2 + 1 / L_v**2)
-    if vals[order[-1]] > tm11 * 0.999:
-        raise NotImplementedError(f"port {plane.face}: the lowest {n_modes} modes include a TM "
-                                  "mode; only TE channels are built")
     n_u = plane.e_u_indices.size
     omega_calc = 2 * math.pi * f_calc
     modes = []
@@ -329,19 +333,128 @@ def solve_te_modes(plane: PortPlane, grid, m_eps, m_mu, pec_e_flat, n_modes: int
         full = _resolve_sign(plane, full)
         e_u, e_v = full[:n_u].copy(), full[n_u:].copy()
         omega_c = math.sqrt(float(vals[i]))
-        z_real = _z_real_te(omega_calc, omega_c, eps_r)
+        z_real = _z_real("TE", omega_calc, omega_c, eps_r)
         h_u, h_v = travelling_wave_h(e_u, e_v, plane, m_mu, z_real)
-        modes.append(TEMode(f"TE_num{rank:02d}", omega_c, eps_r, e_u, e_v, h_u, h_v))
+        modes.append(DiscreteMode(f"TE_num{rank:02d}", omega_c, eps_r, e_u, e_v, h_u, h_v))
     return modes
 
 
-def _z_real_te(omega: float, omega_c: float, eps_r: float) -> float:
+def normal_e_indices(plane: PortPlane, grid) -> np.ndarray:
+    """Flat E indices of the normal edges leaving the plane's nodes into the
+    domain (layer 0 on a MIN face, N - 1 on a MAX face), one per plane node
+    in the (u, v) node raster iu * (N_v + 1) + iv (curl_curl_2d.py:236-292)."""
+    n_ax, is_max, u_ax, v_ax = FACES[plane.face]
+    nx, ny, nz = grid.dims
+    N = (nx, ny, nz)
+    e_off = _offsets([_e_shape(c, nx, ny, nz) for c in range(3)])
+    iu, iv = np.meshgrid(np.arange(N[u_ax] + 1), np.arange(N[v_ax] + 1), indexing="ij")
+    ijk = [None, None, None]
+    ijk[n_ax], ijk[u_ax], ijk[v_ax] = np.full_like(iu, N[n_ax] - 1 if is_max else 0), iu, iv
+    es = _e_shape(n_ax, nx, ny, nz)
+    return (e_off[n_ax] + (ijk[0] * es[1] + ijk[1]) * es[2] + ijk[2]).ravel().astype(np.int64)
+
+
+def gradient_2d(plane: PortPlane) -> sp.csr_matrix:
+    """Plane-node gradient: rows [e_u | e_v] in plane order, columns the
+    node raster of :func:`normal_e_indices`; +1 at the edge's end node (the
+    restriction of the 3-D node gradient, curl_curl_2d.py:295-347)."""
+    n_u_c, n_v_c = plane.u_node_window[1], plane.v_node_window[1]  # cells along u, v
+    nv1 = n_v_c + 1
+    # e_u (iu cell, iv node), iv fastest: nodes (iu, iv) -> (iu + 1, iv)
+    iu, iv = np.meshgrid(np.arange(n_u_c), np.arange(n_v_c + 1), indexing="ij")
+    a_u, b_u = (iu * nv1 + iv).ravel(), ((iu + 1) * nv1 + iv).ravel()
+    # e_v (iv cell, iu node), iu fastest: nodes (iu, iv) -> (iu, iv + 1)
+    jv, ju = np.meshgrid(np.arange(n_v_c), np.arange(n_u_c + 1), indexing="ij")
+    a_v, b_v = (ju * nv1 + jv).ravel(), (ju * nv1 + jv + 1).ravel()
+    a, b = np.concatenate([a_u, a_v]), np.concatenate([b_u, b_v])
+    r = np.arange(a.size)
+    return sp.csr_matrix((np.concatenate([-np.ones(a.size), np.ones(a.size)]),
+                          (np.concatenate([r, r]), np.concatenate([a, b]))),
+                         shape=(a.size, (n_u_c + 1) * nv1))
+
+
+def solve_tm_modes(plane: PortPlane, grid, m_eps, m_mu, pec_e_flat, n_modes: int,
+                   f_calc: float, eps_r: float = 1.0) -> list[DiscreteMode]:
+    """Lowest TM modes of the normal-E node problem K_z phi = w_c^2 M_z phi.
+
+    The normal edges at the plane exchange energy with the transversal H
+    faces of the first cell only: K_z = C_s^T diag(1/M_mu) C_s with C_s the
+    3-D curl restricted to those faces and edges, which on the plane is the
+    node gradient, so K_z = G^T diag(1/M_mu[h co-located with each edge]) G;
+    M_z = M_eps of the normal edges; Dirichlet where the normal edge is PEC
+    or massless (curl_curl_2d.py:127-218).  Shift-invert eigsh below the
+    hollow-rectangle TM11 cut-off (numerical_2d.py:585-753); the transverse
+    profile is e_t = G phi, M_eps-normalised on the plane edges and signed
+    by the port-symmetric rule.
+    """
+    if f_calc <= 0.0:
+        raise ValueError("f_calc must be positive")
+    e_n = normal_e_indices(plane, grid)
+    G = gradient_2d(plane)
+    mh = np.concatenate([m_mu[plane.h_v_indices], m_mu[plane.h_u_indices]])  # [e_u | e_v] rows
+    mu_inv = np.where(mh > 0, 1.0 / np.where(mh > 0, mh, 1.0), 0.0)
+    K = (G.T @ sp.diags(mu_inv) @ G).tocsr()
+    m_z = np.asarray(m_eps[e_n], float)
+    dirichlet = np.asarray(pec_e_flat[e_n], bool) | (m_z <= 0.0)
+    free = np.nonzero(~dirichlet)[0]
+    if free.size == 0:
+        raise RuntimeError(f"port {plane.face}: every plane node is PEC; no TM degrees of freedom")
+    Kf = K[free][:, free]
+    Mf = sp.diags(m_z[free]).tocsr()
+    L_u = float(plane.u_edge_uv[:, 0].max() - plane.u_edge_uv[:, 0].min())
+    L_v = float(plane.v_edge_uv[:, 1].max() - plane.v_edge_uv[:, 1].min())
+    if L_u <= 0.0 or L_v <= 0.0:
+        raise RuntimeError(f"port {plane.face}: zero u/v extent")
+    c_eff = C0 / math.sqrt(eps_r)
+    sigma = 0.95 * 0.95 * ((math.pi * c_eff / L_u) ** 2 + (math.pi * c_eff / L_v) ** 2)
+    n_total = min(max(n_modes + 4, 2 * n_modes), free.size - 2)
+    if n_total < n_modes:
+        raise RuntimeError(f"port {plane.face}: {free.size} free TM nodes are too few for "
+                           f"{n_modes} modes")
+    vals, vecs = eigsh(Kf, k=n_total, M=Mf, sigma=sigma, v0=_arpack_v0(free.size))
+    if np.any(vals <= 0.0):
+        raise RuntimeError(f"port {plane.face}: non-positive TM eigenvalues {vals[vals <= 0.0]}")
+    order = np.argsort(vals)[:n_modes]
+    phi = np.zeros((e_n.size, order.size))
+    phi[free] = vecs[:, order]
+    e_t = G @ phi
+    me2d = m_eps[np.concatenate([plane.e_u_indices, plane.e_v_indices])]
+    norm_sq = np.einsum("ij,i,ij->j", e_t, me2d, e_t)
+    if np.any(norm_sq <= 0.0):
+        raise RuntimeError(f"port {plane.face}: TM profile with a non-positive M_eps norm")
+    e_t = e_t / np.sqrt(norm_sq)[None, :]
+    n_u = plane.e_u_indices.size
+    omega_calc = 2 * math.pi * f_calc
+    modes = []
+    for rank in range(order.size):
+        full = _resolve_sign(plane, e_t[:, rank])
+        e_u, e_v = full[:n_u].copy(), full[n_u:].copy()
+        omega_c = math.sqrt(float(vals[order[rank]]))
+        h_u, h_v = travelling_wave_h(e_u, e_v, plane, m_mu, _z_real("TM", omega_calc, omega_c, eps_r))
+        modes.append(DiscreteMode(f"TM_num{rank:02d}", omega_c, eps_r, e_u, e_v, h_u, h_v, "TM"))
+    return modes
+
+
+def solve_modes(plane: PortPlane, grid, m_eps, m_mu, pec_e_flat, n_modes: int, f_calc: float,
+                eps_r: float = 1.0) -> list[DiscreteMode]:
+    """The n_modes lowest cut-offs over both families: n_modes of each are
+    solved, merged by a stable sort on w_c (TE first on a tie) and cut
+    (factory.py:1627-1660)."""
+    te = solve_te_modes(plane, grid, m_eps, m_mu, pec_e_flat, n_modes, f_calc, eps_r)
+    tm = solve_tm_modes(plane, grid, m_eps, m_mu, pec_e_flat, n_modes, f_calc, eps_r)
+    return sorted(te + tm, key=lambda m: m.omega_c)[:n_modes]
+
+
+def _z_real(mode_type: str, omega: float, omega_c: float, eps_r: float) -> float:
+    """|Z| at omega for the H-profile bake-in (numerical_2d.py:828-853)."""
     k2 = omega**2 * MU0 * EPS0 * eps_r
     kc2 = omega_c**2 * MU0 * EPS0 * eps_r
     diff = kc2 - k2
     if abs(diff) < 1e-30:
         raise ValueError("f_calc coincides with a mode cut-off")
-    return omega * MU0 / math.sqrt(abs(diff))
+    if mode_type == "TE":
+        return omega * MU0 / math.sqrt(abs(diff))
+    return math.sqrt(abs(diff)) / (omega * EPS0 * eps_r)
 
 
 def travelling_wave_h(e_u, e_v, plane: PortPlane, m_mu, z_wave: float):
@@ -379,7 +492,7 @@ class PortModel:
 
     name: str
     plane: PortPlane
-    modes: list[TEMode]
+    modes: list[DiscreteMode]
     dt: float
     omega_calc: float
     me_u_port: np.ndarray
@@ -480,8 +593,7 @@ def _calibrate(plane, modes, me_u, me_v, mh_u, mh_v, omega_calc):
         if i_test != 0.0 and abs(Z.imag) <= abs(Z.real) and zr != 0.0:
             gamma = (v_test / i_test) / zr
             if math.isfinite(gamma) and gamma != 0.0:
-                out.append(TEMode(dm.name, dm.omega_c, dm.epsilon_r, dm.e_u, dm.e_v,
-                                  dm.h_u * gamma, dm.h_v * gamma))
+                out.append(replace(dm, h_u=dm.h_u * gamma, h_v=dm.h_v * gamma))
                 calibrated = True
         if calibrated and p_one != 0.0 and math.isfinite(p_one):
             kappa = math.sqrt(abs(p_one) * abs(zr))
@@ -506,7 +618,7 @@ def _calibrate(plane, modes, me_u, me_v, mh_u, mh_v, omega_calc):
     return out, rec, src, st
 
 
-def _chain_params(dm: TEMode, me_u, me_v, mh_u, mh_v, dt, slab_defect):
+def _chain_params(dm: DiscreteMode, me_u, me_v, mh_u, mh_v, dt, slab_defect):
     q_mode = dm.omega_c * dt
     if slab_defect is not None and slab_defect > DTBC_SLAB_DEFECT_TOL:
         return None, None, None
@@ -525,7 +637,7 @@ def _chain_params(dm: TEMode, me_u, me_v, mh_u, mh_v, dt, slab_defect):
     return r_mean, q_mode, spread
 
 
-def _chain_z0(dm: TEMode, r_mode, me_u, me_v, mh_u, mh_v, dt) -> float:
+def _chain_z0(dm: DiscreteMode, r_mode, me_u, me_v, mh_u, mh_v, dt) -> float:
     num = np.concatenate([mh_v * dm.h_v, mh_u * dm.h_u])
     den = np.concatenate([dm.e_u, dm.e_v])
     weight = np.concatenate([me_u * dm.e_u**2, me_v * dm.e_v**2])
@@ -561,10 +673,29 @@ def chain_slab_defect(dims, m_eps, m_mu, face: str) -> float:
     return defect
 
 
+def warn_degenerate_cutoffs(modes, name: str, rtol: float = DEGENERATE_CUTOFF_RTOL) -> None:
+    """UserWarning for two channels whose cut-offs agree within rtol: their
+    flux inner product is not orthogonal, so the port couples them
+    (factory.py:1318-1364).  TE_mn / TM_mn of a homogeneously filled PEC
+    rectangle are such a pair at any resolution."""
+    import warnings  # noqa: PLC0415
+
+    ws = sorted(modes, key=lambda m: m.omega_c)
+    for a, b in zip(ws, ws[1:]):
+        if a.omega_c <= 0.0 or b.omega_c <= 0.0:
+            continue
+        rel = abs(b.omega_c - a.omega_c) / b.omega_c
+        if rel < rtol:
+            warnings.warn(f"modal port {name!r}: modes {a.name!r} (fc = {a.omega_c / 2e9 / math.pi:.4f} "
+                          f"GHz) and {b.name!r} (fc = {b.omega_c / 2e9 / math.pi:.4f} GHz) are "
+                          f"degenerate within {rel * 100:.3f}% relative cut-off difference; the "
+                          "port couples them", UserWarning, stacklevel=3)
+
+
 def build_port(name: str, face: str, mesh, m_eps: np.ndarray, m_mu: np.ndarray, *, dt: float,
                f_calc: float, n_modes: int = 1, eps_r: float = 1.0,
                termination: str = "auto") -> PortModel:
-    """Numerical TE modal port on a whole bbox face (factory.py:1367-1877 order)."""
+    """Numerical modal port (TE and TM) on a whole bbox face (factory.py:1367-1877 order)."""
     if dt <= 0.0 or f_calc <= 0.0:
         raise ValueError("dt and f_calc must be positive")
     grid = mesh.grid
@@ -574,7 +705,8 @@ def build_port(name: str, face: str, mesh, m_eps: np.ndarray, m_mu: np.ndarray, 
     m_eps = _mesh.flatten_port_mass(m_eps, dims, face)
     m_mu = _mesh.flatten_port_mu(m_mu, dims, face)
     pec = _mesh.pec_flat(_mesh.flatten_port_pec(mesh.pec_mask_edges, dims, face), dims)
-    modes = solve_te_modes(plane, grid, m_eps, m_mu, pec, n_modes, f_calc, eps_r)
+    modes = solve_modes(plane, grid, m_eps, m_mu, pec, n_modes, f_calc, eps_r)
+    warn_degenerate_cutoffs(modes, name)
     omega_calc = 2.0 * math.pi * f_calc
     me_u, me_v = m_eps[plane.e_u_indices], m_eps[plane.e_v_indices]
     me_ui, me_vi = m_eps[plane.e_u_indices_interior], m_eps[plane.e_v_indices_interior]

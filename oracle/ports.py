@@ -4,6 +4,15 @@ Restates (paths relative to the reference repository root):
 
 * plane maps        src/magnelio/ports/_modal/port_plane.py:32-334 and the
                     face frames of src/magnelio/mesh/faces.py:63-71
+* discrete TM modes closed form on a uniform PEC rectangle: the node
+                    potential phi = sin(m pi ia/Na) sin(n pi ib/Nb) of the
+                    normal-E problem (ports/_modal/curl_curl_2d.py:127-218),
+                    transverse profile its plane gradient e_t = G phi
+                    (numerical_2d.py:585-753), w_c^2 = c^2 (km^2 + kn^2)
+                    with the km, kn below; TM wave impedance
+                    gamma/(j w eps0 eps_r) and |Z| bake-in
+                    sqrt|kc^2 - k^2|/(w eps0 eps_r) (mode.py:206-211,
+                    numerical_2d.py:828-853)
 * discrete TE modes closed form on a uniform PEC rectangle: edge voltages
                     e_x = kn cos(m pi (i+1/2)/Nx) sin(n pi j/Ny) dx,
                     e_y = -km sin(m pi i/Nx) cos(n pi (j+1/2)/Ny) dy,
@@ -33,7 +42,10 @@ Parity pins (tests/test_oracle_ports.py): the closed-form modes are
 eigenpairs of an independently assembled curl-curl pencil to 1e-12, are
 M_eps-orthonormal and divergence-free; the calibration makes the unit
 travelling wave satisfy V/I = Z_modal exactly; the discrete TE10 chain
-parameters satisfy r = c dt / dz and q = w_c dt.  ``record_scale`` /
+parameters satisfy r = c dt / dz and q = w_c dt.  The closed-form TM modes
+are eigenpairs of the normal-E pencil assembled from the oracle's own 3-D
+curl operators, curl-free on the plane and M_eps-orthogonal to every TE
+mode.  ``record_scale`` /
 ``source_scale`` / ``state_scale`` have no closed form; they are pinned by
 the reference's own values on the uniform WR-90 guide
 (tests/test_golden_ports.py, tests/golden/ports_wr90.npz), and the whole
@@ -164,6 +176,48 @@ def rect_te_mode(pl: dict, x, y, z, m: int, n: int, m_eps: np.ndarray, eps_r: fl
     return full[:nu].copy(), full[nu:].copy(), wc
 
 
+def rect_tm_mode(pl: dict, x, y, z, m: int, n: int, m_eps: np.ndarray, eps_r: float = 1.0):
+    """Closed-form discrete TM_mn (m, n >= 1) on a uniform PEC-walled plane.
+
+    The transverse profile is the plane gradient of the node potential
+    phi(ia, ib) = sin(m pi ia/Na) sin(n pi ib/Nb): on an edge along axis a
+    at (cell ia, node ib), phi(ia+1, ib) - phi(ia, ib) =
+    km da cos(m pi (ia+1/2)/Na) sin(n pi ib/Nb), and likewise along b.
+    Returns (e_u, e_v, omega_c), M_eps-normalised and signed.
+    """
+    if m < 1 or n < 1:
+        raise ValueError("TM_mn needs m, n >= 1")
+    nodes = (np.asarray(x), np.asarray(y), np.asarray(z))
+    ta = sorted([pl["u_axis"], pl["v_axis"]])
+    Na, Nb = pl["N"][ta[0]], pl["N"][ta[1]]
+    da = float(np.diff(nodes[ta[0]])[0])
+    db = float(np.diff(nodes[ta[1]])[0])
+    a0, b0 = nodes[ta[0]][0], nodes[ta[1]][0]
+    km = (2.0 / da) * math.sin(m * math.pi / (2 * Na))
+    kn = (2.0 / db) * math.sin(n * math.pi / (2 * Nb))
+
+    def prof(uv, along_a):
+        if along_a:
+            ia = (uv[:, 0] - a0) / da
+            ib = np.rint((uv[:, 1] - b0) / db)
+            s = np.where((ib == 0) | (ib == Nb), 0.0, np.sin(n * math.pi * ib / Nb))
+            return km * da * np.cos(m * math.pi * ia / Na) * s
+        ib = (uv[:, 0] - b0) / db
+        ia = np.rint((uv[:, 1] - a0) / da)
+        s = np.where((ia == 0) | (ia == Na), 0.0, np.sin(m * math.pi * ia / Na))
+        return kn * db * s * np.cos(n * math.pi * ib / Nb)
+
+    e_u = prof(pl["u_uv"], pl["u_axis"] == ta[0])
+    e_v = prof(pl["v_uv"][:, ::-1], pl["v_axis"] == ta[0])
+    full = np.concatenate([e_u, e_v])
+    me = np.concatenate([m_eps[pl["e_u"]], m_eps[pl["e_v"]]])
+    full = full / math.sqrt(float(np.sum(me * full * full)))
+    full = resolve_sign(pl, full)
+    nu = pl["e_u"].size
+    wc = C0 / math.sqrt(eps_r) * math.sqrt(km * km + kn * kn)
+    return full[:nu].copy(), full[nu:].copy(), wc
+
+
 def z_te(omega, omega_c, eps_r=1.0):
     k2 = omega**2 * MU0 * EPS0 * eps_r
     kc2 = omega_c**2 * MU0 * EPS0 * eps_r
@@ -172,12 +226,27 @@ def z_te(omega, omega_c, eps_r=1.0):
     return 1j * omega * MU0 / g, g
 
 
+def z_mode(kind, omega, omega_c, eps_r=1.0):
+    """(Z, gamma) of a TE or TM channel."""
+    z, g = z_te(omega, omega_c, eps_r)
+    if kind == "TM":
+        return g / (1j * omega * EPS0 * eps_r), g
+    return z, g
+
+
+def z_real(kind, omega, omega_c, eps_r=1.0):
+    """|Z| at omega used to bake the H profiles."""
+    root = math.sqrt(abs(omega_c**2 * MU0 * EPS0 * eps_r - omega**2 * MU0 * EPS0 * eps_r))
+    return omega * MU0 / root if kind == "TE" else root / (omega * EPS0 * eps_r)
+
+
 class Port:
-    """Oracle modal port: calibrated TE channels, Mur/DTBC termination."""
+    """Oracle modal port: calibrated TE / TM channels, Mur/DTBC termination."""
 
     def __init__(self, name, pl, modes, m_eps, m_mu, dt, omega_calc, eps_r=1.0, termination="auto",
                  complement=None):
-        """modes: list of (e_u, e_v, omega_c) from :func:`rect_te_mode`;
+        """modes: list of (e_u, e_v, omega_c) from :func:`rect_te_mode`, or
+        (e_u, e_v, omega_c, "TE" | "TM") (:func:`rect_tm_mode` is "TM");
         termination "auto" (exact DTBC) or "mur" (modal Mur-1 everywhere);
         complement: (r_u, r_v, live_u, live_v) of :func:`complement_arrays`
         or None."""
@@ -210,14 +279,13 @@ class Port:
         dv = duals(v_nodes)[np.searchsorted(v_nodes, pl["u_uv"][:, 1])]
         du = duals(u_nodes)[np.searchsorted(u_nodes, pl["v_uv"][:, 0])]
         area_u, area_v = pl["u_len"] * dv * 1.0, pl["v_len"] * du * 1.0
-        for mi, (e_u, e_v, wc) in enumerate(modes):
-            k2 = omega_calc**2 * MU0 * EPS0 * eps_r
-            kc2 = wc**2 * MU0 * EPS0 * eps_r
-            z_real = omega_calc * MU0 / math.sqrt(abs(kc2 - k2))
-            sc = MU0 * ndx / z_real
+        for mi, mode in enumerate(modes):
+            e_u, e_v, wc = mode[:3]
+            kind = mode[3] if len(mode) > 3 else "TE"
+            sc = MU0 * ndx / z_real(kind, omega_calc, wc, eps_r)
             h_u = np.where(self.mh_u > 0, -e_v * (sc / np.where(self.mh_u > 0, self.mh_u, 1.0)), 0.0)
             h_v = np.where(self.mh_v > 0, +e_u * (sc / np.where(self.mh_v > 0, self.mh_v, 1.0)), 0.0)
-            Z, gam = z_te(omega_calc, wc, eps_r)
+            Z, gam = z_mode(kind, omega_calc, wc, eps_r)
             zr = float(Z.real)
             v_test = float(np.dot(self.me_u, e_u**2)) + float(np.dot(self.me_v, e_v**2))
             i_test = float(np.dot(self.mh_u, h_u**2)) + float(np.dot(self.mh_v, h_v**2))
@@ -234,7 +302,8 @@ class Port:
             self.state_scale[mi] = c_state
             self.record_scale[mi] = kappa * c_state
             self.source_scale[mi] = math.sqrt(abs(zr)) / kappa / c_state
-            self.modes.append(dict(e_u=e_u, e_v=e_v, h_u=h_u * g, h_v=h_v * g, wc=wc, eps_r=eps_r))
+            self.modes.append(dict(e_u=e_u, e_v=e_v, h_u=h_u * g, h_v=h_v * g, wc=wc, eps_r=eps_r,
+                                   kind=kind))
         # chain parameters (all channels certified on a uniform feed)
         self.dtbc = []
         self.line = {}

@@ -4,10 +4,10 @@ Restates src/magnelio/signals/signal_1d.py:60-118 (direct DFT
 sum_n x_n e^{-2 pi j f n dt}, no dt factor) and
 src/magnelio/post/modal_sparameters.py:118-242, :335-557: I is shifted by
 the temporal half step e^{+j w dt/2}; certified DTBC channels use the exact
-discrete wave impedance (TE: z0 s/rad) and the discrete half-cell factor
-lambda^{1/2} for the two-plane de-stagger; channels on Mur the continuum TE
-wave impedance j w mu0 / gamma and the factor e^{gamma dx_n / 2}
-(mode.py:170-230); S = b / a_excited with |a| < 1e-12 peak -> NaN.
+discrete wave impedance (TE: z0 s/rad, TM: z0 rad/s) and the discrete
+half-cell factor lambda^{1/2} for the two-plane de-stagger; channels on Mur
+the continuum wave impedance (TE j w mu0 / gamma, TM gamma / (j w eps)) and
+the factor e^{gamma dx_n / 2} (mode.py:170-230); S = b / a_excited with |a| < 1e-12 peak -> NaN.
 
 Pinned by tests/test_oracle_sparams.py: the DFT of a sine and of DC against
 closed forms, and a synthetic discrete travelling wave whose a/b split must
@@ -29,19 +29,21 @@ def dft(x, f, dt):
 
 
 def waves(V, I, f, dt, line, mode=None, normal_dx=None):
-    """(a, b) at the port plane of a TE channel: certified (line = (r, q, z0))
-    or on Mur (line None, mode = (omega_c, eps_r), normal_dx the port cell)."""
-    from .ports import z_te  # noqa: PLC0415
+    """(a, b) at the port plane of a TE or TM channel: certified (line =
+    (r, q, z0)) or on Mur (line None, mode = (omega_c, eps_r[, kind]),
+    normal_dx the port cell)."""
+    from .ports import z_mode  # noqa: PLC0415
 
+    kind = mode[2] if mode is not None and len(mode) > 2 else "TE"
     w = 2 * math.pi * np.asarray(f, float)
     Vf = dft(V, f, dt)
     If = dft(I, f, dt) * np.exp(1j * w * dt / 2)
     if line is not None:
         r, q, z0 = line
-        Z = dtbc_wave_impedance(w * dt, q, z0, "TE")
+        Z = dtbc_wave_impedance(w * dt, q, z0, kind)
         th = destagger_theta(w * dt, r, q)
     else:
-        zg = [z_te(float(om), mode[0], mode[1]) for om in w]
+        zg = [z_mode(kind, float(om), mode[0], mode[1]) for om in w]
         Z = np.array([a for a, _ in zg], complex)
         th = 0.5 * float(normal_dx) * np.array([g for _, g in zg], complex)
     sz = np.sqrt(Z)

@@ -150,11 +150,23 @@ def _c1_fixture_module():
     return mk
 
 
+def golden_n_modes(name) -> int:
+    """Modes per port of a golden case (3 unless the fixture says)."""
+    return int(golden_c1(name).get("n_modes", 3))
+
+
+def golden_excite(name) -> int:
+    """The excited channel of port 1 in a golden case (0 unless the fixture says)."""
+    return int(golden_c1(name).get("excite", 0))
+
+
 def golden_c1_oracle(name, dtype=np.float64):
     """Oracle of a golden case: masses, PEC and alpha/beta from oracle.mesh,
     ports from the reference's numerical mode profiles (the oracle's Port
     calibrates them itself), each with its complement absorber; c1_mur's
-    port 1 terminates on Mur.  Returns (Oracle, ports, dt, (me, mu))."""
+    port 1 terminates on Mur; c1_tm's corner block is PEC and its ports carry a
+    TM channel.  Port 1's channel ``golden_excite(name)`` is excited.  Returns
+    (Oracle, ports, dt, (me, mu))."""
     f = golden_c1(name)
     x, y, z = f["x"], f["y"], f["z"]
     dims = (x.size - 1, y.size - 1, z.size - 1)
@@ -165,6 +177,8 @@ def golden_c1_oracle(name, dtype=np.float64):
     elif name == "c1_mur":
         mk = _c1_fixture_module()
         boxes, eps_block = [(1, mk.MUR_BLOCK)], mk.MUR_EPS
+    elif name == "c1_tm":
+        boxes = [(2, _c1_fixture_module().TM_CORNER)]
     mid = om.fill_aabb(x, y, z, boxes)
     eps = [[1.0, eps_block, 1.0]] * 3
     sig = [[0.0, GOLDEN_SIGMA if name == "c1_loaded" else 0.0, 0.0]] * 3
@@ -185,14 +199,15 @@ def golden_c1_oracle(name, dtype=np.float64):
     ports = []
     for label, face in (("port1", "zmin"), ("port2", "zmax")):
         pl = op.plane(face, x, y, z)
-        modes = [(f[f"{label}_e_u{m}"], f[f"{label}_e_v{m}"], float(f[f"{label}_omega_c{m}"]))
-                 for m in range(3)]
+        modes = [(f[f"{label}_e_u{m}"], f[f"{label}_e_v{m}"], float(f[f"{label}_omega_c{m}"]),
+                  str(f.get(f"{label}_type{m}", "TE"))) for m in range(golden_n_modes(name))]
         term = "mur" if (name == "c1_mur" and label == "port1") else "auto"
         comp = op.complement_arrays(pl, me, mvac, pec_mesh, dt)
         ports.append(op.Port(label, pl, modes, me, mm, dt, 2 * math.pi * C1_FMAX, termination=term,
                              complement=comp))
-    f_lo = ports[0].modes[0]["wc"] / (2 * math.pi)
-    ports[0].set_excitation(0, lambda t: modulated_gaussian(t, C1_FMAX, f_lo))
+    exc = golden_excite(name)
+    f_lo = ports[0].modes[exc]["wc"] / (2 * math.pi)
+    ports[0].set_excitation(exc, lambda t: modulated_gaussian(t, C1_FMAX, f_lo))
     orc = ofd.Oracle(dims, ae, be, ah, bh, dt, ports=ports, dtype=dtype)
     return orc, ports, dt, (me, mm)
 
@@ -210,11 +225,13 @@ def golden_c1_mesh(name):
     elif name == "c1_mur":
         mk = _c1_fixture_module()
         regions = [(pm.Material(name="feed_block", epsilon=(mk.MUR_EPS,) * 3), mk.MUR_BLOCK)]
+    elif name == "c1_tm":
+        regions = [(pm.Material.pec(), _c1_fixture_module().TM_CORNER)]
     return pm.Mesh.from_grid(g, regions=regions, pec_faces=("xmin", "xmax", "ymin", "ymax"))
 
 
 def golden_c1_solver(name, precision="double", n_steps=300):
-    """Product HIP solver of a golden case (port 1 mode 0 excited)."""
+    """Product HIP solver of a golden case (port 1's golden channel excited)."""
     from magnelio_amd import mesh as pm  # noqa: PLC0415
     from magnelio_amd import ports as pp  # noqa: PLC0415
     from magnelio_amd.boundaries import PECBoundary  # noqa: PLC0415
@@ -224,13 +241,14 @@ def golden_c1_solver(name, precision="double", n_steps=300):
     mesh = golden_c1_mesh(name)
     dt = pm.courant_dt(mesh.grid, "normal")
     me, mu = pm.build_m_eps(mesh), pm.build_m_mu(mesh)
-    ports = [pp.build_port(lbl, face, mesh, me, mu, dt=dt, f_calc=C1_FMAX, n_modes=3)
+    n_modes, exc = golden_n_modes(name), golden_excite(name)
+    ports = [pp.build_port(lbl, face, mesh, me, mu, dt=dt, f_calc=C1_FMAX, n_modes=n_modes)
              for lbl, face in (("port1", "zmin"), ("port2", "zmax"))]
-    wf = resolve_excitation(ports[0].modes[0].omega_c, 0.0, C1_FMAX, 0)
+    wf = resolve_excitation(ports[0].modes[exc].omega_c, 0.0, C1_FMAX, exc)
     bcs = {f: PECBoundary(f) for f in ("xmin", "xmax", "ymin", "ymax")}
     sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, ports=ports,
                               total_time_steps=n_steps, dt=dt, precision=precision, nsets=1,
-                              excitations=[ChannelExcitation("port1", 0, wf)])
+                              excitations=[ChannelExcitation("port1", exc, wf)])
     return sol, ports, dt
 
 

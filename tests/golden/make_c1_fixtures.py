@@ -20,9 +20,13 @@ plain cases for 300 steps:
   next to port 1: its feed slabs fail the DTBC certificate, its channels run
   on modal Mur and the complement absorber (a per-edge Mur on the
   port-unrepresented remainder of the plane) is live; port 2 keeps the DTBC;
+* ``c1_tm``:     the guide with a PEC block filling a 3 x 2-cell corner of the
+  cross-section along its whole length (an L-shaped section: TE_mn and
+  TM_mn are no longer degenerate), 4-mode ports whose channel 3 is the
+  lowest TM mode; that TM channel of port 1 is the one excited;
 
-port 1 mode 0 excited with the modulated Gaussian from the TE10 cut-off to
-25 GHz, dt = courant_dt(grid, "normal"), float64 (and float32 for the
+port 1 mode 0 (c1_tm: mode 3) excited with the modulated Gaussian from its
+cut-off to 25 GHz, dt = courant_dt(grid, "normal"), float64 (and float32 for the
 guide).  Stored: the ports' numerical mode profiles and cut-offs, alpha/beta
 (E, H) of the solver, e/h after steps 1, 10, 150 and 300 (reference flat
 layout), the solver's state_dict() after step 150 (flattened "a/b" keys),
@@ -68,6 +72,9 @@ def cases():
         # 596-619, :1337-1346); port 2 keeps the DTBC
         "c1_mur": dict(axes=(np.linspace(0.0, A, 13), np.linspace(0.0, B, 9), z), loaded="mur",
                        precisions=("double", "single")),
+        # an L-shaped section: the lowest TM mode is channel 3 of 4
+        "c1_tm": dict(axes=(np.linspace(0.0, A, 13), np.linspace(0.0, B, 9), z), loaded="tm",
+                      precisions=("double", "single"), n_modes=4, excite=3),
     }
 
 
@@ -78,6 +85,8 @@ BLOCK_EPS, BLOCK_SIGMA = 4.0, 0.05
 # the Mur case: a block in z cells 2-4 next to port 1
 MUR_BLOCK = (0.3 * A, 0.0, 2e-3, 0.7 * A, 0.5 * B, 5e-3)
 MUR_EPS = 2.5
+# the TM case: a PEC block over cells x 0-2, y 0-1, the whole length
+TM_CORNER = (0.0, 0.0, 0.0, 0.25 * A, 0.25 * B, 20e-3)
 
 
 def main(ref_src: str, only=None) -> None:
@@ -100,7 +109,10 @@ def main(ref_src: str, only=None) -> None:
         x, y, z = cfg["axes"]
         grid = GridLines(x=x, y=y, z=z)
         regions = []
-        if cfg["loaded"] == "mur":
+        n_modes, excite = cfg.get("n_modes", 3), cfg.get("excite", 0)
+        if cfg["loaded"] == "tm":
+            regions = [(Material.pec(), TM_CORNER)]
+        elif cfg["loaded"] == "mur":
             regions = [(Material(name="feed_block", epsilon=(MUR_EPS,) * 3), MUR_BLOCK)]
         elif cfg["loaded"]:
             regions = [(Material(name="block", epsilon=(BLOCK_EPS,) * 3, sigma=(BLOCK_SIGMA,) * 3), BLOCK),
@@ -108,19 +120,23 @@ def main(ref_src: str, only=None) -> None:
         mesh = Mesh.from_grid(grid, regions=regions).with_boundary_conditions(
             {"xmin": "PEC", "xmax": "PEC", "ymin": "PEC", "ymax": "PEC", "zmin": "PMC", "zmax": "PMC"})
         dt = courant_dt(grid, "normal")
-        out = {"x": x, "y": y, "z": z, "dt": np.float64(dt), "snaps": np.array(SNAPS)}
+        out = {"x": x, "y": y, "z": z, "dt": np.float64(dt), "snaps": np.array(SNAPS),
+               "n_modes": np.int64(n_modes), "excite": np.int64(excite)}
         for prec in cfg["precisions"]:
             m_eps, m_mu = build_M_eps(mesh), build_M_mu(mesh)
-            ops = [build_modal_port(PortSpecNumerical(name=lbl, plane=face, n_modes=3), mesh, m_eps,
-                                    m_mu, dt=dt, f_calc=F_MAX)
-                   for lbl, face in (("port1", BoxFace.Z_MIN), ("port2", BoxFace.Z_MAX))]
-            f_c = ops[0].discrete_modes[0].mode.omega_c / (2 * math.pi)
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                ops = [build_modal_port(PortSpecNumerical(name=lbl, plane=face, n_modes=n_modes), mesh,
+                                        m_eps, m_mu, dt=dt, f_calc=F_MAX)
+                       for lbl, face in (("port1", BoxFace.Z_MIN), ("port2", BoxFace.Z_MAX))]
+            f_c = ops[0].discrete_modes[excite].mode.omega_c / (2 * math.pi)
             if prec == "double":  # the ports' numerical modes (the oracle takes them as given)
                 for op in ops:
                     for m, dm in enumerate(op.discrete_modes):
                         out[f"{op.name}_e_u{m}"] = np.asarray(dm.e_u_profile, float)
                         out[f"{op.name}_e_v{m}"] = np.asarray(dm.e_v_profile, float)
                         out[f"{op.name}_omega_c{m}"] = np.float64(dm.mode.omega_c)
+                        out[f"{op.name}_type{m}"] = np.array(dm.mode.mode_type.value)
                     out[f"{op.name}_mur"] = np.array([t is None for t in op._dtbc])
                     out[f"{op.name}_mur_r"] = np.asarray(op._mur_r, float)
                     if op._comp_r_u is not None:
@@ -130,7 +146,7 @@ def main(ref_src: str, only=None) -> None:
             def wf(t, lo=f_c):
                 return modulated_gaussian(t, F_MAX, lo)
 
-            ops[0].set_excitation(0, wf)
+            ops[0].set_excitation(excite, wf)
             rec = PortSignalRecorder(dt=dt, ports=ops)
             sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions={}, ports=ops, recorder=rec,
                                       total_time_steps=SNAPS[0], dt=dt, verbose=False,
@@ -161,7 +177,7 @@ def main(ref_src: str, only=None) -> None:
             s = compute_s_parameters(
                 recorder_signals=sigs,
                 port_modes={op.name: [dm.mode for dm in op.discrete_modes] for op in ops},
-                excited=("port1", 0), reference_signal=ref, f_axis=f_axis,
+                excited=("port1", excite), reference_signal=ref, f_axis=f_axis,
                 port_normal_dx={op.name: op.plane.normal_dx for op in ops},
                 port_line_params={(op.name, m): lp for op in ops
                                   for m, lp in op.dtbc_line_params.items()})

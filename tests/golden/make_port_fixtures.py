@@ -3,7 +3,7 @@
 Run in the build container only (the reference never travels to the GPU
 machine; only the .npz files this writes do):
 
-    python tests/golden/make_port_fixtures.py /root/reference/src
+    python tests/golden/make_port_fixtures.py /root/reference/src [case ...]
 
 Pins what no closed form fixes: the numerical V/I calibration
 (``record_scale``, ``source_scale``, ``state_scale``;
@@ -15,10 +15,14 @@ ports/_modal/factory.py:1367) on
 * ``wr90``:   a WR-90 guide on a uniform 12 x 6 cross-section, 20 cells long;
 * ``graded``: the same guide on a seeded non-uniform cross-section (adjacent
   spacing ratio <= 1.3), uniform along z;
+* ``corner``: the uniform guide on 12 x 8 with a PEC block over a 3 x 2-cell
+  corner of the cross-section along its whole length (an L-shaped section,
+  so TE_mn and TM_mn split), 4-mode ports whose channel 3 is the lowest TM
+  mode;
 
-both with PEC x/y walls and 3-mode numerical ports (lowest cut-offs) on
-z_min and z_max, dt = courant_dt(grid, "normal"), f_calc = 25 GHz.  Also
-stored, under ``op_``: every attribute of the built ``PortOperatorModal``
+all with PEC x/y walls and numerical ports (lowest cut-offs over TE and TM;
+3 modes, 4 on ``corner``) on z_min and z_max, dt = courant_dt(grid,
+"normal"), f_calc = 25 GHz.  Also stored, under ``op_``: every attribute of the built ``PortOperatorModal``
 that a backend reads to run the port (plane maps and geometry, calibrated
 mode profiles, mass slices, Mur and DTBC chain parameters) — the adapter
 tests build reference-shaped stand-ins from them.
@@ -42,31 +46,43 @@ def graded_axis(n, length, seed):
     return np.concatenate([[0.0], np.cumsum(w)])
 
 
+# the corner case's PEC block (x0, y0, z0, x1, y1, z1): cells x 0-2, y 0-1
+CORNER = (0.0, 0.0, 0.0, 0.25 * A, 0.25 * B, L)
+
+
 def cases():
+    """name -> (x, y, z, corner block?, modes per port)."""
     return {
-        "wr90": (np.linspace(0.0, A, 13), np.linspace(0.0, B, 7), np.linspace(0.0, L, 21)),
-        "graded": (graded_axis(12, A, 11), graded_axis(6, B, 12), np.linspace(0.0, L, 21)),
+        "wr90": (np.linspace(0.0, A, 13), np.linspace(0.0, B, 7), np.linspace(0.0, L, 21), False, 3),
+        "graded": (graded_axis(12, A, 11), graded_axis(6, B, 12), np.linspace(0.0, L, 21), False, 3),
+        "corner": (np.linspace(0.0, A, 13), np.linspace(0.0, B, 9), np.linspace(0.0, L, 21), True, 4),
     }
 
 
 def main(ref_src: str) -> None:
     sys.path.insert(0, ref_src)
     from magnelio._operators.material_matrices import build_M_eps, build_M_mu
+    from magnelio.materials.material import Material
     from magnelio.mesh.grid import GridLines
     from magnelio.mesh.mesher import Mesh
     from magnelio.ports._modal import BoxFace, PortSpecNumerical, build_modal_port
     from magnelio.solver.stability import courant_dt
 
-    for name, (x, y, z) in cases().items():
+    only = sys.argv[2:]
+    for name, (x, y, z, corner, n_modes) in cases().items():
+        if only and name not in only:
+            continue
         grid = GridLines(x=x, y=y, z=z)
-        mesh = Mesh.from_grid(grid).with_boundary_conditions(
+        regions = [(Material.pec(), CORNER)] if corner else []
+        mesh = Mesh.from_grid(grid, regions=regions).with_boundary_conditions(
             {"xmin": "PEC", "xmax": "PEC", "ymin": "PEC", "ymax": "PEC", "zmin": "PMC", "zmax": "PMC"})
         dt = courant_dt(grid, "normal")
         m_eps, m_mu = build_M_eps(mesh), build_M_mu(mesh)
-        out = {"x": x, "y": y, "z": z, "dt": np.float64(dt), "f_calc": np.float64(25e9)}
+        out = {"x": x, "y": y, "z": z, "dt": np.float64(dt), "f_calc": np.float64(25e9),
+               "n_modes": np.int64(n_modes), "corner": np.bool_(corner)}
         for label, face in (("port1", BoxFace.Z_MIN), ("port2", BoxFace.Z_MAX)):
-            op = build_modal_port(PortSpecNumerical(name=label, plane=face, n_modes=3), mesh, m_eps,
-                                  m_mu, dt=dt, f_calc=25e9)
+            op = build_modal_port(PortSpecNumerical(name=label, plane=face, n_modes=n_modes), mesh,
+                                  m_eps, m_mu, dt=dt, f_calc=25e9)
             M = len(op.discrete_modes)
             lp = op.dtbc_line_params
             p = f"{label}_"
@@ -74,6 +90,7 @@ def main(ref_src: str) -> None:
             out[p + "source_scale"] = np.asarray(op._source_scale, float)
             out[p + "state_scale"] = np.asarray(op.state_scale, float)
             out[p + "omega_c"] = np.array([dm.mode.omega_c for dm in op.discrete_modes])
+            out[p + "types"] = np.array([dm.mode.mode_type.value for dm in op.discrete_modes])
             out[p + "dtbc"] = np.array([[lp[m][0], lp[m][1], np.nan if lp[m][2] is None else lp[m][2]]
                                         if m in lp else [np.nan] * 3 for m in range(M)])
             out[p + "kinds"] = np.array(op.termination_kinds)
@@ -109,4 +126,5 @@ def main(ref_src: str) -> None:
 
 
 if __name__ == "__main__":
+    # python make_port_fixtures.py [reference src] [case ...]
     main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src")

@@ -3,7 +3,7 @@
 Stand-ins carry the reference's attribute names and, for the modal port,
 the values a real ``PortOperatorModal`` held (tests/golden/ports_*.npz,
 dumped from the reference's build_modal_port on a uniform and a graded
-WR-90 cross-section).  The adapters must rebuild exactly what this
+WR-90 cross-section and on an L-shaped section with a TM channel).  The adapters must rebuild exactly what this
 backend's own builders make from the same mesh, and the solver fields must
 carry over; ``resolve_backend`` serves "hip" only and never falls back.
 """
@@ -35,7 +35,7 @@ def _operator_standin(f, label):
                                               "e_u_indices_interior", "e_v_indices_interior",
                                               "u_node_window", "v_node_window")})
     modes = []
-    for m in range(3):
+    for m in range(int(f.get("n_modes", 3))):
         mode = NS(name=str(g[f"mode{m}_name"]), mode_type=NS(value=str(g[f"mode{m}_type"])),
                   omega_c=float(g[f"mode{m}_omega_c"]), epsilon_r=float(g[f"mode{m}_epsilon_r"]))
         modes.append(NS(mode=mode, e_u_profile=f[f"{label}_e_u{m}"], e_v_profile=f[f"{label}_e_v{m}"],
@@ -49,16 +49,20 @@ def _operator_standin(f, label):
                                    "_dtbc_pair_spread")})
 
 
-@pytest.mark.parametrize("name", ["wr90", "graded"])
+@pytest.mark.parametrize("name", ["wr90", "graded", "corner"])
 def test_port_from_operator_equals_own_build(name):
     f = dict(np.load(GOLDEN / f"ports_{name}.npz"))
     g = mm.GridLines(f["x"], f["y"], f["z"])
-    mesh = mm.Mesh.from_grid(g, pec_faces=("xmin", "xmax", "ymin", "ymax"))
+    # corner: an L-shaped section whose fourth channel is a TM mode
+    corner = (0.0, 0.0, 0.0, 0.25 * 22.86e-3, 0.25 * 10.16e-3, 20e-3)
+    regions = [(mm.Material.pec(), corner)] if bool(f.get("corner", False)) else []
+    mesh = mm.Mesh.from_grid(g, regions=regions, pec_faces=("xmin", "xmax", "ymin", "ymax"))
     dt = mm.courant_dt(g, "normal")
     me, mu = mm.build_m_eps(mesh), mm.build_m_mu(mesh)
     for label, face in (("port1", "zmin"), ("port2", "zmax")):
-        own = pp.build_port(label, face, mesh, me, mu, dt=dt, f_calc=25e9, n_modes=3)
+        own = pp.build_port(label, face, mesh, me, mu, dt=dt, f_calc=25e9, n_modes=int(f.get("n_modes", 3)))
         got = adapters.port_from_operator(_operator_standin(f, label), f_calc=25e9)
+        assert [m.mode_type for m in got.modes] == [m.mode_type for m in own.modes]
         assert got.face == own.face == face
         for k in ("e_u_indices", "e_v_indices", "h_u_indices", "h_v_indices", "e_u_indices_interior",
                   "e_v_indices_interior"):

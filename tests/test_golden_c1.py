@@ -1,9 +1,10 @@
 """Whole-run golden vectors of the reference's own solver (CPU side).
 
-``tests/golden/c1_{guide,loaded}.npz`` hold 300 steps of the reference's
-NumPy tier on reduced C1 cases (tests/golden/make_c1_fixtures.py): the
-uniform WR-90 guide, and a graded cross-section with a lossy dielectric
-block and a PEC post.  They pin the oracle's whole composition — graded dual
+``tests/golden/c1_{guide,loaded,mur,tm}.npz`` hold 300 steps of the
+reference's NumPy tier on reduced C1 cases (tests/golden/make_c1_fixtures.py):
+the uniform WR-90 guide, a graded cross-section with a lossy dielectric
+block and a PEC post, a dielectric block that puts port 1 on Mur, and an
+L-shaped section whose 4-mode ports carry a TM channel (the one excited).  They pin the oracle's whole composition — graded dual
 masses, material averaging, the PEC edge OR rule, port-plane flattening,
 the curl order, the modal DTBC port step and calibration, the recorder and
 the S-parameter post — and the product's host coefficient build.
@@ -25,7 +26,7 @@ import cases
 from magnelio_amd import mesh as pm
 from oracle import sparams as osp
 
-CASES = ("c1_guide", "c1_loaded", "c1_mur")
+CASES = ("c1_guide", "c1_loaded", "c1_mur", "c1_tm")
 
 
 def _product_coefficients(name):
@@ -60,12 +61,12 @@ def test_oracle_coefficients_equal_reference(name):
         assert np.array_equal(v, f["double_" + k]), k
 
 
-def _sparams(ports, dt, f_axis):
+def _sparams(ports, dt, f_axis, excited=("port1", 0)):
     sig = {(p.name, m): (np.array(p.V[m]), np.array(p.I[m])) for p in ports for m in range(p.n_modes)}
     lines = {(p.name, m): p.line[m] for p in ports for m in range(p.n_modes) if m in p.line}
-    modes = {(p.name, m): (md["wc"], md["eps_r"]) for p in ports for m, md in enumerate(p.modes)}
+    modes = {(p.name, m): (md["wc"], md["eps_r"], md["kind"]) for p in ports for m, md in enumerate(p.modes)}
     ndx = {p.name: p.pl["normal_dx"] for p in ports}
-    return osp.s_params(sig, lines, ("port1", 0), f_axis, dt, modes=modes, normal_dx=ndx)
+    return osp.s_params(sig, lines, excited, f_axis, dt, modes=modes, normal_dx=ndx)
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -83,10 +84,10 @@ def test_oracle_run_matches_reference(name):
     vmax = max(np.abs(f[k]).max() for k in f if k.startswith("double_V_"))
     imax = max(np.abs(f[k]).max() for k in f if k.startswith("double_I_"))
     for p in ports:
-        for m in range(3):
+        for m in range(p.n_modes):
             np.testing.assert_allclose(p.V[m], f[f"double_V_{p.name}_{m}"], rtol=0, atol=1e-10 * vmax)
             np.testing.assert_allclose(p.I[m], f[f"double_I_{p.name}_{m}"], rtol=0, atol=1e-10 * imax)
-    S = _sparams(ports, dt, f["f_axis"])
+    S = _sparams(ports, dt, f["f_axis"], ("port1", cases.golden_excite(name)))
     for (lbl, m), s in S.items():
         want = f[f"double_S_{lbl}_{m}"]
         ok = np.isfinite(want)
@@ -141,7 +142,7 @@ def test_oracle_checkpoint_complement_state_equals_reference():
         assert np.max(np.abs(got - want)) <= 1e-12 * np.abs(want).max(), k
 
 
-@pytest.mark.parametrize("name", ["c1_guide", "c1_mur"])
+@pytest.mark.parametrize("name", ["c1_guide", "c1_mur", "c1_tm"])
 def test_oracle_float32_matches_reference_float32(name):
     f = cases.golden_c1(name)
     orc, _, _, _ = cases.golden_c1_oracle(name, np.float32)

@@ -1,16 +1,19 @@
 """Modal-port calibration against the reference's own values (golden vectors).
 
-``tests/golden/ports_{wr90,graded}.npz`` were written by
+``tests/golden/ports_{wr90,graded,corner}.npz`` were written by
 ``tests/golden/make_port_fixtures.py`` from the reference's
 ``build_modal_port`` (src/magnelio/ports/_modal/factory.py:1367) on a WR-90
 guide with a uniform and with a seeded graded cross-section, 3 numerical
-modes per port on z_min / z_max.  They pin what no closed form fixes: the
+modes per port on z_min / z_max, and on an L-shaped section (a PEC block in
+one corner) whose 4-mode ports carry the lowest TM mode as channel 3.  They pin what no closed form fixes: the
 V/I calibration ``record_scale`` / ``source_scale`` / ``state_scale``
 (operator.py:778-941), the DTBC chain (r, q, z0) (operator.py:671-776), the
 cut-offs and the M_eps-normalised mode profiles (ARPACK start-vector noise
 ~1e-13 of the profile scale).
 
-* the product's port build (``magnelio_amd.ports.build_port``) on both;
+* the product's port build (``magnelio_amd.ports.build_port``) on all
+  three: mode families, cut-offs, E and calibrated H profiles, scales and
+  chain parameters;
 * the oracle's closed-form TE modes and calibration (``oracle.ports``) on
   the uniform guide, where its closed forms apply.
 """
@@ -34,30 +37,45 @@ def _load(name):
     return dict(np.load(GOLDEN / f"ports_{name}.npz"))
 
 
-@pytest.mark.parametrize("name", ["wr90", "graded"])
+def _fixture_module():
+    import sys  # noqa: PLC0415
+
+    if str(GOLDEN) not in sys.path:
+        sys.path.insert(0, str(GOLDEN))
+    import make_port_fixtures as mk  # noqa: PLC0415
+
+    return mk
+
+
+@pytest.mark.parametrize("name", ["wr90", "graded", "corner"])
 def test_product_port_build_matches_reference(name):
     f = _load(name)
     g = mm.GridLines(f["x"], f["y"], f["z"])
-    mesh = mm.Mesh.from_grid(g, pec_faces=("xmin", "xmax", "ymin", "ymax"))
+    regions = [(mm.Material.pec(), _fixture_module().CORNER)] if bool(f.get("corner", False)) else []
+    mesh = mm.Mesh.from_grid(g, regions=regions, pec_faces=("xmin", "xmax", "ymin", "ymax"))
+    n_modes = int(f.get("n_modes", 3))
     dt = mm.courant_dt(g, "normal")
     assert dt == float(f["dt"])
     me, mu = mm.build_m_eps(mesh), mm.build_m_mu(mesh)
     for label, face in PORTS:
-        P = pp.build_port(label, face, mesh, me, mu, dt=dt, f_calc=float(f["f_calc"]), n_modes=3)
+        P = pp.build_port(label, face, mesh, me, mu, dt=dt, f_calc=float(f["f_calc"]), n_modes=n_modes)
         p = label + "_"
         assert P.termination_kinds == [str(k) for k in f[p + "kinds"]]
+        if p + "types" in f:
+            assert [md.mode_type for md in P.modes] == [str(t) for t in f[p + "types"]]
         for key in ("record_scale", "source_scale", "state_scale"):
             np.testing.assert_allclose(getattr(P, key), f[p + key], rtol=1e-12, atol=0, err_msg=key)
         np.testing.assert_allclose([md.omega_c for md in P.modes], f[p + "omega_c"], rtol=1e-12)
-        line = np.array([[P.dtbc_r[m], P.dtbc_q[m], P.dtbc_z0[m]] for m in range(3)])
+        line = np.array([[P.dtbc_r[m], P.dtbc_q[m], P.dtbc_z0[m]] for m in range(n_modes)])
         np.testing.assert_allclose(line, f[p + "dtbc"], rtol=1e-12, atol=0)
         assert np.array_equal(P.plane.e_u_indices, f[p + "e_u_indices"])
         assert np.array_equal(P.plane.e_v_indices, f[p + "e_v_indices"])
-        for m in range(3):
-            scale = max(np.abs(f[f"{p}e_u{m}"]).max(), np.abs(f[f"{p}e_v{m}"]).max())
-            for comp in ("e_u", "e_v"):
-                np.testing.assert_allclose(getattr(P.modes[m], comp), f[f"{p}{comp}{m}"], rtol=0,
-                                           atol=1e-12 * scale, err_msg=f"{label} mode {m} {comp}")
+        for m in range(n_modes):
+            for fam in ("e", "h"):
+                scale = max(np.abs(f[f"{p}{fam}_u{m}"]).max(), np.abs(f[f"{p}{fam}_v{m}"]).max())
+                for comp in (fam + "_u", fam + "_v"):
+                    np.testing.assert_allclose(getattr(P.modes[m], comp), f[f"{p}{comp}{m}"], rtol=0,
+                                               atol=1e-12 * scale, err_msg=f"{label} mode {m} {comp}")
 
 
 # (m, n) of the three lowest TE cut-offs of WR-90: TE10, TE20, TE01

@@ -10,7 +10,10 @@ dielectric block and a PEC post (tests/golden/make_c1_fixtures.py).
   run (the port reductions associate differently in float64), S within
   1e-4 of the reference's float64 S (the single-vs-double gate);
 * c1_mur (tests/golden/c1_mur.npz): port 1 on modal Mur with the complement
-  absorber live, port 2 on the DTBC — the same gates in both precisions.
+  absorber live, port 2 on the DTBC — the same gates in both precisions;
+* c1_tm (tests/golden/c1_tm.npz): an L-shaped section whose 4-mode ports
+  carry the lowest TM mode as channel 3, that TM channel of port 1 excited
+  — the same gates in both precisions.
 """
 
 import numpy as np
@@ -22,17 +25,18 @@ from magnelio_amd.signals import s_parameters
 pytestmark = pytest.mark.gpu
 
 
-def _hip_sparams(sol, ports, dt, f_axis):
+def _hip_sparams(sol, ports, dt, f_axis, excited=("port1", 0)):
     sig = sol.recorded(0)
     modes = {p.name: p.modes for p in ports}
     lines = {(p.name, m): lp for p in ports for m, lp in p.dtbc_line_params.items()}
     ndx = {p.name: p.plane.normal_dx for p in ports}
-    return s_parameters(sig, modes, ("port1", 0), f_axis, dt, port_normal_dx=ndx, port_line_params=lines)
+    return s_parameters(sig, modes, excited, f_axis, dt, port_normal_dx=ndx, port_line_params=lines)
 
 
 @pytest.mark.parametrize("name,precision", [("c1_guide", "double"), ("c1_loaded", "double"),
                                             ("c1_guide", "single"), ("c1_mur", "double"),
-                                            ("c1_mur", "single")])
+                                            ("c1_mur", "single"), ("c1_tm", "double"),
+                                            ("c1_tm", "single")])
 def test_hip_run_matches_reference(name, precision, gpu):
     print(f"HIP golden {name} {precision}", flush=True)
     f = cases.golden_c1(name)
@@ -60,7 +64,7 @@ def test_hip_run_matches_reference(name, precision, gpu):
         for (lbl, m), (V, I) in sig.items():
             assert np.max(np.abs(V - f[f"double_V_{lbl}_{m}"])) <= 1e-9 * vmax
             assert np.max(np.abs(I - f[f"double_I_{lbl}_{m}"])) <= 1e-9 * imax
-    S = _hip_sparams(sol, ports, dt, f["f_axis"])
+    S = _hip_sparams(sol, ports, dt, f["f_axis"], ("port1", cases.golden_excite(name)))
     sgate = 1e-9 if precision == "double" else 1e-4
     dmax = 0.0
     for (lbl, m), s in S.items():

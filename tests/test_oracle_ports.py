@@ -143,3 +143,92 @@ def test_product_port_setup_matches_oracle():
             assert p.dtbc_r[m] == pytest.approx(r, rel=1e-12)
             assert p.dtbc_q[m] == pytest.approx(q, rel=1e-9)
             assert p.dtbc_z0[m] == pytest.approx(z0, rel=1e-9)
+
+
+def _node_gradient_on_plane(phi, dims, kz):
+    """[Ex | Ey] of the plane-node gradient of phi at node layer kz, as a
+    flat E vector (the 3-D gradient of a potential living on one plane)."""
+    g = np.zeros(yee.n_e(*dims))
+    Ex, Ey, _ = yee.split(g, *dims, "E")
+    Ex[:, :, kz] = phi[1:, :] - phi[:-1, :]
+    Ey[:, :, kz] = phi[:, 1:] - phi[:, :-1]
+    return g
+
+
+@pytest.mark.parametrize("face", ["zmin", "zmax"])
+@pytest.mark.parametrize("mn", [(1, 1), (2, 1), (1, 2), (3, 2)])
+def test_closed_form_tm_modes_are_normal_e_pencil_eigenpairs(face, mn):
+    """K_z phi = w_c^2 M_z phi with K_z applied through the oracle's own 3-D
+    curls (normal edges of the first cell -> their transversal H faces ->
+    back), the profile is the plane gradient of phi, curl-free on the plane,
+    M_eps-normalised and M_eps-orthogonal to the TE modes."""
+    (x, y, z), me, mm_ = _c1_masses()
+    dims = nx, ny, nz = (12, 6, 10)
+    pl = op.plane(face, x, y, z)
+    m, n = mn
+    e_u, e_v, wc = op.rect_tm_mode(pl, x, y, z, m, n, me)
+    phi = np.outer(np.sin(m * math.pi * np.arange(nx + 1) / nx), np.sin(n * math.pi * np.arange(ny + 1) / ny))
+    k_e = 0 if face == "zmin" else nz - 1  # normal-edge layer next to the plane
+    e = np.zeros(yee.n_e(*dims))
+    yee.split(e, *dims, "E")[2][:, :, k_e] = phi
+    h = np.zeros(yee.n_h(*dims))
+    yee.update_h(e, h, np.zeros_like(h), -np.ones_like(h), *dims)
+    keep = np.zeros_like(h, dtype=bool)
+    kx, ky, _ = yee.split(keep, *dims, "H")
+    kx[:, :, k_e] = True
+    ky[:, :, k_e] = True
+    h = np.where(keep, h / np.where(mm_ > 0, mm_, 1.0), 0.0)
+    Ke = np.zeros_like(e)
+    yee.update_e(Ke, h, np.zeros_like(e), np.ones_like(e), *dims)
+    lhs = yee.split(Ke, *dims, "E")[2][1:-1, 1:-1, k_e]
+    rhs = wc**2 * (yee.split(me, *dims, "E")[2][:, :, k_e] * phi)[1:-1, 1:-1]
+    assert np.max(np.abs(lhs - rhs)) <= 1e-10 * np.max(np.abs(rhs))
+    # the transverse profile is the plane gradient of phi, normalised and signed
+    kz = 0 if face == "zmin" else nz
+    g = _node_gradient_on_plane(phi, dims, kz)
+    full = np.concatenate([g[pl["e_u"]], g[pl["e_v"]]])
+    mfull = np.concatenate([me[pl["e_u"]], me[pl["e_v"]]])
+    full = op.resolve_sign(pl, full / math.sqrt(float(np.sum(mfull * full * full))))
+    sc = float(np.abs(full).max())
+    np.testing.assert_allclose(np.concatenate([e_u, e_v]), full, rtol=0, atol=1e-13 * sc)
+    # curl-free: no normal H on the plane faces
+    e_t = np.zeros_like(e)
+    e_t[pl["e_u"]], e_t[pl["e_v"]] = e_u, e_v
+    hn = np.zeros(yee.n_h(*dims))
+    yee.update_h(e_t, hn, np.zeros_like(hn), -np.ones_like(hn), *dims)
+    assert np.max(np.abs(yee.split(hn, *dims, "H")[2][:, :, kz])) <= 1e-13 * sc
+    # M_eps-orthogonal to every TE mode of the plane
+    for te in [(1, 0), (2, 0), (0, 1), (1, 1), (2, 1)]:
+        t_u, t_v, _ = op.rect_te_mode(pl, x, y, z, *te, me)
+        ov = float(np.sum(me[pl["e_u"]] * t_u * e_u) + np.sum(me[pl["e_v"]] * t_v * e_v))
+        assert abs(ov) <= 1e-12, te
+
+
+def test_product_tm_modes_match_closed_form():
+    """The product's TM eigen-solver on the uniform guide: the three lowest
+    TM modes equal the closed-form TM_11, TM_21, TM_31 (cut-off 1e-10,
+    profiles 1e-9 of their scale); the merged port keeps TE below TM."""
+    from magnelio_amd import mesh as mm
+    from magnelio_amd import ports as pp
+
+    dims, size = (12, 6, 10), (22.86e-3, 10.16e-3, 10e-3)
+    x, y, z = cases.c1_axes(dims, size)
+    mesh = mm.Mesh.from_grid(mm.GridLines(x, y, z), pec_faces=("xmin", "xmax", "ymin", "ymax"))
+    (_, _, _), me_o, _ = _c1_masses()
+    for face in ("zmin", "zmax"):
+        me = mm.flatten_port_mass(mm.build_m_eps(mesh), dims, face)
+        mu = mm.flatten_port_mu(mm.build_m_mu(mesh), dims, face)
+        pec = mm.pec_flat(mm.flatten_port_pec(mesh.pec_mask_edges, dims, face), dims)
+        plane = pp.PortPlane.from_mesh(face, mesh.grid)
+        got = pp.solve_tm_modes(plane, mesh.grid, me, mu, pec, 3, cases.C1_FMAX)
+        pl = op.plane(face, x, y, z)
+        for md, mn in zip(got, [(1, 1), (2, 1), (3, 1)]):
+            e_u, e_v, wc = op.rect_tm_mode(pl, x, y, z, *mn, me_o)
+            assert md.mode_type == "TM"
+            assert md.omega_c == pytest.approx(wc, rel=1e-10)
+            sc = max(np.abs(e_u).max(), np.abs(e_v).max())
+            np.testing.assert_allclose(md.e_u, e_u, rtol=0, atol=1e-9 * sc)
+            np.testing.assert_allclose(md.e_v, e_v, rtol=0, atol=1e-9 * sc)
+        merged = pp.solve_modes(plane, mesh.grid, me, mu, pec, 6, cases.C1_FMAX)
+        wcs = [md.omega_c for md in merged]
+        assert wcs == sorted(wcs) and "TM" in [md.mode_type for md in merged]
