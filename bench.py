@@ -344,6 +344,109 @@ def main_c4(args) -> int:
 COPY_BYTES = 1 << 30
 
 
+FP64_PEAK_TFLOPS = 78.6
+# per (direction, patch) of a grid face: the four tangential currents' complex
+# multiply-adds against the column phase (the row and plane phases are per
+# row and per face)
+NTFF_FLOPS_PER_PAIR = 32
+
+
+def main_ntff(args) -> int:
+    """Far-field post of the C2 Huygens box: one pattern (all default
+    directions, every patch) per step, on device-resident inputs — the
+    grid-face kernel the monitor path uses, the per-patch kernel beside it."""
+    import torch
+
+    from magnelio_amd import farfield as pf
+    from magnelio_amd import mesh as mm
+    from magnelio_amd.boundaries import CPMLBoundary
+    from magnelio_amd.monitors import MonitorFarField
+
+    mesh = c2_mesh(args.n, length=64e-3)
+    bcs = {f: CPMLBoundary(f, mesh.grid, thickness_cells=8) for f in mm.FACES}
+    mon = MonitorFarField([10e9], margin_cells=2)
+    mon._build(mesh.grid, bcs)
+    rng = np.random.default_rng(11)
+    faces = []
+    for face, axis, _, _, _ in mon.faces:
+        g = mon._geometry[face]
+        shp = (1, g["c1"].size, g["c2"].size)
+        faces.append(dict(name=face, axis=axis, sign=g["sign"], plane=g["plane"], c1=g["c1"], c2=g["c2"],
+                          w1=g["w1"], w2=g["w2"],
+                          bins={c: rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+                                for c in pf.TANGENTIALS[axis]}))
+    dump = dict(freqs=np.array([10e9]), faces=faces, image_planes=[], source_spectrum=np.array([1.0 + 0j]))
+    sets = pf.patch_sets_from_dump(dump, 0)
+    theta, phi = pf.default_angular_grid()
+    frames = pf.direction_frames(theta, phi)
+    n_d = frames.shape[0]
+    tens = pf.DeviceTransform(frames, faces=[pf.tensor_face(s) for s in sets])
+    gen = pf.DeviceTransform(frames, patches=pf.patch_array(pf._concat(sets)))
+    n_p = gen.np_
+    st = torch.cuda.current_stream()
+
+    def timed(t, steps, warm):
+        for _ in range(warm):
+            t.launch(10e9, st)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record(st)
+            t.launch(10e9, st)
+            b.record(st)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    steps, warm = max(1, min(args.steps, 100)), max(1, min(args.warmup, 5))
+    wall, ms = timed(tens, steps, warm)
+    _, ms_gen = timed(gen, max(1, steps // 10), 1)
+    a, b = tens.result()
+    gen.launch(10e9, st)
+    c, d = gen.result()
+    agree = float(max(np.abs(a - c).max(), np.abs(b - d).max()) / max(np.abs(c).max(), np.abs(d).max()))
+    flops = NTFF_FLOPS_PER_PAIR * float(n_p) * n_d
+    achieved = flops / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "far-field patterns/s",
+        "value": steps / wall,
+        "unit": "patterns/s",
+        "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"near-to-far-field transform of the C2 Huygens box ({args.n}^3 grid, "
+                                f"8-layer CPML + 2 margin cells): {len(sets)} grid faces, {n_p} surface "
+                                f"patches with seeded bins, {n_d} directions (2-degree theta x phi grid), "
+                                "one frequency per step, device-resident inputs"),
+                   "patches": n_p, "directions": n_d, "face_jobs": tens.n_jobs,
+                   "per_patch_kernel_ms": ms_gen, "per_patch_vs_face_max_rel": agree},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_ntff_face_tile",
+                     "kernel_ms": ms, "alg_flops_per_launch": flops,
+                     "flops_note": "32 per (direction, patch): four complex multiply-adds of the tangential "
+                                   "currents; phases (one sincos per direction and row or column) not counted"},
+    }
+    if not args.no_cpu:
+        from oracle import farfield as off
+
+        surf = pf._concat(sets)
+        sub = max(1, int(round(n_d / 64)))
+        idx = np.arange(0, n_d, sub)
+        host = dict(centers=surf.centers, normals=surf.normals, areas=surf.areas, E=surf.E, H=surf.H)
+        th_s = np.arccos(np.clip(frames[idx, 2], -1, 1))
+        t1 = time.perf_counter()
+        off.ntff([host], [], 10e9, th_s, np.array([0.0]))
+        t_cpu = time.perf_counter() - t1
+        per_pattern = t_cpu * n_d / idx.size
+        out["cpu_baseline"] = {"value": 1.0 / per_pattern, "unit": "patterns/s", "cores": _host_threads(),
+                               "kind": "port",
+                               "sample": (f"oracle NumPy transform (the reference's algorithm) over all "
+                                          f"{n_p} patches for {idx.size} of the {n_d} directions "
+                                          f"({t_cpu:.1f} s), scaled to the full grid")}
+    print(json.dumps(out))
+    return 0
+
+
 def pmc_probe(args) -> int:
     """Launches whose counters calibrate and measure HBM traffic: a device
     copy of COPY_BYTES (1 GiB read + 1 GiB written) and `--steps` E/H curl
@@ -583,9 +686,10 @@ def main(argv=None) -> int:
     ap.add_argument("--pmc-probe", action="store_true",
                     help="a short stencil-only run of the c2 grid plus a 1 GiB device copy, "
                          "for HBM traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "ntff"), default="c2",
                     help="c2: single large job (default line); c3: batched K=6 sweep vs one "
-                         "excitation per launch; c5: serving loop vs one job at a time")
+                         "excitation per launch; c4: FP32 vs FP64 study; c5: serving loop vs one "
+                         "job at a time; ntff: far-field post of the C2 Huygens box")
     ap.add_argument("--jobs", type=int, default=200, help="c5: number of sweep jobs")
     ap.add_argument("--rate", type=float, default=1.5, help="c5: Poisson arrival rate [jobs/s]")
     ap.add_argument("--step-scale", type=float, default=0.05,
@@ -604,6 +708,8 @@ def main(argv=None) -> int:
         return main_c5(args)
     if args.workload == "c4":
         return main_c4(args)
+    if args.workload == "ntff":
+        return main_ntff(args)
 
     import torch
 

@@ -280,6 +280,36 @@ int mgh_ade_update(int dtype, int nsets, void* e, long long e_set_stride, int n,
                    const int* blk, const int* npoles, const void* tab, int pmax, void* J,
                    void* stream);
 
+/* ---- Far field: near-to-far-field transform of a Huygens patch set --------
+ * Replaces the radiation-vector sums and projections of ntff_transform
+ * (src/magnelio/post/far_field.py:139-271); the host builds the patch set
+ * (the monitor's faces divided by the source spectrum, doubled by each image
+ * plane, monitors/far_field.py:325-355, post/far_field.py:100-122) and the
+ * direction grid.  patches: n_patches x 15 doubles (centre xyz, area (n x H)
+ * xyz and -area (n x E) xyz as (re, im) pairs, library e^{-jwt} phasors);
+ * dirs: n_dirs x 9 doubles (r, theta, phi unit vectors); FP64 throughout:
+ *   N = sum_p e^{-jk r.c_p} wJ_p,  L = sum_p e^{-jk r.c_p} wM_p
+ *   E_theta = pre (eta N.theta + L.phi),  E_phi = pre (eta N.phi - L.theta)
+ * with pre = +jk/(4 pi) — the reference's conjugate-in / conjugate-out
+ * amplitudes.  The patches split into n_chunks ranges whose partial sums
+ * (work: n_chunks x 12 x n_dirs doubles) are added in chunk order.
+ * out: n_dirs x 4 doubles (E_theta re, im, E_phi re, im). */
+int mgh_ntff(int n_patches, const double* patches, int n_dirs, const double* dirs, double k,
+             double pre_re, double pre_im, double eta, int n_chunks, double* work, double* out,
+             void* stream);
+/* The same transform for tensor faces — a Huygens-box face or its image: the
+ * grid c1 x c2 on the plane x_axis = planes[f] (face_desc: n_faces x 6 int64:
+ * axis, n1, n2, offsets of c1 and c2 into coords, offset of W), whose phase
+ * factors into e^{-jk r_a p} e^{-jk r_1 c1_i} e^{-jk r_2 c2_j} (t1 < t2 the
+ * other two axes).  W: per face n1 x n2 x 4 (re, im) pairs: area (n x H)_t1,
+ * area (n x H)_t2, -area (n x E)_t1, -area (n x E)_t2.  jobs: n_jobs x 3 int32
+ * (face, first column, columns <= 16), each column of each face in one job;
+ * work: n_jobs x 8 x n_dirs doubles, added in job order; out as above. */
+int mgh_ntff_faces(int n_faces, const long long* face_desc, const double* planes,
+                   const double* coords, const double* W, int n_jobs, const int* jobs, int n_dirs,
+                   const double* dirs, double k, double pre_re, double pre_im, double eta,
+                   double* work, double* out, void* stream);
+
 /* ---- SIBC walls: lossy walls in the time-domain H update ------------------
  * Replaces SIBCOperator.update_field (src/magnelio/solver/_sibc.py:213-249),
  * called right after the H curl and the mu(w) completion, before the CPML H

@@ -133,6 +133,16 @@ SIGNATURES: dict[str, tuple] = {
          _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int,
          _c_void_p, _c_void_p],
     ),
+    "mgh_ntff": (
+        _c_int,
+        [_c_int, _c_void_p, _c_int, _c_void_p, _c_double, _c_double, _c_double, _c_double, _c_int,
+         _c_void_p, _c_void_p, _c_void_p],
+    ),
+    "mgh_ntff_faces": (
+        _c_int,
+        [_c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p,
+         _c_double, _c_double, _c_double, _c_double, _c_void_p, _c_void_p, _c_void_p],
+    ),
     "mgh_tfsf_apply": (
         _c_int,
         [_c_int, _c_int, _c_void_p, _c_ll, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p,

@@ -33,6 +33,7 @@ import numpy as np
 import torch
 
 from . import _lib, layout
+from .farfield import far_field_from_dump, source_spectrum, wall_position
 
 NT = 8
 BMAX = 512  # longest block (steps per MFMA contraction)
@@ -744,6 +745,8 @@ class MonitorFarField(DFTMonitorBase):
         if margin_cells < 1:
             raise ValueError("margin_cells must be at least 1")
         self.margin_cells = margin_cells
+        self._source_spectrum = None
+        self._accepted_power = None
 
     def _build(self, grid, boundary_conditions) -> None:
         from .boundaries import CPMLBoundary  # noqa: PLC0415
@@ -777,7 +780,7 @@ class MonitorFarField(DFTMonitorBase):
                 axis = "xyz".index(face[0])
                 at_low = face.endswith("min")
                 self._image_planes.append({"axis": axis, "kind": str(kind).upper(), "at_low": at_low,
-                                           "position": float(nodes[axis][0 if at_low else -1]),
+                                           "position": wall_position(face, str(kind).upper(), nodes[axis]),
                                            "physical_halfspace": True})
         for face in open_faces:
             axis = "xyz".index(face[0])
@@ -828,9 +831,43 @@ class MonitorFarField(DFTMonitorBase):
             faces.append({"name": face, "axis": int(g["axis"]), "sign": g["sign"], "plane": g["plane"],
                           "c1": g["c1"], "c2": g["c2"], "w1": g["w1"], "w2": g["w2"],
                           "bins": per_face[face]})
-        return {"name": self.name, "freqs": np.asarray(self.freqs, float),
+        dump = {"name": self.name, "freqs": np.asarray(self.freqs, float),
                 "margin_cells": int(self.margin_cells), "faces": faces,
                 "image_planes": [dict(p) for p in self._image_planes]}
+        if self._source_spectrum is not None:
+            dump["source_spectrum"] = np.asarray(self._source_spectrum)
+        if self._accepted_power is not None:
+            dump["accepted_power"] = np.asarray(self._accepted_power)
+        return dump
+
+    def renormalize(self, source_signal, dt: float | None = None) -> None:
+        """Per-watt normalisation (monitors/far_field.py:250-267): store the
+        source's DFT in the bins' own convention as the divisor of the
+        far-field patch sets; the bins stay as recorded.  source_signal is
+        an object with ``values`` and ``dt`` or the sample array (with dt)."""
+        vals = getattr(source_signal, "values", source_signal)
+        step = getattr(source_signal, "dt", dt)
+        if step is None:
+            raise ValueError("renormalize needs the source's time step")
+        self._source_spectrum = source_spectrum(vals, float(step), self.freqs)
+
+    @property
+    def is_renormalized(self) -> bool:
+        return self._source_spectrum is not None
+
+    def set_accepted_power(self, f_axis, accepted) -> None:
+        """Accepted input power of the run on the monitor frequencies (for gain)."""
+        self._accepted_power = np.interp(self.freqs, np.asarray(f_axis, float), np.asarray(accepted, float))
+
+    def far_field(self, f: float | None = None, *, f_index: int | None = None, theta=None, phi=None,
+                  s: int = 0):
+        """Far-field pattern of field set s at one recorded frequency, the
+        transform summed on the device (the reference monitor's ``result``,
+        monitors/far_field.py:357-391)."""
+        solver = self._solver()
+        dev = solver.e.device if solver is not None else None
+        return far_field_from_dump(self.result_dump(s), f, f_index=f_index, theta=theta, phi=phi,
+                                   device=dev)
 
     def load_result_dump(self, dump: dict, s: int = 0) -> None:
         """Field set s's bins from the reference's dump (resume)."""
