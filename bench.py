@@ -119,43 +119,84 @@ def c3_case(dims=(192, 96, 288), seed=3, size=C3_SIZE, n_modes=3, f_max=C3_FMAX)
 
 
 C5_SIZE = (22.86e-3, 10.16e-3, 45.72e-3)  # WR-90 cross-section, 2a long
+# 3-4 ports: a cross of two WR-90 guides, the main one along z over
+# x in [a/2, 3a/2], the arm along x over z in [a/2, 3a/2]; PEC blocks fill the
+# four corners (with 3 ports the x_max end of the arm is a PEC wall)
+C5_CROSS = (2 * 22.86e-3, 10.16e-3, 2 * 22.86e-3)
+C5_FACES = ("zmin", "zmax", "xmin", "xmax")
+
+
+def _c5_size(n_ports):
+    return C5_SIZE if n_ports <= 2 else C5_CROSS
 
 
 def _c5_walls(n_ports):
-    return ("xmin", "xmax", "ymin", "ymax") + (("zmax",) if n_ports == 1 else ())
+    if n_ports <= 2:
+        return ("xmin", "xmax", "ymin", "ymax") + (("zmax",) if n_ports == 1 else ())
+    return ("ymin", "ymax") + (("xmax",) if n_ports == 3 else ())
+
+
+def _c5_corners():
+    """PEC corner blocks of the cross guide."""
+    a, b = C5_SIZE[0], C5_SIZE[1]
+    lo, hi, end = 0.5 * a, 1.5 * a, 2 * a
+    return [(x0, 0.0, z0, x1, b, z1) for x0, x1 in ((0.0, lo), (hi, end)) for z0, z1 in ((0.0, lo), (hi, end))]
+
+
+def _c5_dims(c, n_ports):
+    """Grid of a job of about c cells: uniform, 8-cell multiples across the guide."""
+    a, b = C5_SIZE[0], C5_SIZE[1]
+    if n_ports <= 2:
+        n = max(16, int(round((c / 0.889) ** (1 / 3) / 8.0)) * 8)
+        return (n, max(8, int(round(n * b / a))), 2 * n)
+    m = max(16, int(round((c / 1.778) ** (1 / 3) / 8.0)) * 8)
+    return (2 * m, max(8, int(round(m * b / a))), 2 * m)
+
+
+def _c5_keepout(n_ports):
+    """(lo, hi) corners of the region the dielectric objects are clipped to:
+    clear of every port's feed slab."""
+    if n_ports <= 2:
+        return np.array([0.0, 0.0, 0.2 * C5_SIZE[2]]), np.array([C5_SIZE[0], C5_SIZE[1], 0.8 * C5_SIZE[2]])
+    a = C5_SIZE[0]
+    return np.array([0.5 * a, 0.0, 0.5 * a]), np.array([1.5 * a, C5_SIZE[1], 1.5 * a])
 
 
 def _c5_mesh(dims, n_ports, objs):
     """Mesh and dt of one serving job (module level: a process pool runs it)."""
     from magnelio_amd import mesh as mm
 
-    axes = [np.linspace(0.0, L, k + 1) for L, k in zip(C5_SIZE, dims)]
+    axes = [np.linspace(0.0, L, k + 1) for L, k in zip(_c5_size(n_ports), dims)]
     grid = mm.GridLines(*axes)
-    boxes, shapes = [], []
-    zlo, zhi = 0.2 * C5_SIZE[2], 0.8 * C5_SIZE[2]  # keep the port feeds homogeneous
+    boxes = [(mm.Material.pec(), c) for c in _c5_corners()] if n_ports > 2 else []
+    shapes = []
+    klo, khi = _c5_keepout(n_ports)
     for eps, sphere, ctr, half in objs:
         mat = mm.Material.dielectric(eps)
         if sphere:
             r = float(half.min())
             shapes.append((mat, lambda X, Y, Z, c=ctr, r=r: ((X - c[0]) ** 2 + (Y - c[1]) ** 2
-                           + (Z - c[2]) ** 2 < r * r) & (Z > zlo) & (Z < zhi)))
+                           + (Z - c[2]) ** 2 < r * r) & (X > klo[0]) & (X < khi[0])
+                           & (Z > klo[2]) & (Z < khi[2])))
         else:
-            lo, hi = ctr - half, ctr + half
-            boxes.append((mat, (lo[0], lo[1], max(lo[2], zlo), hi[0], hi[1], min(hi[2], zhi))))
+            lo, hi = np.maximum(ctr - half, klo), np.minimum(ctr + half, khi)
+            boxes.append((mat, (lo[0], lo[1], lo[2], hi[0], hi[1], hi[2])))
     mesh = mm.Mesh.from_grid(grid, regions=boxes, shapes=shapes, pec_faces=_c5_walls(n_ports))
     return mesh, mm.courant_dt(grid, "normal")
 
 
 def _c5_port(dims, n_ports, name, face, n_modes, f_max):
-    """Modal port of a serving job.  The feed slab is air in every job, so the
-    model depends on (grid, walls, face, modes) only and is built on the empty
-    guide of that grid (identical to the build on the job's own mesh)."""
+    """Modal port of a serving job.  The feed slab holds no dielectric in any
+    job, so the model depends on (grid, walls, face, modes) only and is built
+    on the empty guide of that grid (the cross's PEC corners included),
+    identical to the build on the job's own mesh."""
     from magnelio_amd import mesh as mm
     from magnelio_amd import ports as pp
 
-    axes = [np.linspace(0.0, L, k + 1) for L, k in zip(C5_SIZE, dims)]
+    axes = [np.linspace(0.0, L, k + 1) for L, k in zip(_c5_size(n_ports), dims)]
     grid = mm.GridLines(*axes)
-    mesh = mm.Mesh.from_grid(grid, pec_faces=_c5_walls(n_ports))
+    regions = [(mm.Material.pec(), c) for c in _c5_corners()] if n_ports > 2 else []
+    mesh = mm.Mesh.from_grid(grid, regions=regions, pec_faces=_c5_walls(n_ports))
     return pp.build_port_local(name, face, mesh, dt=mm.courant_dt(grid, "normal"), f_calc=f_max,
                                n_modes=n_modes)
 
@@ -181,9 +222,10 @@ def _c5_assemble(parts):
 def c5_jobs(n_jobs=200, seed=5, rate=1.5, cells=(32 ** 3, 384 ** 3), steps=(2000, 20000),
             step_scale=1.0, f_max=25e9, n_freq=101):
     """Serving workload: Poisson arrivals (rate jobs/s), cell counts log-uniform
-    in `cells`, WR-90 guides with 1-4 seeded dielectric bricks/spheres
-    (eps_r U[1, 10]) clear of 1-2 modal ports (z faces) x 1-3 modes, steps
-    U{steps} x step_scale."""
+    in `cells`, 1-4 seeded dielectric bricks/spheres (eps_r U[1, 10]) clear of
+    1-4 modal ports x 1-3 modes — a WR-90 guide with ports on its z ends for
+    1-2 ports, a cross of two WR-90 guides with ports on z and x ends for 3-4
+    — steps U{steps} x step_scale."""
     import functools
 
     from magnelio_amd.serve import BuildPlan, SweepJob
@@ -194,18 +236,23 @@ def c5_jobs(n_jobs=200, seed=5, rate=1.5, cells=(32 ** 3, 384 ** 3), steps=(2000
     jobs = []
     for i in range(n_jobs):
         c = math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1])))
-        n = max(16, int(round((c / 0.889) ** (1 / 3) / 8.0)) * 8)
-        dims = (n, max(8, int(round(n * C5_SIZE[1] / C5_SIZE[0]))), 2 * n)
-        n_ports = int(rng.integers(1, 3))
+        n_ports = int(rng.integers(1, 5))
+        dims = _c5_dims(c, n_ports)
         n_modes = int(rng.integers(1, 4))
         n_steps = max(64, int(round(int(rng.integers(steps[0], steps[1] + 1)) * step_scale)))
+        size = np.array(_c5_size(n_ports))
         objs = []
         for _ in range(int(rng.integers(1, 5))):
-            ctr = rng.uniform([0.2, 0.2, 0.25], [0.8, 0.8, 0.75]) * np.array(C5_SIZE)
-            half = rng.uniform(0.05, 0.15, 3) * np.array(C5_SIZE)
+            if n_ports <= 2:
+                ctr = rng.uniform([0.2, 0.2, 0.25], [0.8, 0.8, 0.75]) * size
+                half = rng.uniform(0.05, 0.15, 3) * size
+            else:
+                ctr = rng.uniform([0.3, 0.2, 0.3], [0.7, 0.8, 0.7]) * size
+                half = rng.uniform(0.05, 0.15, 3) * np.array(C5_SIZE)
             objs.append((float(rng.uniform(1.0, 10.0)), bool(rng.random() < 0.5), ctr, half))
         tasks = {"mesh": (None, functools.partial(_c5_mesh, dims, n_ports, objs))}
-        for name, face in zip(("port1", "port2"), ("zmin", "zmax")[:n_ports]):
+        for k, face in enumerate(C5_FACES[:n_ports]):
+            name = f"port{k + 1}"
             tasks[name] = ((tuple(dims), n_ports, face, n_modes),
                            functools.partial(_c5_port, dims, n_ports, name, face, n_modes, f_max))
         build = BuildPlan(tasks, _c5_assemble)
@@ -695,6 +742,8 @@ def main(argv=None) -> int:
     ap.add_argument("--step-scale", type=float, default=0.05,
                     help="c5: fraction of the 2000-20000 steps each job runs")
     ap.add_argument("--max-active", type=int, default=8, help="c5: resident jobs at once")
+    ap.add_argument("--c5-mode", choices=("both", "continuous", "sequential"), default="both",
+                    help="c5: serving modes to run (both: continuous, then one job at a time)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine advances one job per device "
@@ -895,19 +944,24 @@ def main_c5(args) -> int:
     warm = c5_jobs(n_jobs=1, seed=99, cells=(24 ** 3, 24 ** 3), steps=(200, 200))
     SweepServer(build_pool=pool).run(warm)
     reports = {}
-    for mode in ("continuous", "sequential"):
+    modes = ("continuous", "sequential") if args.c5_mode == "both" else (args.c5_mode,)
+    for mode in modes:
         print(f"c5 {mode}: {args.jobs} jobs", file=sys.stderr, flush=True)
-        srv = SweepServer(max_active=args.max_active, build_pool=pool, progress=20)
+        srv = SweepServer(max_active=args.max_active, build_pool=pool, progress=max(1, args.jobs // 20))
         reports[mode] = srv.run(stream_of_jobs(), mode=mode)
         r = reports[mode]
         print(f"c5 {mode}: {r.jobs_per_s:.3f} jobs/s p50 {r.p50:.2f} s p99 {r.p99:.2f} s",
               file=sys.stderr, flush=True)
-    c, q = reports["continuous"], reports["sequential"]
-    same = all(np.array_equal(a.result.matrix, b.result.matrix, equal_nan=True)
-               for a, b in zip(c.records, q.records))
+    c = reports.get("continuous", reports.get("sequential"))
+    q = reports.get("sequential")
+    same = (all(np.array_equal(a.result.matrix, b.result.matrix, equal_nan=True)
+                for a, b in zip(reports["continuous"].records, q.records))
+            if len(reports) == 2 else None)
     recs = c.records
     out = {
-        "metric": "serving sweep jobs/s with p50/p99 job latency (continuous batching)",
+        "metric": ("serving sweep jobs/s with p50/p99 job latency (continuous batching)"
+                   if "continuous" in reports else "serving sweep jobs/s with p50/p99 job latency "
+                   "(one job at a time)"),
         "value": c.jobs_per_s,
         "unit": "jobs/s",
         "n_gpus": 1,
@@ -921,8 +975,10 @@ def main_c5(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": (f"{args.jobs} sweep jobs, Poisson arrivals {args.rate}/s (seed 5), cells "
-                         "log-uniform 32^3-384^3 (WR-90 guides, 1-4 dielectric bricks/spheres eps_r "
-                         "U[1,10]), 1-2 modal DTBC ports x 1-3 modes, steps U{2000..20000} x "
+                         "log-uniform 32^3-384^3 (a WR-90 guide with z-end ports for 1-2 ports, a "
+                         "cross of two WR-90 guides with z- and x-end ports for 3-4; 1-4 dielectric "
+                         "bricks/spheres eps_r U[1,10]), 1-4 modal DTBC ports x 1-3 modes, steps "
+                         "U{2000..20000} x "
                          f"{args.step_scale}; admission by HBM footprint, up to {args.max_active} "
                          "resident jobs on separate streams, graph-replayed 256-step chunks, mesh and "
                          f"port builds in {workers} host processes"),
@@ -934,12 +990,16 @@ def main_c5(args) -> int:
         "gcell_per_s": c.gcell_per_s,
         "peak_resident": c.peak_resident,
         "peak_footprint_gb": round(c.peak_bytes / 1e9, 2),
-        "one_job_at_a_time": {"jobs_per_s": q.jobs_per_s, "p50": q.p50, "p99": q.p99,
-                              "gcell_per_s": q.gcell_per_s, **q.breakdown(),
-                              # device time of the jobs run alone: n / this is the
-                              # throughput ceiling of one-at-a-time service
-                              "device_s": float(sum(r.finished - r.admitted for r in q.records))},
-        "results_identical": bool(same),
+        "one_job_at_a_time": ({"jobs_per_s": q.jobs_per_s, "p50": q.p50, "p99": q.p99,
+                               "gcell_per_s": q.gcell_per_s, **q.breakdown(),
+                               # device time of the jobs run alone: n / this is the
+                               # throughput ceiling of one-at-a-time service
+                               "device_s": float(sum(r.finished - r.admitted for r in q.records))}
+                              if q is not None else None),
+        "results_identical": same,
+        "ports_per_job": {str(k): int(v) for k, v in zip(*np.unique(
+            [len([t for t in j.build.tasks if t.startswith("port")]) for j in stream_of_jobs()],
+            return_counts=True))},
         "host_build_workers": workers,
     }
     pool.shutdown()

@@ -118,14 +118,17 @@ class CPMLBoundary:
         return self.sigma.copy()
 
     def psi_shapes(self, dims) -> tuple[list, list]:
-        """(E-term shapes, H-term shapes) of the psi slabs, (n0, n1, n2) order."""
+        """(E-term shapes, H-term shapes) of the psi slabs, (n0, n1, n2) order
+        (known before ``initialize``: the slab is min(thickness, cells) deep)."""
         nx, ny, nz = dims
         e_sh = [(nx, ny + 1, nz + 1), (nx + 1, ny, nz + 1), (nx + 1, ny + 1, nz)]
         h_sh = [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
 
+        n = self.n if self._initialized else min(self.thickness_cells, dims[self.axis])
+
         def cut(s):
             s = list(s)
-            s[self.axis] = self.n
+            s[self.axis] = n
             return tuple(s)
 
         return ([cut(e_sh[c]) for c in E_TERM_COMPS[self.axis]],

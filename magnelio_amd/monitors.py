@@ -371,6 +371,19 @@ class DFTMonitorBase:
             raise ValueError("field vectors too large for 32-bit sample indices")
         return dev_idx, w
 
+    def footprint(self, grid, boundary_conditions, K: int) -> int:
+        """Device bytes ``attach`` allocates for K field sets: the float64 bins
+        and their float32 compensation terms, the float32 sample buffers, the
+        sample map (int32 index + float32 weight per term) and the phasor
+        tables — from the host-side sample map, before any device work."""
+        idx, _ = self.sample_map(grid, boundary_conditions)
+        F = int(self.freqs.size)
+        bins = K * F * self.S * 2 * (8 + (4 if self.compensated else 0))
+        nbuf = 2 if self.pipelined else 1
+        X = nbuf * K * self.period * self.S_pad * 4
+        tables = 2 * int(math.ceil(F / 16) * 16) * block_pad(self.block) * 4 + 8 * idx.size
+        return int(bins + X + tables)
+
     def attach(self, solver) -> None:
         dev_idx, w = self.device_map(solver.mesh.grid, solver.boundary_conditions)
         dev = solver.e.device

@@ -66,19 +66,38 @@ class SweepJob:
     f_max: float
     precision: str = "single"
     waveforms: dict | None = None
+    # optional: CPML faces (default: PEC walls on the non-port faces) and a
+    # factory of DFT monitors recorded in every field set
+    boundary_conditions: dict | None = None
+    monitors_factory: object = None
 
 
-def footprint_bytes(dims, n_channels: int, K: int, n_steps: int, precision: str = "single") -> int:
+def footprint_bytes(dims, n_channels: int, K: int, n_steps: int, precision: str = "single",
+                    psi_words: int = 0, dft_bytes: int = 0) -> int:
     """Device bytes of a resident sweep: K field sets plus the shared alpha/beta
     and masses, the DTBC histories (w, s) and V/I records of every channel of
-    every set in float64, and the port work areas."""
+    every set in float64, the port work areas, the CPML psi slabs (psi_words
+    per set, plus their one-byte air masks) and the DFT monitors' bins and
+    sample buffers (dft_bytes, all sets)."""
     nx, ny, nz = dims
     ne, nh = kernels.n_e(nx, ny, nz), kernels.n_h(nx, ny, nz)
     w = 4 if precision == "single" else 8
     fields = w * (ne + nh) * K
     coeffs = w * (ne + nh) * 3  # alpha, beta and the mass vector of each side
     per_channel = 8 * 4 * max(n_steps, 1)  # w_hist, s_hist, V_rec, I_rec
-    return int(fields + coeffs + per_channel * n_channels * K + (64 << 20))
+    psi = (w * K + 1) * int(psi_words)
+    return int(fields + coeffs + per_channel * n_channels * K + psi + int(dft_bytes) + (64 << 20))
+
+
+def job_extras(boundary_conditions, monitors, grid, K: int) -> tuple[int, int]:
+    """(psi words per set, DFT monitor bytes) of a job's CPML faces and monitors."""
+    psi = 0
+    for bc in (boundary_conditions or {}).values():
+        if getattr(bc, "kind", "") == "CPML":
+            e_sh, h_sh = bc.psi_shapes(grid.dims)
+            psi += sum(int(np.prod(sh)) for sh in e_sh + h_sh)
+    dft = sum(int(m.footprint(grid, boundary_conditions, K)) for m in (monitors or []))
+    return psi, dft
 
 
 @dataclass
@@ -187,13 +206,17 @@ class SweepServer:
         mesh, ports, dt = built if built is not None else job.build()
         sw = ScatteringSweep(mesh, ports, dt=dt, n_steps=job.n_steps, f_axis=job.f_axis,
                              f_max=job.f_max, waveforms=job.waveforms, precision=job.precision,
-                             use_graphs=self.use_graphs, device=str(self.device))
+                             use_graphs=self.use_graphs, device=str(self.device),
+                             boundary_conditions=job.boundary_conditions,
+                             monitors_factory=job.monitors_factory)
         solver = sw.make_solver(sw.excited)
         solver.prepare_host()
         rec.cells = int(np.prod(mesh.dims))
         rec.K = len(sw.excited)
         rec.n_steps = job.n_steps
-        rec.footprint = footprint_bytes(mesh.dims, len(sw.channels), rec.K, job.n_steps, job.precision)
+        psi, dft = job_extras(sw.boundary_conditions, solver.monitors, mesh.grid, rec.K)
+        rec.footprint = footprint_bytes(mesh.dims, len(sw.channels), rec.K, job.n_steps, job.precision,
+                                        psi_words=psi, dft_bytes=dft)
         rec.prepared = time.perf_counter() - t0
         return sw, solver
 

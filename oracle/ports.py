@@ -135,15 +135,20 @@ def resolve_sign(pl: dict, v: np.ndarray) -> np.ndarray:
     return -v if v[j] < 0 else v
 
 
-def rect_te_mode(pl: dict, x, y, z, m: int, n: int, m_eps: np.ndarray, eps_r: float = 1.0):
+def rect_te_mode(pl: dict, x, y, z, m: int, n: int, m_eps: np.ndarray, eps_r: float = 1.0,
+                 window=None):
     """Closed-form discrete TE_mn on a uniform PEC-walled plane.
 
     m counts along the lower-numbered global tangential axis, n along the
-    higher one.  Returns (e_u, e_v, omega_c), M_eps-normalised and signed.
+    higher one.  ``window``: ((a0, a1), (b0, b1)) node-index ranges of a
+    rectangular guide inside the plane (PEC around it; default the whole
+    plane); the mode vanishes outside.  Returns (e_u, e_v, omega_c),
+    M_eps-normalised and signed.
     """
     nodes = (np.asarray(x), np.asarray(y), np.asarray(z))
     ta = sorted([pl["u_axis"], pl["v_axis"]])  # (a, b) global tangential axes
-    Na, Nb = pl["N"][ta[0]], pl["N"][ta[1]]
+    (ia0, ia1), (ib0, ib1) = window if window is not None else ((0, pl["N"][ta[0]]), (0, pl["N"][ta[1]]))
+    Na, Nb = ia1 - ia0, ib1 - ib0
     da = float(np.diff(nodes[ta[0]])[0])
     db = float(np.diff(nodes[ta[1]])[0])
     a0, b0 = nodes[ta[0]][0], nodes[ta[1]][0]
@@ -152,15 +157,15 @@ def rect_te_mode(pl: dict, x, y, z, m: int, n: int, m_eps: np.ndarray, eps_r: fl
 
     def prof(uv, along_a):
         # uv columns are (primal-axis midpoint, secondary node) of the family
-        # the sine factor sits on wall nodes (index 0 or N): exactly zero there
+        # the sine factor sits on wall nodes (window index 0 or N): exactly zero there
         if along_a:  # edges along axis a: E_a = kn cos(m pi ia/Na) sin(n pi ib/Nb)
-            ia = (uv[:, 0] - a0) / da
-            ib = np.rint((uv[:, 1] - b0) / db)
-            s = np.where((ib == 0) | (ib == Nb), 0.0, np.sin(n * math.pi * ib / Nb))
+            ia = (uv[:, 0] - a0) / da - ia0
+            ib = np.rint((uv[:, 1] - b0) / db) - ib0
+            s = np.where((ib <= 0) | (ib >= Nb) | (ia < 0) | (ia > Na), 0.0, np.sin(n * math.pi * ib / Nb))
             return kn * np.cos(m * math.pi * ia / Na) * s * da
-        ib = (uv[:, 0] - b0) / db
-        ia = np.rint((uv[:, 1] - a0) / da)
-        s = np.where((ia == 0) | (ia == Na), 0.0, np.sin(m * math.pi * ia / Na))
+        ib = (uv[:, 0] - b0) / db - ib0
+        ia = np.rint((uv[:, 1] - a0) / da) - ia0
+        s = np.where((ia <= 0) | (ia >= Na) | (ib < 0) | (ib > Nb), 0.0, np.sin(m * math.pi * ia / Na))
         return -km * s * np.cos(n * math.pi * ib / Nb) * db
 
     u_uv = pl["u_uv"]

@@ -400,36 +400,42 @@ def c3_oracle(excite, dims=C3_SMALL, dtype=np.float64):
 
 def c5_job_oracle(job, excite, dtype=np.float64):
     """Oracle of a serving job (bench.c5_jobs) with one excited (port,
-    mode): the WR-90 guide of the job's grid, its bricks (z clamped to the
-    middle 60 %) and spheres (cell-centre test) filled in the job's order,
-    PEC x/y walls (and z_max with one port), closed-form TE ports.
-    Returns (Oracle, ports, dt)."""
-    from bench import C5_SIZE  # noqa: PLC0415
+    mode): the job's grid — a WR-90 guide (1-2 ports on its z ends) or a
+    cross of two (3-4 ports on z and x ends, PEC corner blocks) — its bricks
+    (clipped to the region clear of the feeds) and spheres (cell-centre test)
+    filled in the job's order, PEC walls, closed-form TE ports on each
+    port's rectangular window.  Returns (Oracle, ports, dt)."""
+    import bench  # noqa: PLC0415
 
     dims, n_ports, objs = job.build.tasks["mesh"][1].args
     n_modes = job.build.tasks["port1"][0][3]
-    x, y, z = (np.linspace(0.0, s, n + 1) for s, n in zip(C5_SIZE, dims))
-    zlo, zhi = 0.2 * C5_SIZE[2], 0.8 * C5_SIZE[2]
+    size = bench._c5_size(n_ports)
+    x, y, z = (np.linspace(0.0, s, n + 1) for s, n in zip(size, dims))
+    klo, khi = bench._c5_keepout(n_ports)
     boxes, spheres, eps = [], [], [1.0]
+    if n_ports > 2:
+        eps.append(1.0)
+        boxes += [(1, c) for c in bench._c5_corners()]
+    pec_id = 1 if n_ports > 2 else -1
     for e, sphere, ctr, half in objs:
         eps.append(e)
         if sphere:
             spheres.append((len(eps) - 1, ctr, float(half.min())))
         else:
-            lo, hi = ctr - half, ctr + half
-            boxes.append((len(eps) - 1, (lo[0], lo[1], max(lo[2], zlo), hi[0], hi[1], min(hi[2], zhi))))
+            lo, hi = np.maximum(ctr - half, klo), np.minimum(ctr + half, khi)
+            boxes.append((len(eps) - 1, tuple(lo) + tuple(hi)))
     mid = om.fill_aabb(x, y, z, boxes)
     cx, cy, cz = np.meshgrid(*(0.5 * (a[:-1] + a[1:]) for a in (x, y, z)), indexing="ij")
     for m, c, r in spheres:
-        inside = ((cx - c[0]) ** 2 + (cy - c[1]) ** 2 + (cz - c[2]) ** 2 < r * r) & (cz > zlo) & (cz < zhi)
+        inside = (((cx - c[0]) ** 2 + (cy - c[1]) ** 2 + (cz - c[2]) ** 2 < r * r) & (cx > klo[0])
+                  & (cx < khi[0]) & (cz > klo[2]) & (cz < khi[2]))
         mid[inside] = m
     me = om.mass_e(x, y, z, mid, [eps] * 3, om.EPS0)
     mmu = om.mass_h(x, y, z, mid, [[1.0] * len(eps)] * 3, om.MU0)
     nx, ny, nz = dims
-    walls = ["xmin", "xmax", "ymin", "ymax"] + (["zmax"] if n_ports == 1 else [])
-    pec = om.bbox_walls(np.zeros(me.size, bool), nx, ny, nz, walls)
+    pec = om.bbox_walls(om.pec_edge_mask(mid == pec_id), nx, ny, nz, list(bench._c5_walls(n_ports)))
     pec_mesh = pec
-    faces = ("zmin", "zmax")[:n_ports]
+    faces = bench.C5_FACES[:n_ports]
     for f in faces:
         me = om.flatten_port(me, nx, ny, nz, f, "E")
         mmu = om.flatten_port(mmu, nx, ny, nz, f, "H")
@@ -438,10 +444,16 @@ def c5_job_oracle(job, excite, dtype=np.float64):
     ae, be = om.alpha_beta(me, np.zeros_like(me), dt, pec)
     ah, bh = om.alpha_beta(mmu, np.zeros_like(mmu), dt)
     ports = []
-    for name, face in zip(("port1", "port2"), faces):
+    for k, face in enumerate(faces):
         pl = op.plane(face, x, y, z)
-        modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES[:n_modes]]
-        ports.append(op.Port(name, pl, modes, me, mmu, dt, 2 * math.pi * job.f_max,
+        if face[0] == "z":  # global tangential axes (x, y): the guide is x-wide
+            window = ((nx // 4, 3 * nx // 4), (0, ny)) if n_ports > 2 else None
+            mns = C1_MODES
+        else:  # (y, z): the arm is z-wide
+            window = ((0, ny), (nz // 4, 3 * nz // 4))
+            mns = [(n, m) for m, n in C1_MODES]
+        modes = [op.rect_te_mode(pl, x, y, z, m, n, me, window=window) for m, n in mns[:n_modes]]
+        ports.append(op.Port(f"port{k + 1}", pl, modes, me, mmu, dt, 2 * math.pi * job.f_max,
                              complement=oracle_complement(pl, x, y, z, mid, me, pec_mesh, dt)))
     p = next(q for q in ports if q.name == excite[0])
     f_lo = p.modes[excite[1]]["wc"] / (2 * math.pi)

@@ -1,8 +1,9 @@
 """Serving jobs (C5) in the continuous-batching mix against the oracle.
 
-Three seeded sweep jobs from the C5 generator (bench.c5_jobs: WR-90 guides
-with seeded dielectric bricks / spheres, 1-2 modal DTBC ports x 1-3 modes,
-every channel excited), small grids (24^3-32^3 cells) and 300-400 steps, all
+Three seeded sweep jobs from the C5 generator (bench.c5_jobs: a cross of two
+WR-90 guides with 3 ports on its z and x ends x 3 modes, and two single
+guides with one port x 2 modes; seeded dielectric bricks / spheres, every
+channel excited), small grids (24^3-32^3 cells) and 300-400 steps, all
 arriving at once so they run resident together on their own streams; the
 oracle runs each job's excitations one at a time on cells filled from the
 job's seeded geometry.  Every job's S-matrix (all channels x all

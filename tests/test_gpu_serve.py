@@ -1,7 +1,8 @@
 """GPU serving loop: continuous batching of sweep jobs.
 
-Small seeded jobs of the serving workload (WR-90 guides with dielectric
-bricks/spheres, 1-2 modal ports x 1-3 modes) arrive close together:
+Small seeded jobs of the serving workload (WR-90 guides and crosses of two
+with dielectric bricks/spheres, 1-4 modal ports x 1-3 modes) arrive close
+together:
 
 * in the mix (several jobs resident, their chunks overlapping on separate
   streams) every job's S-matrix equals the same job run alone, bit for bit;
@@ -9,7 +10,10 @@ bricks/spheres, 1-2 modal ports x 1-3 modes) arrive close together:
   resident footprint within the budget and every job still completes with
   the same S-matrix;
 * the one-job-at-a-time mode gives the same S-matrices;
-* latency percentiles and throughput are reported.
+* latency percentiles and throughput are reported;
+* the admission footprint of a job with a CPML face and a DFT monitor
+  covers what its solver allocates on the device (psi slabs, bins and sample
+  buffers included) within the 64 MiB slack plus 10 %.
 """
 
 import numpy as np
@@ -72,3 +76,37 @@ def test_sequential_mode(jobs, alone, gpu):
     assert rep.peak_resident == 1
     for r in rep.records:
         assert _same(r.result, alone[r.job_id]), r.job_id
+
+
+def test_footprint_covers_cpml_and_monitors(jobs, gpu):
+    import time
+
+    import torch
+
+    from magnelio_amd.boundaries import CPMLBoundary, PECBoundary
+    from magnelio_amd.monitors import MonitorFieldFrequency
+    from magnelio_amd.serve import JobRecord, SweepJob
+
+    # a single guide with one port on z_min and a CPML on z_max
+    base = next(j for j in jobs if j.build.tasks["mesh"][1].args[1] == 2)
+    dims, _, objs = base.build.tasks["mesh"][1].args
+    mesh, dt = bench._c5_mesh(dims, 2, objs)
+    ports = [bench._c5_port(dims, 2, "port1", "zmin", 1, base.f_max)]
+    bcs = {f: PECBoundary(f) for f in ("xmin", "xmax", "ymin", "ymax")}
+    bcs["zmax"] = CPMLBoundary("zmax", mesh.grid, thickness_cells=8)
+    job = SweepJob(99, 0.0, lambda: (mesh, ports, dt), base.n_steps, base.f_axis, base.f_max,
+                   boundary_conditions=bcs,
+                   monitors_factory=lambda: [MonitorFieldFrequency(np.linspace(8e9, 20e9, 41), fields=("E",))])
+    srv = SweepServer()
+    rec = JobRecord(99, 0.0)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    sw, solver = srv._prepare(job, rec, time.perf_counter())
+    solver.setup()
+    torch.cuda.synchronize()
+    used = torch.cuda.memory_allocated() - before
+    print(f"footprint {rec.footprint / 1e6:.1f} MB, allocated by setup {used / 1e6:.1f} MB", flush=True)
+    assert rec.footprint >= used
+    assert rec.footprint <= 1.1 * used + (64 << 20)
+    mon = solver.monitors[0]
+    assert mon.footprint(mesh.grid, bcs, solver.nsets) >= mon.bytes()
