@@ -130,12 +130,13 @@ def port_from_operator(op, f_calc: float | None = None) -> PortModel:
     for dm in op.discrete_modes:
         md = dm.mode
         kind = str(getattr(md.mode_type, "value", md.mode_type))
-        if kind not in ("TE", "TM"):
+        if kind not in ("TE", "TM", "TEM"):
             raise NotImplementedError(f"port {op.name!r}: {kind} modes are not served by this backend")
+        z_line = getattr(md, "z_line", None)
         modes.append(DiscreteMode(name=str(md.name), omega_c=float(md.omega_c), epsilon_r=float(md.epsilon_r),
                             e_u=np.asarray(dm.e_u_profile, float), e_v=np.asarray(dm.e_v_profile, float),
                             h_u=np.asarray(dm.h_u_profile, float), h_v=np.asarray(dm.h_v_profile, float),
-                            mode_type=kind))
+                            mode_type=kind, z_line=None if z_line is None else float(z_line)))
     comp_r = comp_live = None
     if getattr(op, "_comp_r_u", None) is not None:  # the complement absorber (operator.py:596-619)
         comp_r = np.concatenate([np.asarray(op._comp_r_u, float), np.asarray(op._comp_r_v, float)])

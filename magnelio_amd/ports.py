@@ -47,6 +47,7 @@ from . import mesh as _mesh
 from .dtbc import packed_kernels
 
 C0, MU0, EPS0 = _mesh.C0, _mesh.MU0, _mesh.EPS0
+ETA0 = math.sqrt(MU0 / EPS0)
 
 # face -> (normal axis, is_max, u_axis, v_axis); u x v points into the domain
 FACES = {
@@ -254,8 +255,15 @@ class DiscreteMode:
     h_u: np.ndarray
     h_v: np.ndarray
     mode_type: str = "TE"
+    z_line: float | None = None  # TEM: the line impedance
 
     def z_modal(self, omega: float) -> complex:
+        """Z of the power-wave split: the line impedance of a TEM channel,
+        else the wave impedance (mode.py:215-231)."""
+        if self.z_line is not None:
+            return complex(self.z_line)
+        if self.mode_type == "TEM":
+            return complex(ETA0 / math.sqrt(self.epsilon_r))
         if self.mode_type == "TM":
             return z_tm(omega, self.omega_c, self.epsilon_r)
         return z_te(omega, self.omega_c, self.epsilon_r)
@@ -696,22 +704,34 @@ def build_port(name: str, face: str, mesh, m_eps: np.ndarray, m_mu: np.ndarray, 
                f_calc: float, n_modes: int = 1, eps_r: float = 1.0,
                termination: str = "auto") -> PortModel:
     """Numerical modal port (TE and TM) on a whole bbox face (factory.py:1367-1877 order)."""
+    plane, m_eps, m_mu, pec = _port_inputs(face, mesh, m_eps, m_mu, dt, f_calc)
+    modes = solve_modes(plane, mesh.grid, m_eps, m_mu, pec, n_modes, f_calc, eps_r)
+    warn_degenerate_cutoffs(modes, name)
+    return _assemble_port(name, plane, mesh, m_eps, m_mu, modes, dt, f_calc, termination)
+
+
+def _port_inputs(face, mesh, m_eps, m_mu, dt, f_calc):
+    """Feed check, plane, and the masses and PEC mask flattened at the port
+    face (the builder-local flatten of factory.py:1419-1458)."""
     if dt <= 0.0 or f_calc <= 0.0:
         raise ValueError("dt and f_calc must be positive")
-    grid = mesh.grid
-    validate_feed(face, grid)
-    plane = PortPlane.from_mesh(face, grid)
+    validate_feed(face, mesh.grid)
+    plane = PortPlane.from_mesh(face, mesh.grid)
     dims = mesh.dims
     m_eps = _mesh.flatten_port_mass(m_eps, dims, face)
     m_mu = _mesh.flatten_port_mu(m_mu, dims, face)
     pec = _mesh.pec_flat(_mesh.flatten_port_pec(mesh.pec_mask_edges, dims, face), dims)
-    modes = solve_modes(plane, grid, m_eps, m_mu, pec, n_modes, f_calc, eps_r)
-    warn_degenerate_cutoffs(modes, name)
+    return plane, m_eps, m_mu, pec
+
+
+def _assemble_port(name, plane, mesh, m_eps, m_mu, modes, dt, f_calc, termination) -> PortModel:
+    """Calibration, Mur and DTBC chain parameters and the complement absorber
+    of a port whose modes are solved (factory.py:1803-1877, operator.py:596-941)."""
     omega_calc = 2.0 * math.pi * f_calc
     me_u, me_v = m_eps[plane.e_u_indices], m_eps[plane.e_v_indices]
     me_ui, me_vi = m_eps[plane.e_u_indices_interior], m_eps[plane.e_v_indices_interior]
     mh_u, mh_v = m_mu[plane.h_u_indices], m_mu[plane.h_v_indices]
-    defect = chain_slab_defect(dims, m_eps, m_mu, face)
+    defect = chain_slab_defect(mesh.dims, m_eps, m_mu, plane.face)
     modes, rec, src, st = _calibrate(plane, modes, me_u, me_v, mh_u, mh_v, omega_calc)
     M = len(modes)
     v_p = np.empty(M)
@@ -738,6 +758,210 @@ def build_port(name: str, face: str, mesh, m_eps: np.ndarray, m_mu: np.ndarray, 
                      mh_v=mh_v, record_scale=rec, source_scale=src, state_scale=st, dtbc_r=rs,
                      dtbc_q=qs, dtbc_z0=z0s, pair_spread=spreads, mur_r=mur_r, tau=tau,
                      slab_defect=defect, comp_r=comp_r, comp_live=comp_live)
+
+
+# ---------------------------------------------------------------------------
+# TEM ports of multi-conductor feeds (coax)
+
+def _components(a: np.ndarray, b: np.ndarray, n: int) -> dict:
+    """Connected components of the node links a-b: label -> sorted nodes
+    (nodes without a link are in none)."""
+    from scipy.sparse.csgraph import connected_components  # noqa: PLC0415
+
+    if a.size == 0:
+        return {}
+    rows, cols = np.concatenate([a, b]), np.concatenate([b, a])
+    adj = sp.csr_matrix((np.ones(rows.size, np.int8), (rows, cols)), shape=(n, n))
+    _, labels = connected_components(adj, directed=False)
+    out: dict = {}
+    for node in np.unique(rows):
+        out.setdefault(int(labels[node]), []).append(int(node))
+    return out
+
+
+def conductor_groups(plane: PortPlane, mesh, pec_e_flat: np.ndarray) -> list[np.ndarray]:
+    """[ground, signal_1, ...] plane-node groups of the conductors on a port
+    plane (auto_conductors.py:54-247): connected components of the nodes
+    joined by PEC plane edges; PEC cells of the port-adjacent slab link their
+    four corners, which only fuses edge components into one conductor (or,
+    with fewer than two edge components, supplies the node sets, with a
+    warning).  Ground is the largest component touching the plane's
+    boundary (else the largest), the rest by descending size."""
+    import warnings  # noqa: PLC0415
+
+    G = gradient_2d(plane)
+    nu1, nv1 = plane.u_node_window[1] + 1, plane.v_node_window[1] + 1
+    prim = np.concatenate([plane.e_u_indices, plane.e_v_indices])
+    rows = G[np.asarray(pec_e_flat, bool)[prim]].tocoo()
+    order = np.lexsort((rows.data, rows.row))  # per edge: start (-1) then end (+1)
+    cols = rows.col[order].reshape(-1, 2)
+    ea, eb = cols[:, 0].astype(np.int64), cols[:, 1].astype(np.int64)
+    comps = _components(ea, eb, nu1 * nv1)
+    # PEC cells of the slab next to the plane link their corners
+    n_ax, is_max, u_ax, v_ax = FACES[plane.face]
+    slab = np.take(mesh.material_id, mesh.dims[n_ax] - 1 if is_max else 0, axis=n_ax)
+    if u_ax > v_ax:
+        slab = slab.T
+    pec_ids = [k for k, m in mesh.material_library.items() if getattr(m, "is_pec", False)]
+    cu, cv = np.nonzero(np.isin(slab, pec_ids)) if pec_ids else (np.zeros(0, int), np.zeros(0, int))
+    if cu.size:
+        c00 = cu * nv1 + cv
+        ca = np.concatenate([c00, c00, c00])
+        cb = np.concatenate([c00 + nv1, c00 + 1, c00 + nv1 + 1])
+        merged = _components(np.concatenate([ea, ca]), np.concatenate([eb, cb]), nu1 * nv1)
+        if len(comps) >= 2:
+            lab = {n: l for l, nodes in merged.items() for n in nodes}
+            fused: dict = {}
+            for nodes in comps.values():
+                fused.setdefault(lab[nodes[0]], []).extend(nodes)
+            comps = {k: sorted(v) for k, v in fused.items()}
+        else:
+            comps = merged
+            if len(merged) >= 2:
+                warnings.warn(f"port {plane.face}: conductors resolved from PEC cells only; refine the "
+                              "transversal mesh", UserWarning, stacklevel=2)
+    if len(comps) < 2:
+        raise ValueError(f"port {plane.face}: {len(comps)} PEC-connected conductor(s) on the plane; "
+                         "a TEM port needs a ground and at least one signal conductor")
+
+    def touches(nodes):
+        iu, iv = np.divmod(np.asarray(nodes), nv1)
+        return bool(np.any((iu == 0) | (iu == nu1 - 1) | (iv == 0) | (iv == nv1 - 1)))
+
+    wall = [k for k, v in comps.items() if touches(v)]
+    ground = max(wall or list(comps), key=lambda k: len(comps[k]))
+    rest = sorted((k for k in comps if k != ground), key=lambda k: -len(comps[k]))
+    return [np.asarray(comps[ground], np.int64)] + [np.asarray(comps[k], np.int64) for k in rest]
+
+
+def _boundary_window_factor(plane: PortPlane, grid, pmc_faces=()) -> np.ndarray:
+    """Per plane edge [u | v]: the share of its dual cell the plane's
+    capacitance owns (tem_laplace.py:170-306): 1 inside, 1/2 on the plane's
+    boundary lines (a wall on the line), 1 on a PMC face's line (the
+    magnetic wall sits half a cell beyond it)."""
+    n_ax, is_max, u_ax, v_ax = FACES[plane.face]
+    names = (("xmin", "xmax"), ("ymin", "ymax"), ("zmin", "zmax"))
+    nu, nv = plane.u_node_window[1], plane.v_node_window[1]
+
+    def f(axis, lo):
+        return 1.0 if names[axis][0 if lo else 1] in pmc_faces else 0.5
+
+    fu = np.ones((nu, nv + 1))
+    fu[:, 0], fu[:, -1] = f(v_ax, True), f(v_ax, False)
+    fv = np.ones((nv, nu + 1))
+    fv[:, 0], fv[:, -1] = f(u_ax, True), f(u_ax, False)
+    return np.concatenate([fu.ravel(), fv.ravel()])
+
+
+def solve_tem_modes(plane: PortPlane, grid, m_eps, m_mu, groups, eps_r: float = 1.0,
+                    pmc_faces=()) -> list[DiscreteMode]:
+    """TEM channels of a multi-conductor plane by the FIT Laplace with
+    conductor potentials (tem_laplace.py:321-488, :640-771): per signal
+    conductor k, phi = 1 on k, 0 on the others, L phi = 0 elsewhere with
+    L = G^T M_eps G; e = -G phi, M_eps-normalised, signed so the D flux out
+    of conductor k is positive; C' = e_raw^T M_cap e_raw / dx_n with the
+    boundary-corrected mass, Z_line = sqrt(eps_r) / (c C').  Several signal
+    conductors: the eigenvectors of the raw fields' Gram matrix (descending,
+    the leading near-largest conductor weight positive)."""
+    from scipy.sparse.linalg import spsolve  # noqa: PLC0415
+
+    if eps_r <= 0.0:
+        raise ValueError("epsilon_r must be positive")
+    if len(groups) < 2:
+        raise ValueError("a TEM port needs a ground and at least one signal conductor")
+    G = gradient_2d(plane)
+    prim = np.concatenate([plane.e_u_indices, plane.e_v_indices])
+    Md = np.asarray(m_eps[prim], float)
+    Mcap = Md * _boundary_window_factor(plane, grid, pmc_faces)
+    n_nodes = G.shape[1]
+    fixed = np.concatenate(groups)
+    if np.unique(fixed).size != fixed.size:
+        raise ValueError("conductor groups must be disjoint")
+    free = np.setdiff1d(np.arange(n_nodes), fixed)
+    if free.size == 0:
+        raise RuntimeError("every plane node is on a conductor")
+    L = (G.T @ sp.diags(Md) @ G).tocsc()
+    L_ff = L[free][:, free]
+    L_fb = L[free][:, fixed]
+    owner = np.concatenate([np.full(g.size, k) for k, g in enumerate(groups)])
+    raw = []
+    for k in range(1, len(groups)):
+        phi_b = (owner == k).astype(float)
+        phi = np.zeros(n_nodes)
+        phi[free] = spsolve(L_ff, -(L_fb @ phi_b))
+        phi[fixed] = phi_b
+        e_raw = -(G @ phi)
+        norm_sq = float(e_raw @ (Md * e_raw))
+        if norm_sq <= 0.0:
+            raise RuntimeError(f"signal conductor {k}: degenerate TEM field")
+        e_n = e_raw / math.sqrt(norm_sq)
+        outward = -float((G.T @ (Md * e_n))[groups[k]].sum())
+        if outward < 0.0:
+            e_n, e_raw = -e_n, -e_raw
+        raw.append((e_n, float(e_raw @ (Mcap * e_raw)), e_raw))
+    ndx = float(plane.normal_dx)
+    if len(raw) == 1:
+        channels = [(raw[0][0], raw[0][1] / ndx)]
+    else:
+        E = np.stack([r[2] for r in raw], axis=1)
+        g_fit = E.T @ (Md[:, None] * E)
+        g_cap = E.T @ (Mcap[:, None] * E)
+        w_all, W = np.linalg.eigh(g_fit)
+        order = np.argsort(w_all)[::-1]
+        w_all, W = w_all[order], W[:, order]
+        channels = []
+        for m in range(W.shape[1]):
+            w = W[:, m]
+            aw = np.abs(w)
+            lead = int(np.flatnonzero(aw >= (1.0 - 1e-9) * aw.max())[0])
+            if w[lead] < 0.0:
+                w = -w
+            channels.append(((E @ w) / math.sqrt(float(w_all[m])), float(w @ g_cap @ w) / ndx))
+    n_u = plane.e_u_indices.size
+    z_tem = ETA0 / math.sqrt(eps_r)
+    out = []
+    for i, (e, c_pl) in enumerate(channels):
+        e_u, e_v = e[:n_u].copy(), e[n_u:].copy()
+        h_u, h_v = travelling_wave_h(e_u, e_v, plane, m_mu, z_tem)
+        out.append(DiscreteMode(f"TEM_lap{i:02d}", 0.0, eps_r, e_u, e_v, h_u, h_v, "TEM",
+                                z_line=math.sqrt(eps_r) / (C0 * c_pl)))
+    return out
+
+
+def conductor_centroid(plane: PortPlane, grid, nodes) -> tuple[float, float]:
+    """(u, v) mean of a conductor's plane-node coordinates (factory.py:953-971)."""
+    _, _, u_ax, v_ax = FACES[plane.face]
+    nv1 = plane.v_node_window[1] + 1
+    coords = (grid.x, grid.y, grid.z)
+    iu, iv = np.divmod(np.asarray(nodes), nv1)
+    return float(np.mean(coords[u_ax][iu])), float(np.mean(coords[v_ax][iv]))
+
+
+def build_coax_port(name: str, face: str, mesh, m_eps: np.ndarray, m_mu: np.ndarray, *, dt: float,
+                    f_calc: float, center, eps_r: float = 1.0, n_modes: int = 1,
+                    termination: str = "auto", pmc_faces=()) -> PortModel:
+    """Coaxial (TEM) modal port on a whole bbox face: the FIT path of the
+    reference's coax port (factory.py:1470-1520) — conductors found on the
+    flattened PEC mask, exactly two of them centred within one cell of
+    ``center`` (global tangential frame, lower axis first; :974-1020), the
+    Laplace TEM channel, then the same calibration, chain and absorber set-up
+    as every modal port."""
+    plane, m_eps, m_mu, pec = _port_inputs(face, mesh, m_eps, m_mu, dt, f_calc)
+    groups = conductor_groups(plane, mesh, pec)
+    if len(groups) != 2:
+        raise ValueError(f"coax port {name!r} on {face!r}: {len(groups)} conductor groups found; "
+                         "expected 2 (inner and outer)")
+    _, _, u_ax, v_ax = FACES[face]
+    cu, cv = (center[0], center[1]) if u_ax < v_ax else (center[1], center[0])
+    d = (mesh.grid.dx, mesh.grid.dy, mesh.grid.dz)
+    tol = max(float(np.max(d[u_ax])), float(np.max(d[v_ax])))
+    for k, g in enumerate(groups):
+        gu, gv = conductor_centroid(plane, mesh.grid, g)
+        if math.hypot(gu - cu, gv - cv) > tol:
+            raise ValueError(f"coax port {name!r}: {'outer' if k == 0 else 'inner'} conductor centroid "
+                             f"({gu:.6g}, {gv:.6g}) is more than one cell from the centre ({cu:.6g}, {cv:.6g})")
+    modes = solve_tem_modes(plane, mesh.grid, m_eps, m_mu, groups, eps_r, pmc_faces)[:n_modes]
+    return _assemble_port(name, plane, mesh, m_eps, m_mu, modes, dt, f_calc, termination)
 
 
 def complement_absorber(plane: PortPlane, mesh, m_eps_flat: np.ndarray, dt: float):

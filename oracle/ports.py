@@ -4,6 +4,11 @@ Restates (paths relative to the reference repository root):
 
 * plane maps        src/magnelio/ports/_modal/port_plane.py:32-334 and the
                     face frames of src/magnelio/mesh/faces.py:63-71
+* TEM channels     taken as given (profiles and line impedance); the H
+                    profiles use the wave impedance eta0/sqrt(eps_r), the
+                    calibration the line impedance, the chain q = 0 with no
+                    static z0 (ports/_modal/tem_laplace.py:474-482,
+                    operator.py:666-668, :695-697)
 * discrete TM modes closed form on a uniform PEC rectangle: the node
                     potential phi = sin(m pi ia/Na) sin(n pi ib/Nb) of the
                     normal-E problem (ports/_modal/curl_curl_2d.py:127-218),
@@ -231,16 +236,22 @@ def z_te(omega, omega_c, eps_r=1.0):
     return 1j * omega * MU0 / g, g
 
 
-def z_mode(kind, omega, omega_c, eps_r=1.0):
-    """(Z, gamma) of a TE or TM channel."""
+def z_mode(kind, omega, omega_c, eps_r=1.0, z_line=None):
+    """(Z, gamma) of a TE, TM or TEM channel (a TEM channel's Z is its line
+    impedance, mode.py:215-231)."""
     z, g = z_te(omega, omega_c, eps_r)
+    if kind == "TEM":
+        return complex(z_line if z_line is not None else math.sqrt(MU0 / (EPS0 * eps_r))), g
     if kind == "TM":
         return g / (1j * omega * EPS0 * eps_r), g
     return z, g
 
 
 def z_real(kind, omega, omega_c, eps_r=1.0):
-    """|Z| at omega used to bake the H profiles."""
+    """|Z| at omega used to bake the H profiles (TEM: the wave impedance
+    eta0 / sqrt(eps_r), tem_laplace.py:474-482)."""
+    if kind == "TEM":
+        return math.sqrt(MU0 / (EPS0 * eps_r))
     root = math.sqrt(abs(omega_c**2 * MU0 * EPS0 * eps_r - omega**2 * MU0 * EPS0 * eps_r))
     return omega * MU0 / root if kind == "TE" else root / (omega * EPS0 * eps_r)
 
@@ -287,10 +298,11 @@ class Port:
         for mi, mode in enumerate(modes):
             e_u, e_v, wc = mode[:3]
             kind = mode[3] if len(mode) > 3 else "TE"
+            z_line = mode[4] if len(mode) > 4 else None
             sc = MU0 * ndx / z_real(kind, omega_calc, wc, eps_r)
             h_u = np.where(self.mh_u > 0, -e_v * (sc / np.where(self.mh_u > 0, self.mh_u, 1.0)), 0.0)
             h_v = np.where(self.mh_v > 0, +e_u * (sc / np.where(self.mh_v > 0, self.mh_v, 1.0)), 0.0)
-            Z, gam = z_mode(kind, omega_calc, wc, eps_r)
+            Z, gam = z_mode(kind, omega_calc, wc, eps_r, z_line)
             zr = float(Z.real)
             v_test = float(np.dot(self.me_u, e_u**2)) + float(np.dot(self.me_v, e_v**2))
             i_test = float(np.dot(self.mh_u, h_u**2)) + float(np.dot(self.mh_v, h_v**2))
@@ -308,7 +320,7 @@ class Port:
             self.record_scale[mi] = kappa * c_state
             self.source_scale[mi] = math.sqrt(abs(zr)) / kappa / c_state
             self.modes.append(dict(e_u=e_u, e_v=e_v, h_u=h_u * g, h_v=h_v * g, wc=wc, eps_r=eps_r,
-                                   kind=kind))
+                                   kind=kind, z_line=z_line))
         # chain parameters (all channels certified on a uniform feed)
         self.dtbc = []
         self.line = {}
@@ -329,7 +341,9 @@ class Port:
                 self.dtbc.append(None)
             else:
                 self.dtbc.append(DTBC(r, q))
-                self.line[mi] = (r, q, z0)
+                # no static z0 on a massless (q = 0) chain: its S-parameters
+                # take the channel's own Z (operator.py:666-668)
+                self.line[mi] = (r, q, z0 if q > 0.0 else None)
         self.mur_r = np.empty(M)
         self.tau = np.empty(M)
         for mi, md in enumerate(self.modes):

@@ -5,9 +5,11 @@ sum_n x_n e^{-2 pi j f n dt}, no dt factor) and
 src/magnelio/post/modal_sparameters.py:118-242, :335-557: I is shifted by
 the temporal half step e^{+j w dt/2}; certified DTBC channels use the exact
 discrete wave impedance (TE: z0 s/rad, TM: z0 rad/s) and the discrete
-half-cell factor lambda^{1/2} for the two-plane de-stagger; channels on Mur
-the continuum wave impedance (TE j w mu0 / gamma, TM gamma / (j w eps)) and
-the factor e^{gamma dx_n / 2} (mode.py:170-230); S = b / a_excited with |a| < 1e-12 peak -> NaN.
+half-cell factor lambda^{1/2} for the two-plane de-stagger (a massless TEM
+chain has no static z0 and takes the line impedance); channels on Mur the
+continuum wave impedance (TE j w mu0 / gamma, TM gamma / (j w eps), TEM the
+line impedance) and the factor e^{gamma dx_n / 2} (mode.py:170-231);
+S = b / a_excited with |a| < 1e-12 peak -> NaN.
 
 Pinned by tests/test_oracle_sparams.py: the DFT of a sine and of DC against
 closed forms, and a synthetic discrete travelling wave whose a/b split must
@@ -29,23 +31,26 @@ def dft(x, f, dt):
 
 
 def waves(V, I, f, dt, line, mode=None, normal_dx=None):
-    """(a, b) at the port plane of a TE or TM channel: certified (line =
-    (r, q, z0)) or on Mur (line None, mode = (omega_c, eps_r[, kind]),
-    normal_dx the port cell)."""
+    """(a, b) at the port plane of a TE, TM or TEM channel: certified (line =
+    (r, q, z0); z0 None on a massless chain, whose Z is the channel's own) or
+    on Mur (line None, normal_dx the port cell); mode = (omega_c, eps_r[,
+    kind[, z_line]])."""
     from .ports import z_mode  # noqa: PLC0415
 
     kind = mode[2] if mode is not None and len(mode) > 2 else "TE"
+    z_line = mode[3] if mode is not None and len(mode) > 3 else None
     w = 2 * math.pi * np.asarray(f, float)
     Vf = dft(V, f, dt)
     If = dft(I, f, dt) * np.exp(1j * w * dt / 2)
-    if line is not None:
-        r, q, z0 = line
-        Z = dtbc_wave_impedance(w * dt, q, z0, kind)
-        th = destagger_theta(w * dt, r, q)
+    if line is not None and line[2] is not None:
+        Z = dtbc_wave_impedance(w * dt, line[1], line[2], kind)
     else:
-        zg = [z_mode(kind, float(om), mode[0], mode[1]) for om in w]
-        Z = np.array([a for a, _ in zg], complex)
-        th = 0.5 * float(normal_dx) * np.array([g for _, g in zg], complex)
+        Z = np.array([z_mode(kind, float(om), mode[0], mode[1], z_line)[0] for om in w], complex)
+    if line is not None:
+        th = destagger_theta(w * dt, line[0], line[1])
+    else:
+        th = 0.5 * float(normal_dx) * np.array([z_mode(kind, float(om), mode[0], mode[1])[1] for om in w],
+                                               complex)
     sz = np.sqrt(Z)
     ep, em = np.exp(th), np.exp(-th)
     den = ep + em

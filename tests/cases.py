@@ -26,6 +26,13 @@ def c1_axes(dims=C1_DIMS, size=C1_SIZE):
     return tuple(np.linspace(0.0, L, n + 1) for L, n in zip(size, dims))
 
 
+def gaussian(t: float, f_max: float) -> float:
+    """Plain Gaussian pulse, peak 1 at 4 / f_max (the TEM waveform)."""
+    sigma = 2.0 / (math.pi * f_max)
+    x = (t - 4.0 / f_max) / sigma
+    return math.exp(-x * x)
+
+
 def modulated_gaussian(t: float, f_max: float, f_min: float) -> float:
     bw = f_max - f_min
     sigma = 2.0 / (math.pi * bw)
@@ -179,6 +186,8 @@ def golden_c1_oracle(name, dtype=np.float64):
         boxes, eps_block = [(1, mk.MUR_BLOCK)], mk.MUR_EPS
     elif name == "c1_tm":
         boxes = [(2, _c1_fixture_module().TM_CORNER)]
+    elif name == "c1_coax":
+        boxes = [(2, _c1_fixture_module().COAX_INNER)]
     mid = om.fill_aabb(x, y, z, boxes)
     eps = [[1.0, eps_block, 1.0]] * 3
     sig = [[0.0, GOLDEN_SIGMA if name == "c1_loaded" else 0.0, 0.0]] * 3
@@ -200,14 +209,19 @@ def golden_c1_oracle(name, dtype=np.float64):
     for label, face in (("port1", "zmin"), ("port2", "zmax")):
         pl = op.plane(face, x, y, z)
         modes = [(f[f"{label}_e_u{m}"], f[f"{label}_e_v{m}"], float(f[f"{label}_omega_c{m}"]),
-                  str(f.get(f"{label}_type{m}", "TE"))) for m in range(golden_n_modes(name))]
+                  str(f.get(f"{label}_type{m}", "TE")),
+                  float(f[f"{label}_z_line{m}"]) if f"{label}_z_line{m}" in f else None)
+                 for m in range(golden_n_modes(name))]
         term = "mur" if (name == "c1_mur" and label == "port1") else "auto"
         comp = op.complement_arrays(pl, me, mvac, pec_mesh, dt)
         ports.append(op.Port(label, pl, modes, me, mm, dt, 2 * math.pi * C1_FMAX, termination=term,
                              complement=comp))
     exc = golden_excite(name)
     f_lo = ports[0].modes[exc]["wc"] / (2 * math.pi)
-    ports[0].set_excitation(exc, lambda t: modulated_gaussian(t, C1_FMAX, f_lo))
+    if f_lo > 0.0:
+        ports[0].set_excitation(exc, lambda t: modulated_gaussian(t, C1_FMAX, f_lo))
+    else:  # a TEM channel: the plain Gaussian
+        ports[0].set_excitation(exc, lambda t: gaussian(t, C1_FMAX))
     orc = ofd.Oracle(dims, ae, be, ah, bh, dt, ports=ports, dtype=dtype)
     return orc, ports, dt, (me, mm)
 
@@ -227,7 +241,20 @@ def golden_c1_mesh(name):
         regions = [(pm.Material(name="feed_block", epsilon=(mk.MUR_EPS,) * 3), mk.MUR_BLOCK)]
     elif name == "c1_tm":
         regions = [(pm.Material.pec(), _c1_fixture_module().TM_CORNER)]
+    elif name == "c1_coax":
+        regions = [(pm.Material.pec(), _c1_fixture_module().COAX_INNER)]
     return pm.Mesh.from_grid(g, regions=regions, pec_faces=("xmin", "xmax", "ymin", "ymax"))
+
+
+def golden_port(name, label, face, mesh, me, mu, dt, n_modes):
+    """Product port of a golden case: coaxial (TEM) for c1_coax, else numerical."""
+    from magnelio_amd import ports as pp  # noqa: PLC0415
+
+    if name == "c1_coax":
+        w = _c1_fixture_module().COAX_W
+        return pp.build_coax_port(label, face, mesh, me, mu, dt=dt, f_calc=C1_FMAX, center=(0.5 * w, 0.5 * w),
+                                  n_modes=n_modes)
+    return pp.build_port(label, face, mesh, me, mu, dt=dt, f_calc=C1_FMAX, n_modes=n_modes)
 
 
 def golden_c1_solver(name, precision="double", n_steps=300):
@@ -242,7 +269,7 @@ def golden_c1_solver(name, precision="double", n_steps=300):
     dt = pm.courant_dt(mesh.grid, "normal")
     me, mu = pm.build_m_eps(mesh), pm.build_m_mu(mesh)
     n_modes, exc = golden_n_modes(name), golden_excite(name)
-    ports = [pp.build_port(lbl, face, mesh, me, mu, dt=dt, f_calc=C1_FMAX, n_modes=n_modes)
+    ports = [golden_port(name, lbl, face, mesh, me, mu, dt, n_modes)
              for lbl, face in (("port1", "zmin"), ("port2", "zmax"))]
     wf = resolve_excitation(ports[0].modes[exc].omega_c, 0.0, C1_FMAX, exc)
     bcs = {f: PECBoundary(f) for f in ("xmin", "xmax", "ymin", "ymax")}

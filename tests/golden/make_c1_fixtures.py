@@ -24,9 +24,13 @@ plain cases for 300 steps:
   cross-section along its whole length (an L-shaped section: TE_mn and
   TM_mn are no longer degenerate), 4-mode ports whose channel 3 is the
   lowest TM mode; that TM channel of port 1 is the one excited;
+* ``c1_coax``:   a square coaxial line, 12 x 12 x 20 cells of 1 mm: PEC box
+  walls around a 4 x 4-cell PEC inner conductor along the whole length,
+  coaxial (TEM) ports on z_min / z_max (``PortSpecCoax``);
 
 port 1 mode 0 (c1_tm: mode 3) excited with the modulated Gaussian from its
-cut-off to 25 GHz, dt = courant_dt(grid, "normal"), float64 (and float32 for the
+cut-off to 25 GHz (the TEM channel of c1_coax: the plain Gaussian to
+25 GHz), dt = courant_dt(grid, "normal"), float64 (and float32 for the
 guide).  Stored: the ports' numerical mode profiles and cut-offs, alpha/beta
 (E, H) of the solver, e/h after steps 1, 10, 150 and 300 (reference flat
 layout), the solver's state_dict() after step 150 (flattened "a/b" keys),
@@ -75,6 +79,9 @@ def cases():
         # an L-shaped section: the lowest TM mode is channel 3 of 4
         "c1_tm": dict(axes=(np.linspace(0.0, A, 13), np.linspace(0.0, B, 9), z), loaded="tm",
                       precisions=("double", "single"), n_modes=4, excite=3),
+        # a square coax: TEM ports
+        "c1_coax": dict(axes=(np.linspace(0.0, COAX_W, 13), np.linspace(0.0, COAX_W, 13), z),
+                        loaded="coax", precisions=("double", "single"), n_modes=1, excite=0),
     }
 
 
@@ -87,6 +94,9 @@ MUR_BLOCK = (0.3 * A, 0.0, 2e-3, 0.7 * A, 0.5 * B, 5e-3)
 MUR_EPS = 2.5
 # the TM case: a PEC block over cells x 0-2, y 0-1, the whole length
 TM_CORNER = (0.0, 0.0, 0.0, 0.25 * A, 0.25 * B, 20e-3)
+# the coax case: a 12 mm square box, inner conductor over cells 4-7 in x and y
+COAX_W = 12e-3
+COAX_INNER = (4e-3, 4e-3, 0.0, 8e-3, 8e-3, 20e-3)
 
 
 def main(ref_src: str, only=None) -> None:
@@ -95,11 +105,11 @@ def main(ref_src: str, only=None) -> None:
     from magnelio.mesh.grid import GridLines
     from magnelio.mesh.mesher import Mesh
     from magnelio._operators.material_matrices import build_M_eps, build_M_mu
-    from magnelio.ports._modal import BoxFace, PortSpecNumerical, build_modal_port
+    from magnelio.ports._modal import BoxFace, PortSpecCoax, PortSpecNumerical, build_modal_port
     from magnelio.ports.recorder import PortSignalRecorder
     from magnelio.post.modal_sparameters import compute_s_parameters
     from magnelio.signals.signal_1d import Signal1D
-    from magnelio.signals.waveforms import modulated_gaussian
+    from magnelio.signals.waveforms import gaussian, modulated_gaussian
     from magnelio.solver.fit_td import FITTimeDomainSolver
     from magnelio.solver.stability import courant_dt
 
@@ -110,7 +120,9 @@ def main(ref_src: str, only=None) -> None:
         grid = GridLines(x=x, y=y, z=z)
         regions = []
         n_modes, excite = cfg.get("n_modes", 3), cfg.get("excite", 0)
-        if cfg["loaded"] == "tm":
+        if cfg["loaded"] == "coax":
+            regions = [(Material.pec(), COAX_INNER)]
+        elif cfg["loaded"] == "tm":
             regions = [(Material.pec(), TM_CORNER)]
         elif cfg["loaded"] == "mur":
             regions = [(Material(name="feed_block", epsilon=(MUR_EPS,) * 3), MUR_BLOCK)]
@@ -126,8 +138,13 @@ def main(ref_src: str, only=None) -> None:
             m_eps, m_mu = build_M_eps(mesh), build_M_mu(mesh)
             with warnings.catch_warnings():
                 warnings.simplefilter("ignore")
-                ops = [build_modal_port(PortSpecNumerical(name=lbl, plane=face, n_modes=n_modes), mesh,
-                                        m_eps, m_mu, dt=dt, f_calc=F_MAX)
+                def spec(lbl, face):
+                    if cfg["loaded"] == "coax":
+                        return PortSpecCoax(name=lbl, plane=face, inner_radius=2e-3, outer_radius=6e-3,
+                                            center=(0.5 * COAX_W, 0.5 * COAX_W), n_modes=n_modes)
+                    return PortSpecNumerical(name=lbl, plane=face, n_modes=n_modes)
+
+                ops = [build_modal_port(spec(lbl, face), mesh, m_eps, m_mu, dt=dt, f_calc=F_MAX)
                        for lbl, face in (("port1", BoxFace.Z_MIN), ("port2", BoxFace.Z_MAX))]
             f_c = ops[0].discrete_modes[excite].mode.omega_c / (2 * math.pi)
             if prec == "double":  # the ports' numerical modes (the oracle takes them as given)
@@ -137,6 +154,14 @@ def main(ref_src: str, only=None) -> None:
                         out[f"{op.name}_e_v{m}"] = np.asarray(dm.e_v_profile, float)
                         out[f"{op.name}_omega_c{m}"] = np.float64(dm.mode.omega_c)
                         out[f"{op.name}_type{m}"] = np.array(dm.mode.mode_type.value)
+                        if dm.mode.z_line is not None:
+                            out[f"{op.name}_z_line{m}"] = np.float64(dm.mode.z_line)
+                    for k in ("record_scale", "state_scale"):
+                        out[f"{op.name}_{k}"] = np.asarray(getattr(op, k), float)
+                    out[f"{op.name}_source_scale"] = np.asarray(op._source_scale, float)
+                    out[f"{op.name}_dtbc"] = np.array([[np.nan if v is None else v for v in
+                                                        (op._dtbc_r[m], op._dtbc_q[m], op._dtbc_z0[m])]
+                                                       for m in range(len(op.discrete_modes))])
                     out[f"{op.name}_mur"] = np.array([t is None for t in op._dtbc])
                     out[f"{op.name}_mur_r"] = np.asarray(op._mur_r, float)
                     if op._comp_r_u is not None:
@@ -144,7 +169,7 @@ def main(ref_src: str, only=None) -> None:
                             out[f"{op.name}_comp_{k}"] = np.asarray(getattr(op, f"_comp_{k}"), float)
 
             def wf(t, lo=f_c):
-                return modulated_gaussian(t, F_MAX, lo)
+                return gaussian(t, F_MAX) if lo <= 0.0 else modulated_gaussian(t, F_MAX, lo)
 
             ops[0].set_excitation(excite, wf)
             rec = PortSignalRecorder(dt=dt, ports=ops)
@@ -170,7 +195,7 @@ def main(ref_src: str, only=None) -> None:
             for (lbl, m), (V, I) in sigs.items():
                 out[f"{p}V_{lbl}_{m}"] = np.asarray(V.values, float)
                 out[f"{p}I_{lbl}_{m}"] = np.asarray(I.values, float)
-            f_axis = np.linspace(1.1 * f_c, 24e9, 41)
+            f_axis = np.linspace(1.1 * f_c if f_c > 0 else 1e9, 24e9, 41)
             t_axis = np.arange(SNAPS[-1]) * dt
             ref = Signal1D(t=t_axis, values=np.array([wf(float(t)) for t in t_axis]), dt=dt,
                            label="excitation")

@@ -26,7 +26,7 @@ import cases
 from magnelio_amd import mesh as pm
 from oracle import sparams as osp
 
-CASES = ("c1_guide", "c1_loaded", "c1_mur", "c1_tm")
+CASES = ("c1_guide", "c1_loaded", "c1_mur", "c1_tm", "c1_coax")
 
 
 def _product_coefficients(name):
@@ -64,7 +64,8 @@ def test_oracle_coefficients_equal_reference(name):
 def _sparams(ports, dt, f_axis, excited=("port1", 0)):
     sig = {(p.name, m): (np.array(p.V[m]), np.array(p.I[m])) for p in ports for m in range(p.n_modes)}
     lines = {(p.name, m): p.line[m] for p in ports for m in range(p.n_modes) if m in p.line}
-    modes = {(p.name, m): (md["wc"], md["eps_r"], md["kind"]) for p in ports for m, md in enumerate(p.modes)}
+    modes = {(p.name, m): (md["wc"], md["eps_r"], md["kind"], md["z_line"]) for p in ports
+             for m, md in enumerate(p.modes)}
     ndx = {p.name: p.pl["normal_dx"] for p in ports}
     return osp.s_params(sig, lines, excited, f_axis, dt, modes=modes, normal_dx=ndx)
 
@@ -73,11 +74,15 @@ def _sparams(ports, dt, f_axis, excited=("port1", 0)):
 def test_oracle_run_matches_reference(name):
     f = cases.golden_c1(name)
     orc, ports, dt, _ = cases.golden_c1_oracle(name)
+    # relative to the snapshot's norm, or to a tenth of the run's largest
+    # one when the field has decayed below that (the coax pulse has left its
+    # line by step 300: field 1e-12 of its peak)
+    peak = {c: max(np.linalg.norm(f[f"double_{c}{n}"]) for n in f["snaps"]) for c in "eh"}
     for n in f["snaps"]:
         orc.run(int(n) - orc.n)
         for comp, got in (("e", orc.e), ("h", orc.h)):
             want = f[f"double_{comp}{n}"]
-            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            err = np.linalg.norm(got - want) / max(np.linalg.norm(want), 0.1 * peak[comp])
             assert err <= 1e-12, (n, comp, err)
     # channels that carry no mode stay at round-off: gate every channel
     # against the largest V (I) of the run
@@ -142,7 +147,7 @@ def test_oracle_checkpoint_complement_state_equals_reference():
         assert np.max(np.abs(got - want)) <= 1e-12 * np.abs(want).max(), k
 
 
-@pytest.mark.parametrize("name", ["c1_guide", "c1_mur", "c1_tm"])
+@pytest.mark.parametrize("name", ["c1_guide", "c1_mur", "c1_tm", "c1_coax"])
 def test_oracle_float32_matches_reference_float32(name):
     f = cases.golden_c1(name)
     orc, _, _, _ = cases.golden_c1_oracle(name, np.float32)
@@ -150,5 +155,36 @@ def test_oracle_float32_matches_reference_float32(name):
         orc.run(int(n) - orc.n)
         for comp, got in (("e", orc.e), ("h", orc.h)):
             want = f[f"single_{comp}{n}"]
+            if name == "c1_coax":
+                # the float64 port projections of the TEM profile differ from
+                # the reference's NumPy dots in the last bit; once the pulse
+                # has left the line (step 300) such a flip shows in a few
+                # float32 words of the 1e-9 residue
+                assert np.abs(got.astype(np.float64) - want).max() <= 1e-6 * np.abs(want).max(), (n, comp)
+                continue
             # same operation order in float32: bit for bit
             assert np.array_equal(got.astype(np.float64), want), (n, comp)
+
+
+def test_product_coax_port_matches_reference():
+    """The product's coaxial port (conductors from the mesh, Laplace TEM
+    channel, line impedance, calibration, q = 0 chain) on c1_coax equals the
+    reference's PortSpecCoax build."""
+    f = cases.golden_c1("c1_coax")
+    mesh = cases.golden_c1_mesh("c1_coax")
+    dt = pm.courant_dt(mesh.grid, "normal")
+    me, mu = pm.build_m_eps(mesh), pm.build_m_mu(mesh)
+    for label, face in (("port1", "zmin"), ("port2", "zmax")):
+        P = cases.golden_port("c1_coax", label, face, mesh, me, mu, dt, 1)
+        (md,) = P.modes
+        assert md.mode_type == "TEM" and md.omega_c == 0.0
+        assert md.z_line == pytest.approx(float(f[f"{label}_z_line0"]), rel=1e-12)
+        sc = max(np.abs(f[f"{label}_e_u0"]).max(), np.abs(f[f"{label}_e_v0"]).max())
+        np.testing.assert_allclose(md.e_u, f[f"{label}_e_u0"], rtol=0, atol=1e-12 * sc)
+        np.testing.assert_allclose(md.e_v, f[f"{label}_e_v0"], rtol=0, atol=1e-12 * sc)
+        for k in ("record_scale", "source_scale", "state_scale"):
+            np.testing.assert_allclose(getattr(P, k), f[f"{label}_{k}"], rtol=1e-12, err_msg=k)
+        r, q, z0 = f[f"{label}_dtbc"][0]
+        assert P.dtbc_r[0] == pytest.approx(r, rel=1e-12) and P.dtbc_q[0] == q == 0.0
+        assert P.dtbc_z0[0] is None and np.isnan(z0)
+        np.testing.assert_allclose(P.mur_r, f[f"{label}_mur_r"], rtol=1e-12)
