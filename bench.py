@@ -40,6 +40,62 @@ HBM_PEAK_GBS = 8000.0
 K_UPDATE_E_TRAFFIC = {(256, 1, True): 1.308e9}
 
 
+def stream_ceiling(n: int, n_read: int, n_write: int, reps: int = 20) -> dict:
+    """The device's streaming rate for a curl launch's access mix: n_read
+    input and n_write output arrays of n float32 words, one element per lane
+    and nothing else in the kernel (mgh_stream_probe), median of `reps`
+    launches timed with HIP events on the current stream."""
+    import torch
+
+    from magnelio_amd import _lib
+
+    src = torch.ones(n_read * n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty(n_write * n, dtype=torch.float32, device="cuda:0")
+    st = torch.cuda.current_stream()
+    sh = int(st.cuda_stream)
+
+    def launch():
+        _lib.call("mgh_stream_probe", n_read, n_write, src.data_ptr(), dst.data_ptr(), n, sh)
+
+    for _ in range(3):
+        launch()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record(st)
+        launch()
+        b.record(st)
+    torch.cuda.synchronize()
+    us = float(np.median([a.elapsed_time(b) * 1e3 for a, b in ev]))
+    byt = 4.0 * n * (n_read + n_write)
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"reads": n_read, "writes": n_write, "elements": n, "bytes": byt, "us": us,
+            "GBps": byt / us / 1e3, "frac_of_peak": byt / us / 1e3 / HBM_PEAK_GBS}
+
+
+def curl_launch_times(sol, reps: int = 20) -> tuple[float, float]:
+    """Median HIP-event times (us) of the solver's E and H curl launches on
+    its current state (the fields change; call after the timed steps)."""
+    import torch
+
+    from magnelio_amd import kernels
+
+    st = torch.cuda.current_stream()
+    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(reps)]
+    for a, b, c in ev:
+        a.record(st)
+        kernels.update_e(sol.e, sol.h, sol.alpha_e, sol.beta_e, sol.dims, curl64=sol.curl64,
+                         kchunk=sol.kchunk, stream=st)
+        b.record(st)
+        kernels.update_h(sol.e, sol.h, sol.alpha_h, sol.beta_h, sol.dims, kchunk=sol.kchunk,
+                         stream=st)
+        c.record(st)
+    torch.cuda.synchronize()
+    return (float(np.median([a.elapsed_time(b) * 1e3 for a, b, _ in ev])),
+            float(np.median([b.elapsed_time(c) * 1e3 for _, b, c in ev])))
+
+
 def c2_mesh(n: int = 256, seed: int = 2024, n_objects: int = 40, length: float = 64e-3):
     """Seeded non-uniform n^3 grid (adjacent spacing ratio <= 1.5) + objects."""
     from magnelio_amd.mesh import GridLines, Material, Mesh
@@ -680,6 +736,7 @@ def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plai
     t_b, ok = timed(sol)
     samples = sol.monitors[0].S if sol.monitors else 0
     bins_gb = sol.monitors[0].bytes() / 1e9 if sol.monitors else 0.0
+    e_us, h_us = curl_launch_times(sol)
     del sol
     torch.cuda.empty_cache()
     t_s = 0.0
@@ -704,6 +761,17 @@ def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plai
            "K": K, "value": rate_b, "unit": "Gcell-updates/s", "ms_per_step": t_b / steps * 1e3,
            "steps": steps, "warmup": warmup + extra["steps"],
            "roofline_gcell": roof, "frac_of_roofline": rate_b / roof, "dft_bins_gb": round(bins_gb, 2)}
+    # the K-set curl launches alone: algorithmic bytes 4 (6 + 9 K) per cell
+    alg = 4.0 * cells * (6 + 9 * K)
+    curl = {"e_us": e_us, "h_us": h_us, "alg_bytes_per_launch": alg,
+            "frac_of_peak_e": alg / e_us / 1e3 / HBM_PEAK_GBS,
+            "frac_of_peak_h": alg / h_us / 1e3 / HBM_PEAK_GBS}
+    if K == 6:
+        ceil = stream_ceiling(cells, 42, 18)
+        curl["stream_ceiling"] = ceil
+        curl["frac_of_stream_ceiling_e"] = ceil["us"] / e_us
+        curl["frac_of_stream_ceiling_h"] = ceil["us"] / h_us
+    out["curl_launches"] = curl
     if single_path:
         rate_s = K * cells * steps / t_s / 1e9
         out["single_excitation_path"] = {"value": rate_s, "ms_per_step_all_K": t_s / steps * 1e3,
@@ -873,6 +941,11 @@ def main(argv=None) -> int:
             "cpml_psi_bytes_per_launch": psi_e,
         },
     }
+    if K == 1:
+        # what the device streams for this launch's algorithmic words alone
+        ceil = stream_ceiling(cells, 12, 3)
+        out["roofline"]["stream_ceiling"] = ceil
+        out["roofline"]["frac_of_stream_ceiling"] = achieved / ceil["GBps"]
     coeffs = sol._coeffs  # float64 host alpha/beta (host-built: the mesh has a TF/SF source)
     del sol, monitors, mon
     torch.cuda.empty_cache()

@@ -39,6 +39,16 @@ int mgh_version(void);
 int mgh_device_info(char* name, int name_len, int* cu_count, long long* hbm_bytes, char* arch,
                     int arch_len);
 
+/* HBM streaming probe (no counterpart in the reference: a measurement aid
+ * for callers that report the curl launches against what the device can
+ * stream).  Streams n_read input arrays and n_write output arrays of n floats
+ * (in: the inputs back to back, out: the outputs back to back) the way a curl
+ * launch streams its algorithmic words, with nothing else in the kernel:
+ * out[w][i] = sum of in[r][i] over r = w mod n_write.  Mixes: 12 + 3 (the
+ * E or H launch of one field set) and 42 + 18 (six sets). */
+int mgh_stream_probe(int n_read, int n_write, const void* in, void* out, long long n,
+                     void* stream);
+
 /* ---- (a3) E phase --------------------------------------------------------
  * Replaces update_E_fused_cuda / update_E_fused / update_E_stencil
  * (src/magnelio/_operators/numba_kernels.py:518-536, :36-93, :137-171),

@@ -31,6 +31,7 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_char_p, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_ll), _c_char_p, _c_int],
     ),
+    "mgh_stream_probe": (_c_int, [_c_int, _c_int, _c_void_p, _c_void_p, _c_ll, _c_void_p]),
     "mgh_update_e": (
         _c_int,
         [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p,

@@ -3,7 +3,7 @@
 The tunable is the z-chunk of the curl kernels (planes a thread marches,
 keeping the z neighbour in registers): shorter chunks give more work-groups,
 longer ones fewer re-read chunk-start planes.  The sets per thread (two, or
-three for odd multiples of 3) and the XCD band mapping are fixed in the
+three for multiples of 3) and the XCD band mapping are fixed in the
 library from measurements; the chunk depends on the grid, K and the CPML the
 launch carries, so it is measured per configuration.
 

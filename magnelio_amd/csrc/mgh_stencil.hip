@@ -35,8 +35,8 @@
 // are read once for the three updates.  Nodes are numbered x-fastest, 256 to
 // a work-group (four 64-wide waves over consecutive nodes), which keeps every
 // wave's loads on contiguous rows and leaves no partly empty tile at i = Nx.
-// With two or more field sets a thread advances two sets (three when K is an
-// odd multiple of 3), so the coefficient words serve all of them, and bands of
+// With two or more field sets a thread advances two sets (three when K is a
+// multiple of 3), so the coefficient words serve all of them, and bands of
 // consecutive units — the set groups of one tile first, then the next tiles —
 // run on one XCD and share its L2 (BAND below).
 //
@@ -333,75 +333,76 @@ __global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __r
       if (iny) ey = es[iEy];
       if (knz) ez = es[iEz];
 
+      // All three components are computed before the first store (each
+      // store ahead of a later component's arithmetic held the wave on it).
+      T vx = T(0), vy = T(0), vz = T(0);
       if (inx) {  // Ex(i,j,k): dHz/dy - dHy/dz
-        T v;
         if constexpr (CURL64) {
           double c = 0.0;
           if (iny) c += (double)hz;
           if (j > 0) c -= (double)hz_jm;
           if (knz) c -= (double)hy;
           if (k > 0) c += (double)hy_prev[q];
-          v = (T)((double)(a_x * ex) + (double)b_x * c);
+          vx = (T)((double)(a_x * ex) + (double)b_x * c);
         } else {
           T c = T(0);
           if (iny) c += hz;
           if (j > 0) c -= hz_jm;
           if (knz) c -= hy;
           if (k > 0) c += hy_prev[q];
-          v = (a_x * ex) + (b_x * c);
+          vx = (a_x * ex) + (b_x * c);
         }
-        if constexpr (PML) {  // Ex: z faces d = Hy(k) - Hy(k-1) (+), y faces Hz(j) - Hz(j-1) (-)
-          if (hit) pml_comp<T, 2, 0, +1, 1, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q,
-                                                   hy - hy_prev[q], hz - hz_jm, b_x, &v);
-        }
-        es[iEx] = v;
       }
       if (iny) {  // Ey(i,j,k): dHx/dz - dHz/dx
-        T v;
         if constexpr (CURL64) {
           double c = 0.0;
           if (knz) c += (double)hx;
           if (k > 0) c -= (double)hx_prev[q];
           if (inx) c -= (double)hz;
           if (i > 0) c += (double)hz_im;
-          v = (T)((double)(a_y * ey) + (double)b_y * c);
+          vy = (T)((double)(a_y * ey) + (double)b_y * c);
         } else {
           T c = T(0);
           if (knz) c += hx;
           if (k > 0) c -= hx_prev[q];
           if (inx) c -= hz;
           if (i > 0) c += hz_im;
-          v = (a_y * ey) + (b_y * c);
+          vy = (a_y * ey) + (b_y * c);
         }
-        if constexpr (PML) {  // Ey: z faces d = Hx(k) - Hx(k-1) (-), x faces Hz(i) - Hz(i-1) (+)
-          if (hit) pml_comp<T, 2, 1, -1, 0, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q,
-                                                   hx - hx_prev[q], hz - hz_im, b_y, &v);
-        }
-        es[iEy] = v;
       }
       if (knz) {  // Ez(i,j,k): dHy/dx - dHx/dy
-        T v;
         if constexpr (CURL64) {
           double c = 0.0;
           if (inx) c += (double)hy;
           if (i > 0) c -= (double)hy_im;
           if (iny) c -= (double)hx;
           if (j > 0) c += (double)hx_jm;
-          v = (T)((double)(a_z * ez) + (double)b_z * c);
+          vz = (T)((double)(a_z * ez) + (double)b_z * c);
         } else {
           T c = T(0);
           if (inx) c += hy;
           if (i > 0) c -= hy_im;
           if (iny) c -= hx;
           if (j > 0) c += hx_jm;
-          v = (a_z * ez) + (b_z * c);
+          vz = (a_z * ez) + (b_z * c);
         }
-        if constexpr (PML) {  // Ez: y faces d = Hx(j) - Hx(j-1) (+), x faces Hy(i) - Hy(i-1) (-)
-          if (hit) pml_comp<T, 1, 1, +1, 0, 1, -1>(pa, nx + 1, ny + 1, i, j, k, s0 + q,
-                                                   hx - hx_jm, hy - hy_im, b_z, &v);
-        }
-        es[iEz] = v;
       }
+      if constexpr (PML) {
+        if (hit) {
+          // Ex: z faces d = Hy(k) - Hy(k-1) (+), y faces Hz(j) - Hz(j-1) (-)
+          if (inx) pml_comp<T, 2, 0, +1, 1, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q,
+                                                   hy - hy_prev[q], hz - hz_jm, b_x, &vx);
+          // Ey: z faces d = Hx(k) - Hx(k-1) (-), x faces Hz(i) - Hz(i-1) (+)
+          if (iny) pml_comp<T, 2, 1, -1, 0, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q,
+                                                   hx - hx_prev[q], hz - hz_im, b_y, &vy);
+          // Ez: y faces d = Hx(j) - Hx(j-1) (+), x faces Hy(i) - Hy(i-1) (-)
+          if (knz) pml_comp<T, 1, 1, +1, 0, 1, -1>(pa, nx + 1, ny + 1, i, j, k, s0 + q,
+                                                   hx - hx_jm, hy - hy_im, b_z, &vz);
+        }
+      }
+      if (inx) es[iEx] = vx;
+      if (iny) es[iEy] = vy;
+      if (knz) es[iEz] = vz;
       hy_prev[q] = hy;
       hx_prev[q] = hx;
     }
@@ -480,33 +481,35 @@ __global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __r
         ex_jp = es[(int64_t)k * pEx + rN + nx];
         hzv = hs[iHz];
       }
+      T vx = T(0), vy = T(0), vz = T(0);
       if (hasHx) {  // Hx(i,j,k): dEz/dy - dEy/dz
         const T c = ((ez_jp - ez) - ey_kp) + ey_k[q];
-        T v = (a_x * hxv) - (b_x * c);
-        if constexpr (PML) {  // Hx: z faces d = Ey(k+1) - Ey(k) (-), y faces Ez(j+1) - Ez(j) (+)
-          if (hit) pml_comp<T, 2, 0, -1, 1, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q,
-                                                   ey_kp - ey_k[q], ez_jp - ez, b_x, &v);
-        }
-        hs[iHx] = v;
+        vx = (a_x * hxv) - (b_x * c);
       }
       if (hasHy) {  // Hy(i,j,k): dEx/dz - dEz/dx
         const T c = ((ex_kp - ex_k[q]) - ez_ip) + ez;
-        T v = (a_y * hyv) - (b_y * c);
-        if constexpr (PML) {  // Hy: z faces d = Ex(k+1) - Ex(k) (+), x faces Ez(i+1) - Ez(i) (-)
-          if (hit) pml_comp<T, 2, 1, +1, 0, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q,
-                                                   ex_kp - ex_k[q], ez_ip - ez, b_y, &v);
-        }
-        hs[iHy] = v;
+        vy = (a_y * hyv) - (b_y * c);
       }
       if (hasHz) {  // Hz(i,j,k): dEy/dx - dEx/dy
         const T c = ((ey_ip - ey_k[q]) - ex_jp) + ex_k[q];
-        T v = (a_z * hzv) - (b_z * c);
-        if constexpr (PML) {  // Hz: y faces d = Ex(j+1) - Ex(j) (-), x faces Ey(i+1) - Ey(i) (+)
-          if (hit) pml_comp<T, 1, 1, -1, 0, 1, +1>(pa, nx, ny, i, j, k, s0 + q,
-                                                   ex_jp - ex_k[q], ey_ip - ey_k[q], b_z, &v);
-        }
-        hs[iHz] = v;
+        vz = (a_z * hzv) - (b_z * c);
       }
+      if constexpr (PML) {
+        if (hit) {
+          // Hx: z faces d = Ey(k+1) - Ey(k) (-), y faces Ez(j+1) - Ez(j) (+)
+          if (hasHx) pml_comp<T, 2, 0, -1, 1, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q,
+                                                     ey_kp - ey_k[q], ez_jp - ez, b_x, &vx);
+          // Hy: z faces d = Ex(k+1) - Ex(k) (+), x faces Ez(i+1) - Ez(i) (-)
+          if (hasHy) pml_comp<T, 2, 1, +1, 0, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q,
+                                                     ex_kp - ex_k[q], ez_ip - ez, b_y, &vy);
+          // Hz: y faces d = Ex(j+1) - Ex(j) (-), x faces Ey(i+1) - Ey(i) (+)
+          if (hasHz) pml_comp<T, 1, 1, -1, 0, 1, +1>(pa, nx, ny, i, j, k, s0 + q,
+                                                     ex_jp - ex_k[q], ey_ip - ey_k[q], b_z, &vz);
+        }
+      }
+      if (hasHx) hs[iHx] = vx;
+      if (hasHy) hs[iHy] = vy;
+      if (hasHz) hs[iHz] = vz;
       ey_k[q] = ey_kp;
       ex_k[q] = ex_kp;
     }
@@ -956,15 +959,16 @@ __global__ __launch_bounds__(WG) void k_cpml_xcols_h(const T* __restrict__ e, T*
 // host side
 
 // field sets per thread: each coefficient word a thread loads serves them
-// all.  Two for K >= 2; three when K is an odd multiple of 3, where pairs
-// would leave one group half empty (K = 3 on 192x96x288: 158 against
-// 181 us per E launch).  Three for every K, or six, measured slower at
-// K = 6 (303 / 471 against 297 us): the extra registers cost occupancy.
+// all.  Three when K is a multiple of 3 (K = 3 on 192x96x288: 158 against
+// 181 us per E launch with pairs, which leave one group half empty; K = 6:
+// E / H 266-275 / 277-281 us against 268-277 / 273-288 with pairs, each
+// component computed before the first store), else two.  Six at K = 6:
+// 372 / 321 us (140-153 VGPRs, 3 waves per SIMD).
 constexpr int kSetsPerThread = 2;
 
 inline int sets_per_thread(int nsets) {
   if (nsets < 2) return 1;
-  return (nsets % 3 == 0 && nsets % 2 == 1) ? 3 : kSetsPerThread;
+  return nsets % 3 == 0 ? 3 : kSetsPerThread;
 }
 
 // a0 < a1: split the planes at the interior columns [a0, a1) (see Plan)
