@@ -989,6 +989,7 @@ def main_c3(args) -> int:
                    "wall_s": round(time.perf_counter() - t_setup, 1)},
         "roofline_gcell": r["roofline_gcell"],
         "frac_of_roofline": r["frac_of_roofline"],
+        "curl_launches": r["curl_launches"],
         "single_excitation_path": r["single_excitation_path"],
     }
     print(json.dumps(out))

@@ -44,7 +44,7 @@ import scipy.sparse as sp
 from scipy.sparse.linalg import eigsh
 
 from . import mesh as _mesh
-from .dtbc import packed_kernels
+from .dtbc import N_KERNEL_INIT, packed_kernels
 
 C0, MU0, EPS0 = _mesh.C0, _mesh.MU0, _mesh.EPS0
 ETA0 = math.sqrt(MU0 / EPS0)
@@ -565,10 +565,10 @@ class PortModel:
         """(M, n_u + n_v) H profiles co-located with [u | v] (h_v | h_u)."""
         return np.stack([np.concatenate([m.h_v, m.h_u]) for m in self.modes])
 
-    def packed_dtbc_kernels(self, n_steps: int) -> np.ndarray:
+    def packed_dtbc_kernels(self, n_steps: int, init: int = N_KERNEL_INIT) -> np.ndarray:
         rows = []
         for r, q in zip(self.dtbc_r, self.dtbc_q):
-            rows.append(packed_kernels(r, q, n_steps) if r is not None else None)
+            rows.append(packed_kernels(r, q, n_steps, init) if r is not None else None)
         length = max((a.size for a in rows if a is not None), default=0)
         out = np.zeros((len(rows), length))
         for i, a in enumerate(rows):

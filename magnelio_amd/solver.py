@@ -44,7 +44,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import _lib, autotune, kernels, layout
+from . import _lib, autotune, dtbc, kernels, layout
 from . import mesh as _mesh
 from .dispersion import DeviceDispersion
 from .sibc import DeviceSIBC
@@ -126,7 +126,13 @@ class DevicePorts:
     WORDS = 32
 
     def __init__(self, ports: list[PortModel], dims, nsets: int, n_steps: int, device,
-                 excitations: list[ChannelExcitation | None], dt: float, record: bool = True):
+                 excitations: list[ChannelExcitation | None], dt: float, record: bool = True,
+                 kernel_init: int = dtbc.N_KERNEL_INIT):
+        """kernel_init: length of the first DTBC kernel version; version v
+        (length kernel_init * 2^v) is in force while the step index is below
+        it (the reference's schedule, dtbc.py:317-363)."""
+        if kernel_init < 2:
+            raise ValueError(f"kernel_init must be >= 2, got {kernel_init}")
         self.ports = ports
         self.nsets = nsets
         self.n_steps = n_steps
@@ -161,7 +167,7 @@ class DevicePorts:
                     params[m, 5] = 2.0 - 2.0 * r2 - q2
                 params[m, 3] = P.mur_r[m]
                 params[m, 4] = P.record_scale[m]
-            kern = P.packed_dtbc_kernels(max(n_steps, 1))
+            kern = P.packed_dtbc_kernels(max(n_steps, 1), kernel_init)
             state = dev(np.zeros((nsets, M, 4)))
             cap = max(n_steps, 1)
             any_dtbc = any(r is not None for r in P.dtbc_r)
@@ -205,7 +211,7 @@ class DevicePorts:
             w[9], w[10], w[11] = t_pe.data_ptr(), t_ph.data_ptr(), t_par.data_ptr()
             w[12] = t_kern.data_ptr() if t_kern is not None else 0
             w[13] = kern.shape[1] if kern.size else 0
-            w[14] = 4096
+            w[14] = kernel_init
             w[15], w[16], w[17] = state.data_ptr(), w_hist.data_ptr(), s_hist.data_ptr()
             w[18] = w_hist.shape[2] if any_dtbc else 0
             w[19] = t_exc.data_ptr()
@@ -297,6 +303,7 @@ class FITTimeDomainSolver:
     kchunk: int | str = kernels.DEFAULT_KCHUNK  # z planes per thread, or "auto" (autotune.py)
     use_graphs: bool = False
     graph_block: int = 1  # minimum steps per captured graph
+    dtbc_kernel_init: int = dtbc.N_KERNEL_INIT  # first DTBC kernel length (then doubled)
     coefficients: str = "auto"  # alpha/beta built on the "device", the "host", or "auto"
     tile_skip: str = "auto"  # dead-unit skipping: "auto" (when it skips >= 2 %), "on", "off"
     device: str = "cuda:0"
@@ -518,7 +525,8 @@ class FITTimeDomainSolver:
                      if cpml_faces else None)
         n_steps = self.horizon
         self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
-        self.dports = (DevicePorts(self.ports, dims, K, n_steps, dev, self.excitations, dt, self.record)
+        self.dports = (DevicePorts(self.ports, dims, K, n_steps, dev, self.excitations, dt, self.record,
+                                   self.dtbc_kernel_init)
                        if self.ports else None)
         for src in self.sources:
             src.attach(self)

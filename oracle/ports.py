@@ -260,12 +260,13 @@ class Port:
     """Oracle modal port: calibrated TE / TM channels, Mur/DTBC termination."""
 
     def __init__(self, name, pl, modes, m_eps, m_mu, dt, omega_calc, eps_r=1.0, termination="auto",
-                 complement=None):
+                 complement=None, kernel_init=None):
         """modes: list of (e_u, e_v, omega_c) from :func:`rect_te_mode`, or
         (e_u, e_v, omega_c, "TE" | "TM") (:func:`rect_tm_mode` is "TM");
         termination "auto" (exact DTBC) or "mur" (modal Mur-1 everywhere);
         complement: (r_u, r_v, live_u, live_v) of :func:`complement_arrays`
-        or None."""
+        or None; kernel_init: first DTBC kernel length (None: the
+        reference's 4096)."""
         self.name, self.pl, self.dt = name, pl, dt
         self.comp = None if complement is None else tuple(np.asarray(a, float) for a in complement)
         self.comp_int_u = np.zeros(pl["e_u"].size)
@@ -340,7 +341,8 @@ class Port:
             if termination == "mur":
                 self.dtbc.append(None)
             else:
-                self.dtbc.append(DTBC(r, q))
+                self.dtbc.append(DTBC(r, q) if kernel_init is None
+                                 else DTBC(r, q, n_kernel_init=kernel_init))
                 # no static z0 on a massless (q = 0) chain: its S-parameters
                 # take the channel's own Z (operator.py:666-668)
                 self.line[mi] = (r, q, z0 if q > 0.0 else None)

@@ -53,7 +53,7 @@ def oracle_complement(pl, x, y, z, mid, me_flat, pec_mesh, dt):
 
 
 def c1_oracle(dtype=np.float64, dims=C1_DIMS, size=C1_SIZE, excite=("port1", 0), order="stencil",
-              termination="auto", band=None):
+              termination="auto", band=None, kernel_init=None):
     """Oracle side: coefficients, two ports, excitation; returns (Oracle, ports, dt)."""
     x, y, z = c1_axes(dims, size)
     nx, ny, nz = dims
@@ -75,7 +75,8 @@ def c1_oracle(dtype=np.float64, dims=C1_DIMS, size=C1_SIZE, excite=("port1", 0),
         pl = op.plane(face, x, y, z)
         modes = [op.rect_te_mode(pl, x, y, z, m, n, me) for m, n in C1_MODES]
         ports.append(op.Port(name, pl, modes, me, mm, dt, omega_calc, termination=termination,
-                             complement=oracle_complement(pl, x, y, z, mid, me, pec_mesh, dt)))
+                             complement=oracle_complement(pl, x, y, z, mid, me, pec_mesh, dt),
+                             kernel_init=kernel_init))
     if excite is not None:
         p = ports[0] if excite[0] == "port1" else ports[1]
         f_lo, f_hi = band if band is not None else (p.modes[excite[1]]["wc"] / (2 * math.pi), C1_FMAX)
@@ -100,7 +101,8 @@ def c1_mesh_ports(dims=C1_DIMS, size=C1_SIZE, termination="auto"):
 
 
 def c1_solver(dims=C1_DIMS, size=C1_SIZE, excitations=(("port1", 0),), n_steps=4000,
-              precision="single", use_graphs=False, termination="auto", curl64=False, monitors=()):
+              precision="single", use_graphs=False, termination="auto", curl64=False, monitors=(),
+              kernel_init=None):
     """Product side: mesh, ports, batched HIP solver (one set per excitation)."""
     from magnelio_amd import mesh as mm
     from magnelio_amd import ports as pp
@@ -126,6 +128,8 @@ def c1_solver(dims=C1_DIMS, size=C1_SIZE, excitations=(("port1", 0),), n_steps=4
                               total_time_steps=n_steps, dt=dt, precision=precision,
                               nsets=len(exc), excitations=exc, use_graphs=use_graphs,
                               curl64=curl64, monitors=list(monitors))
+    if kernel_init is not None:
+        sol.dtbc_kernel_init = kernel_init
     return sol, ports, dt
 
 

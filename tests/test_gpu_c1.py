@@ -161,6 +161,35 @@ def test_c1_fp64_vs_oracle(c1_oracle64, gpu):
             assert np.max(np.abs(V - ref_v)) <= 1e-9 * np.max(np.abs(oports[0].V[0]))
 
 
+def test_dtbc_kernel_versions_vs_oracle(gpu):
+    """The DTBC kernel schedule on the device: with a first kernel length of
+    64 the run crosses the switches to 128, 256 and 512 within 300 steps
+    (the default 4096 is first crossed at step 4096, in runs longer than C1);
+    the FP64 fields and port voltages follow the oracle's DTBC with the same
+    schedule (dtbc.py:317-363).  A history term past a version's length read
+    from the wrong version block would break the convolution; kernels of
+    different lengths agree to ~1e-14 in their common terms, so the schedule
+    itself does not show in the fields."""
+    steps, init = 300, 64
+    orc, oports, _, _ = cases.c1_oracle(np.float64, kernel_init=init)
+    orc.run(steps)
+    sol, _, _ = cases.c1_solver(n_steps=steps, precision="double", kernel_init=init)
+    sol.run()
+    e, h = sol.fields(0)
+    err_e, err_h = _rel_l2(e, orc.e), _rel_l2(h, orc.h)
+    print(f"C1 fp64, DTBC kernel init {init}: rel L2 e {err_e:.3e} h {err_h:.3e}", flush=True)
+    assert err_e <= 1e-10 and err_h <= 1e-10
+    sig = sol.recorded(0)
+    scale = np.max(np.abs(oports[0].V[0]))
+    for p in oports:
+        for m in range(p.n_modes):
+            V, _ = sig[(p.name, m)]
+            assert np.max(np.abs(V - np.array(p.V[m]))) <= 1e-9 * scale
+    # the device table holds the four versions 64 + 128 + 256 + 512
+    words = sol.dports.desc.reshape(len(oports), -1)
+    assert all(int(w[14]) == init and int(w[13]) == 960 for w in words)
+
+
 def test_batched_k6_equals_single_runs(gpu):
     exc = [("port1", 0), ("port1", 1), ("port1", 2), ("port2", 0), ("port2", 1), ("port2", 2)]
     sol, _, _ = cases.c1_solver(excitations=exc, n_steps=500)

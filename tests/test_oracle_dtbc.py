@@ -114,3 +114,23 @@ def test_terminated_chain_is_transparent():
     err = np.max(np.abs(short - ref)) / np.max(np.abs(ref))
     assert err < 1e-5  # -100 dB
     del long_ref
+
+
+def test_product_packed_kernels_follow_the_schedule():
+    """The device's packed kernel table: version v (length init * 2^v) at
+    offset init (2^v - 1), each equal to the oracle's kernel of that length,
+    enough versions for the run; a first length below 2 is refused."""
+    from magnelio_amd import dtbc as pd
+    from magnelio_amd.solver import DevicePorts
+
+    r, q, init, steps = 0.61, 0.03, 64, 300
+    packed = pd.packed_kernels(r, q, steps, init)
+    lengths = pd.kernel_versions(steps, init)
+    assert lengths == [64, 128, 256, 512]
+    off = 0
+    for n in lengths:
+        np.testing.assert_allclose(packed[off: off + n], od.dtbc_kernel(r, q, n), rtol=0, atol=1e-15)
+        off += n
+    assert packed.size == off
+    with pytest.raises(ValueError, match="kernel_init"):
+        DevicePorts([], (4, 4, 4), 1, 10, "cpu", [None], 1e-12, kernel_init=1)
