@@ -276,7 +276,7 @@ def _c5_assemble(parts):
 
 
 def c5_jobs(n_jobs=200, seed=5, rate=1.5, cells=(32 ** 3, 384 ** 3), steps=(2000, 20000),
-            step_scale=1.0, f_max=25e9, n_freq=101):
+            step_scale=1.0, f_max=25e9, n_freq=101, fp64_check="off"):
     """Serving workload: Poisson arrivals (rate jobs/s), cell counts log-uniform
     in `cells`, 1-4 seeded dielectric bricks/spheres (eps_r U[1, 10]) clear of
     1-4 modal ports x 1-3 modes — a WR-90 guide with ports on its z ends for
@@ -314,7 +314,7 @@ def c5_jobs(n_jobs=200, seed=5, rate=1.5, cells=(32 ** 3, 384 ** 3), steps=(2000
         build = BuildPlan(tasks, _c5_assemble)
         f_c = 3e8 / (2 * C5_SIZE[0])
         jobs.append(SweepJob(i, float(t[i]), build, n_steps, np.linspace(1.1 * f_c, 0.96 * f_max, n_freq),
-                             f_max))
+                             f_max, fp64_check=fp64_check))
     return jobs
 
 
@@ -810,6 +810,9 @@ def main(argv=None) -> int:
     ap.add_argument("--step-scale", type=float, default=0.05,
                     help="c5: fraction of the 2000-20000 steps each job runs")
     ap.add_argument("--max-active", type=int, default=8, help="c5: resident jobs at once")
+    ap.add_argument("--fp64-check", choices=("off", "column", "all"), default="off",
+                    help="c5: float64 run of each job's first excitation (column) or all, "
+                         "its S-parameter error reported per job")
     ap.add_argument("--c5-mode", choices=("both", "continuous", "sequential"), default="both",
                     help="c5: serving modes to run (both: continuous, then one job at a time)")
     args = ap.parse_args(argv)
@@ -1012,7 +1015,8 @@ def main_c5(args) -> int:
     list(pool.map(abs, range(4 * workers)))  # start every worker now
 
     def stream_of_jobs():
-        return c5_jobs(n_jobs=args.jobs, rate=args.rate, step_scale=args.step_scale)
+        return c5_jobs(n_jobs=args.jobs, rate=args.rate, step_scale=args.step_scale,
+                       fp64_check=args.fp64_check)
 
     # warm-up: one small job loads the library and the runtime
     warm = c5_jobs(n_jobs=1, seed=99, cells=(24 ** 3, 24 ** 3), steps=(200, 200))
@@ -1071,6 +1075,15 @@ def main_c5(args) -> int:
                                "device_s": float(sum(r.finished - r.admitted for r in q.records))}
                               if q is not None else None),
         "results_identical": same,
+        # each job's single-precision S-matrix against a float64 run of its
+        # first excitation (or all) on the same kernels, when asked for
+        "fp64_check": ({"mode": args.fp64_check,
+                        "max_abs_dS_max": max(r.fp64["max_abs_dS"] for r in recs),
+                        "max_abs_dS_p50": float(np.median([r.fp64["max_abs_dS"] for r in recs])),
+                        "db_max": max(r.fp64["db"] for r in recs),
+                        "check_cell_updates": float(sum(r.cells * r.check_sets * r.n_steps
+                                                        for r in recs))}
+                       if args.fp64_check != "off" else None),
         "ports_per_job": {str(k): int(v) for k, v in zip(*np.unique(
             [len([t for t in j.build.tasks if t.startswith("port")]) for j in stream_of_jobs()],
             return_counts=True))},

@@ -195,8 +195,8 @@ int mgh_step_advance(long long* step_ctr, long long by, void* stream);
  * float64 n_mat x 16 (eps xyz, mu xyz, sigma xyz, sigma_m xyz with 0 for PEC
  * materials, is_pec, 3 unused); geo: device float64 dx[nx] dy[ny] dz[nz]
  * then dual widths ax[nx+1] ay[ny+1] az[nz+1]; face bits xmin 1, xmax 2,
- * ymin 4, ymax 8, zmin 16, zmax 32 for PEC walls and port faces (port faces
- * on one axis).  Outputs in the device layout: alpha/beta/M_eps on the E
+ * ymin 4, ymax 8, zmin 16, zmax 32 for PEC walls and port faces (on any
+ * axes).  Outputs in the device layout: alpha/beta/M_eps on the E
  * side, alpha/beta/M_mu on the H side (m_eps / m_mu may be NULL).  No ADE
  * W term (dispersive meshes use the host construction). */
 int mgh_build_coefficients(int dtype, int nx, int ny, int nz, const int* mat_id, int n_mat,

@@ -36,7 +36,10 @@ __device__ inline bool on_face(int bits, int axis, int pos, int n) {
 }
 
 // position of an entity after port flattening: the boundary plane of a port
-// face reads the first interior plane
+// face reads the first interior plane.  The reference flattens face by face,
+// each a copy of the first interior slab into the boundary slab; copies
+// along different axes commute, so an entity on the boundary planes of two
+// port axes reads the entity one plane in along both, whatever the order.
 __device__ inline int flat_pos(int bits, int axis, int pos, int n) {
   if ((bits >> (2 * axis)) & 1 && pos == 0) return 1;
   if ((bits >> (2 * axis + 1)) & 1 && pos == n) return n - 1;
@@ -193,9 +196,6 @@ int mgh_build_coefficients(int dtype, int nx, int ny, int nz, const int* mat_id,
   MGH_REQUIRE(mat_id && tab && geo && alpha_e && beta_e && alpha_h && beta_h, "null pointer");
   MGH_REQUIRE(dt > 0.0, "dt must be positive");
   MGH_REQUIRE((wall_faces & ~63) == 0 && (port_faces & ~63) == 0, "bad face bits");
-  int port_axes = 0;
-  for (int a = 0; a < 3; ++a) port_axes += ((port_faces >> (2 * a)) & 3) ? 1 : 0;
-  MGH_REQUIRE(port_axes <= 1, "port faces must share one axis");
   const int dims[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a)
     MGH_REQUIRE(!((port_faces >> (2 * a)) & 3) || dims[a] >= 2, "port axis needs >= 2 cells");

@@ -16,7 +16,13 @@ The reference has no serving tier; its unit of work is one
 which a job here reproduces as one batched sweep (``sweep.ScatteringSweep``).
 Results do not depend on the mix: every job has its own buffers and every
 kernel is deterministic, so a job's S-matrix in the mix equals its S-matrix
-alone bit for bit.
+alone bit for bit.  A job may ask for a float64 check of its result
+(``SweepJob.fp64_check``): the check solver is admitted with the job (its
+footprint counted), advanced on the job's stream and its S-parameter error
+recorded with the job.  Host preparation (material matrices, coefficients)
+runs for at most twice ``max_active`` waiting jobs at a time, and admitted
+solvers drop their float64 host copies once on the device, so host memory
+stays bounded however far arrivals outrun service.
 """
 
 from __future__ import annotations
@@ -70,6 +76,9 @@ class SweepJob:
     # factory of DFT monitors recorded in every field set
     boundary_conditions: dict | None = None
     monitors_factory: object = None
+    # float64 check of the result ("off", "column", "all"; ScatteringSweep):
+    # its solver is admitted and stepped with the job, its footprint counted
+    fp64_check: str = "off"
 
 
 def footprint_bytes(dims, n_channels: int, K: int, n_steps: int, precision: str = "single",
@@ -104,7 +113,7 @@ def job_extras(boundary_conditions, monitors, grid, K: int) -> tuple[int, int]:
 class JobRecord:
     job_id: int
     arrival: float
-    prepared: float = 0.0  # host preparation done (worker thread)
+    prepared: float = 0.0  # host preparation done (built and prepared)
     admitted: float = 0.0
     finished: float = 0.0
     cells: int = 0
@@ -112,6 +121,8 @@ class JobRecord:
     n_steps: int = 0
     footprint: int = 0
     result: SParameterResult | None = None
+    fp64: dict | None = None  # the float64 check of the result, when asked for
+    check_sets: int = 0  # field sets of the float64 check solver
 
     @property
     def latency(self) -> float:
@@ -165,7 +176,12 @@ class _Active:
     sweep: ScatteringSweep
     solver: object
     stream: torch.cuda.Stream
+    check: object = None  # float64 check solver, stepped after the job's solver
     events: deque = field(default_factory=deque)
+
+    def completed(self) -> bool:
+        n = self.rec.n_steps
+        return self.solver.n_completed >= n and (self.check is None or self.check.n_completed >= n)
 
 
 class SweepServer:
@@ -208,17 +224,24 @@ class SweepServer:
                              f_max=job.f_max, waveforms=job.waveforms, precision=job.precision,
                              use_graphs=self.use_graphs, device=str(self.device),
                              boundary_conditions=job.boundary_conditions,
-                             monitors_factory=job.monitors_factory)
+                             monitors_factory=job.monitors_factory, fp64_check=job.fp64_check)
         solver = sw.make_solver(sw.excited)
         solver.prepare_host()
+        # the float64 check solver takes its host half at admission
+        check = sw.make_check_solver()
         rec.cells = int(np.prod(mesh.dims))
         rec.K = len(sw.excited)
         rec.n_steps = job.n_steps
         psi, dft = job_extras(sw.boundary_conditions, solver.monitors, mesh.grid, rec.K)
         rec.footprint = footprint_bytes(mesh.dims, len(sw.channels), rec.K, job.n_steps, job.precision,
                                         psi_words=psi, dft_bytes=dft)
+        if check is not None:
+            rec.check_sets = len(sw.check_channels())
+            psi64, _ = job_extras(sw.boundary_conditions, [], mesh.grid, rec.check_sets)
+            rec.footprint += footprint_bytes(mesh.dims, len(sw.channels), rec.check_sets,
+                                             job.n_steps, "double", psi_words=psi64)
         rec.prepared = time.perf_counter() - t0
-        return sw, solver
+        return sw, solver, check
 
     @staticmethod
     def _submit_build(executor, build, shared: dict) -> dict:
@@ -244,7 +267,10 @@ class SweepServer:
     @staticmethod
     def _finish(a: _Active) -> SParameterResult:
         cols = [a.sweep.s_column(a.solver.recorded(k), ch) for k, ch in enumerate(a.sweep.excited)]
-        return SParameterResult.merge(cols)
+        res = SParameterResult.merge(cols)
+        if a.check is not None:
+            a.rec.fp64 = a.sweep.fp64_error(res, a.check)
+        return res
 
     # -- main loop -------------------------------------------------------
     def run(self, jobs: list[SweepJob], mode: str = "continuous") -> ServeReport:
@@ -269,28 +295,26 @@ class SweepServer:
         try:
             while done < len(order):
                 now = time.perf_counter() - t0
-                # arrivals start their host-side build
+                # arrivals start their host-side build (mesh, port modes)
                 while arrivals and arrivals[0][0] <= now:
                     _, jid, job = heapq.heappop(arrivals)
-                    if external is not None:
-                        futures[jid] = (job, self._submit_build(external, job.build, shared))
-                    else:
-                        futures[jid] = (job, pool.submit(self._prepare, job, recs[jid], t0))
-                # prepared jobs join the admission queue in arrival order
+                    futures[jid] = (job, self._submit_build(external or pool, job.build, shared))
+                # built jobs are prepared (material matrices, coefficients:
+                # gigabytes of host memory for a large job) and join the
+                # admission queue in arrival order, while fewer than
+                # 2 max_active prepared jobs wait
                 for jid in sorted(futures, key=lambda i: (recs[i].arrival, i)):
+                    if len(waiting) >= 2 * max_active:
+                        break
                     job, fut = futures[jid]
-                    if external is not None:
-                        if all(f.done() for f in fut.values()):
-                            built = self._assemble(job.build, fut)
-                            waiting.append((job, *self._prepare(job, recs[jid], t0, built)))
-                            del futures[jid]
-                    elif fut.done():
-                        waiting.append((job, *fut.result()))
+                    if all(f.done() for f in fut.values()):
+                        built = self._assemble(job.build, fut)
+                        waiting.append((job, *self._prepare(job, recs[jid], t0, built)))
                         del futures[jid]
                 waiting.sort(key=lambda x: (x[0].arrival, x[0].job_id))
                 # admission: first fit in arrival order
                 for item in list(waiting):
-                    job, sw, solver = item
+                    job, sw, solver, check = item
                     rec = recs[job.job_id]
                     if len(active) >= max_active:
                         break
@@ -305,10 +329,14 @@ class SweepServer:
                     # job's memory is reused by the next one; the job's stream
                     # waits for the set-up copies before its first step
                     solver.setup()
+                    solver.release_host()
+                    if check is not None:
+                        check.setup()
+                        check.release_host()
                     stream = torch.cuda.Stream(device=self.device)
                     stream.wait_stream(torch.cuda.current_stream(self.device))
                     rec.admitted = time.perf_counter() - t0
-                    active.append(_Active(rec, sw, solver, stream))
+                    active.append(_Active(rec, sw, solver, stream, check))
                     used += rec.footprint
                     waiting.remove(item)
                 peak_resident = max(peak_resident, len(active))
@@ -318,16 +346,19 @@ class SweepServer:
                 for a in active:
                     while a.events and a.events[0].query():
                         a.events.popleft()
-                    if len(a.events) < self.inflight and a.solver.n_completed < a.rec.n_steps:
+                    if len(a.events) < self.inflight and not a.completed():
                         with torch.cuda.stream(a.stream):
-                            a.solver.launch(self.chunk)
+                            if a.solver.n_completed < a.rec.n_steps:
+                                a.solver.launch(self.chunk)
+                            if a.check is not None and a.check.n_completed < a.rec.n_steps:
+                                a.check.launch(self.chunk)
                             ev = torch.cuda.Event()
                             ev.record(a.stream)
                         a.events.append(ev)
                         progressed = True
                 # retire finished jobs
                 for a in list(active):
-                    if a.solver.n_completed >= a.rec.n_steps and all(e.query() for e in a.events):
+                    if a.completed() and all(e.query() for e in a.events):
                         a.stream.synchronize()
                         a.rec.result = self._finish(a)
                         a.rec.finished = time.perf_counter() - t0

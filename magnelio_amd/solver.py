@@ -57,7 +57,6 @@ from .ports import PortModel
 PORT_CHUNK, PORT_MAX_MODES = 1024, 8
 PORT_PQ = 4 * PORT_MAX_MODES
 
-_PORT_AXIS = {"xmin": 0, "xmax": 0, "ymin": 1, "ymax": 1, "zmin": 2, "zmax": 2}
 
 
 def _face_bits(faces) -> int:
@@ -411,11 +410,21 @@ class FITTimeDomainSolver:
                       "alpha_h": lay(co.alpha_h, "H"), "beta_h": lay(co.beta_h, "H"),
                       "m_eps": lay(m_eps, "E"), "m_mu": lay(m_mu, "H")}
 
+    def release_host(self) -> None:
+        """Drop the float64 host arrays that ``prepare_host`` keeps after
+        ``setup`` (coefficients and masses; the set-up's binds have read
+        them).  For callers that hold many solvers, such as the serving
+        loop; the device state is untouched."""
+        if not self._setup_done:
+            raise RuntimeError("release_host() before setup()")
+        self._coeffs = None
+        self._m_eps_host = self._m_mu_host = None
+
     def _device_coefficients(self) -> bool:
         """Whether alpha/beta and the masses are built on the device: not for
-        given mass matrices, ADE materials or TF/SF sources (their host
-        set-up reads the float64 coefficients) nor for port faces on more
-        than one axis."""
+        given mass matrices, SIBC walls, ADE materials or TF/SF sources
+        (their host set-up reads the float64 coefficients) nor for a
+        hand-edited PEC mask."""
         if self.coefficients not in ("auto", "host", "device"):
             raise ValueError(f"coefficients must be 'auto', 'host' or 'device', got {self.coefficients!r}")
         if self.coefficients == "host":
@@ -432,8 +441,6 @@ class FITTimeDomainSolver:
             why = "sources"
         elif not getattr(self.mesh, "pec_rule", False):
             why = "a hand-edited PEC mask"
-        elif len({_PORT_AXIS[P.face] for P in self.ports}) > 1:
-            why = "port faces on more than one axis"
         if why is not None and self.coefficients == "device":
             raise ValueError(f"device-built coefficients do not cover {why}")
         return why is None

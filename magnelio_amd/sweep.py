@@ -18,6 +18,7 @@ sets independently.
 
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass, field
 
@@ -114,11 +115,17 @@ class SweepResult:
     seconds: float  # device wall time of the marching (set-up excluded)
     batches: list = field(default_factory=list)  # excited channels per solver
     n_cells: int = 0
+    # error of the single-precision S-parameters against a float64 run of
+    # the same kernels (ScatteringSweep fp64_check), or None
+    fp64: dict | None = None
 
     @property
     def cell_updates(self) -> float:
         """Cell-updates of the sweep: cells x steps x excitations."""
         return float(self.n_cells) * self.n_steps * sum(len(b) for b in self.batches)
+
+
+FP64_CHECKS = ("off", "column", "all")
 
 
 class ScatteringSweep:
@@ -128,12 +135,17 @@ class ScatteringSweep:
     the reference's auto waveform for its mode (``resolve_excitation``,
     scattering_td.py:2322-2366) unless ``waveforms`` maps it to a callable.
     ``batch``: field sets per solver (None = all channels in one solver).
+    ``fp64_check``: "off", "column" (the first excited channel) or "all" —
+    those excitations are run again in float64 (same ports, waveforms and
+    kernels) and the result carries max |S_fp32 - S_fp64| over every
+    frequency and observed channel of those columns (``SweepResult.fp64``).
     """
 
     def __init__(self, mesh, ports, *, dt: float, n_steps: int, f_axis, f_max: float,
                  f_min: float = 0.0, excited=None, waveforms=None, boundary_conditions=None,
                  precision: str = "single", batch: int | None = None, use_graphs: bool = True,
-                 monitors_factory=None, device: str = "cuda:0", graph_block: int = 32):
+                 monitors_factory=None, device: str = "cuda:0", graph_block: int = 32,
+                 fp64_check: str = "off"):
         self.mesh, self.ports = mesh, list(ports)
         self.dt, self.n_steps = float(dt), int(n_steps)
         self.f_axis = np.asarray(f_axis, dtype=float)
@@ -157,6 +169,11 @@ class ScatteringSweep:
         self.monitors_factory = monitors_factory
         self.device = device
         self.graph_block = int(graph_block)
+        if fp64_check not in FP64_CHECKS:
+            raise ValueError(f"fp64_check must be one of {FP64_CHECKS}, got {fp64_check!r}")
+        if fp64_check != "off" and precision != "single":
+            raise ValueError("fp64_check applies to single-precision sweeps")
+        self.fp64_check = fp64_check
         self.solvers: list[FITTimeDomainSolver] = []
 
     def waveform(self, ch):
@@ -165,14 +182,41 @@ class ScatteringSweep:
         port = next(p for p in self.ports if p.name == ch[0])
         return resolve_excitation(port.modes[ch[1]].omega_c, self.f_min, self.f_max, ch[1])
 
-    def make_solver(self, chans) -> FITTimeDomainSolver:
+    def make_solver(self, chans, precision: str | None = None,
+                    monitors: bool = True) -> FITTimeDomainSolver:
         exc = [ChannelExcitation(p, m, self.waveform((p, m))) for p, m in chans]
-        mons = list(self.monitors_factory()) if self.monitors_factory is not None else []
+        mons = (list(self.monitors_factory())
+                if monitors and self.monitors_factory is not None else [])
         return FITTimeDomainSolver(mesh=self.mesh, boundary_conditions=self.boundary_conditions,
                                    ports=self.ports, total_time_steps=self.n_steps, dt=self.dt,
-                                   precision=self.precision, nsets=len(chans), excitations=exc,
-                                   use_graphs=self.use_graphs, monitors=mons, device=self.device,
-                                   graph_block=self.graph_block)
+                                   precision=precision or self.precision, nsets=len(chans),
+                                   excitations=exc, use_graphs=self.use_graphs, monitors=mons,
+                                   device=self.device, graph_block=self.graph_block)
+
+    def check_channels(self) -> tuple:
+        """Excitations the float64 check runs again (empty when off)."""
+        if self.fp64_check == "off":
+            return ()
+        return self.excited[:1] if self.fp64_check == "column" else self.excited
+
+    def make_check_solver(self) -> FITTimeDomainSolver | None:
+        """The float64 solver of the check channels (no DFT monitors: only
+        the port signals are compared), or None when the check is off."""
+        chans = self.check_channels()
+        return self.make_solver(chans, precision="double", monitors=False) if chans else None
+
+    def fp64_error(self, s32: SParameterResult, check: FITTimeDomainSolver) -> dict:
+        """max |S_fp32 - S_fp64| per checked column and over all of them."""
+        cols = []
+        worst = 0.0
+        for k, ch in enumerate(self.check_channels()):
+            c64 = self.s_column(check.recorded(k), ch).matrix[:, :, 0]
+            c32 = s32.matrix[:, :, s32.excitations.index(ch)]
+            d = float(np.max(np.abs(c32 - c64)))
+            cols.append({"excited": list(ch), "max_abs_dS": d})
+            worst = max(worst, d)
+        return {"mode": self.fp64_check, "columns": cols, "max_abs_dS": worst,
+                "db": 20.0 * math.log10(worst) if worst > 0 else -math.inf}
 
     def s_column(self, signals: dict, ch) -> SParameterResult:
         modes = {p.name: p.modes for p in self.ports}
@@ -202,5 +246,12 @@ class ScatteringSweep:
                 rec = s.recorded(k)
                 sigs[ch] = rec
                 cols.append(self.s_column(rec, ch))
-        return SweepResult(SParameterResult.merge(cols), sigs, self.dt, self.n_steps, seconds,
-                           [list(b) for b in batches], int(np.prod(self.mesh.dims)))
+        res = SweepResult(SParameterResult.merge(cols), sigs, self.dt, self.n_steps, seconds,
+                          [list(b) for b in batches], int(np.prod(self.mesh.dims)))
+        check = self.make_check_solver()
+        if check is not None:
+            check.setup()
+            check.run()
+            torch.cuda.synchronize(dev)
+            res.fp64 = self.fp64_error(res.s_params, check)
+        return res

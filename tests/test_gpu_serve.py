@@ -13,7 +13,10 @@ together:
 * latency percentiles and throughput are reported;
 * the admission footprint of a job with a CPML face and a DFT monitor
   covers what its solver allocates on the device (psi slabs, bins and sample
-  buffers included) within the 64 MiB slack plus 10 %.
+  buffers included) within the 64 MiB slack plus 10 %;
+* jobs that ask for the float64 check carry its error (every S entry of the
+  checked column within 1e-4 of the float64 run) and the same S-matrices as
+  without it, the check solver stepped in the mix.
 """
 
 import numpy as np
@@ -60,6 +63,21 @@ def test_mix_equals_alone(jobs, alone, gpu):
           f"peak resident {rep.peak_resident}")
 
 
+def test_fp64_check_in_the_mix(jobs, alone, gpu):
+    import dataclasses
+
+    checked = [dataclasses.replace(j, fp64_check="column") for j in jobs]
+    print("continuous mix with float64 checks", flush=True)
+    rep = SweepServer(max_active=4, chunk=64).run(checked, mode="continuous")
+    for r in rep.records:
+        assert _same(r.result, alone[r.job_id]), r.job_id
+        assert r.fp64 is not None and r.fp64["mode"] == "column" and r.check_sets == 1
+        assert len(r.fp64["columns"]) == 1 and 0.0 < r.fp64["max_abs_dS"] < 1e-4, r.fp64
+        print(f"job {r.job_id}: fp32 vs fp64 {r.fp64['db']:.1f} dB")
+    plain = SweepServer(max_active=4, chunk=64).run(jobs[:2]).records
+    assert all(r.footprint < c.footprint for r, c in zip(plain, rep.records[:2]))
+
+
 def test_budget_admission(jobs, alone, gpu):
     fps = sorted(r.footprint for r in SweepServer(max_active=8).run(jobs[:2]).records)
     budget = int(1.2 * max(fps))
@@ -101,7 +119,8 @@ def test_footprint_covers_cpml_and_monitors(jobs, gpu):
     rec = JobRecord(99, 0.0)
     torch.cuda.synchronize()
     before = torch.cuda.memory_allocated()
-    sw, solver = srv._prepare(job, rec, time.perf_counter())
+    sw, solver, check = srv._prepare(job, rec, time.perf_counter())
+    assert check is None
     solver.setup()
     torch.cuda.synchronize()
     used = torch.cuda.memory_allocated() - before
@@ -110,3 +129,17 @@ def test_footprint_covers_cpml_and_monitors(jobs, gpu):
     assert rec.footprint <= 1.1 * used + (64 << 20)
     mon = solver.monitors[0]
     assert mon.footprint(mesh.grid, bcs, solver.nsets) >= mon.bytes()
+    # with the float64 check the footprint also covers the check solver
+    import dataclasses
+
+    rec2 = JobRecord(100, 0.0)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    _, solver2, check2 = srv._prepare(dataclasses.replace(job, job_id=100, fp64_check="column"),
+                                      rec2, time.perf_counter())
+    solver2.setup()
+    check2.setup()
+    torch.cuda.synchronize()
+    used2 = torch.cuda.memory_allocated() - before
+    print(f"checked: footprint {rec2.footprint / 1e6:.1f} MB, allocated {used2 / 1e6:.1f} MB", flush=True)
+    assert rec2.footprint >= used2 and rec2.footprint > rec.footprint

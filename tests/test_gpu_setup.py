@@ -7,6 +7,7 @@ update_coefficients), FP32 and FP64:
   faces, modal ports on zmin and zmax (port-plane flattening of M_eps, M_mu
   and the PEC mask);
 * the same materials with ports on xmin / xmax and walls on y and z;
+* ports on zmin, zmax and xmin together (two port axes), walls on y and xmax;
 * no ports, walls on all six faces.
 """
 
@@ -21,22 +22,22 @@ from magnelio_amd.solver import FITTimeDomainSolver
 pytestmark = pytest.mark.gpu
 
 
-def _axes(n, seed, port_axis):
-    """Seeded non-uniform spacings; along the port axis the three cells at
+def _axes(n, seed, port_axes):
+    """Seeded non-uniform spacings; along a port axis the three cells at
     each end are equal (the port feed rule)."""
     rng = np.random.default_rng(seed)
     out = []
     for a, k in enumerate(n):
         d = rng.uniform(0.8e-3, 1.2e-3, k)
-        if a == port_axis:
+        if a in port_axes:
             d[:3] = d[-3:] = 1e-3
         out.append(np.concatenate([[0.0], np.cumsum(d)]))
     return out
 
 
-def _mesh(port_axis, walls):
-    n = (22, 14, 18) if port_axis == 2 else (18, 14, 16)
-    x, y, z = _axes(n, 3, port_axis)
+def _mesh(port_axes, walls):
+    n = (22, 14, 18) if port_axes == (2,) else (18, 14, 16)
+    x, y, z = _axes(n, 3, port_axes)
     L = [a[-1] for a in (x, y, z)]
 
     def box(f0, f1):
@@ -52,9 +53,12 @@ def _mesh(port_axis, walls):
 
 
 CASES = {
-    "z_ports": (2, ("xmin", "xmax", "ymin", "ymax"), ("zmin", "zmax")),
-    "x_ports": (0, ("ymin", "ymax", "zmin", "zmax"), ("xmin", "xmax")),
-    "no_ports": (2, mm.FACES, ()),
+    "z_ports": ((2,), ("xmin", "xmax", "ymin", "ymax"), ("zmin", "zmax")),
+    "x_ports": ((0,), ("ymin", "ymax", "zmin", "zmax"), ("xmin", "xmax")),
+    # ports on two axes (the serving workload's crosses): the boundary-plane
+    # remaps of the two axes compose at the shared edges
+    "zx_ports": ((0, 2), ("ymin", "ymax", "xmax"), ("zmin", "zmax", "xmin")),
+    "no_ports": ((2,), mm.FACES, ()),
 }
 
 

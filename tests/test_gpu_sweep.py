@@ -5,6 +5,9 @@ On the C1 guide (two 3-mode DTBC ports, K = 6 excitations):
 * the batched sweep's S-matrix equals the one-excitation-per-solver sweep's
   bit for bit (every kernel treats the sets independently);
 * FP32 against an FP64 sweep of the same kernels: every S entry within 1e-4;
+* a sweep with the float64 check of every column reports, per column, the
+  largest |S_fp32 - S_fp64| of the separate FP64 sweep (the check runs the
+  same kernels), within 1e-4;
 * physics of the empty guide: |S_{2m,1m}| = 1 and S_{1m,1m} ~ 0 for every
   propagating mode, reciprocity S_{ij} = S_{ji}, no mode conversion.  Every
   channel is driven with a modulated Gaussian on 18-25 GHz, clear of the
@@ -29,7 +32,8 @@ def sweeps():
     mesh, ports, dt = cases.c1_mesh_ports()
     wf = {(p.name, m): Excitation(18e9, 25e9, m) for p in ports for m in range(p.n_modes)}
     out = {}
-    for key, kw in {"batched": {}, "single": {"batch": 1}, "fp64": {"precision": "double"}}.items():
+    for key, kw in {"batched": {}, "single": {"batch": 1}, "fp64": {"precision": "double"},
+                    "checked": {"fp64_check": "all"}}.items():
         print(f"C1 sweep {key}", flush=True)
         sw = ScatteringSweep(mesh, ports, dt=dt, n_steps=4000, f_axis=F, f_max=cases.C1_FMAX, waveforms=wf, **kw)
         out[key] = sw.run()
@@ -47,6 +51,19 @@ def test_fp32_within_1e4_of_fp64(sweeps, gpu):
     b, d = sweeps["batched"].s_params.matrix, sweeps["fp64"].s_params.matrix
     assert np.isfinite(d).all()
     assert np.abs(b - d).max() < 1e-4
+
+
+def test_fp64_check_reports_the_fp64_sweep(sweeps, gpu):
+    chk = sweeps["checked"].fp64
+    b, d = sweeps["checked"].s_params, sweeps["fp64"].s_params
+    assert np.array_equal(b.matrix, sweeps["batched"].s_params.matrix, equal_nan=True)
+    assert chk["mode"] == "all" and len(chk["columns"]) == 6
+    for col in chk["columns"]:
+        k = b.excitations.index(tuple(col["excited"]))
+        assert col["max_abs_dS"] == float(np.abs(b.matrix[:, :, k] - d.matrix[:, :, k]).max())
+    assert 0.0 < chk["max_abs_dS"] < 1e-4
+    print(f"C1 sweep fp32 vs fp64: {chk['db']:.1f} dB")
+    assert sweeps["batched"].fp64 is None
 
 
 def test_empty_guide_physics(sweeps, gpu):

@@ -65,3 +65,24 @@ def test_device_coefficient_eligibility():
     edited = mm.Mesh.from_grid(g)
     edited.pec_rule = False
     assert not FITTimeDomainSolver(mesh=edited)._device_coefficients()
+
+
+def test_sweep_fp64_check_arguments():
+    """The float64 check: its modes, the columns each runs, and refusals."""
+    import cases
+    from magnelio_amd.sweep import ScatteringSweep
+
+    mesh, ports, dt = cases.c1_mesh_ports(dims=(12, 6, 16))
+    f = np.linspace(8e9, 20e9, 5)
+    kw = dict(dt=dt, n_steps=10, f_axis=f, f_max=cases.C1_FMAX)
+    assert ScatteringSweep(mesh, ports, **kw).check_channels() == ()
+    assert ScatteringSweep(mesh, ports, **kw).make_check_solver() is None
+    col = ScatteringSweep(mesh, ports, fp64_check="column", **kw)
+    assert col.check_channels() == (("port1", 0),)
+    assert col.make_check_solver().precision == "double"
+    every = ScatteringSweep(mesh, ports, fp64_check="all", **kw)
+    assert every.check_channels() == every.excited and len(every.excited) == 6
+    with pytest.raises(ValueError, match="fp64_check must be"):
+        ScatteringSweep(mesh, ports, fp64_check="some", **kw)
+    with pytest.raises(ValueError, match="single-precision"):
+        ScatteringSweep(mesh, ports, fp64_check="column", precision="double", **kw)
