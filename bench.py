@@ -707,7 +707,7 @@ def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plai
     # short graphs (a multiple of the DFT slice cadence), so that a short
     # timed window replays graphs too
     sweep = ScatteringSweep(mesh, ports, dt=dt, n_steps=total, f_axis=f_axis, f_max=C3_FMAX,
-                            monitors_factory=monitors, graph_block=8)
+                            monitors_factory=monitors, graph_block=8, kchunk="auto")
     K = len(sweep.excited)
 
     extra = {"steps": 0}
@@ -737,6 +737,9 @@ def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plai
     samples = sol.monitors[0].S if sol.monitors else 0
     bins_gb = sol.monitors[0].bytes() / 1e9 if sol.monitors else 0.0
     e_us, h_us = curl_launch_times(sol)
+    from magnelio_amd import kernels
+
+    shape = dict(zip(("kchunk", "sets_per_thread", "band"), kernels.shape_parts(sol.kchunk)))
     del sol
     torch.cuda.empty_cache()
     t_s = 0.0
@@ -763,7 +766,7 @@ def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plai
            "roofline_gcell": roof, "frac_of_roofline": rate_b / roof, "dft_bins_gb": round(bins_gb, 2)}
     # the K-set curl launches alone: algorithmic bytes 4 (6 + 9 K) per cell
     alg = 4.0 * cells * (6 + 9 * K)
-    curl = {"e_us": e_us, "h_us": h_us, "alg_bytes_per_launch": alg,
+    curl = {"e_us": e_us, "h_us": h_us, "launch_shape": shape, "alg_bytes_per_launch": alg,
             "frac_of_peak_e": alg / e_us / 1e3 / HBM_PEAK_GBS,
             "frac_of_peak_h": alg / h_us / 1e3 / HBM_PEAK_GBS}
     if K == 6:
@@ -861,8 +864,11 @@ def main(argv=None) -> int:
                                         slice_steps=args.dft_slice_steps))
     sol = FITTimeDomainSolver(mesh=mesh, boundary_conditions=bcs, total_time_steps=total, dt=dt,
                               precision="single", nsets=args.k, record=False, sources=sources,
-                              monitors=monitors)
+                              monitors=monitors, kchunk="auto")
     sol.setup()
+    from magnelio_amd import kernels
+
+    shape = dict(zip(("kchunk", "sets_per_thread", "band"), kernels.shape_parts(sol.kchunk)))
     seed_mid_run(sol, n0)
     setup_s = time.perf_counter() - t_setup
 
@@ -919,6 +925,7 @@ def main(argv=None) -> int:
                          + f"; timed steps from a seeded mid-run state at step {n0} "
                          "(random fields, full sample buffers)"),
             "K": K,
+            "launch_shape": shape,  # autotuned at setup (0: the library's choice)
             "dft_bins_gb": round(mon.bytes() / 1e9, 2) if mon is not None else 0.0,
             "dft_slice": ({"steps_between": mon.cadence, "frequency_groups": mon.fsplit,
                            "pipelined": mon.pipelined} if mon is not None else None),
@@ -1025,7 +1032,7 @@ def main_c5(args) -> int:
     modes = ("continuous", "sequential") if args.c5_mode == "both" else (args.c5_mode,)
     for mode in modes:
         print(f"c5 {mode}: {args.jobs} jobs", file=sys.stderr, flush=True)
-        srv = SweepServer(max_active=args.max_active, build_pool=pool, progress=max(1, args.jobs // 20))
+        srv = SweepServer(max_active=args.max_active, build_pool=pool, progress=max(1, args.jobs // 60))
         reports[mode] = srv.run(stream_of_jobs(), mode=mode)
         r = reports[mode]
         print(f"c5 {mode}: {r.jobs_per_s:.3f} jobs/s p50 {r.p50:.2f} s p99 {r.p99:.2f} s",

@@ -57,8 +57,12 @@ int mgh_stream_probe(int n_read, int n_write, const void* in, void* out, long lo
  * E_s <- alpha_e * E_s + beta_e * (C^T H_s) for s = 0..nsets-1.
  * curl64 = 0: array-stencil operation order in the field dtype;
  * curl64 = 1 (float32 only): the Numba tier's float64 curl accumulator.
- * kchunk: z planes marched per thread (launch shape; results do not depend
- * on it). */
+ * kchunk: the launch shape word (results do not depend on it): z planes
+ * marched per thread in bits 0-15; optionally field sets per thread (1-3)
+ * in bits 16-19 and the XCD band in launch units (1-255) in bits 20-27,
+ * 0 leaving each to the library (sets: 3 when nsets is a multiple of 3,
+ * else 2, 1 for one set; band: 8).  Every curl entry point below takes the
+ * same word. */
 int mgh_update_e(int dtype, int nx, int ny, int nz, int nsets, void* e, long long e_set_stride,
                  const void* h, long long h_set_stride, const void* alpha_e, const void* beta_e,
                  int curl64, int kchunk, void* stream);

@@ -1,20 +1,22 @@
 """Launch-shape tuning of the curl launches, cached on disk.
 
-The tunable is the z-chunk of the curl kernels (planes a thread marches,
-keeping the z neighbour in registers): shorter chunks give more work-groups,
-longer ones fewer re-read chunk-start planes.  The sets per thread (two, or
-three for multiples of 3) and the XCD band mapping are fixed in the
-library from measurements; the chunk depends on the grid, K and the CPML the
-launch carries, so it is measured per configuration.
+The shape of a curl launch is three numbers (``kernels.launch_shape``): the
+z-chunk (planes a thread marches, keeping the z neighbour in registers —
+shorter chunks give more work-groups, longer ones fewer re-read chunk-start
+planes), the field sets a thread advances (each coefficient word it loads
+serves them all; more sets cost registers and occupancy) and the XCD band
+(consecutive launch units a chiplet takes, which share its L2).  The best
+shape depends on the grid, K and the CPML the launch carries, so it is
+measured per configuration.
 
-``FITTimeDomainSolver(kchunk="auto")`` calls :func:`kchunk_for` at setup,
+``FITTimeDomainSolver(kchunk="auto")`` calls :func:`shape_for` at setup,
 before any state is loaded: the curl launches (with the solver's CPML, as a
 step runs them) are timed on the solver's own zeroed buffers — zero fields
 and zero psi stay zero, so the timing changes nothing — and the fastest
 candidate is stored in a JSON cache (``MAGNELIO_HIP_AUTOTUNE_CACHE``, default
 ``~/.cache/magnelio_amd/autotune.json``) keyed by device, the library's
 content hash, grid, K, dtype and CPML faces; later solvers of the same key
-read it.
+read it.  Every shape gives the same fields bit for bit.
 
 The reference's CUDA tier has a fixed block shape (``numba_kernels.py:446``);
 this module has no counterpart there.
@@ -33,8 +35,16 @@ import torch
 
 from . import _lib, kernels
 
-CANDIDATES = (1, 2, 3, 4)
+KCHUNKS = (1, 2, 3, 4)
+SETS = (1, 2, 3)  # for K >= 2; one set is always 1
+BANDS = (4, 8, 16)
 REPS = 5
+
+
+def candidates(nsets: int) -> list[int]:
+    """Shape words tried for K = nsets field sets."""
+    sets = SETS if nsets >= 2 else (1,)
+    return [kernels.launch_shape(kc, s, b) for kc in KCHUNKS for s in sets for b in BANDS]
 
 
 def cache_path() -> Path:
@@ -80,7 +90,7 @@ def cache_key(device: torch.device, dims, nsets: int, dtype, cpml_faces) -> str:
 def time_curl(e, h, ae, be, ah, bh, dims, kchunk: int, cpml=None, curl64: bool = False,
               stream=None, reps: int = REPS) -> float:
     """Mean milliseconds of one E + H curl step (with ``cpml`` as the step
-    launches it) at z-chunk ``kchunk``, after one untimed step."""
+    launches it) at launch shape ``kchunk``, after one untimed step."""
     stream = stream or torch.cuda.current_stream(e.device)
 
     def step():
@@ -99,27 +109,31 @@ def time_curl(e, h, ae, be, ah, bh, dims, kchunk: int, cpml=None, curl64: bool =
 
 
 def choose(timings: dict) -> int:
-    """Fastest candidate; ties go to the smaller chunk."""
-    return min(sorted(timings), key=lambda kc: (timings[kc], kc))
+    """Fastest shape; ties go to the shorter chunk, then fewer sets per
+    thread, then the smaller band."""
+    return min(sorted(timings), key=lambda w: (timings[w], kernels.shape_parts(w)))
 
 
-def kchunk_for(solver, candidates=CANDIDATES, path: Path | None = None) -> int:
-    """z-chunk for ``solver`` (set up, fields still zero): from the cache, or
-    timed now and stored."""
+def shape_for(solver, path: Path | None = None) -> int:
+    """Launch shape word for ``solver`` (set up, fields still zero): from the
+    cache, or timed now and stored."""
     path = path or cache_path()
     cpml = solver.cpml if (solver.cpml is not None and solver._dispersion is None
                           and solver._dispersion_mu is None and solver._sibc is None) else None
     faces = [f.face for f in solver.cpml.faces] if solver.cpml is not None else []
     key = cache_key(solver.e.device, solver.dims, solver.nsets, solver.e.dtype, faces)
+    cands = candidates(solver.nsets)
     data = _load(path)
     hit = data.get(key)
-    if isinstance(hit, dict) and hit.get("kchunk") in candidates:
-        return int(hit["kchunk"])
-    timings = {kc: time_curl(solver.e, solver.h, solver.alpha_e, solver.beta_e, solver.alpha_h,
-                             solver.beta_h, solver.dims, kc, cpml=cpml, curl64=solver.curl64)
-               for kc in candidates}
+    if isinstance(hit, dict) and hit.get("shape") in cands:
+        return int(hit["shape"])
+    timings = {w: time_curl(solver.e, solver.h, solver.alpha_e, solver.beta_e, solver.alpha_h,
+                            solver.beta_h, solver.dims, w, cpml=cpml, curl64=solver.curl64)
+               for w in cands}
     best = choose(timings)
+    kc, sets, band = kernels.shape_parts(best)
     data = _load(path)  # re-read: another process may have added entries meanwhile
-    data[key] = {"kchunk": best, "ms": {str(k): round(v, 4) for k, v in timings.items()}}
+    data[key] = {"shape": best, "kchunk": kc, "sets": sets, "band": band,
+                 "ms": {str(k): round(v, 4) for k, v in timings.items()}}
     _store(path, data)
     return best

@@ -81,25 +81,26 @@ struct Plan {
   int kchunk, ngroups;
   int ntp, nkc, units;
   int split, a0, a1, tpa, wb;
+  int band;         // units per XCD band
   const int* live;  // live spatial units (kc * ntp + tp), or null for all
 };
 
 // Work-groups are dealt to the 8 XCDs round robin (work-group b to XCD
-// b mod 8).  Units go to XCDs in bands of 8 consecutive ones: XCD x runs
-// units [64 s + 8 x, 64 s + 8 x + 8) for s = 0, 1, ..., so consecutive node
-// tiles (and the set groups of one tile) share an XCD's L2 — the previous
-// row a tile reads as its y neighbour was just fetched there — while all
-// XCDs sweep the same z region together.  On 256^3 (K = 1) the curl
-// launches ran 2-4 % faster than with units dealt one by one, and faster
-// than a contiguous unit range per XCD, which spreads the XCDs over eight
-// z regions.
+// b mod 8).  Units go to XCDs in bands of B consecutive ones (B = 8 unless
+// the launch shape asks otherwise): XCD x runs units [8 B s + B x,
+// 8 B s + B x + B) for s = 0, 1, ..., so consecutive node tiles (and the set
+// groups of one tile) share an XCD's L2 — the previous row a tile reads as
+// its y neighbour was just fetched there — while all XCDs sweep the same z
+// region together.  On 256^3 (K = 1) the curl launches ran 2-4 % faster with
+// bands of 8 than with units dealt one by one, and faster than a contiguous
+// unit range per XCD, which spreads the XCDs over eight z regions.
 constexpr int BAND = 8;
 
 __device__ __forceinline__ bool unit_of(const Plan& p, int& grp, int& i, int& j, int& k0,
                                         int& k1) {
   const int blk = blockIdx.x;
   const int sq = blk >> 3;  // this work-group's place in its XCD's sequence
-  const int u = (sq / BAND) * (8 * BAND) + (blk & 7) * BAND + (sq % BAND);
+  const int u = (sq / p.band) * (8 * p.band) + (blk & 7) * p.band + (sq % p.band);
   if (u >= p.units) return false;
   grp = u % p.ngroups;
   const int t0 = u / p.ngroups;
@@ -972,10 +973,11 @@ inline int sets_per_thread(int nsets) {
 }
 
 // a0 < a1: split the planes at the interior columns [a0, a1) (see Plan)
-Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, const int* live = nullptr,
-               int nlive = 0, int a0 = 0, int a1 = 0) {
+Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, int band,
+               const int* live = nullptr, int nlive = 0, int a0 = 0, int a1 = 0) {
   Plan p;
   std::memset(&p, 0, sizeof(p));
+  p.band = band;
   p.nx = nx;
   p.ny = ny;
   p.nz = nz;
@@ -997,7 +999,27 @@ Plan make_plan(int nx, int ny, int nz, int nsets, int kchunk, int ns, const int*
 }
 
 // work-groups of a launch: units rounded up to whole bands on every XCD
-int plan_blocks(const Plan& p) { return (p.units + 8 * BAND - 1) / (8 * BAND) * (8 * BAND); }
+int plan_blocks(const Plan& p) {
+  const int64_t w = 8 * (int64_t)p.band;
+  return (int)((p.units + w - 1) / w * w);
+}
+
+// The launch shape word of the curl entry points: z planes marched per
+// thread in bits 0-15, field sets per thread in bits 16-19 (0: the
+// library's choice, sets_per_thread), the XCD band in launch units in bits
+// 20-27 (0: BAND).  Results do not depend on it.
+struct Shape {
+  int kchunk, ns, band;
+};
+
+inline Shape decode_shape(int word, int nsets) {
+  Shape s;
+  s.kchunk = word & 0xffff;
+  const int ns = (word >> 16) & 0xf, band = (word >> 20) & 0xff;
+  s.ns = ns ? ns : sets_per_thread(nsets);
+  s.band = band ? band : BAND;
+  return s;
+}
 
 template <typename T>
 CFace<T> face_of(const mgh::CpmlFace& F, bool e_side) {
@@ -1108,10 +1130,11 @@ template <typename T, bool C64>
 int run_e(void* e, const void* h, const void* ae, const void* be, int nx, int ny, int nz,
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
           int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
-  const int ns = sets_per_thread(nsets);
+  const Shape sh = decode_shape(kchunk, nsets);
+  const int ns = sh.ns;
   int a0 = 0, a1 = 0;
   const bool split = !live && x_split(pml, nx, &a0, &a1);
-  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive, a0, a1);
+  const Plan pl = make_plan(nx, ny, nz, nsets, sh.kchunk, ns, sh.band, live, nlive, a0, a1);
   if (pl.units == 0) return MGH_OK;
   PmlArgs<T> pa = pml_args<T>(pml, true, nx, ny, nz);
   pa.xin = split;
@@ -1161,10 +1184,11 @@ template <typename T>
 int run_h(const void* e, void* h, const void* ah, const void* bh, int nx, int ny, int nz,
           int nsets, int kchunk, int64_t es, int64_t hs, hipStream_t st, const int* live = nullptr,
           int nlive = 0, const mgh::CpmlSet* pml = nullptr) {
-  const int ns = sets_per_thread(nsets);
+  const Shape sh = decode_shape(kchunk, nsets);
+  const int ns = sh.ns;
   int a0 = 0, a1 = 0;
   const bool split = !live && x_split(pml, nx, &a0, &a1);
-  const Plan pl = make_plan(nx, ny, nz, nsets, kchunk, ns, live, nlive, a0, a1);
+  const Plan pl = make_plan(nx, ny, nz, nsets, sh.kchunk, ns, sh.band, live, nlive, a0, a1);
   if (pl.units == 0) return MGH_OK;
   PmlArgs<T> pa = pml_args<T>(pml, false, nx, ny, nz);
   pa.xin = split;
@@ -1240,17 +1264,20 @@ int run_xcols_h(const void* e, void* h, const void* bh, int nx, int ny, int nz, 
   return MGH_OK;
 }
 
-int check_common(int dtype, int nx, int ny, int nz, int nsets, int kchunk) {
+int check_common(int dtype, int nx, int ny, int nz, int nsets, int shape) {
   MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 (float32) or 8 (float64), got %d",
               dtype);
   MGH_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "grid must have at least one cell per axis");
   MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "nsets must be in 1..65535, got %d", nsets);
+  MGH_REQUIRE(shape >= 0 && (shape >> 28) == 0, "bad launch shape word %d", shape);
+  const int kchunk = shape & 0xffff, ns = (shape >> 16) & 0xf;
   MGH_REQUIRE(kchunk >= 1, "kchunk must be >= 1, got %d", kchunk);
+  MGH_REQUIRE(ns <= 3, "sets per thread must be 1, 2 or 3 (0: default), got %d", ns);
   // node numbers, unit counts and slab thread blocks are 32-bit
   const int64_t npl = (int64_t)(nx + 1) * (ny + 1);
   const int64_t units =
       ((npl + WG - 1) / WG) * ((int64_t)(nz + 1 + kchunk - 1) / kchunk) * (int64_t)nsets;
-  MGH_REQUIRE(npl < (1ll << 31) && units + 8 < (1ll << 31),
+  MGH_REQUIRE(npl < (1ll << 31) && units + 8 * 256 < (1ll << 31),
               "grid too large for the launch plan");
   return MGH_OK;
 }
@@ -1298,7 +1325,8 @@ int mgh_update_e_tiles(int dtype, int nx, int ny, int nz, int nsets, void* e,
   if (rc) return rc;
   MGH_REQUIRE(e && h && alpha_e && beta_e && (live || n_live == 0), "null device pointer");
   MGH_REQUIRE(!(curl64 && dtype == 8), "curl64 applies to float32 fields only");
-  const int64_t all = (((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG) * ((nz + kchunk) / kchunk);
+  const int kc = kchunk & 0xffff;
+  const int64_t all = (((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG) * ((nz + kc) / kc);
   MGH_REQUIRE(n_live >= 0 && n_live <= all, "n_live %d outside 0..%lld", n_live, (long long)all);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == 4)
@@ -1317,7 +1345,8 @@ int mgh_update_h_tiles(int dtype, int nx, int ny, int nz, int nsets, const void*
   int rc = check_common(dtype, nx, ny, nz, nsets, kchunk);
   if (rc) return rc;
   MGH_REQUIRE(e && h && alpha_h && beta_h && (live || n_live == 0), "null device pointer");
-  const int64_t all = (((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG) * ((nz + kchunk) / kchunk);
+  const int kc = kchunk & 0xffff;
+  const int64_t all = (((int64_t)(nx + 1) * (ny + 1) + WG - 1) / WG) * ((nz + kc) / kc);
   MGH_REQUIRE(n_live >= 0 && n_live <= all, "n_live %d outside 0..%lld", n_live, (long long)all);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == 4)

@@ -20,6 +20,22 @@ from . import _lib
 DEFAULT_KCHUNK = 2
 
 
+def launch_shape(kchunk: int, sets: int = 0, band: int = 0) -> int:
+    """The curl launches' shape word (the ``kchunk`` argument of every curl
+    entry point): z planes marched per thread (bits 0-15), field sets per
+    thread (bits 16-19; 0: the library's choice) and the XCD band in launch
+    units (bits 20-27; 0: 8).  Results do not depend on it."""
+    if not (1 <= kchunk <= 0xFFFF and 0 <= sets <= 3 and 0 <= band <= 0xFF):
+        raise ValueError(f"bad launch shape kchunk={kchunk} sets={sets} band={band}")
+    return int(kchunk) | (int(sets) << 16) | (int(band) << 20)
+
+
+def shape_parts(word: int) -> tuple[int, int, int]:
+    """(z planes per thread, sets per thread, band) of a shape word; 0 for a
+    part left to the library."""
+    return word & 0xFFFF, (word >> 16) & 0xF, (word >> 20) & 0xFF
+
+
 def _dtype_code(t: torch.Tensor) -> int:
     if t.dtype == torch.float32:
         return 4

@@ -299,7 +299,9 @@ class FITTimeDomainSolver:
     excitations: list | None = None  # per set: ChannelExcitation | None
     record: bool = True
     curl64: bool = False
-    kchunk: int | str = kernels.DEFAULT_KCHUNK  # z planes per thread, or "auto" (autotune.py)
+    # curl launch shape word (kernels.launch_shape: z planes per thread, sets
+    # per thread, XCD band), or "auto" (timed at setup, autotune.py)
+    kchunk: int | str = kernels.DEFAULT_KCHUNK
     use_graphs: bool = False
     graph_block: int = 1  # minimum steps per captured graph
     dtbc_kernel_init: int = dtbc.N_KERNEL_INIT  # first DTBC kernel length (then doubled)
@@ -317,8 +319,9 @@ class FITTimeDomainSolver:
     def __post_init__(self):
         if self.precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {list(PRECISIONS)}, got {self.precision!r}")
-        if self.kchunk != "auto" and not (isinstance(self.kchunk, int) and self.kchunk >= 1):
-            raise ValueError(f"kchunk must be a positive int or 'auto', got {self.kchunk!r}")
+        if self.kchunk != "auto" and not (isinstance(self.kchunk, int)
+                                          and kernels.shape_parts(self.kchunk)[0] >= 1):
+            raise ValueError(f"kchunk must be a launch shape word or 'auto', got {self.kchunk!r}")
         if self.excitations is None:
             self.excitations = [None] * self.nsets
         if len(self.excitations) != self.nsets:
@@ -543,7 +546,7 @@ class FITTimeDomainSolver:
         self._sqsum = kernels.SqSum(K, dev)
         self._energy = torch.zeros(K, dtype=torch.float64, device=dev)
         if self.kchunk == "auto":  # timed on the zeroed buffers, before any state is loaded
-            self.kchunk = autotune.kchunk_for(self)
+            self.kchunk = autotune.shape_for(self)
         self.tile_plan = self._build_tile_plan()
         self._setup_done = True
 
@@ -557,7 +560,7 @@ class FITTimeDomainSolver:
             return None
         force = self.dlumped.edges if self.dlumped is not None else None
         plan = build_tile_plan(self.alpha_e, self.beta_e, self.alpha_h, self.beta_h, self.dims,
-                               self.kchunk, force_live_e=force)
+                               kernels.shape_parts(self.kchunk)[0], force_live_e=force)
         if self.tile_skip == "auto" and max(plan.skipped_e, plan.skipped_h) < 0.02:
             return None
         return plan

@@ -135,6 +135,8 @@ class ScatteringSweep:
     the reference's auto waveform for its mode (``resolve_excitation``,
     scattering_td.py:2322-2366) unless ``waveforms`` maps it to a callable.
     ``batch``: field sets per solver (None = all channels in one solver).
+    ``kchunk``: the solvers' curl launch shape (a ``kernels.launch_shape``
+    word or "auto"; None: the solver's default).
     ``fp64_check``: "off", "column" (the first excited channel) or "all" —
     those excitations are run again in float64 (same ports, waveforms and
     kernels) and the result carries max |S_fp32 - S_fp64| over every
@@ -145,7 +147,7 @@ class ScatteringSweep:
                  f_min: float = 0.0, excited=None, waveforms=None, boundary_conditions=None,
                  precision: str = "single", batch: int | None = None, use_graphs: bool = True,
                  monitors_factory=None, device: str = "cuda:0", graph_block: int = 32,
-                 fp64_check: str = "off"):
+                 fp64_check: str = "off", kchunk=None):
         self.mesh, self.ports = mesh, list(ports)
         self.dt, self.n_steps = float(dt), int(n_steps)
         self.f_axis = np.asarray(f_axis, dtype=float)
@@ -174,6 +176,7 @@ class ScatteringSweep:
         if fp64_check != "off" and precision != "single":
             raise ValueError("fp64_check applies to single-precision sweeps")
         self.fp64_check = fp64_check
+        self.kchunk = kchunk  # the solvers' curl launch shape (None: the solver's default)
         self.solvers: list[FITTimeDomainSolver] = []
 
     def waveform(self, ch):
@@ -191,7 +194,8 @@ class ScatteringSweep:
                                    ports=self.ports, total_time_steps=self.n_steps, dt=self.dt,
                                    precision=precision or self.precision, nsets=len(chans),
                                    excitations=exc, use_graphs=self.use_graphs, monitors=mons,
-                                   device=self.device, graph_block=self.graph_block)
+                                   device=self.device, graph_block=self.graph_block,
+                                   **({} if self.kchunk is None else {"kchunk": self.kchunk}))
 
     def check_channels(self) -> tuple:
         """Excitations the float64 check runs again (empty when off)."""

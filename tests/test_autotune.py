@@ -2,12 +2,31 @@
 
 import json
 
-from magnelio_amd import autotune
+import pytest
+
+from magnelio_amd import autotune, kernels
 
 
-def test_choose_fastest_ties_to_smaller_chunk():
+def test_choose_fastest_ties_to_smaller_shape():
+    shape = kernels.launch_shape
     assert autotune.choose({1: 0.30, 2: 0.21, 3: 0.25, 4: 0.22}) == 2
     assert autotune.choose({1: 0.2, 2: 0.2, 4: 0.3}) == 1
+    # ties: shorter chunk, then fewer sets per thread, then the smaller band
+    t = {shape(2, 3, 4): 0.2, shape(2, 2, 16): 0.2, shape(4, 1, 4): 0.2, shape(1, 3, 8): 0.3}
+    assert autotune.choose(t) == shape(2, 2, 16)
+
+
+def test_shape_words_and_candidates():
+    w = kernels.launch_shape(3, 2, 16)
+    assert kernels.shape_parts(w) == (3, 2, 16) and kernels.shape_parts(2) == (2, 0, 0)
+    for bad in ((0, 0, 0), (1, 4, 0), (1, 0, 256), (70000, 0, 0)):
+        with pytest.raises(ValueError):
+            kernels.launch_shape(*bad)
+    one, six = autotune.candidates(1), autotune.candidates(6)
+    assert len(one) == len(autotune.KCHUNKS) * len(autotune.BANDS)
+    assert len(six) == len(autotune.KCHUNKS) * len(autotune.SETS) * len(autotune.BANDS)
+    assert {kernels.shape_parts(x)[1] for x in one} == {1}
+    assert len(set(six)) == len(six)
 
 
 def test_cache_roundtrip_and_bad_files(tmp_path):
