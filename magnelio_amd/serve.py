@@ -123,6 +123,8 @@ class JobRecord:
     result: SParameterResult | None = None
     fp64: dict | None = None  # the float64 check of the result, when asked for
     check_sets: int = 0  # field sets of the float64 check solver
+    status: str = "pending"  # "done", "rejected" (footprint over the budget) or "cancelled"
+    reason: str = ""
 
     @property
     def latency(self) -> float:
@@ -136,20 +138,27 @@ class JobRecord:
 @dataclass
 class ServeReport:
     mode: str
-    records: list
+    records: list  # every job, in arrival order (see JobRecord.status)
     wall: float
     peak_resident: int = 0
     peak_bytes: int = 0
 
+    @property
+    def completed(self) -> list:
+        """Records of the jobs that ran to their S-matrix; the statistics
+        below are over these."""
+        return [r for r in self.records if r.status == "done"]
+
     def latencies(self) -> np.ndarray:
-        return np.array([r.latency for r in self.records])
+        return np.array([r.latency for r in self.completed])
 
     def breakdown(self) -> dict:
         """Median seconds per phase: host preparation, wait for admission,
         device run (admission to S-matrix)."""
-        prep = np.array([r.prepared - r.arrival for r in self.records])
-        wait = np.array([r.admitted - r.prepared for r in self.records])
-        run = np.array([r.finished - r.admitted for r in self.records])
+        done = self.completed
+        prep = np.array([r.prepared - r.arrival for r in done])
+        wait = np.array([r.admitted - r.prepared for r in done])
+        run = np.array([r.finished - r.admitted for r in done])
         return {"prep_p50": float(np.median(prep)), "wait_p50": float(np.median(wait)),
                 "run_p50": float(np.median(run)), "run_p99": float(np.percentile(run, 99))}
 
@@ -163,11 +172,11 @@ class ServeReport:
 
     @property
     def jobs_per_s(self) -> float:
-        return len(self.records) / self.wall
+        return len(self.completed) / self.wall
 
     @property
     def gcell_per_s(self) -> float:
-        return sum(r.cell_updates for r in self.records) / self.wall / 1e9
+        return sum(r.cell_updates for r in self.completed) / self.wall / 1e9
 
 
 @dataclass
@@ -207,6 +216,16 @@ class SweepServer:
         # bound and hold the GIL); None = a thread pool of build_workers
         self.build_pool = build_pool
         self.progress = int(progress)  # report every `progress` finished jobs (0: never)
+        self._cancel: set = set()
+
+    def cancel(self, job_id: int) -> None:
+        """Cancel a job of the running ``run`` (or, called before it, of the
+        next one) — from any thread.
+        A job not yet arrived or built is dropped, a waiting one leaves the
+        queue, a resident one is retired once its queued steps drain (its
+        memory returns to the budget); it gets no S-matrix and the status
+        "cancelled".  A job already done is left alone."""
+        self._cancel.add(int(job_id))
 
     # -- helpers ---------------------------------------------------------
     def _budget(self) -> int:
@@ -295,9 +314,22 @@ class SweepServer:
         try:
             while done < len(order):
                 now = time.perf_counter() - t0
+                # cancellations: jobs not yet resident leave at once
+                for jid in list(self._cancel):
+                    rec = recs.get(jid)
+                    if rec is None or rec.status != "pending":
+                        continue
+                    if any(a.rec is rec for a in active):
+                        continue  # retired below, once its queued steps drain
+                    futures.pop(jid, None)
+                    waiting[:] = [w for w in waiting if w[0].job_id != jid]
+                    rec.status, rec.reason = "cancelled", "cancelled before admission"
+                    done += 1
                 # arrivals start their host-side build (mesh, port modes)
                 while arrivals and arrivals[0][0] <= now:
                     _, jid, job = heapq.heappop(arrivals)
+                    if recs[jid].status != "pending":
+                        continue
                     futures[jid] = (job, self._submit_build(external or pool, job.build, shared))
                 # built jobs are prepared (material matrices, coefficients:
                 # gigabytes of host memory for a large job) and join the
@@ -318,9 +350,13 @@ class SweepServer:
                     rec = recs[job.job_id]
                     if len(active) >= max_active:
                         break
-                    if rec.footprint > budget:
-                        raise RuntimeError(f"job {job.job_id} needs {rec.footprint / 1e9:.1f} GB, "
-                                           f"more than the {budget / 1e9:.1f} GB budget")
+                    if rec.footprint > budget:  # can never fit: rejected, the others go on
+                        rec.status = "rejected"
+                        rec.reason = (f"needs {rec.footprint / 1e9:.1f} GB, more than the "
+                                      f"{budget / 1e9:.1f} GB budget")
+                        waiting.remove(item)
+                        done += 1
+                        continue
                     if used + rec.footprint > budget:
                         if mode == "sequential":
                             break
@@ -346,7 +382,8 @@ class SweepServer:
                 for a in active:
                     while a.events and a.events[0].query():
                         a.events.popleft()
-                    if len(a.events) < self.inflight and not a.completed():
+                    if (len(a.events) < self.inflight and not a.completed()
+                            and a.rec.job_id not in self._cancel):
                         with torch.cuda.stream(a.stream):
                             if a.solver.n_completed < a.rec.n_steps:
                                 a.solver.launch(self.chunk)
@@ -356,11 +393,16 @@ class SweepServer:
                             ev.record(a.stream)
                         a.events.append(ev)
                         progressed = True
-                # retire finished jobs
+                # retire finished (or cancelled) jobs
                 for a in list(active):
-                    if a.completed() and all(e.query() for e in a.events):
+                    cancelled = a.rec.job_id in self._cancel
+                    if (cancelled or a.completed()) and all(e.query() for e in a.events):
                         a.stream.synchronize()
-                        a.rec.result = self._finish(a)
+                        if cancelled:
+                            a.rec.status, a.rec.reason = "cancelled", "cancelled while resident"
+                        else:
+                            a.rec.result = self._finish(a)
+                            a.rec.status = "done"
                         a.rec.finished = time.perf_counter() - t0
                         used -= a.rec.footprint
                         active.remove(a)
@@ -374,5 +416,6 @@ class SweepServer:
                     time.sleep(2e-4)
         finally:
             pool.shutdown(wait=True)
+            self._cancel.clear()
         wall = time.perf_counter() - t0 - min(j.arrival for j in order)
         return ServeReport(mode, [recs[j.job_id] for j in order], wall, peak_resident, peak_bytes)

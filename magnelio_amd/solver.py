@@ -44,7 +44,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import _lib, autotune, dtbc, kernels, layout
+from . import _lib, autotune, config, dtbc, kernels, layout
 from . import mesh as _mesh
 from .dispersion import DeviceDispersion
 from .sibc import DeviceSIBC
@@ -294,19 +294,22 @@ class FITTimeDomainSolver:
     energy_check_interval: int | None = None
     monitors: list = field(default_factory=list)
     lumped: list = field(default_factory=list)
-    precision: str = "single"
+    precision: str | None = None  # "single" / "double"; None: MAGNELIO_PRECISION, else single
     nsets: int = 1
     excitations: list | None = None  # per set: ChannelExcitation | None
     record: bool = True
     curl64: bool = False
     # curl launch shape word (kernels.launch_shape: z planes per thread, sets
     # per thread, XCD band), or "auto" (timed at setup, autotune.py)
-    kchunk: int | str = kernels.DEFAULT_KCHUNK
-    use_graphs: bool = False
+    # (None: MAGNELIO_HIP_LAUNCH_SHAPE, else two z planes per thread)
+    kchunk: int | str | None = None
+    use_graphs: bool | None = None  # None: MAGNELIO_GPU_GRAPHS, else off
     graph_block: int = 1  # minimum steps per captured graph
     dtbc_kernel_init: int = dtbc.N_KERNEL_INIT  # first DTBC kernel length (then doubled)
     coefficients: str = "auto"  # alpha/beta built on the "device", the "host", or "auto"
-    tile_skip: str = "auto"  # dead-unit skipping: "auto" (when it skips >= 2 %), "on", "off"
+    # dead-unit skipping: "auto" (when it skips >= 2 %), "on", "off"; None:
+    # MAGNELIO_TILE_SKIP ("0" off), else auto
+    tile_skip: str | None = None
     device: str = "cuda:0"
     # (m_eps, m_mu, m_sigma, m_sigma_m) flat float64 vectors in the reference
     # layout, unflattened — e.g. the reference's own build_M_* of a conformal
@@ -317,8 +320,11 @@ class FITTimeDomainSolver:
     sibc: object | None = None
 
     def __post_init__(self):
-        if self.precision not in PRECISIONS:
-            raise ValueError(f"precision must be one of {list(PRECISIONS)}, got {self.precision!r}")
+        # an explicit argument wins over the environment (config.py)
+        self.precision = config.precision(self.precision)
+        self.use_graphs = config.graphs(self.use_graphs)
+        self.tile_skip = config.tile_skip(self.tile_skip)
+        self.kchunk = config.launch_shape(self.kchunk)
         if self.kchunk != "auto" and not (isinstance(self.kchunk, int)
                                           and kernels.shape_parts(self.kchunk)[0] >= 1):
             raise ValueError(f"kchunk must be a launch shape word or 'auto', got {self.kchunk!r}")

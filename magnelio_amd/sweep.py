@@ -25,6 +25,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
+from . import config
 from .boundaries import PECBoundary
 from .signals import resolve_excitation, s_parameters
 from .solver import ChannelExcitation, FITTimeDomainSolver
@@ -145,7 +146,8 @@ class ScatteringSweep:
 
     def __init__(self, mesh, ports, *, dt: float, n_steps: int, f_axis, f_max: float,
                  f_min: float = 0.0, excited=None, waveforms=None, boundary_conditions=None,
-                 precision: str = "single", batch: int | None = None, use_graphs: bool = True,
+                 precision: str | None = None, batch: int | None = None,
+                 use_graphs: bool | None = None,
                  monitors_factory=None, device: str = "cuda:0", graph_block: int = 32,
                  fp64_check: str = "off", kchunk=None):
         self.mesh, self.ports = mesh, list(ports)
@@ -163,17 +165,19 @@ class ScatteringSweep:
             boundary_conditions = {f: PECBoundary(f) for f in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
                                    if f not in port_faces}
         self.boundary_conditions = boundary_conditions
-        self.precision = precision
+        # explicit arguments win; else MAGNELIO_PRECISION / MAGNELIO_GPU_GRAPHS,
+        # else single precision with graphs
+        self.precision = config.precision(precision)
         self.batch = len(self.excited) if batch is None else int(batch)
         if self.batch < 1:
             raise ValueError("batch must be >= 1")
-        self.use_graphs = use_graphs
+        self.use_graphs = config.graphs(use_graphs, default=True)
         self.monitors_factory = monitors_factory
         self.device = device
         self.graph_block = int(graph_block)
         if fp64_check not in FP64_CHECKS:
             raise ValueError(f"fp64_check must be one of {FP64_CHECKS}, got {fp64_check!r}")
-        if fp64_check != "off" and precision != "single":
+        if fp64_check != "off" and self.precision != "single":
             raise ValueError("fp64_check applies to single-precision sweeps")
         self.fp64_check = fp64_check
         self.kchunk = kchunk  # the solvers' curl launch shape (None: the solver's default)

@@ -14,6 +14,10 @@ together:
 * the admission footprint of a job with a CPML face and a DFT monitor
   covers what its solver allocates on the device (psi slabs, bins and sample
   buffers included) within the 64 MiB slack plus 10 %;
+* a job whose footprint exceeds the budget is rejected (recorded, no
+  S-matrix) while the others complete unchanged; a job cancelled before the
+  run and one cancelled from another thread while the loop runs end as
+  "cancelled", and the rest still equal their runs alone;
 * jobs that ask for the float64 check carry its error (every S entry of the
   checked column within 1e-4 of the float64 run) and the same S-matrices as
   without it, the check solver stepped in the mix.
@@ -76,6 +80,47 @@ def test_fp64_check_in_the_mix(jobs, alone, gpu):
         print(f"job {r.job_id}: fp32 vs fp64 {r.fp64['db']:.1f} dB")
     plain = SweepServer(max_active=4, chunk=64).run(jobs[:2]).records
     assert all(r.footprint < c.footprint for r, c in zip(plain, rep.records[:2]))
+
+
+def test_rejection_and_cancellation(jobs, alone, gpu):
+    import threading
+
+    first = SweepServer(max_active=8).run(jobs[:3]).records
+    fps = sorted(r.footprint for r in first)
+    budget = (fps[0] + fps[-1]) // 2  # the largest of the first three cannot fit
+    big = max(first, key=lambda r: r.footprint).job_id
+    fit = [j for j in jobs if j.job_id != big]
+    early, resident = fit[-2].job_id, fit[-1].job_id
+    srv = SweepServer(hbm_budget=budget, max_active=2, chunk=16)
+    srv.cancel(early)  # before the run
+    # the other is cancelled from another thread the moment it is admitted
+    orig = srv._prepare
+
+    def prepare(job, rec, t0, built=None):
+        sw, solver, check = orig(job, rec, t0, built)
+        if job.job_id == resident:
+            setup = solver.setup
+
+            def admitted():
+                setup()
+                t = threading.Thread(target=srv.cancel, args=(resident,))
+                t.start()
+                t.join()
+
+            solver.setup = admitted
+        return sw, solver, check
+
+    srv._prepare = prepare
+    print(f"rejection / cancellation, budget {budget / 1e6:.0f} MB", flush=True)
+    rep = srv.run(jobs)
+    by = {r.job_id: r for r in rep.records}
+    assert by[big].status == "rejected" and by[big].result is None and "budget" in by[big].reason
+    assert by[early].status == "cancelled" and by[early].result is None
+    assert by[resident].status == "cancelled" and "resident" in by[resident].reason
+    assert by[resident].result is None
+    for r in rep.completed:
+        assert _same(r.result, alone[r.job_id]), r.job_id
+    assert len(rep.completed) == len(jobs) - 3 and all(r.status != "pending" for r in rep.records)
 
 
 def test_budget_admission(jobs, alone, gpu):
