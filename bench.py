@@ -819,6 +819,10 @@ def main(argv=None) -> int:
     ap.add_argument("--step-scale", type=float, default=0.05,
                     help="c5: fraction of the 2000-20000 steps each job runs")
     ap.add_argument("--max-active", type=int, default=8, help="c5: resident jobs at once")
+    ap.add_argument("--cells-max", type=int, default=384 ** 3,
+                    help="c5: largest job (cells log-uniform from 32^3 to this)")
+    ap.add_argument("--pack-cells", type=int, default=0,
+                    help="c5: jobs of at most this many cells step in shared launches (0: none)")
     ap.add_argument("--fp64-check", choices=("off", "column", "all"), default="off",
                     help="c5: float64 run of each job's first excitation (column) or all, "
                          "its S-parameter error reported per job")
@@ -1029,7 +1033,7 @@ def main_c5(args) -> int:
 
     def stream_of_jobs():
         return c5_jobs(n_jobs=args.jobs, rate=args.rate, step_scale=args.step_scale,
-                       fp64_check=args.fp64_check)
+                       fp64_check=args.fp64_check, cells=(32 ** 3, args.cells_max))
 
     # warm-up: one small job loads the library and the runtime
     warm = c5_jobs(n_jobs=1, seed=99, cells=(24 ** 3, 24 ** 3), steps=(200, 200))
@@ -1038,7 +1042,8 @@ def main_c5(args) -> int:
     modes = ("continuous", "sequential") if args.c5_mode == "both" else (args.c5_mode,)
     for mode in modes:
         print(f"c5 {mode}: {args.jobs} jobs", file=sys.stderr, flush=True)
-        srv = SweepServer(max_active=args.max_active, build_pool=pool, progress=max(1, args.jobs // 60))
+        srv = SweepServer(max_active=args.max_active, build_pool=pool, progress=max(1, args.jobs // 60),
+                          pack_cells=args.pack_cells)
         reports[mode] = srv.run(stream_of_jobs(), mode=mode)
         r = reports[mode]
         print(f"c5 {mode}: {r.jobs_per_s:.3f} jobs/s p50 {r.p50:.2f} s p99 {r.p99:.2f} s",
@@ -1066,7 +1071,7 @@ def main_c5(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": (f"{args.jobs} sweep jobs, Poisson arrivals {args.rate}/s (seed 5), cells "
-                         "log-uniform 32^3-384^3 (a WR-90 guide with z-end ports for 1-2 ports, a "
+                         f"log-uniform 32^3-{round(args.cells_max ** (1 / 3))}^3 (a WR-90 guide with z-end ports for 1-2 ports, a "
                          "cross of two WR-90 guides with z- and x-end ports for 3-4; 1-4 dielectric "
                          "bricks/spheres eps_r U[1,10]), 1-4 modal DTBC ports x 1-3 modes, steps "
                          "U{2000..20000} x "
@@ -1088,6 +1093,8 @@ def main_c5(args) -> int:
                                "device_s": float(sum(r.finished - r.admitted for r in q.records))}
                               if q is not None else None),
         "results_identical": same,
+        "pack_cells": args.pack_cells,
+        "packed_jobs": int(sum(r.packed for r in recs)),
         # each job's single-precision S-matrix against a float64 run of its
         # first excitation (or all) on the same kernels, when asked for
         "fp64_check": ({"mode": args.fp64_check,

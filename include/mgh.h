@@ -188,6 +188,30 @@ int mgh_port_step(int dtype, int nsets, void* e, long long e_set_stride, const v
  * (fit_td.py:950). */
 int mgh_step_advance(long long* step_ctr, long long by, void* stream);
 
+/* ---- packed small jobs ----------------------------------------------------
+ * No reference counterpart (the reference runs one solver at a time; its
+ * unit of work is AnalysisScatteringTD.run, scattering_td.py:1252-1330): the
+ * per-step launches of several small solvers, as a serving loop holds them,
+ * in one launch each.  Results equal each solver stepped alone, bit for bit.
+ *
+ * mgh_update_e_packed / mgh_update_h_packed: the E / H curl of n_jobs
+ * (1..16) grids, plain (no CPML, dense units), one field set per thread;
+ * jobs = n_jobs x 10 int64 words: nx, ny, nz, nsets, E, E set stride, H,
+ * H set stride, alpha and beta of the phase's side; shape: the launch shape
+ * word of mgh_update_e with a sets-per-thread part of 0 or 1. */
+int mgh_update_e_packed(int dtype, int n_jobs, const long long* jobs, int shape, void* stream);
+int mgh_update_h_packed(int dtype, int n_jobs, const long long* jobs, int shape, void* stream);
+
+/* mgh_port_step_packed: mgh_port_step for ports of several solvers (1..8
+ * ports in all): port_desc as for mgh_port_step, jobs = n_ports x 6 int64
+ * words: E, E set stride, H, H set stride, field sets and step counter of
+ * the solver each port belongs to. */
+int mgh_port_step_packed(int dtype, int n_ports, const long long* port_desc,
+                         const long long* jobs, void* stream);
+
+/* mgh_step_advance_packed: *ctrs[i] += by for n (1..64) step counters. */
+int mgh_step_advance_packed(int n, const long long* ctrs, long long by, void* stream);
+
 /* ---- (a2)+(a6) material matrices and coefficients on the device ----------
  * Replaces build_M_eps / build_M_mu / build_M_sigma / build_M_sigma_m
  * (src/magnelio/_operators/material_matrices.py:334-757, category-0 cells),

@@ -88,6 +88,10 @@ SIGNATURES: dict[str, tuple] = {
          _c_void_p],
     ),
     "mgh_step_advance": (_c_int, [_c_void_p, _c_ll, _c_void_p]),
+    "mgh_update_e_packed": (_c_int, [_c_int, _c_int, _c_void_p, _c_int, _c_void_p]),
+    "mgh_update_h_packed": (_c_int, [_c_int, _c_int, _c_void_p, _c_int, _c_void_p]),
+    "mgh_port_step_packed": (_c_int, [_c_int, _c_int, _c_void_p, _c_void_p, _c_void_p]),
+    "mgh_step_advance_packed": (_c_int, [_c_int, _c_void_p, _c_ll, _c_void_p]),
     "mgh_weighted_sq_sum_workspace": (_c_ll, [_c_int]),
     "mgh_weighted_sq_sum": (
         _c_int,

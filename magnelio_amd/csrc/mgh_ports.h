@@ -69,6 +69,14 @@ struct PortDev {
   const double* comp_r;
   const double* comp_live;
   double* comp_state;
+  // the field sets this port acts on (set by the step call, not the
+  // descriptor): E and H of set 0 and their set strides, the set count and
+  // the step counter of the solver the port belongs to
+  void* e;
+  const void* h;
+  long long e_set, h_set;
+  const long long* ctr;
+  int nsets, pad2;
 };
 
 // chunks of the projection pass: the plane and the history, whichever is longer

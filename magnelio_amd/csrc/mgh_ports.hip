@@ -73,16 +73,14 @@ __device__ inline bool kernel_slice(const mgh::PortDev& P, long long n, long lon
 }
 
 template <typename T>
-__global__ __launch_bounds__(PB) void k_port_project(const T* __restrict__ e, int64_t e_set,
-                                                     mgh::PortSet ps,
-                                                     const long long* __restrict__ step_ctr) {
+__global__ __launch_bounds__(PB) void k_port_project(mgh::PortSet ps) {
   const mgh::PortDev& P = ps.p[blockIdx.y];
   const int s = blockIdx.z, c = blockIdx.x;
   const int n_pts = P.n_u + P.n_v, M = P.M;
   const int nch = mgh::port_chunks(n_pts, P.cap);
-  if (c >= nch) return;  // whole work-group
-  const long long n = *step_ctr;
-  const T* es = e + (int64_t)s * e_set;
+  if (c >= nch || s >= P.nsets) return;  // whole work-group
+  const long long n = *P.ctr;
+  const T* es = (const T*)P.e + (int64_t)s * P.e_set;
   double acc[PQ];
 #pragma unroll
   for (int q = 0; q < PQ; ++q) acc[q] = 0.0;
@@ -133,13 +131,13 @@ __global__ __launch_bounds__(PB) void k_port_project(const T* __restrict__ e, in
   store_partials(acc, M, P.work + ((int64_t)s * nch + c) * PQ);
 }
 
-__global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps,
-                                                     const long long* __restrict__ step_ctr) {
+__global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps) {
   const mgh::PortDev& P = ps.p[blockIdx.x];
   const int s = blockIdx.y;
+  if (s >= P.nsets) return;  // whole work-group
   const int n_pts = P.n_u + P.n_v, M = P.M;
   const int nch = mgh::port_chunks(n_pts, P.cap);
-  const long long n = *step_ctr;
+  const long long n = *P.ctr;
   __shared__ double tot[PQ];
   const int q = threadIdx.x;
   if (q < PQ) {
@@ -157,7 +155,7 @@ __global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps,
   const int m = threadIdx.x;
   if (m < M) {
     double* st = P.state + ((int64_t)s * M) * 4;
-    double* vcs = P.work + vc_offset(nch, gridDim.y) + (int64_t)s * MGH_PORT_MAX_MODES;
+    double* vcs = P.work + vc_offset(nch, P.nsets) + (int64_t)s * MGH_PORT_MAX_MODES;
     const int exc = P.exc_mode ? P.exc_mode[s] : -1;
     const double* prm = P.params + 8 * m;
     double* sm = st + 4 * m;  // u, u_prev, V_port_prev, V_int_prev
@@ -207,23 +205,20 @@ __global__ __launch_bounds__(64) void k_port_advance(mgh::PortSet ps,
   if (P.absmax && env > P.absmax[s]) P.absmax[s] = env;
 }
 template <typename T>
-__global__ __launch_bounds__(PB) void k_port_write(T* __restrict__ e, int64_t e_set,
-                                                   const T* __restrict__ h, int64_t h_set,
-                                                   mgh::PortSet ps,
-                                                   const long long* __restrict__ step_ctr) {
+__global__ __launch_bounds__(PB) void k_port_write(mgh::PortSet ps) {
   const mgh::PortDev& P = ps.p[blockIdx.y];
   const int s = blockIdx.z, c = blockIdx.x;
   const int n_pts = P.n_u + P.n_v, M = P.M;
-  if (c >= mgh::port_plane_chunks(n_pts)) return;  // whole work-group
+  if (c >= mgh::port_plane_chunks(n_pts) || s >= P.nsets) return;  // whole work-group
   const int nch = mgh::port_chunks(n_pts, P.cap);
-  const long long n = *step_ctr;
+  const long long n = *P.ctr;
   const bool rec = P.V_rec != nullptr && n < P.rec_cap;
-  const double* vcs = P.work + vc_offset(nch, gridDim.z) + (int64_t)s * MGH_PORT_MAX_MODES;
+  const double* vcs = P.work + vc_offset(nch, P.nsets) + (int64_t)s * MGH_PORT_MAX_MODES;
   double vc[MGH_PORT_MAX_MODES];
 #pragma unroll
   for (int m = 0; m < MGH_PORT_MAX_MODES; ++m) vc[m] = m < M ? vcs[m] : 0.0;
-  T* es = e + (int64_t)s * e_set;
-  const T* hs = h + (int64_t)s * h_set;
+  T* es = (T*)P.e + (int64_t)s * P.e_set;
+  const T* hs = (const T*)P.h + (int64_t)s * P.h_set;
   // complement absorber: the interior projections of this step (V_int_prev
   // after the advance) remove the port's channels from the interior plane
   const bool comp = P.comp_r != nullptr;
@@ -273,19 +268,19 @@ __global__ __launch_bounds__(PB) void k_port_write(T* __restrict__ e, int64_t e_
       }
     }
   }
-  if (rec) store_partials(acc, M, P.work + rec_offset(nch, gridDim.z) + ((int64_t)s * nch + c) * PQ);
+  if (rec) store_partials(acc, M, P.work + rec_offset(nch, P.nsets) + ((int64_t)s * nch + c) * PQ);
 }
 
-__global__ __launch_bounds__(64) void k_port_record(mgh::PortSet ps,
-                                                    const long long* __restrict__ step_ctr) {
+__global__ __launch_bounds__(64) void k_port_record(mgh::PortSet ps) {
   const mgh::PortDev& P = ps.p[blockIdx.x];
   const int s = blockIdx.y;
-  const long long n = *step_ctr;
+  if (s >= P.nsets) return;  // whole work-group
+  const long long n = *P.ctr;
   if (P.V_rec == nullptr || n >= P.rec_cap) return;  // whole work-group
   const int n_pts = P.n_u + P.n_v, M = P.M;
   const int nch = mgh::port_chunks(n_pts, P.cap);
   const int npc = mgh::port_plane_chunks(n_pts);
-  const double* part = P.work + rec_offset(nch, gridDim.y);
+  const double* part = P.work + rec_offset(nch, P.nsets);
   __shared__ double tot[PQ];
   const int q = threadIdx.x;
   if (q < PQ) {
@@ -308,6 +303,44 @@ __global__ void k_step_advance(long long* ctr, long long by) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *ctr += by;
 }
 
+// the step counters of packed solvers, one lane each
+constexpr int MGH_PACK_CTRS = 64;
+struct CtrTable {
+  long long* c[MGH_PACK_CTRS];
+};
+
+__global__ void k_step_advance_packed(CtrTable t, int n, long long by) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) *t.c[threadIdx.x] += by;
+}
+
+// The four launches of a port step over the ports of ps (each carrying its
+// field sets and step counter); max_sets: the largest set count among them.
+int launch_port_step(int dtype, mgh::PortSet& ps, int max_sets, hipStream_t st) {
+  int max_ch = 1, max_pch = 1;
+  for (int i = 0; i < ps.n; ++i) {
+    const int n_pts = ps.p[i].n_u + ps.p[i].n_v;
+    max_ch = std::max(max_ch, mgh::port_chunks(n_pts, ps.p[i].cap));
+    max_pch = std::max(max_pch, mgh::port_plane_chunks(n_pts));
+  }
+  const dim3 g_proj(max_ch, ps.n, max_sets), g_write(max_pch, ps.n, max_sets),
+      g_one(ps.n, max_sets, 1);
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_port_project<float>, g_proj, dim3(PB), 0, st, ps);
+  else
+    hipLaunchKernelGGL(k_port_project<double>, g_proj, dim3(PB), 0, st, ps);
+  MGH_CHECK_LAUNCH("k_port_project");
+  hipLaunchKernelGGL(k_port_advance, g_one, dim3(64), 0, st, ps);
+  MGH_CHECK_LAUNCH("k_port_advance");
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_port_write<float>, g_write, dim3(PB), 0, st, ps);
+  else
+    hipLaunchKernelGGL(k_port_write<double>, g_write, dim3(PB), 0, st, ps);
+  MGH_CHECK_LAUNCH("k_port_write");
+  hipLaunchKernelGGL(k_port_record, g_one, dim3(64), 0, st, ps);
+  MGH_CHECK_LAUNCH("k_port_record");
+  return MGH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -322,41 +355,70 @@ int mgh_port_step(int dtype, int nsets, void* e, long long e_set_stride, const v
   MGH_REQUIRE(e && h && port_desc && step_ctr, "null pointer");
   mgh::PortSet ps{};
   ps.n = n_ports;
-  int max_ch = 1, max_pch = 1;
   for (int i = 0; i < n_ports; ++i) {
-    int rc = mgh::load_port(port_desc + (int64_t)i * MGH_PORT_WORDS, &ps.p[i], nsets);
+    mgh::PortDev& P = ps.p[i];
+    int rc = mgh::load_port(port_desc + (int64_t)i * MGH_PORT_WORDS, &P, nsets);
     if (rc) return rc;
-    const int n_pts = ps.p[i].n_u + ps.p[i].n_v;
-    max_ch = std::max(max_ch, mgh::port_chunks(n_pts, ps.p[i].cap));
-    max_pch = std::max(max_pch, mgh::port_plane_chunks(n_pts));
+    P.e = e;
+    P.h = h;
+    P.e_set = e_set_stride;
+    P.h_set = h_set_stride;
+    P.ctr = step_ctr;
+    P.nsets = nsets;
   }
-  hipStream_t st = (hipStream_t)stream;
-  const dim3 g_proj(max_ch, n_ports, nsets), g_write(max_pch, n_ports, nsets), g_one(n_ports, nsets, 1);
-  if (dtype == 4)
-    hipLaunchKernelGGL(k_port_project<float>, g_proj, dim3(PB), 0, st, (const float*)e, e_set_stride,
-                       ps, step_ctr);
-  else
-    hipLaunchKernelGGL(k_port_project<double>, g_proj, dim3(PB), 0, st, (const double*)e,
-                       e_set_stride, ps, step_ctr);
-  MGH_CHECK_LAUNCH("k_port_project");
-  hipLaunchKernelGGL(k_port_advance, g_one, dim3(64), 0, st, ps, step_ctr);
-  MGH_CHECK_LAUNCH("k_port_advance");
-  if (dtype == 4)
-    hipLaunchKernelGGL(k_port_write<float>, g_write, dim3(PB), 0, st, (float*)e, e_set_stride,
-                       (const float*)h, h_set_stride, ps, step_ctr);
-  else
-    hipLaunchKernelGGL(k_port_write<double>, g_write, dim3(PB), 0, st, (double*)e, e_set_stride,
-                       (const double*)h, h_set_stride, ps, step_ctr);
-  MGH_CHECK_LAUNCH("k_port_write");
-  hipLaunchKernelGGL(k_port_record, g_one, dim3(64), 0, st, ps, step_ctr);
-  MGH_CHECK_LAUNCH("k_port_record");
-  return MGH_OK;
+  return launch_port_step(dtype, ps, nsets, (hipStream_t)stream);
+}
+
+// The port steps of several solvers (a serving loop's packed small jobs) in
+// the four launches of one step: port_desc holds n_ports descriptors, jobs
+// per port 6 words: E, E set stride, H, H set stride, field sets, step counter
+// of the solver the port belongs to.
+int mgh_port_step_packed(int dtype, int n_ports, const long long* port_desc,
+                         const long long* jobs, void* stream) {
+  MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 or 8, got %d", dtype);
+  MGH_REQUIRE(n_ports >= 1 && n_ports <= MGH_PORT_MAX_PORTS, "n_ports must be 1..%d, got %d",
+              MGH_PORT_MAX_PORTS, n_ports);
+  MGH_REQUIRE(port_desc && jobs, "null pointer");
+  mgh::PortSet ps{};
+  ps.n = n_ports;
+  int max_sets = 1;
+  for (int i = 0; i < n_ports; ++i) {
+    const long long* w = jobs + (int64_t)i * 6;
+    const int nsets = (int)w[4];
+    MGH_REQUIRE(nsets >= 1 && nsets <= 65535, "port %d: bad nsets %d", i, nsets);
+    MGH_REQUIRE(w[0] && w[2] && w[5], "port %d: null field or counter pointer", i);
+    mgh::PortDev& P = ps.p[i];
+    int rc = mgh::load_port(port_desc + (int64_t)i * MGH_PORT_WORDS, &P, nsets);
+    if (rc) return rc;
+    P.e = (void*)w[0];
+    P.e_set = w[1];
+    P.h = (const void*)w[2];
+    P.h_set = w[3];
+    P.nsets = nsets;
+    P.ctr = (const long long*)w[5];
+    max_sets = std::max(max_sets, nsets);
+  }
+  return launch_port_step(dtype, ps, max_sets, (hipStream_t)stream);
 }
 
 int mgh_step_advance(long long* step_ctr, long long by, void* stream) {
   MGH_REQUIRE(step_ctr, "null step counter");
   hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, (hipStream_t)stream, step_ctr, by);
   MGH_CHECK_LAUNCH("k_step_advance");
+  return MGH_OK;
+}
+
+// Advance the step counters of several solvers (packed jobs) by `by`.
+int mgh_step_advance_packed(int n, const long long* ctrs, long long by, void* stream) {
+  MGH_REQUIRE(n >= 1 && n <= MGH_PACK_CTRS, "n must be 1..%d, got %d", MGH_PACK_CTRS, n);
+  MGH_REQUIRE(ctrs, "null counter table");
+  CtrTable t{};
+  for (int i = 0; i < n; ++i) {
+    MGH_REQUIRE(ctrs[i], "null step counter %d", i);
+    t.c[i] = (long long*)ctrs[i];
+  }
+  hipLaunchKernelGGL(k_step_advance_packed, dim3(1), dim3(64), 0, (hipStream_t)stream, t, n, by);
+  MGH_CHECK_LAUNCH("k_step_advance_packed");
   return MGH_OK;
 }
 

@@ -264,11 +264,13 @@ __device__ __forceinline__ void pml_comp(const PmlArgs<T>& pa, int n0, int n1, i
   }
 }
 
+// The E phase of one launch unit (a device function shared by the single
+// launch and the packed launch of several grids).
 template <typename T, int NS, bool CURL64, bool PML>
-__global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __restrict__ h,
-                                                 const T* __restrict__ ae,
-                                                 const T* __restrict__ be, Plan pl, int nsets,
-                                                 int64_t e_set, int64_t h_set, PmlArgs<T> pa) {
+__device__ __forceinline__ void curl_e(T* __restrict__ e, const T* __restrict__ h,
+                                       const T* __restrict__ ae, const T* __restrict__ be,
+                                       Plan pl, int nsets, int64_t e_set, int64_t h_set,
+                                       const PmlArgs<T>& pa) {
   int grp, i, j, k0, k1;
   if (!unit_of(pl, grp, i, j, k0, k1)) return;
   const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
@@ -410,11 +412,12 @@ __global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __r
   }
 }
 
+// The H phase of one launch unit.
 template <typename T, int NS, bool PML>
-__global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __restrict__ h,
-                                                 const T* __restrict__ ah,
-                                                 const T* __restrict__ bh, Plan pl, int nsets,
-                                                 int64_t e_set, int64_t h_set, PmlArgs<T> pa) {
+__device__ __forceinline__ void curl_h(const T* __restrict__ e, T* __restrict__ h,
+                                       const T* __restrict__ ah, const T* __restrict__ bh,
+                                       Plan pl, int nsets, int64_t e_set, int64_t h_set,
+                                       const PmlArgs<T>& pa) {
   int grp, i, j, k0, k1;
   if (!unit_of(pl, grp, i, j, k0, k1)) return;
   const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
@@ -515,6 +518,58 @@ __global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __r
       ex_k[q] = ex_kp;
     }
   }
+}
+
+template <typename T, int NS, bool CURL64, bool PML>
+__global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __restrict__ h,
+                                                 const T* __restrict__ ae,
+                                                 const T* __restrict__ be, Plan pl, int nsets,
+                                                 int64_t e_set, int64_t h_set, PmlArgs<T> pa) {
+  curl_e<T, NS, CURL64, PML>(e, h, ae, be, pl, nsets, e_set, h_set, pa);
+}
+
+template <typename T, int NS, bool PML>
+__global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __restrict__ h,
+                                                 const T* __restrict__ ah,
+                                                 const T* __restrict__ bh, Plan pl, int nsets,
+                                                 int64_t e_set, int64_t h_set, PmlArgs<T> pa) {
+  curl_h<T, NS, PML>(e, h, ah, bh, pl, nsets, e_set, h_set, pa);
+}
+
+// Packed launches: the curl phase of several grids (jobs of a serving loop)
+// in one launch, work-group row y for job y.  Each job keeps its own plan,
+// buffers and XCD bands; one set per thread; no CPML.
+constexpr int PACK_MAX = 16;
+
+template <typename T>
+struct PackJob {
+  T* e;
+  T* h;
+  const T* a;  // alpha of the phase's side
+  const T* b;  // beta
+  int64_t e_set, h_set;
+  int nsets, pad;
+  Plan pl;
+};
+
+template <typename T>
+struct Pack {
+  int n, pad;
+  PackJob<T> j[PACK_MAX];
+};
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_update_e_pack(Pack<T> pk) {
+  const PackJob<T>& J = pk.j[blockIdx.y];
+  const PmlArgs<T> none{};
+  curl_e<T, 1, false, false>(J.e, J.h, J.a, J.b, J.pl, J.nsets, J.e_set, J.h_set, none);
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_update_h_pack(Pack<T> pk) {
+  const PackJob<T>& J = pk.j[blockIdx.y];
+  const PmlArgs<T> none{};
+  curl_h<T, 1, false>(J.e, J.h, J.a, J.b, J.pl, J.nsets, J.e_set, J.h_set, none);
 }
 
 // ---------------------------------------------------------------------------
@@ -1282,6 +1337,42 @@ int check_common(int dtype, int nx, int ny, int nz, int nsets, int shape) {
   return MGH_OK;
 }
 
+// The packed launch of one phase (E: side 0, H: side 1) over n jobs given as
+// host words per job: nx, ny, nz, nsets, E, E set stride, H, H set stride,
+// alpha and beta of the phase's side.
+template <typename T>
+int run_packed(int side, int n, const long long* w, int shape, hipStream_t st) {
+  Pack<T> pk;
+  std::memset(&pk, 0, sizeof(pk));
+  pk.n = n;
+  int blocks = 0;
+  for (int q = 0; q < n; ++q) {
+    const long long* r = w + (int64_t)q * 10;
+    const int nx = (int)r[0], ny = (int)r[1], nz = (int)r[2], nsets = (int)r[3];
+    int rc = check_common(sizeof(T), nx, ny, nz, nsets, shape);
+    if (rc) return rc;
+    PackJob<T>& J = pk.j[q];
+    J.e = (T*)r[4];
+    J.h = (T*)r[6];
+    J.a = (const T*)r[8];
+    J.b = (const T*)r[9];
+    MGH_REQUIRE(J.e && J.h && J.a && J.b, "packed job %d: null device pointer", q);
+    J.e_set = r[5];
+    J.h_set = r[7];
+    J.nsets = nsets;
+    const Shape sh = decode_shape(shape, nsets);
+    J.pl = make_plan(nx, ny, nz, nsets, sh.kchunk, 1, sh.band);
+    blocks = std::max(blocks, plan_blocks(J.pl));
+  }
+  if (blocks == 0) return MGH_OK;
+  if (side == 0)
+    hipLaunchKernelGGL((k_update_e_pack<T>), dim3(blocks, n), dim3(WG), 0, st, pk);
+  else
+    hipLaunchKernelGGL((k_update_h_pack<T>), dim3(blocks, n), dim3(WG), 0, st, pk);
+  MGH_CHECK_LAUNCH("k_update_pack");
+  return MGH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1481,6 +1572,34 @@ int mgh_update_h_pml(int dtype, int nx, int ny, int nz, int nsets, const void* e
   if (rc || !cpml) return rc;
   return mgh_cpml_xcols_h(dtype, nx, ny, nz, nsets, e, e_set_stride, h, h_set_stride, beta_h,
                           cpml, stream);
+}
+
+// Packed E / H phases of several grids in one launch each (a serving loop's
+// small jobs): jobs = n_jobs x 10 words (nx, ny, nz, nsets, E, E set stride,
+// H, H set stride, alpha and beta of the phase's side), shape = the launch
+// shape word (its sets-per-thread part must be 0 or 1: one set per thread).
+int mgh_update_e_packed(int dtype, int n_jobs, const long long* jobs, int shape, void* stream) {
+  MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 (float32) or 8 (float64), got %d",
+              dtype);
+  MGH_REQUIRE(n_jobs >= 1 && n_jobs <= PACK_MAX, "n_jobs must be 1..%d, got %d", PACK_MAX,
+              n_jobs);
+  MGH_REQUIRE(jobs, "null job table");
+  MGH_REQUIRE(((shape >> 16) & 0xf) <= 1, "packed launches take one set per thread");
+  hipStream_t st = (hipStream_t)stream;
+  return dtype == 4 ? run_packed<float>(0, n_jobs, jobs, shape, st)
+                    : run_packed<double>(0, n_jobs, jobs, shape, st);
+}
+
+int mgh_update_h_packed(int dtype, int n_jobs, const long long* jobs, int shape, void* stream) {
+  MGH_REQUIRE(dtype == 4 || dtype == 8, "dtype must be 4 (float32) or 8 (float64), got %d",
+              dtype);
+  MGH_REQUIRE(n_jobs >= 1 && n_jobs <= PACK_MAX, "n_jobs must be 1..%d, got %d", PACK_MAX,
+              n_jobs);
+  MGH_REQUIRE(jobs, "null job table");
+  MGH_REQUIRE(((shape >> 16) & 0xf) <= 1, "packed launches take one set per thread");
+  hipStream_t st = (hipStream_t)stream;
+  return dtype == 4 ? run_packed<float>(1, n_jobs, jobs, shape, st)
+                    : run_packed<double>(1, n_jobs, jobs, shape, st);
 }
 
 }  // extern "C"

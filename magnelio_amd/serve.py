@@ -19,7 +19,10 @@ kernel is deterministic, so a job's S-matrix in the mix equals its S-matrix
 alone bit for bit.  A job may ask for a float64 check of its result
 (``SweepJob.fp64_check``): the check solver is admitted with the job (its
 footprint counted), advanced on the job's stream and its S-parameter error
-recorded with the job.  Host preparation (material matrices, coefficients)
+recorded with the job.  Small jobs (``pack_cells``) step together: one
+packed group (``pack.PackedGroup``) advances all of them per step in shared
+launches on a stream of its own, rebuilt whenever a member joins or leaves.
+Host preparation (material matrices, coefficients)
 runs for at most twice ``max_active`` waiting jobs at a time, and admitted
 solvers drop their float64 host copies once on the device, so host memory
 stays bounded however far arrivals outrun service.
@@ -37,7 +40,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import kernels
+from . import kernels, pack
 from .sweep import SParameterResult, ScatteringSweep
 
 
@@ -123,6 +126,7 @@ class JobRecord:
     result: SParameterResult | None = None
     fp64: dict | None = None  # the float64 check of the result, when asked for
     check_sets: int = 0  # field sets of the float64 check solver
+    packed: bool = False  # stepped in the packed group of small jobs
     status: str = "pending"  # "done", "rejected" (footprint over the budget) or "cancelled"
     reason: str = ""
 
@@ -186,6 +190,7 @@ class _Active:
     solver: object
     stream: torch.cuda.Stream
     check: object = None  # float64 check solver, stepped after the job's solver
+    packed: bool = False  # stepped in the shared packed group, not on its own stream
     events: deque = field(default_factory=deque)
 
     def completed(self) -> bool:
@@ -204,7 +209,8 @@ class SweepServer:
 
     def __init__(self, device: str = "cuda:0", hbm_budget: int | None = None, max_active: int = 8,
                  chunk: int = 256, inflight: int = 2, build_workers: int = 2,
-                 use_graphs: bool = True, build_pool=None, progress: int = 0):
+                 use_graphs: bool = True, build_pool=None, progress: int = 0,
+                 pack_cells: int = 0):
         self.device = torch.device(device)
         self.hbm_budget = hbm_budget
         self.max_active = int(max_active)
@@ -216,6 +222,10 @@ class SweepServer:
         # bound and hold the GIL); None = a thread pool of build_workers
         self.build_pool = build_pool
         self.progress = int(progress)  # report every `progress` finished jobs (0: never)
+        # resident jobs of at most this many cells (and packable: no CPML,
+        # sources or monitors, no float64 check) step together in shared
+        # launches (pack.PackedGroup); 0: every job on its own
+        self.pack_cells = int(pack_cells)
         self._cancel: set = set()
 
     def cancel(self, job_id: int) -> None:
@@ -307,6 +317,9 @@ class SweepServer:
         heapq.heapify(arrivals)
         waiting: list[tuple] = []  # (job, sweep, solver prepared on the host)
         active: list[_Active] = []
+        pack_stream = torch.cuda.Stream(device=self.device)
+        pack_events: deque = deque()
+        pack_key, group = None, None
         used = 0
         peak_resident = peak_bytes = 0
         done = 0
@@ -372,14 +385,37 @@ class SweepServer:
                     stream = torch.cuda.Stream(device=self.device)
                     stream.wait_stream(torch.cuda.current_stream(self.device))
                     rec.admitted = time.perf_counter() - t0
-                    active.append(_Active(rec, sw, solver, stream, check))
+                    packed = (self.pack_cells > 0 and rec.cells <= self.pack_cells
+                              and check is None and pack.packable(solver))
+                    rec.packed = packed
+                    active.append(_Active(rec, sw, solver, stream, check, packed))
                     used += rec.footprint
                     waiting.remove(item)
                 peak_resident = max(peak_resident, len(active))
                 peak_bytes = max(peak_bytes, used)
                 # one chunk per resident job per round, up to `inflight` queued
                 progressed = False
+                # small jobs: one packed group, rebuilt when its members change
+                members = [a for a in active if a.packed and not a.completed()
+                           and a.rec.job_id not in self._cancel][:pack.MAX_JOBS]
+                while pack_events and pack_events[0].query():
+                    pack_events.popleft()
+                if members and len(pack_events) < self.inflight:
+                    key = tuple(a.rec.job_id for a in members)
+                    if key != pack_key:
+                        group = pack.PackedGroup([a.solver for a in members],
+                                                 use_graphs=self.use_graphs)
+                        pack_key = key
+                        pack_stream.wait_stream(torch.cuda.current_stream(self.device))
+                    with torch.cuda.stream(pack_stream):
+                        group.launch(self.chunk)
+                        ev = torch.cuda.Event()
+                        ev.record(pack_stream)
+                    pack_events.append(ev)
+                    progressed = True
                 for a in active:
+                    if a.packed:
+                        continue
                     while a.events and a.events[0].query():
                         a.events.popleft()
                     if (len(a.events) < self.inflight and not a.completed()
@@ -396,8 +432,10 @@ class SweepServer:
                 # retire finished (or cancelled) jobs
                 for a in list(active):
                     cancelled = a.rec.job_id in self._cancel
-                    if (cancelled or a.completed()) and all(e.query() for e in a.events):
-                        a.stream.synchronize()
+                    drained = (not pack_events or pack_events[-1].query()) if a.packed else \
+                        all(e.query() for e in a.events)
+                    if (cancelled or a.completed()) and drained:
+                        (pack_stream if a.packed else a.stream).synchronize()
                         if cancelled:
                             a.rec.status, a.rec.reason = "cancelled", "cancelled while resident"
                         else:

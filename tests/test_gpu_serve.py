@@ -18,6 +18,8 @@ together:
   S-matrix) while the others complete unchanged; a job cancelled before the
   run and one cancelled from another thread while the loop runs end as
   "cancelled", and the rest still equal their runs alone;
+* with packing (jobs up to 40^3 cells share launches, the group rebuilt as
+  jobs come and go) every job's S-matrix still equals its run alone;
 * jobs that ask for the float64 check carry its error (every S entry of the
   checked column within 1e-4 of the float64 run) and the same S-matrices as
   without it, the check solver stepped in the mix.
@@ -80,6 +82,15 @@ def test_fp64_check_in_the_mix(jobs, alone, gpu):
         print(f"job {r.job_id}: fp32 vs fp64 {r.fp64['db']:.1f} dB")
     plain = SweepServer(max_active=4, chunk=64).run(jobs[:2]).records
     assert all(r.footprint < c.footprint for r, c in zip(plain, rep.records[:2]))
+
+
+def test_packed_mix_equals_alone(jobs, alone, gpu):
+    print("continuous mix, small jobs packed", flush=True)
+    rep = SweepServer(max_active=4, chunk=64, pack_cells=40 ** 3).run(jobs, mode="continuous")
+    assert sum(r.packed for r in rep.records) == len(jobs)
+    for r in rep.records:
+        assert _same(r.result, alone[r.job_id]), r.job_id
+    print(f"packed: p50 {rep.p50:.3f} s  p99 {rep.p99:.3f} s  {rep.jobs_per_s:.2f} jobs/s")
 
 
 def test_rejection_and_cancellation(jobs, alone, gpu):
