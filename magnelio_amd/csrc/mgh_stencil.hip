@@ -412,114 +412,6 @@ __device__ __forceinline__ void curl_e(T* __restrict__ e, const T* __restrict__ 
   }
 }
 
-// The H phase of one launch unit.
-template <typename T, int NS, bool PML>
-__device__ __forceinline__ void curl_h(const T* __restrict__ e, T* __restrict__ h,
-                                       const T* __restrict__ ah, const T* __restrict__ bh,
-                                       Plan pl, int nsets, int64_t e_set, int64_t h_set,
-                                       const PmlArgs<T>& pa) {
-  int grp, i, j, k0, k1;
-  if (!unit_of(pl, grp, i, j, k0, k1)) return;
-  const int nx = pl.nx, ny = pl.ny, nz = pl.nz;
-  const int s0 = grp * NS;
-
-  const mgh::Dims d{nx, ny, nz};
-  const int64_t oEy = d.n_ex(), oEz = oEy + d.n_ey();
-  const int64_t oHy = d.n_hx(), oHz = oHy + d.n_hy();
-  const int64_t pEx = (int64_t)(ny + 1) * nx, pEy = (int64_t)ny * (nx + 1),
-                pEz = (int64_t)(ny + 1) * (nx + 1);
-  const int64_t pHx = (int64_t)ny * (nx + 1), pHy = (int64_t)(ny + 1) * nx,
-                pHz = (int64_t)ny * nx;
-  const int64_t rN = (int64_t)j * nx + i;
-  const int64_t rN1 = (int64_t)j * (nx + 1) + i;
-  const bool inx = i < nx, iny = j < ny;
-
-  // E at plane k (carried from the previous iteration's k+1 load)
-  T ey_k[NS], ex_k[NS];
-#pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    ey_k[q] = T(0);
-    ex_k[q] = T(0);
-    if (s0 + q < nsets) {
-      const T* es = e + (int64_t)(s0 + q) * e_set;
-      if (iny) ey_k[q] = es[oEy + (int64_t)k0 * pEy + rN1];
-      if (inx) ex_k[q] = es[(int64_t)k0 * pEx + rN];
-    }
-  }
-
-  for (int k = k0; k < k1; ++k) {
-    const bool knz = k < nz;
-    bool hit = false;
-    if constexpr (PML) hit = pml_hit(pa, i, j, k);
-    const int64_t iHx = (int64_t)k * pHx + rN1;
-    const int64_t iHy = oHy + (int64_t)k * pHy + rN;
-    const int64_t iHz = oHz + (int64_t)k * pHz + rN;
-    const bool hasHx = iny && knz, hasHy = inx && knz, hasHz = inx && iny;
-    T a_x = T(0), b_x = T(0), a_y = T(0), b_y = T(0), a_z = T(0), b_z = T(0);
-    if (hasHx) { a_x = ah[iHx]; b_x = bh[iHx]; }
-    if (hasHy) { a_y = ah[iHy]; b_y = bh[iHy]; }
-    if (hasHz) { a_z = ah[iHz]; b_z = bh[iHz]; }
-
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      if (s0 + q >= nsets) break;
-      const T* es = e + (int64_t)(s0 + q) * e_set;
-      T* hs = h + (int64_t)(s0 + q) * h_set;
-      // Every word of this set at this plane is loaded before its first store
-      // (see k_update_e).  E at plane k+1 exists for k+1 <= nz.
-      T ey_kp = T(0), ex_kp = T(0), ez = T(0), ez_jp = T(0), ez_ip = T(0), ey_ip = T(0),
-        ex_jp = T(0), hxv = T(0), hyv = T(0), hzv = T(0);
-      if (iny && knz) ey_kp = es[oEy + (int64_t)(k + 1) * pEy + rN1];
-      if (inx && knz) ex_kp = es[(int64_t)(k + 1) * pEx + rN];
-      if (knz) ez = es[oEz + (int64_t)k * pEz + rN1];
-      if (hasHx) {
-        ez_jp = es[oEz + (int64_t)k * pEz + rN1 + (nx + 1)];
-        hxv = hs[iHx];
-      }
-      if (hasHy) {
-        ez_ip = es[oEz + (int64_t)k * pEz + rN1 + 1];
-        hyv = hs[iHy];
-      }
-      if (hasHz) {
-        ey_ip = es[oEy + (int64_t)k * pEy + rN1 + 1];
-        ex_jp = es[(int64_t)k * pEx + rN + nx];
-        hzv = hs[iHz];
-      }
-      T vx = T(0), vy = T(0), vz = T(0);
-      if (hasHx) {  // Hx(i,j,k): dEz/dy - dEy/dz
-        const T c = ((ez_jp - ez) - ey_kp) + ey_k[q];
-        vx = (a_x * hxv) - (b_x * c);
-      }
-      if (hasHy) {  // Hy(i,j,k): dEx/dz - dEz/dx
-        const T c = ((ex_kp - ex_k[q]) - ez_ip) + ez;
-        vy = (a_y * hyv) - (b_y * c);
-      }
-      if (hasHz) {  // Hz(i,j,k): dEy/dx - dEx/dy
-        const T c = ((ey_ip - ey_k[q]) - ex_jp) + ex_k[q];
-        vz = (a_z * hzv) - (b_z * c);
-      }
-      if constexpr (PML) {
-        if (hit) {
-          // Hx: z faces d = Ey(k+1) - Ey(k) (-), y faces Ez(j+1) - Ez(j) (+)
-          if (hasHx) pml_comp<T, 2, 0, -1, 1, 0, +1>(pa, nx + 1, ny, i, j, k, s0 + q,
-                                                     ey_kp - ey_k[q], ez_jp - ez, b_x, &vx);
-          // Hy: z faces d = Ex(k+1) - Ex(k) (+), x faces Ez(i+1) - Ez(i) (-)
-          if (hasHy) pml_comp<T, 2, 1, +1, 0, 0, -1>(pa, nx, ny + 1, i, j, k, s0 + q,
-                                                     ex_kp - ex_k[q], ez_ip - ez, b_y, &vy);
-          // Hz: y faces d = Ex(j+1) - Ex(j) (-), x faces Ey(i+1) - Ey(i) (+)
-          if (hasHz) pml_comp<T, 1, 1, -1, 0, 1, +1>(pa, nx, ny, i, j, k, s0 + q,
-                                                     ex_jp - ex_k[q], ey_ip - ey_k[q], b_z, &vz);
-        }
-      }
-      if (hasHx) hs[iHx] = vx;
-      if (hasHy) hs[iHy] = vy;
-      if (hasHz) hs[iHz] = vz;
-      ey_k[q] = ey_kp;
-      ex_k[q] = ex_kp;
-    }
-  }
-}
-
 template <typename T, int NS, bool CURL64, bool PML>
 __global__ __launch_bounds__(WG) void k_update_e(T* __restrict__ e, const T* __restrict__ h,
                                                  const T* __restrict__ ae,
@@ -533,7 +425,7 @@ __global__ __launch_bounds__(WG) void k_update_h(const T* __restrict__ e, T* __r
                                                  const T* __restrict__ ah,
                                                  const T* __restrict__ bh, Plan pl, int nsets,
                                                  int64_t e_set, int64_t h_set, PmlArgs<T> pa) {
-  curl_h<T, NS, PML>(e, h, ah, bh, pl, nsets, e_set, h_set, pa);
+#include "mgh_curl_h_body.inc"
 }
 
 // Packed launches: the curl phase of several grids (jobs of a serving loop)
@@ -567,9 +459,18 @@ __global__ __launch_bounds__(WG) void k_update_e_pack(Pack<T> pk) {
 
 template <typename T>
 __global__ __launch_bounds__(WG) void k_update_h_pack(Pack<T> pk) {
+  constexpr int NS = 1;
+  constexpr bool PML = false;
   const PackJob<T>& J = pk.j[blockIdx.y];
-  const PmlArgs<T> none{};
-  curl_h<T, 1, false>(J.e, J.h, J.a, J.b, J.pl, J.nsets, J.e_set, J.h_set, none);
+  const T* __restrict__ e = J.e;
+  T* __restrict__ h = J.h;
+  const T* __restrict__ ah = J.a;
+  const T* __restrict__ bh = J.b;
+  const Plan pl = J.pl;
+  const int nsets = J.nsets;
+  const int64_t e_set = J.e_set, h_set = J.h_set;
+  const PmlArgs<T> pa{};
+#include "mgh_curl_h_body.inc"
 }
 
 // ---------------------------------------------------------------------------
