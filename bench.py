@@ -26,6 +26,7 @@ import math
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -785,7 +786,23 @@ def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plai
         rate_s = K * cells * steps / t_s / 1e9
         out["single_excitation_path"] = {"value": rate_s, "ms_per_step_all_K": t_s / steps * 1e3,
                                          "speedup_batched": rate_b / rate_s}
+        out["batch_tuner"] = c3_batch_tuner(sweep)
     return out
+
+
+def c3_batch_tuner(sweep) -> dict:
+    """The sweep's batch="auto" choice on this workload: one throw-away
+    solver per candidate K timed over whole graph replays (a cache of its
+    own, so the choice is measured here)."""
+    import tempfile
+
+    from magnelio_amd import autotune
+
+    with tempfile.TemporaryDirectory() as td:
+        path = Path(td) / "autotune.json"
+        best = autotune.batch_for(sweep, path)
+        (entry,) = [v for k, v in autotune._load(path).items() if "|batch|" in k]
+    return {"batch": best, "step_ms": entry["step_ms"], "sweep_step_ms": entry["sweep_step_ms"]}
 
 
 def main(argv=None) -> int:
@@ -1011,6 +1028,7 @@ def main_c3(args) -> int:
         "frac_of_roofline": r["frac_of_roofline"],
         "curl_launches": r["curl_launches"],
         "single_excitation_path": r["single_excitation_path"],
+        "batch_tuner": r["batch_tuner"],
     }
     print(json.dumps(out))
     return 0

@@ -18,8 +18,19 @@ candidate is stored in a JSON cache (``MAGNELIO_HIP_AUTOTUNE_CACHE``, default
 content hash, grid, K, dtype and CPML faces; later solvers of the same key
 read it.  Every shape gives the same fields bit for bit.
 
-The reference's CUDA tier has a fixed block shape (``numba_kernels.py:446``);
-this module has no counterpart there.
+``ScatteringSweep(batch="auto")`` calls :func:`batch_for`: the number of
+excitations per solver (K) is picked the same way.  For each batch size
+``ceil(n / m)`` (the smallest largest batch of m solvers) one solver of that
+many excitations is set up and its full step (curl, ports, monitors, as a
+captured graph when the sweep uses graphs) is timed; the sweep's step costs
+``ceil(n / k)`` such steps, and the cheapest k is cached under the grid, n,
+precision, CPML faces, port channels and monitors.  Every batch size gives
+the same S-parameters bit for bit (the kernels treat the sets
+independently).
+
+The reference's CUDA tier has a fixed block shape (``numba_kernels.py:446``)
+and runs one excitation at a time (``scattering_td.py:1252-1273``); this
+module has no counterpart there.
 """
 
 from __future__ import annotations
@@ -135,5 +146,86 @@ def shape_for(solver, path: Path | None = None) -> int:
     data = _load(path)  # re-read: another process may have added entries meanwhile
     data[key] = {"shape": best, "kchunk": kc, "sets": sets, "band": band,
                  "ms": {str(k): round(v, 4) for k, v in timings.items()}}
+    _store(path, data)
+    return best
+
+
+BATCH_STEPS = 64  # steps timed per candidate batch (whole graphs where graphs are used)
+
+
+def batch_candidates(n: int) -> list[int]:
+    """Batch sizes timed for n excitations, largest first: ceil(n / m) for
+    m = 1..n solvers."""
+    return sorted({-(-n // m) for m in range(1, n + 1)}, reverse=True)
+
+
+def time_steps(solver, steps: int = BATCH_STEPS) -> float:
+    """Mean milliseconds of one full step of a set-up solver (its own
+    launches; graph replays where the solver uses graphs), after two eager
+    steps and one graph's capture and replay.  Advances the solver."""
+    dev = solver.e.device
+    stream = torch.cuda.current_stream(dev)
+    left = solver.horizon - solver.n_completed - 1  # stay short of the monitors' finalize
+    warm = min(2, left)
+    solver.launch(warm)
+    left -= warm
+    G = solver.graph_steps if solver.use_graphs else 1
+    if solver.use_graphs and left >= 2 * G:
+        solver.launch(G)  # capture and first replay, untimed
+        left -= G
+        n = G * max(1, min(left // G, -(-steps // G)))
+    else:
+        n = min(left, steps)
+    if n < 1:
+        raise ValueError("the sweep is too short to time its steps")
+    stream.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    done = solver.launch(n)
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / done
+
+
+def choose_batch(n: int, per_step: dict) -> tuple[int, dict]:
+    """Cheapest batch for n excitations from per-step times of one solver of
+    k excitations: a sweep step costs ceil(n / k) of them (the last solver
+    may hold fewer sets; it is costed as a full one).  Ties go to the larger
+    batch (fewer solvers).  Returns the batch and every candidate's cost."""
+    cost = {k: -(-n // k) * t for k, t in per_step.items()}
+    return min(cost, key=lambda k: (cost[k], -k)), cost
+
+
+def batch_key(sweep, n_monitors: int) -> str:
+    faces = sorted(f for f, bc in sweep.boundary_conditions.items()
+                   if type(bc).__name__ == "CPMLBoundary")
+    dtype = torch.float32 if sweep.precision == "single" else torch.float64
+    base = cache_key(torch.device(sweep.device), sweep.mesh.dims, len(sweep.excited), dtype, faces)
+    return f"{base}|batch|ch{len(sweep.channels)}|mon{n_monitors}|g{int(sweep.use_graphs)}"
+
+
+def batch_for(sweep, path: Path | None = None) -> int:
+    """Excitations per solver for ``sweep`` (a ScatteringSweep): from the
+    cache, or timed now (one throw-away solver per candidate) and stored."""
+    n = len(sweep.excited)
+    if n <= 1:
+        return 1
+    path = path or cache_path()
+    n_mon = len(list(sweep.monitors_factory())) if sweep.monitors_factory is not None else 0
+    key = batch_key(sweep, n_mon)
+    cands = batch_candidates(n)
+    hit = _load(path).get(key)
+    if isinstance(hit, dict) and hit.get("batch") in cands:
+        return int(hit["batch"])
+    per_step = {}
+    for k in cands:
+        s = sweep.make_solver(sweep.excited[:k])
+        s.setup()
+        per_step[k] = time_steps(s)
+        del s
+    best, cost = choose_batch(n, per_step)
+    data = _load(path)
+    data[key] = {"batch": best, "step_ms": {str(k): round(v, 4) for k, v in per_step.items()},
+                 "sweep_step_ms": {str(k): round(v, 4) for k, v in cost.items()}}
     _store(path, data)
     return best

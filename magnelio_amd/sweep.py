@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import config
+from . import autotune, config
 from .boundaries import PECBoundary
 from .signals import resolve_excitation, s_parameters
 from .solver import ChannelExcitation, FITTimeDomainSolver
@@ -135,7 +135,8 @@ class ScatteringSweep:
     ``excited``: channels to excite (default: every (port, mode)); each gets
     the reference's auto waveform for its mode (``resolve_excitation``,
     scattering_td.py:2322-2366) unless ``waveforms`` maps it to a callable.
-    ``batch``: field sets per solver (None = all channels in one solver).
+    ``batch``: field sets per solver (None = all channels in one solver;
+    "auto": timed at the first run, ``autotune.batch_for``).
     ``kchunk``: the solvers' curl launch shape (a ``kernels.launch_shape``
     word or "auto"; None: the solver's default).
     ``fp64_check``: "off", "column" (the first excited channel) or "all" —
@@ -146,7 +147,7 @@ class ScatteringSweep:
 
     def __init__(self, mesh, ports, *, dt: float, n_steps: int, f_axis, f_max: float,
                  f_min: float = 0.0, excited=None, waveforms=None, boundary_conditions=None,
-                 precision: str | None = None, batch: int | None = None,
+                 precision: str | None = None, batch: int | str | None = None,
                  use_graphs: bool | None = None,
                  monitors_factory=None, device: str = "cuda:0", graph_block: int = 32,
                  fp64_check: str = "off", kchunk=None):
@@ -168,9 +169,9 @@ class ScatteringSweep:
         # explicit arguments win; else MAGNELIO_PRECISION / MAGNELIO_GPU_GRAPHS,
         # else single precision with graphs
         self.precision = config.precision(precision)
-        self.batch = len(self.excited) if batch is None else int(batch)
-        if self.batch < 1:
-            raise ValueError("batch must be >= 1")
+        self.batch = len(self.excited) if batch is None else batch if batch == "auto" else int(batch)
+        if self.batch != "auto" and self.batch < 1:
+            raise ValueError("batch must be >= 1 or 'auto'")
         self.use_graphs = config.graphs(use_graphs, default=True)
         self.monitors_factory = monitors_factory
         self.device = device
@@ -236,6 +237,8 @@ class ScatteringSweep:
         return SParameterResult(self.f_axis, self.channels, (ch,), mat)
 
     def run(self) -> SweepResult:
+        if self.batch == "auto":
+            self.batch = autotune.batch_for(self)
         batches = [self.excited[i:i + self.batch] for i in range(0, len(self.excited), self.batch)]
         solvers = [self.make_solver(b) for b in batches]
         for s in solvers:

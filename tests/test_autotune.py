@@ -29,6 +29,22 @@ def test_shape_words_and_candidates():
     assert len(set(six)) == len(six)
 
 
+def test_batch_candidates_and_choice():
+    for n in range(1, 13):
+        c = autotune.batch_candidates(n)
+        assert c[0] == n and c[-1] == 1 and c == sorted(set(c), reverse=True)
+        # every solver count m has its smallest largest batch in the list
+        for m in range(1, n + 1):
+            assert -(-n // m) in c
+    assert autotune.batch_candidates(6) == [6, 3, 2, 1]
+    # K = 6 in one solver: 1.0 ms a step beats two solvers of 3 (2 x 0.55)
+    best, cost = autotune.choose_batch(6, {6: 1.0, 3: 0.55, 2: 0.4, 1: 0.3})
+    assert best == 6 and cost == {6: 1.0, 3: 1.1, 2: pytest.approx(1.2), 1: pytest.approx(1.8)}
+    assert autotune.choose_batch(6, {6: 1.3, 3: 0.5, 2: 0.4, 1: 0.3})[0] == 3
+    assert autotune.choose_batch(6, {6: 1.0, 3: 0.5, 2: 0.5, 1: 0.3})[0] == 6  # tie: larger batch
+    assert autotune.choose_batch(7, {7: 1.0, 4: 0.45, 3: 0.4, 2: 0.3, 1: 0.2})[0] == 4  # 2 x 0.45
+
+
 def test_cache_roundtrip_and_bad_files(tmp_path):
     path = tmp_path / "sub" / "autotune.json"
     assert autotune._load(path) == {}  # missing

@@ -77,3 +77,34 @@ def test_empty_guide_physics(sweeps, gpu):
         for n in range(3):
             if n != m:
                 assert np.abs(r.S("port2", "port1", n, m)).max() < 1e-8
+
+
+def test_auto_batch_times_caches_and_matches(tmp_path, monkeypatch, gpu):
+    """batch="auto": each candidate K is timed once (a throw-away solver),
+    the choice is cached, a second sweep reads it, and the S-matrix equals
+    the all-in-one-solver sweep's bit for bit."""
+    from magnelio_amd import autotune
+
+    monkeypatch.setenv("MAGNELIO_HIP_AUTOTUNE_CACHE", str(tmp_path / "autotune.json"))
+    mesh, ports, dt = cases.c1_mesh_ports()
+    wf = {(p.name, m): Excitation(18e9, 25e9, m) for p in ports for m in range(p.n_modes)}
+    kw = dict(dt=dt, n_steps=600, f_axis=F, f_max=cases.C1_FMAX, waveforms=wf)
+    print("C1 sweep, all excitations in one solver", flush=True)
+    ref = ScatteringSweep(mesh, ports, **kw).run()
+    print("C1 sweep, batch='auto' (times K = 6, 3, 2, 1)", flush=True)
+    sw = ScatteringSweep(mesh, ports, batch="auto", **kw)
+    res = sw.run()
+    assert sw.batch in autotune.batch_candidates(6)
+    assert [len(b) for b in res.batches] == [len(sw.excited[i:i + sw.batch]) for i in range(0, 6, sw.batch)]
+    (entry,) = [v for k, v in autotune._load(tmp_path / "autotune.json").items() if "|batch|" in k]
+    assert entry["batch"] == sw.batch and set(entry["step_ms"]) == {"6", "3", "2", "1"}
+    print(f"C1 batch choice {sw.batch}: step ms {entry['step_ms']}", flush=True)
+    assert np.array_equal(res.s_params.matrix, ref.s_params.matrix, equal_nan=True)
+
+    def no_timing(*args, **kwargs):
+        raise AssertionError("cached batch timed again")
+
+    monkeypatch.setattr(autotune, "time_steps", no_timing)
+    again = ScatteringSweep(mesh, ports, batch="auto", **kw)
+    again.run()
+    assert again.batch == sw.batch

@@ -86,3 +86,20 @@ def test_sweep_fp64_check_arguments():
         ScatteringSweep(mesh, ports, fp64_check="some", **kw)
     with pytest.raises(ValueError, match="single-precision"):
         ScatteringSweep(mesh, ports, fp64_check="column", precision="double", **kw)
+
+
+def test_sweep_batch_arguments():
+    """batch: None (every excitation in one solver), an int, or "auto"."""
+    import cases
+    from magnelio_amd.sweep import ScatteringSweep
+
+    mesh, ports, dt = cases.c1_mesh_ports(dims=(12, 6, 16))
+    kw = dict(dt=dt, n_steps=10, f_axis=np.linspace(8e9, 20e9, 5), f_max=cases.C1_FMAX)
+    assert ScatteringSweep(mesh, ports, **kw).batch == 6
+    assert ScatteringSweep(mesh, ports, batch=2, **kw).batch == 2
+    assert ScatteringSweep(mesh, ports, batch="auto", **kw).batch == "auto"
+    for bad in (0, -1):
+        with pytest.raises(ValueError, match="batch must be"):
+            ScatteringSweep(mesh, ports, batch=bad, **kw)
+    with pytest.raises(ValueError):
+        ScatteringSweep(mesh, ports, batch="some", **kw)
