@@ -197,8 +197,9 @@ def choose_batch(n: int, per_step: dict) -> tuple[int, dict]:
 
 
 def batch_key(sweep, n_monitors: int) -> str:
-    faces = sorted(f for f, bc in sweep.boundary_conditions.items()
-                   if type(bc).__name__ == "CPMLBoundary")
+    from .boundaries import CPMLBoundary  # noqa: PLC0415
+
+    faces = sorted(f for f, bc in sweep.boundary_conditions.items() if isinstance(bc, CPMLBoundary))
     dtype = torch.float32 if sweep.precision == "single" else torch.float64
     base = cache_key(torch.device(sweep.device), sweep.mesh.dims, len(sweep.excited), dtype, faces)
     return f"{base}|batch|ch{len(sweep.channels)}|mon{n_monitors}|g{int(sweep.use_graphs)}"
