@@ -207,10 +207,11 @@ def batch_key(sweep, n_monitors: int) -> str:
 
 def batch_for(sweep, path: Path | None = None) -> int:
     """Excitations per solver for ``sweep`` (a ScatteringSweep): from the
-    cache, or timed now (one throw-away solver per candidate) and stored."""
+    cache, or timed now (one throw-away solver per candidate) and stored.
+    A sweep of fewer than 8 steps keeps every excitation in one solver."""
     n = len(sweep.excited)
-    if n <= 1:
-        return 1
+    if n <= 1 or sweep.n_steps < 8:  # nothing to choose, or too short a run to time
+        return n
     path = path or cache_path()
     n_mon = len(list(sweep.monitors_factory())) if sweep.monitors_factory is not None else 0
     key = batch_key(sweep, n_mon)
