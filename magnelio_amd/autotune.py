@@ -211,7 +211,7 @@ def batch_for(sweep, path: Path | None = None) -> int:
     A sweep of fewer than 8 steps keeps every excitation in one solver."""
     n = len(sweep.excited)
     if n <= 1 or sweep.n_steps < 8:  # nothing to choose, or too short a run to time
-        return n
+        return max(n, 1)
     path = path or cache_path()
     n_mon = len(list(sweep.monitors_factory())) if sweep.monitors_factory is not None else 0
     key = batch_key(sweep, n_mon)
