@@ -686,7 +686,7 @@ MID_RUN_STEP = 1024
 
 
 def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plain: bool = False,
-               single_path: bool = True) -> dict:
+               single_path: bool = True, batch_tuner: bool = False) -> dict:
     """The batched K = 6 sweep of the C3 guide (2 ports x 3 modes as six
     field sets of one solver, graph replays, pipelined 2001-frequency
     field-plane DFT) and the same six excitations run one per solver, each
@@ -786,6 +786,7 @@ def c3_batched(warmup: int, steps: int, freqs: int = 2001, block: int = 64, plai
         rate_s = K * cells * steps / t_s / 1e9
         out["single_excitation_path"] = {"value": rate_s, "ms_per_step_all_K": t_s / steps * 1e3,
                                          "speedup_batched": rate_b / rate_s}
+    if batch_tuner:
         out["batch_tuner"] = c3_batch_tuner(sweep)
     return out
 
@@ -824,6 +825,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-tfsf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="leave the batched K=6 sweep out of the line")
+    ap.add_argument("--batch-tuner", action="store_true",
+                    help="c3: also time the sweep's batch=\"auto\" candidates (K = 6, 3, 2, 1) and report the choice")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="a short stencil-only run of the c2 grid plus a 1 GiB device copy, "
                          "for HBM traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
@@ -1006,7 +1009,8 @@ def main_c3(args) -> int:
     t_setup = time.perf_counter()
     freqs = 2001 if args.freqs == 1001 else args.freqs
     block = 64 if args.dft_block == 256 else args.dft_block
-    r = c3_batched(args.warmup, args.steps, freqs=freqs, block=block, plain=args.dft_plain)
+    r = c3_batched(args.warmup, args.steps, freqs=freqs, block=block, plain=args.dft_plain,
+                   batch_tuner=args.batch_tuner)
     out = {
         "metric": "batched sweep Gcell-updates/s (K excitations per launch) vs one excitation per launch",
         "value": r["value"],
@@ -1028,8 +1032,9 @@ def main_c3(args) -> int:
         "frac_of_roofline": r["frac_of_roofline"],
         "curl_launches": r["curl_launches"],
         "single_excitation_path": r["single_excitation_path"],
-        "batch_tuner": r["batch_tuner"],
     }
+    if "batch_tuner" in r:
+        out["batch_tuner"] = r["batch_tuner"]
     print(json.dumps(out))
     return 0
 
